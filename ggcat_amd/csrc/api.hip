@@ -1,0 +1,536 @@
+// api.hip -- C ABI (include/ggcat_gpu.h) and host orchestration of the
+// bucketing and merge kernels.  Unity build: the kernel files are included.
+#include "../../include/ggcat_gpu.h"
+#include "bucketing.hip"
+#include "merge.hip"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace gg;
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+}  // namespace
+
+struct ggcat_gpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    hipEvent_t ev[16];
+    // buffers
+    DevBuf input, seg_bits, sub_count, sub_cursor, sub_off, sub_kmers, slots, pool, pool_cursor, slot_color,
+        color_starts, colors;
+    DevBuf work, big_list, big_count, out_key, out_val, out_sub, out_cursor, g_klo, g_khi, g_st, g_val, g_off,
+        g_cap;
+    // state
+    ggcat_gpu_params P{};
+    bool have_records = false, have_solid = false;
+    uint32_t n_sub = 0, slot_words = 0, key_words = 1;
+    uint64_t n_records = 0, pool_words = 0, n_solid = 0, n_input = 0, kmer_occ = 0, n_big = 0, n_long = 0;
+    bool colored = false;
+    double ms[8] = {0};
+};
+
+namespace {
+
+int fail(ggcat_gpu_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                 \
+    do {                                                                                             \
+        hipError_t e_ = (expr);                                                                      \
+        if (e_ != hipSuccess)                                                                        \
+            return fail(ctx, GGCAT_GPU_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+int ensure(ggcat_gpu_ctx* ctx, DevBuf& b, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (b.cap >= bytes) return GGCAT_GPU_OK;
+    if (b.p) {
+        hipError_t e = hipFree(b.p);
+        if (e != hipSuccess) return fail(ctx, GGCAT_GPU_EHIP, std::string("hipFree: ") + hipGetErrorString(e));
+        b.p = nullptr;
+        b.cap = 0;
+    }
+    size_t want = bytes + bytes / 8;
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(ctx, GGCAT_GPU_ENOMEM, "hipMalloc(" + std::to_string(want) + "): " + hipGetErrorString(e));
+    }
+    b.cap = want;
+    return GGCAT_GPU_OK;
+}
+
+#define ENSURE(buf, bytes)                                  \
+    do {                                                    \
+        int r_ = ensure(ctx, (buf), (bytes));               \
+        if (r_ != GGCAT_GPU_OK) return r_;                  \
+    } while (0)
+
+template <class T>
+T* P_(DevBuf& b) {
+    return reinterpret_cast<T*>(b.p);
+}
+
+// ntHash rolling tables, index out*4+in with out == 4 meaning "no base leaves"
+// (crates/hashes/src/nthash_base.rs:49-61 on 2-bit codes A=0,C=1,T=2,G=3).
+void nthash_tables(uint32_t m, uint64_t* gtab) {
+    auto rotl = [](uint64_t x, unsigned r) { r &= 63; return r ? (x << r) | (x >> (64 - r)) : x; };
+    auto rotr = [](uint64_t x, unsigned r) { r &= 63; return r ? (x >> r) | (x << (64 - r)) : x; };
+    auto h = [](uint32_t c) { return (uint64_t)((c << 1) + 1) * NT_MULT; };
+    auto rcc = [](uint32_t c) { return (uint64_t)(((c << 1) ^ 4) + 1) * NT_MULT; };
+    for (uint32_t out = 0; out < 5; out++) {
+        for (uint32_t in = 0; in < 4; in++) {
+            uint32_t e = out * 4 + in;
+            uint64_t f = h(in), r = rotl(rcc(in), m - 1);
+            if (out < 4) {
+                f ^= rotl(h(out), m);
+                r ^= rotr(rcc(out), 1);
+            }
+            gtab[2 * e] = f;
+            gtab[2 * e + 1] = r;
+        }
+    }
+}
+
+u128_t mk128(uint64_t hi, uint64_t lo) { return ((u128_t)hi << 64) | lo; }
+
+const uint32_t TILE_T = 1024;
+
+}  // namespace
+
+extern "C" {
+
+int ggcat_gpu_create(int device, ggcat_gpu_ctx** out) {
+    if (!out) return GGCAT_GPU_EINVAL;
+    *out = nullptr;
+    ggcat_gpu_ctx* ctx = new ggcat_gpu_ctx();
+    ctx->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    for (int i = 0; i < 16 && e == hipSuccess; i++) e = hipEventCreate(&ctx->ev[i]);
+    if (e != hipSuccess) {
+        delete ctx;
+        return GGCAT_GPU_EHIP;
+    }
+    *out = ctx;
+    return GGCAT_GPU_OK;
+}
+
+void ggcat_gpu_destroy(ggcat_gpu_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    DevBuf* bufs[] = {&ctx->input, &ctx->seg_bits, &ctx->sub_count, &ctx->sub_cursor, &ctx->sub_off,
+                      &ctx->sub_kmers, &ctx->slots, &ctx->pool, &ctx->pool_cursor, &ctx->slot_color,
+                      &ctx->color_starts, &ctx->colors, &ctx->work, &ctx->big_list, &ctx->big_count,
+                      &ctx->out_key, &ctx->out_val, &ctx->out_sub, &ctx->out_cursor, &ctx->g_klo,
+                      &ctx->g_khi, &ctx->g_st, &ctx->g_val, &ctx->g_off, &ctx->g_cap};
+    for (DevBuf* b : bufs)
+        if (b->p) (void)hipFree(b->p);
+    for (int i = 0; i < 16; i++) (void)hipEventDestroy(ctx->ev[i]);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* ggcat_gpu_last_error(const ggcat_gpu_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_bytes, int input_on_device,
+                        const uint64_t* color_starts, const uint32_t* colors, uint32_t n_colors,
+                        const ggcat_gpu_params* params) {
+    if (!ctx) return GGCAT_GPU_EINVAL;
+    if (!params || (!data && n_bytes)) return fail(ctx, GGCAT_GPU_EINVAL, "null params or data");
+    const ggcat_gpu_params p = *params;
+    if (p.k < 2 || p.k > 64) return fail(ctx, GGCAT_GPU_EINVAL, "k must be in [2, 64]");
+    if (p.m < 1 || p.m + 2 > p.k) return fail(ctx, GGCAT_GPU_EINVAL, "m must satisfy 1 <= m <= k-2");
+    if (p.buckets_log < 1 || p.buckets_log > 13) return fail(ctx, GGCAT_GPU_EINVAL, "buckets_log must be in [1, 13]");
+    if (p.second_buckets_log > 8) return fail(ctx, GGCAT_GPU_EINVAL, "second_buckets_log must be <= 8");
+    if (p.hash_kind > 1) return fail(ctx, GGCAT_GPU_EINVAL, "unknown hash kind");
+    if (p.hash_kind == GGCAT_GPU_HASH_SEQ && p.k > 64) return fail(ctx, GGCAT_GPU_EINVAL, "seq-hash needs k <= 64");
+    if (n_colors && (!color_starts || !colors)) return fail(ctx, GGCAT_GPU_EINVAL, "null color arrays");
+    if (input_on_device && (reinterpret_cast<uintptr_t>(data) & 15))
+        return fail(ctx, GGCAT_GPU_EINVAL, "device input must be 16-byte aligned");
+    HIPCHK(hipSetDevice(ctx->device));
+    ctx->have_records = ctx->have_solid = false;
+    ctx->P = p;
+    const uint32_t w = p.k - p.m;
+    const uint32_t n_sub = ((1u << p.buckets_log) + 1) << p.second_buckets_log;
+    // slot: meta word + inline bases for the longest unique-minimizer record (2k - m)
+    uint32_t sw = 1 + (2 * p.k - p.m + 15) / 16;
+    uint32_t slot_words = 4;
+    while (slot_words < sw) slot_words <<= 1;
+    ctx->n_sub = n_sub;
+    ctx->slot_words = slot_words;
+    ctx->n_input = n_bytes;
+    ctx->colored = n_colors > 0;
+
+    const uint8_t* d_in = data;
+    if (!input_on_device) {
+        ENSURE(ctx->input, n_bytes + 16);
+        HIPCHK(hipMemcpyAsync(ctx->input.p, data, n_bytes, hipMemcpyHostToDevice, ctx->stream));
+        d_in = P_<uint8_t>(ctx->input);
+    }
+    const uint64_t n_tiles = (n_bytes + TILE_T - 1) / TILE_T;
+    const uint64_t seg_words = n_tiles * TILE_T / 32 + 2;
+    ENSURE(ctx->seg_bits, seg_words * 4);
+    ENSURE(ctx->sub_count, (size_t)n_sub * 4);
+    ENSURE(ctx->sub_cursor, (size_t)n_sub * 4);
+    ENSURE(ctx->sub_kmers, (size_t)n_sub * 4);
+    ENSURE(ctx->sub_off, ((size_t)n_sub + 1) * 8);
+    ENSURE(ctx->pool_cursor, 8);
+    if (n_colors) {
+        ENSURE(ctx->color_starts, (size_t)n_colors * 8);
+        ENSURE(ctx->colors, (size_t)n_colors * 4);
+        HIPCHK(hipMemcpyAsync(ctx->color_starts.p, color_starts, (size_t)n_colors * 8, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->colors.p, colors, (size_t)n_colors * 4, hipMemcpyHostToDevice, ctx->stream));
+    }
+
+    BucketArgs A{};
+    A.seq = d_in;
+    A.n = n_bytes;
+    A.k = p.k;
+    A.m = p.m;
+    A.w = w;
+    A.b = p.buckets_log;
+    A.sb = p.second_buckets_log;
+    A.canonical = p.forward_only ? 0 : 1;
+    A.T = TILE_T;
+    A.n_tiles = n_tiles;
+    A.seg_bits = P_<uint32_t>(ctx->seg_bits);
+    A.sub_count = P_<uint32_t>(ctx->sub_count);
+    A.sub_cursor = P_<uint32_t>(ctx->sub_cursor);
+    A.sub_off = P_<uint64_t>(ctx->sub_off);
+    A.sub_kmers = P_<uint32_t>(ctx->sub_kmers);
+    A.slot_words = slot_words;
+    A.pool_cursor = P_<unsigned long long>(ctx->pool_cursor);
+    A.color_starts = n_colors ? P_<uint64_t>(ctx->color_starts) : nullptr;
+    A.colors = n_colors ? P_<uint32_t>(ctx->colors) : nullptr;
+    A.n_colors = n_colors;
+    nthash_tables(p.m, A.gtab);
+    const TileGeom G = tile_geom(TILE_T, w, p.k, p.m);
+
+    HIPCHK(hipMemsetAsync(ctx->sub_count.p, 0, (size_t)n_sub * 4, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->sub_cursor.p, 0, (size_t)n_sub * 4, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->sub_kmers.p, 0, (size_t)n_sub * 4, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->seg_bits.p, 0, seg_words * 4, ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+    if (n_tiles) {
+        hipLaunchKernelGGL(bucket_count_kernel, dim3((uint32_t)n_tiles), dim3(64), G.bytes, ctx->stream, A);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, ctx->stream, A.sub_count, P_<uint64_t>(ctx->sub_off), n_sub);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
+    uint64_t total = 0;
+    HIPCHK(hipMemcpyAsync(&total, P_<uint64_t>(ctx->sub_off) + n_sub, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->n_records = total;
+    ENSURE(ctx->slots, total * slot_words * 4);
+    if (n_colors) ENSURE(ctx->slot_color, total * 4);
+    A.slots = P_<uint32_t>(ctx->slots);
+    A.slot_color = n_colors ? P_<uint32_t>(ctx->slot_color) : nullptr;
+
+    uint64_t pool_cap = std::max<uint64_t>(ctx->pool.cap / 4, n_bytes / 64 + 65536);
+    for (int attempt = 0; attempt < 2; attempt++) {
+        ENSURE(ctx->pool, pool_cap * 4);
+        A.pool = P_<uint32_t>(ctx->pool);
+        A.pool_cap = pool_cap;
+        HIPCHK(hipMemsetAsync(ctx->sub_cursor.p, 0, (size_t)n_sub * 4, ctx->stream));
+        HIPCHK(hipMemsetAsync(ctx->sub_kmers.p, 0, (size_t)n_sub * 4, ctx->stream));
+        HIPCHK(hipMemsetAsync(ctx->pool_cursor.p, 0, 8, ctx->stream));
+        HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
+        if (n_tiles) {
+            hipLaunchKernelGGL(bucket_emit_kernel, dim3((uint32_t)n_tiles), dim3(64), G.bytes, ctx->stream, A);
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
+        uint64_t used = 0;
+        HIPCHK(hipMemcpyAsync(&used, ctx->pool_cursor.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        ctx->pool_words = used;
+        if (used <= pool_cap) break;
+        if (attempt == 1) return fail(ctx, GGCAT_GPU_ENOMEM, "long-record pool overflow");
+        pool_cap = used + 1024;
+    }
+    float t;
+    HIPCHK(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[1]));
+    ctx->ms[0] = t;
+    HIPCHK(hipEventElapsedTime(&t, ctx->ev[1], ctx->ev[2]));
+    ctx->ms[1] = t;
+    HIPCHK(hipEventElapsedTime(&t, ctx->ev[3], ctx->ev[4]));
+    ctx->ms[2] = t;
+    HIPCHK(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[4]));
+    ctx->ms[5] = t;
+    // k-mer occurrence total (sizes the merge output)
+    std::vector<uint32_t> occ(n_sub);
+    HIPCHK(hipMemcpy(occ.data(), ctx->sub_kmers.p, (size_t)n_sub * 4, hipMemcpyDeviceToHost));
+    uint64_t s = 0;
+    for (uint32_t v : occ) s += v;
+    ctx->kmer_occ = s;
+    ctx->have_records = true;
+    return GGCAT_GPU_OK;
+}
+
+int ggcat_gpu_records_info(ggcat_gpu_ctx* ctx, uint64_t* n_records, uint32_t* n_sub, uint64_t* n_bases) {
+    if (!ctx) return GGCAT_GPU_EINVAL;
+    if (!ctx->have_records) return fail(ctx, GGCAT_GPU_ESTATE, "no bucketing result");
+    if (n_records) *n_records = ctx->n_records;
+    if (n_sub) *n_sub = ctx->n_sub;
+    if (n_bases) *n_bases = ctx->kmer_occ + ctx->n_records * (ctx->P.k - 1);
+    return GGCAT_GPU_OK;
+}
+
+int ggcat_gpu_records_copy(ggcat_gpu_ctx* ctx, uint32_t* sub, uint32_t* len, uint8_t* flags, uint16_t* minpos,
+                           uint32_t* color, uint64_t* off, uint8_t* bases) {
+    if (!ctx) return GGCAT_GPU_EINVAL;
+    if (!ctx->have_records) return fail(ctx, GGCAT_GPU_ESTATE, "no bucketing result");
+    HIPCHK(hipSetDevice(ctx->device));
+    const uint32_t SW = ctx->slot_words;
+    std::vector<uint32_t> slots(ctx->n_records * SW), pool(ctx->pool_words), cols;
+    std::vector<uint64_t> soff(ctx->n_sub + 1);
+    HIPCHK(hipMemcpy(slots.data(), ctx->slots.p, slots.size() * 4, hipMemcpyDeviceToHost));
+    if (!pool.empty()) HIPCHK(hipMemcpy(pool.data(), ctx->pool.p, pool.size() * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(soff.data(), ctx->sub_off.p, soff.size() * 8, hipMemcpyDeviceToHost));
+    if (ctx->colored) {
+        cols.resize(ctx->n_records);
+        HIPCHK(hipMemcpy(cols.data(), ctx->slot_color.p, cols.size() * 4, hipMemcpyDeviceToHost));
+    }
+    static const char LET[4] = {'A', 'C', 'T', 'G'};
+    uint64_t bpos = 0;
+    uint32_t s = 0;
+    for (uint64_t r = 0; r < ctx->n_records; r++) {
+        while (s + 1 < soff.size() && soff[s + 1] <= r) s++;
+        const uint32_t* sl = slots.data() + r * SW;
+        uint32_t meta = sl[0];
+        uint32_t L;
+        const uint32_t* bw;
+        if (meta & META_LONG) {
+            L = sl[1];
+            bw = pool.data() + ((uint64_t)sl[2] | ((uint64_t)sl[3] << 32));
+        } else {
+            L = meta & 0xFF;
+            bw = sl + 1;
+        }
+        if (sub) sub[r] = s;
+        if (len) len[r] = L;
+        if (flags) flags[r] = (meta >> 8) & 3;
+        if (minpos) minpos[r] = (meta >> 10) & 0xFF;
+        if (color) color[r] = ctx->colored ? cols[r] : 0;
+        if (off) off[r] = bpos;
+        if (bases)
+            for (uint32_t i = 0; i < L; i++) bases[bpos + i] = LET[(bw[i >> 4] >> (2 * (i & 15))) & 3];
+        bpos += L;
+    }
+    return GGCAT_GPU_OK;
+}
+
+int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
+    if (!ctx) return GGCAT_GPU_EINVAL;
+    if (!ctx->have_records) return fail(ctx, GGCAT_GPU_ESTATE, "no bucketing result");
+    HIPCHK(hipSetDevice(ctx->device));
+    const ggcat_gpu_params& p = ctx->P;
+    const uint32_t key_words = (p.hash_kind == GGCAT_GPU_HASH_SEQ && p.k <= 32) ? 1 : 2;
+    ctx->key_words = key_words;
+    MergeArgs M{};
+    M.slots = P_<uint32_t>(ctx->slots);
+    M.slot_words = ctx->slot_words;
+    M.pool = P_<uint32_t>(ctx->pool);
+    M.sub_off = P_<uint64_t>(ctx->sub_off);
+    M.sub_kmers = P_<uint32_t>(ctx->sub_kmers);
+    M.n_sub = ctx->n_sub;
+    M.k = p.k;
+    M.min_mult = p.min_multiplicity;
+    M.kind = p.hash_kind;
+    M.key_words = key_words;
+    M.lds_cap = key_words == 1 ? 4096 : 2048;
+    M.big_thresh = M.lds_cap / 2;
+    const size_t lds_bytes = (size_t)M.lds_cap * (8 * key_words + 4 + (key_words == 2 ? 4 : 0));
+    // rabin-karp128 constants (crates/hashes/src/cn_rkhash.rs:65-78)
+    M.rk_mult = mk128(0x3eb9402f3e733993ULL, 0xadd64d3ca00e1b6bULL);
+    M.rk_mult_inv = mk128(0x09cb6ff6f1b1a6d7ULL, 0x33e0952e899c3943ULL);
+    const u128_t A_ = mk128(0x4751137d01d863c5ULL, 0xb8c36de2b7d399dfULL);
+    const u128_t C_ = mk128(0x37ea3a13226503fbULL, 0x783f5cb69f4552bdULL);
+    const u128_t G_ = mk128(0x50796b285343f09aULL, 0x0c53113ae736572bULL);
+    const u128_t T_ = mk128(0x1e62d96a5e1f5adeULL, 0x2d4e68d8f88110b7ULL);
+    M.rk_fwd[0] = A_; M.rk_fwd[1] = C_; M.rk_fwd[2] = T_; M.rk_fwd[3] = G_;
+    M.rk_bkw[0] = T_; M.rk_bkw[1] = G_; M.rk_bkw[2] = A_; M.rk_bkw[3] = C_;
+    u128_t r = 1, b = M.rk_mult;
+    for (uint32_t e = p.k - 1; e; e >>= 1, b *= b)
+        if (e & 1) r *= b;
+    M.rk_rmmult = r;
+
+    ENSURE(ctx->work, 8);
+    ENSURE(ctx->big_count, 8);
+    ENSURE(ctx->big_list, (size_t)ctx->n_sub * 4);
+    ENSURE(ctx->out_cursor, 8);
+    uint64_t out_cap = std::max<uint64_t>(ctx->kmer_occ / 6 + (1u << 20), ctx->out_val.cap / 4);
+    for (int attempt = 0; attempt < 2; attempt++) {
+        ENSURE(ctx->out_key, out_cap * 8 * key_words);
+        ENSURE(ctx->out_val, out_cap * 4);
+        ENSURE(ctx->out_sub, out_cap * 4);
+        M.work = P_<unsigned int>(ctx->work);
+        M.big_list = P_<uint32_t>(ctx->big_list);
+        M.big_count = P_<unsigned int>(ctx->big_count);
+        M.out_key = P_<uint64_t>(ctx->out_key);
+        M.out_val = P_<uint32_t>(ctx->out_val);
+        M.out_sub = P_<uint32_t>(ctx->out_sub);
+        M.out_cursor = P_<unsigned long long>(ctx->out_cursor);
+        M.out_cap = out_cap;
+        HIPCHK(hipMemsetAsync(ctx->work.p, 0, 8, ctx->stream));
+        HIPCHK(hipMemsetAsync(ctx->big_count.p, 0, 8, ctx->stream));
+        HIPCHK(hipMemsetAsync(ctx->out_cursor.p, 0, 8, ctx->stream));
+        HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
+        int n_cu = 256;
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) n_cu = prop.multiProcessorCount;
+        hipLaunchKernelGGL(merge_lds_kernel, dim3(n_cu * 2), dim3(256), lds_bytes, ctx->stream, M);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(ctx->ev[6], ctx->stream));
+        uint32_t n_big = 0;
+        HIPCHK(hipMemcpyAsync(&n_big, ctx->big_count.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        ctx->n_big = n_big;
+        if (n_big) {
+            std::vector<uint32_t> list(n_big), occ(ctx->n_sub);
+            HIPCHK(hipMemcpy(list.data(), ctx->big_list.p, n_big * 4, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(occ.data(), ctx->sub_kmers.p, (size_t)ctx->n_sub * 4, hipMemcpyDeviceToHost));
+            std::vector<uint64_t> goff(n_big);
+            std::vector<uint32_t> gcap(n_big);
+            uint64_t tot = 0;
+            for (uint32_t i = 0; i < n_big; i++) {
+                uint64_t c = 1024;
+                while (c < 2ull * occ[list[i]]) c <<= 1;
+                goff[i] = tot;
+                gcap[i] = (uint32_t)c;
+                tot += c;
+            }
+            ENSURE(ctx->g_klo, tot * 8);
+            ENSURE(ctx->g_val, tot * 4);
+            ENSURE(ctx->g_off, (size_t)n_big * 8);
+            ENSURE(ctx->g_cap, (size_t)n_big * 4);
+            if (key_words == 2) {
+                ENSURE(ctx->g_khi, tot * 8);
+                ENSURE(ctx->g_st, tot * 4);
+            }
+            HIPCHK(hipMemcpyAsync(ctx->g_off.p, goff.data(), n_big * 8, hipMemcpyHostToDevice, ctx->stream));
+            HIPCHK(hipMemcpyAsync(ctx->g_cap.p, gcap.data(), n_big * 4, hipMemcpyHostToDevice, ctx->stream));
+            hipLaunchKernelGGL(fill_u64_kernel, dim3(1024), dim3(256), 0, ctx->stream, P_<uint64_t>(ctx->g_klo), EMPTY64, tot);
+            hipLaunchKernelGGL(fill_u32_kernel, dim3(1024), dim3(256), 0, ctx->stream, P_<uint32_t>(ctx->g_val), 0u, tot);
+            if (key_words == 2)
+                hipLaunchKernelGGL(fill_u32_kernel, dim3(1024), dim3(256), 0, ctx->stream, P_<uint32_t>(ctx->g_st), 0u, tot);
+            M.g_klo = P_<uint64_t>(ctx->g_klo);
+            M.g_khi = key_words == 2 ? P_<uint64_t>(ctx->g_khi) : nullptr;
+            M.g_st = key_words == 2 ? P_<uint32_t>(ctx->g_st) : nullptr;
+            M.g_val = P_<uint32_t>(ctx->g_val);
+            M.g_off = P_<uint64_t>(ctx->g_off);
+            M.g_cap = P_<uint32_t>(ctx->g_cap);
+            HIPCHK(hipEventRecord(ctx->ev[7], ctx->stream));
+            hipLaunchKernelGGL(merge_global_kernel, dim3(std::min<uint32_t>(n_big, 4096)), dim3(256), 0, ctx->stream, M, n_big);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipEventRecord(ctx->ev[8], ctx->stream));
+        }
+        uint64_t n_out = 0;
+        HIPCHK(hipMemcpyAsync(&n_out, ctx->out_cursor.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        float t;
+        HIPCHK(hipEventElapsedTime(&t, ctx->ev[5], ctx->ev[6]));
+        ctx->ms[3] = t;
+        ctx->ms[4] = 0;
+        if (n_big) {
+            HIPCHK(hipEventElapsedTime(&t, ctx->ev[7], ctx->ev[8]));
+            ctx->ms[4] = t;
+            HIPCHK(hipEventElapsedTime(&t, ctx->ev[5], ctx->ev[8]));
+            ctx->ms[6] = t;
+        } else {
+            ctx->ms[6] = ctx->ms[3];
+        }
+        ctx->n_solid = n_out;
+        if (n_out <= out_cap) break;
+        if (attempt == 1) return fail(ctx, GGCAT_GPU_ENOMEM, "solid output overflow");
+        out_cap = n_out + 1024;
+    }
+    ctx->have_solid = true;
+    return GGCAT_GPU_OK;
+}
+
+int ggcat_gpu_solid_info(ggcat_gpu_ctx* ctx, uint64_t* n_solid, uint32_t* key_words) {
+    if (!ctx) return GGCAT_GPU_EINVAL;
+    if (!ctx->have_solid) return fail(ctx, GGCAT_GPU_ESTATE, "no merge result");
+    if (n_solid) *n_solid = ctx->n_solid;
+    if (key_words) *key_words = ctx->key_words;
+    return GGCAT_GPU_OK;
+}
+
+int ggcat_gpu_solid_copy(ggcat_gpu_ctx* ctx, uint64_t* keys, uint32_t* multiplicity, uint8_t* flags, uint32_t* sub) {
+    if (!ctx) return GGCAT_GPU_EINVAL;
+    if (!ctx->have_solid) return fail(ctx, GGCAT_GPU_ESTATE, "no merge result");
+    HIPCHK(hipSetDevice(ctx->device));
+    const uint64_t n = ctx->n_solid;
+    if (keys && n) HIPCHK(hipMemcpy(keys, ctx->out_key.p, n * 8 * ctx->key_words, hipMemcpyDeviceToHost));
+    if (sub && n) HIPCHK(hipMemcpy(sub, ctx->out_sub.p, n * 4, hipMemcpyDeviceToHost));
+    if ((multiplicity || flags) && n) {
+        std::vector<uint32_t> v(n);
+        HIPCHK(hipMemcpy(v.data(), ctx->out_val.p, n * 4, hipMemcpyDeviceToHost));
+        for (uint64_t i = 0; i < n; i++) {
+            if (multiplicity) multiplicity[i] = v[i] & 0x3FFFFFFFu;
+            if (flags) flags[i] = (uint8_t)(v[i] >> 30);
+        }
+    }
+    return GGCAT_GPU_OK;
+}
+
+int ggcat_gpu_timings(ggcat_gpu_ctx* ctx, double* ms, uint32_t n) {
+    if (!ctx || !ms) return GGCAT_GPU_EINVAL;
+    for (uint32_t i = 0; i < n && i < 7; i++) ms[i] = ctx->ms[i];
+    return GGCAT_GPU_OK;
+}
+
+int ggcat_gpu_counters(ggcat_gpu_ctx* ctx, uint64_t* out, uint32_t n) {
+    if (!ctx || !out) return GGCAT_GPU_EINVAL;
+    uint64_t v[7] = {ctx->n_input, ctx->n_records, ctx->kmer_occ + ctx->n_records * (ctx->P.k - 1), ctx->kmer_occ,
+                     ctx->n_big, ctx->n_solid, 0};
+    for (uint32_t i = 0; i < n && i < 7; i++) out[i] = v[i];
+    return GGCAT_GPU_OK;
+}
+
+int ggcat_gpu_device_alloc(ggcat_gpu_ctx* ctx, uint64_t bytes, void** out) {
+    if (!ctx || !out) return GGCAT_GPU_EINVAL;
+    *out = nullptr;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipError_t e = hipMalloc(out, bytes ? bytes : 16);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *out = nullptr;
+        return fail(ctx, GGCAT_GPU_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    return GGCAT_GPU_OK;
+}
+
+int ggcat_gpu_device_free(ggcat_gpu_ctx* ctx, void* p) {
+    if (!ctx) return GGCAT_GPU_EINVAL;
+    HIPCHK(hipSetDevice(ctx->device));
+    if (p) HIPCHK(hipFree(p));
+    return GGCAT_GPU_OK;
+}
+
+int ggcat_gpu_copy_to_device(ggcat_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return GGCAT_GPU_EINVAL;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return GGCAT_GPU_OK;
+}
+
+}  // extern "C"

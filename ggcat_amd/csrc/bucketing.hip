@@ -1,0 +1,488 @@
+// bucketing.hip -- minimizer bucketing on gfx950.
+//
+// Restates, for the device, the per-run work of
+//   crates/assembler_minimizer_bucketing/src/lib.rs:171-270  (process_sequence)
+//   crates/hashes/src/cn_nthash.rs:19-109                     (canonical ntHash m-mers)
+//   crates/hashes/src/rolling/batch_minqueue.rs:37-189        (window minima + splits)
+//   crates/minimizer_bucketing/src/sequences_splitter.rs:15-40 (runs of >= k ACGT)
+//
+// Input layout: one byte array in HBM holding every record followed by a
+// separator byte.  Any byte that is not A/C/G/T (either case) ends an ACGT run,
+// exactly like 'N' after the reference's normalisation, so separators need no
+// index.
+//
+// Geometry.  A "window" j is the (k-1)-mer starting at base j; its minimum is
+// taken over the w = k - m m-mers j .. j+w-1.  A tile owns windows [t0, t0+T)
+// and is processed by one wavefront.  The wave also computes the w windows
+// before t0 (halo), enough to find the start of every group whose minimizer is
+// unique (such groups span at most w windows).  Groups with a non-unique
+// minimizer may be longer; their start is found in the global group-start
+// bitmap written by pass A.
+//
+//   pass A (bucket_count_kernel): group-start bitmap + records per sub-bucket
+//   scan   (scan_kernel)        : sub-bucket offsets
+//   pass B (bucket_emit_kernel) : recompute, write one slot per super-k-mer
+//
+// A group = maximal run of consecutive windows sharing the minimizer (value,
+// and position when unique).  Its super-k-mer is the set of k-mers touching
+// the group: bases [a-1, b+k) for a group of windows [a, b], clipped to the
+// ACGT run (include_first / include_last when clipped).
+#include "common.h"
+
+namespace gg {
+
+struct BucketArgs {
+    const uint8_t* seq;
+    uint64_t n;  // bytes in seq
+    uint32_t k, m, w;
+    uint32_t b, sb;
+    uint32_t canonical;
+    uint32_t T;  // windows per tile (multiple of 64)
+    uint64_t n_tiles;
+    uint32_t* seg_bits;  // [ceil(n/32)+2] group-start bitmap (pass A writes)
+    uint32_t* sub_count;  // [n_sub] records per sub-bucket (pass A)
+    // pass B
+    uint32_t* sub_cursor;  // [n_sub]
+    const uint64_t* sub_off;  // [n_sub+1] slot offsets
+    uint32_t* slots;
+    uint32_t slot_words;
+    uint32_t* pool;
+    unsigned long long* pool_cursor;
+    uint64_t pool_cap;  // words
+    uint32_t* sub_kmers;  // [n_sub] k-mer occurrences
+    const uint64_t* color_starts;
+    const uint32_t* colors;
+    uint32_t n_colors;
+    uint32_t* slot_color;
+    uint64_t gtab[40];  // interleaved (fw, rc) rolling tables, index out*4+in
+};
+
+// LDS layout of one tile (host and device agree through this struct).
+struct TileGeom {
+    uint32_t NW, NWB, NM, NB;
+    uint32_t off_mm, off_wv, off_wp, off_pk, off_inv, off_fwd, off_seg, off_ok, off_last, off_g;
+    uint32_t bytes;
+};
+
+__host__ __device__ inline TileGeom tile_geom(uint32_t T, uint32_t w, uint32_t k, uint32_t m) {
+    TileGeom g;
+    g.NW = T + w + 1;
+    g.NWB = (g.NW + w - 1) / w;
+    g.NM = (g.NWB + 1) * w;
+    // bases relative to borg: d0 in [1,16]; the furthest read is the word
+    // after the one holding relative base d0 + max(NM+m-2, NW+k-2)
+    uint32_t need = 48 + (g.NM + m > g.NW + k ? g.NM + m : g.NW + k);
+    g.NB = (need + 31) / 32 * 32;
+    uint32_t o = 0;
+    g.off_g = o; o += 40 * 8;
+    g.off_mm = o; o += g.NM * 8;
+    g.off_wv = o; o += g.NWB * w * 8;
+    g.off_pk = o; o += g.NB / 16 * 4;
+    g.off_inv = o; o += g.NB / 32 * 4;
+    g.off_seg = o; o += (g.NW / 64 + 2) * 8;
+    g.off_ok = o; o += (g.NW / 64 + 2) * 8;
+    g.off_last = o; o += (g.NW / 64 + 2) * 8;
+    g.off_wp = o; o += g.NWB * w * 2;
+    g.off_fwd = o; o += g.NM;
+    g.bytes = (o + 15) / 16 * 16;
+    return g;
+}
+
+GG_DEV bool is_acgt(uint8_t c) {
+    uint8_t u = c & 0xDF;
+    return u == 'A' || u == 'C' || u == 'G' || u == 'T';
+}
+
+struct Tile {
+    const BucketArgs* A;
+    TileGeom G;
+    int64_t t0, wlo, borg;
+    uint32_t d0;  // wlo - borg
+    uint8_t* lds;
+    uint64_t* mm;
+    uint64_t* wv;
+    uint16_t* wp;
+    uint32_t* pk;
+    uint32_t* inv;
+    uint8_t* fwd;
+    uint32_t* seg;
+    uint32_t* ok;
+    uint32_t* last;
+    uint64_t* gt;
+
+    GG_DEV uint32_t code_rel(int64_t r) const { return (pk[r >> 4] >> ((r & 15) * 2)) & 3; }
+    GG_DEV bool inv_rel(int64_t r) const { return (inv[r >> 5] >> (r & 31)) & 1; }
+    // true when any base in [r, r+len) (relative) is not ACGT
+    GG_DEV bool any_inv(int64_t r, uint32_t len) const {
+        int64_t e = r + len;  // exclusive
+        int64_t wi = r >> 5;
+        int64_t we = (e - 1) >> 5;
+        uint32_t lo_mask = ~0u << (r & 31);
+        if (wi == we) {
+            uint32_t hi_mask = ((e & 31) == 0) ? ~0u : ((1u << (e & 31)) - 1);
+            return (inv[wi] & lo_mask & hi_mask) != 0;
+        }
+        if (inv[wi] & lo_mask) return true;
+        for (int64_t q = wi + 1; q < we; q++)
+            if (inv[q]) return true;
+        uint32_t hi_mask = ((e & 31) == 0) ? ~0u : ((1u << (e & 31)) - 1);
+        return (inv[we] & hi_mask) != 0;
+    }
+    GG_DEV bool bit(const uint32_t* a, uint32_t x) const { return (a[x >> 5] >> (x & 31)) & 1; }
+};
+
+// Phases 0-3: load, hash, window minima, per-window flags.
+GG_DEV void tile_compute(Tile& t) {
+    const BucketArgs& A = *t.A;
+    const TileGeom& G = t.G;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t k = A.k, m = A.m, w = A.w;
+
+    for (uint32_t i = lane; i < 40; i += 64) t.gt[i] = A.gtab[i];
+
+    // ---- phase 0: bases -> 2-bit codes + invalid bits --------------------
+    const uint32_t nchunk = G.NB / 16;
+    for (uint32_t c = lane; c < nchunk; c += 64) {
+        int64_t a = t.borg + (int64_t)c * 16;
+        uint32_t packed = 0, invm = 0;
+        if (a >= 0 && a + 16 <= (int64_t)A.n) {
+            uint4 v = *reinterpret_cast<const uint4*>(A.seq + a);
+            uint32_t wds[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+#pragma unroll
+                for (int bb = 0; bb < 4; bb++) {
+                    uint8_t ch = (wds[q] >> (8 * bb)) & 0xFF;
+                    uint32_t idx = q * 4 + bb;
+                    packed |= (uint32_t)((ch >> 1) & 3) << (2 * idx);
+                    invm |= (is_acgt(ch) ? 0u : 1u) << idx;
+                }
+            }
+        } else {
+            for (uint32_t idx = 0; idx < 16; idx++) {
+                int64_t p = a + idx;
+                uint8_t ch = (p >= 0 && p < (int64_t)A.n) ? A.seq[p] : (uint8_t)'\n';
+                packed |= (uint32_t)((ch >> 1) & 3) << (2 * idx);
+                invm |= (is_acgt(ch) ? 0u : 1u) << idx;
+            }
+        }
+        t.pk[c] = packed;
+        reinterpret_cast<uint16_t*>(t.inv)[c] = (uint16_t)invm;
+    }
+    __syncthreads();
+
+    // ---- phase 1: canonical ntHash of every m-mer (rolling strips) -------
+    // fw_i = rotl1(fw_{i-1}) ^ rotl(h(s_{i-1}), m) ^ h(s_{i+m-1})
+    // rc_i = rotr1(rc_{i-1}) ^ rotr1(rc(s_{i-1})) ^ rotl(rc(s_{i+m-1}), m-1)
+    // with the "out" term dropped while the first m bases are fed.
+    {
+        const uint32_t S = (G.NM + 63) / 64;
+        uint32_t p0 = lane * S, p1 = p0 + S;
+        if (p1 > G.NM) p1 = G.NM;
+        if (p0 < p1) {
+            uint64_t fw = 0, rc = 0;
+            const int64_t rb = (int64_t)t.d0;  // relative base of m-mer 0
+            for (uint32_t j = 0; j + 1 < m; j++) {
+                uint32_t e = 16 + t.code_rel(rb + p0 + j);
+                fw = rotl1_64(fw) ^ t.gt[2 * e];
+                rc = rotr1_64(rc) ^ t.gt[2 * e + 1];
+            }
+            for (uint32_t p = p0; p < p1; p++) {
+                uint32_t in = t.code_rel(rb + p + m - 1);
+                uint32_t out = (p == p0) ? 4u : t.code_rel(rb + p - 1);
+                uint32_t e = out * 4 + in;
+                fw = rotl1_64(fw) ^ t.gt[2 * e];
+                rc = rotr1_64(rc) ^ t.gt[2 * e + 1];
+                uint64_t mn = fw < rc ? fw : rc;
+                t.mm[p] = (mn << 1) | (fw != rc ? 1ULL : 0ULL);
+                t.fwd[p] = fw < rc ? 1 : 0;
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 2: window minima, van Herk / Gil-Werman blocks of w -------
+    for (uint32_t q = lane; q < G.NWB; q += 64) {
+        const uint32_t base = q * w;
+        uint64_t sv = t.mm[base + w - 1];
+        uint32_t sp = base + w - 1;
+        t.wv[base + w - 1] = sv;
+        t.wp[base + w - 1] = (uint16_t)sp;
+        for (int i = (int)w - 2; i >= 0; i--) {
+            mm_combine(t.mm[base + i], base + i, sv, sp, sv, sp);
+            t.wv[base + i] = sv;
+            t.wp[base + i] = (uint16_t)sp;
+        }
+        uint64_t pv = 0;
+        uint32_t pp = 0;
+        const uint32_t nb = base + w;
+        for (uint32_t i = 1; i < w; i++) {
+            if (i == 1) {
+                pv = t.mm[nb];
+                pp = nb;
+            } else {
+                mm_combine(pv, pp, t.mm[nb + i - 1], nb + i - 1, pv, pp);
+            }
+            uint64_t rv;
+            uint32_t rp;
+            mm_combine(t.wv[base + i], t.wp[base + i], pv, pp, rv, rp);
+            t.wv[base + i] = rv;
+            t.wp[base + i] = (uint16_t)rp;
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 3: per-window flags ----------------------------------------
+    const uint32_t nwords64 = (G.NW + 63) / 64;
+    for (uint32_t c = 0; c < nwords64; c++) {
+        uint32_t x = c * 64 + lane;
+        bool okf = false, lastf = false, segf = false;
+        if (x >= 1 && x < G.NW) {
+            int64_t r = (int64_t)t.d0 + x;  // relative base of window start
+            bool valid = !t.any_inv(r, k - 1);
+            bool first = t.inv_rel(r - 1);
+            bool lst = t.inv_rel(r + k - 1);
+            okf = valid && !(first && lst);
+            lastf = okf && lst;
+            if (okf) {
+                uint64_t v0 = t.wv[x - 1], v1 = t.wv[x];
+                bool differs = (v0 != v1) || ((v1 & 1) && t.wp[x] != t.wp[x - 1]);
+                segf = first || differs;
+            }
+        }
+        uint64_t bs = __ballot(segf), bo = __ballot(okf), bl = __ballot(lastf);
+        if (lane == 0) {
+            reinterpret_cast<uint64_t*>(t.seg)[c] = bs;
+            reinterpret_cast<uint64_t*>(t.ok)[c] = bo;
+            reinterpret_cast<uint64_t*>(t.last)[c] = bl;
+        }
+    }
+    __syncthreads();
+}
+
+GG_DEV bool tile_is_end(const Tile& t, uint32_t x) {
+    if (!t.bit(t.ok, x)) return false;
+    return t.bit(t.last, x) || t.bit(t.seg, x + 1);
+}
+
+GG_DEV uint32_t sub_of(uint64_t v, uint32_t b, uint32_t sb) {
+    uint32_t bucket = (v & 1) ? (uint32_t)((v >> 1) & ((1ULL << b) - 1)) : (1u << b);
+    uint32_t second = (uint32_t)((v >> (b + 1)) & ((1ULL << sb) - 1));
+    return (bucket << sb) | second;
+}
+
+GG_DEV void tile_init(Tile& t, const BucketArgs* A, uint64_t tile, uint8_t* lds) {
+    t.A = A;
+    t.G = tile_geom(A->T, A->w, A->k, A->m);
+    t.t0 = (int64_t)tile * A->T;
+    t.wlo = t.t0 - A->w;
+    int64_t bo = t.wlo - 1;
+    t.borg = (bo >= 0) ? (bo / 16) * 16 : -(((-bo) + 15) / 16) * 16;
+    t.d0 = (uint32_t)(t.wlo - t.borg);
+    t.lds = lds;
+    t.gt = reinterpret_cast<uint64_t*>(lds + t.G.off_g);
+    t.mm = reinterpret_cast<uint64_t*>(lds + t.G.off_mm);
+    t.wv = reinterpret_cast<uint64_t*>(lds + t.G.off_wv);
+    t.wp = reinterpret_cast<uint16_t*>(lds + t.G.off_wp);
+    t.pk = reinterpret_cast<uint32_t*>(lds + t.G.off_pk);
+    t.inv = reinterpret_cast<uint32_t*>(lds + t.G.off_inv);
+    t.fwd = lds + t.G.off_fwd;
+    t.seg = reinterpret_cast<uint32_t*>(lds + t.G.off_seg);
+    t.ok = reinterpret_cast<uint32_t*>(lds + t.G.off_ok);
+    t.last = reinterpret_cast<uint32_t*>(lds + t.G.off_last);
+}
+
+__global__ __launch_bounds__(64) void bucket_count_kernel(BucketArgs A) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    Tile t;
+    tile_init(t, &A, blockIdx.x, lds);
+    tile_compute(t);
+    const uint32_t lane = threadIdx.x;
+    const uint32_t w = A.w;
+    for (uint32_t c = 0; c < A.T; c += 64) {
+        uint32_t x = w + c + lane;
+        int64_t j = t.t0 + c + lane;
+        bool s = t.bit(t.seg, x) && j < (int64_t)A.n;
+        uint64_t bs = __ballot(s);
+        if (lane == 0) {
+            uint64_t wi = (uint64_t)(t.t0 + c) >> 5;
+            A.seg_bits[wi] = (uint32_t)bs;
+            A.seg_bits[wi + 1] = (uint32_t)(bs >> 32);
+        }
+        if (j < (int64_t)A.n && tile_is_end(t, x)) {
+            uint32_t sub = sub_of(t.wv[x], A.b, A.sb);
+            atomicAdd(&A.sub_count[sub], 1u);
+        }
+    }
+}
+
+// Code of the base at absolute position p (LDS when inside the tile, else HBM).
+GG_DEV uint32_t code_abs(const Tile& t, int64_t p) {
+    int64_t r = p - t.borg;
+    if (r >= 0 && r < (int64_t)t.G.NB) return t.code_rel(r);
+    return (t.A->seq[p] >> 1) & 3;
+}
+GG_DEV bool invalid_abs(const Tile& t, int64_t p) {
+    if (p < 0 || p >= (int64_t)t.A->n) return true;
+    int64_t r = p - t.borg;
+    if (r >= 0 && r < (int64_t)t.G.NB) return t.inv_rel(r);
+    return !is_acgt(t.A->seq[p]);
+}
+
+// 16 codes of relative positions [r, r+16) packed (position r at bits 0..1);
+// positions below 0 read as zero.
+GG_DEV uint32_t extract16(const Tile& t, int64_t r) {
+    if (r >= 0) {
+        uint32_t wi = (uint32_t)(r >> 4), s = (uint32_t)(r & 15) * 2;
+        uint32_t lo = t.pk[wi];
+        if (s == 0) return lo;
+        return (lo >> s) | (t.pk[wi + 1] << (32 - s));
+    }
+    if (r <= -16) return 0;
+    uint32_t sh = (uint32_t)(-r) * 2;
+    return t.pk[0] << sh;
+}
+
+GG_DEV uint32_t rev_comp16(uint32_t x) {
+    uint32_t y = __brev(x);
+    y = ((y >> 1) & 0x55555555u) | ((y & 0x55555555u) << 1);
+    return y ^ 0xAAAAAAAAu;
+}
+
+__global__ __launch_bounds__(64) void bucket_emit_kernel(BucketArgs A) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    Tile t;
+    tile_init(t, &A, blockIdx.x, lds);
+    tile_compute(t);
+    const uint32_t lane = threadIdx.x;
+    const uint32_t w = A.w, k = A.k, m = A.m;
+    const uint32_t SW = A.slot_words;
+    const uint32_t inline_cap = (SW - 1) * 16;
+    for (uint32_t c = 0; c < A.T; c += 64) {
+        uint32_t x = w + c + lane;
+        int64_t bwin = t.t0 + c + lane;
+        if (bwin >= (int64_t)A.n || !tile_is_end(t, x)) continue;
+        // ---- group start: latest group-start bit at or before x ----
+        int64_t a = -1;
+        {
+            int32_t wi = (int32_t)(x >> 5);
+            uint32_t msk = t.seg[wi] & ((2u << (x & 31)) - 1u);
+            while (true) {
+                if (msk) {
+                    a = t.wlo + (int64_t)wi * 32 + (31 - __clz(msk));
+                    break;
+                }
+                if (--wi < 0) break;
+                msk = t.seg[wi];
+            }
+            if (a < 0 && t.wlo >= 0) {
+                // far start: scan the global bitmap below the tile's halo
+                int64_t g = t.wlo;
+                int64_t gw = g >> 5;
+                uint32_t gm = A.seg_bits[gw] & ((2u << (g & 31)) - 1u);
+                while (gm == 0 && gw > 0) gm = A.seg_bits[--gw];
+                if (gm) a = gw * 32 + (31 - __clz(gm));
+            }
+            if (a < 0) continue;  // unreachable for well-formed input (every run starts a group)
+        }
+        const uint64_t v = t.wv[x];
+        const bool unique = v & 1;
+        const bool include_first = invalid_abs(t, a - 1);
+        const bool include_last = t.bit(t.last, x);
+        const int64_t start = include_first ? a : a - 1;
+        const int64_t end = include_last ? bwin + k - 1 : bwin + k;
+        const uint32_t len = (uint32_t)(end - start);
+        const uint32_t prel = t.wp[x];
+        const int64_t mmabs = t.wlo + prel;
+        const bool rc = A.canonical && unique && !t.fwd[prel];
+        uint32_t minpos = 0;
+        if (unique) minpos = rc ? (uint32_t)(end - mmabs - m) : (uint32_t)(mmabs - start);
+        const uint32_t flags = ((include_first ? 1u : 0u) << (rc ? 1 : 0)) | ((include_last ? 1u : 0u) << (rc ? 0 : 1));
+        const uint32_t sub = sub_of(v, A.b, A.sb);
+        const uint64_t slot = A.sub_off[sub] + atomicAdd(&A.sub_cursor[sub], 1u);
+        atomicAdd(&A.sub_kmers[sub], len - k + 1);
+        uint32_t* out = A.slots + slot * SW;
+        const int64_t rs = start - t.borg;
+        if (len <= inline_cap && rs >= 0) {
+            out[0] = len | (flags << 8) | (minpos << 10) | (rc ? META_RC : 0u);
+            const uint32_t nw = (len + 15) / 16;
+            for (uint32_t q = 1; q < SW; q++) {
+                uint32_t wq = 0;
+                uint32_t qi = q - 1;
+                if (qi < nw) {
+                    if (!rc) {
+                        wq = extract16(t, rs + 16 * qi);
+                    } else {
+                        wq = rev_comp16(extract16(t, (end - t.borg) - 16 * (int64_t)qi - 16));
+                    }
+                    uint32_t rem = len - 16 * qi;
+                    if (rem < 16) wq &= (1u << (2 * rem)) - 1u;
+                }
+                out[q] = wq;
+            }
+        } else {
+            // generic path: bases possibly outside the tile, or a long record
+            const bool is_long = len > inline_cap;
+            uint32_t* dst;
+            if (is_long) {
+                uint64_t words = (len + 15) / 16;
+                unsigned long long off = atomicAdd(A.pool_cursor, (unsigned long long)words);
+                out[0] = (flags << 8) | (minpos << 10) | META_LONG | (rc ? META_RC : 0u);
+                out[1] = len;
+                out[2] = (uint32_t)off;
+                out[3] = (uint32_t)(off >> 32);
+                for (uint32_t q = 4; q < SW; q++) out[q] = 0;
+                if (off + words > A.pool_cap) continue;  // overflow: host retries with a larger pool
+                dst = A.pool + off;
+            } else {
+                out[0] = len | (flags << 8) | (minpos << 10) | (rc ? META_RC : 0u);
+                dst = out + 1;
+                for (uint32_t q = 1; q < SW; q++) out[q] = 0;
+            }
+            uint32_t acc = 0;
+            for (uint32_t i = 0; i < len; i++) {
+                uint32_t cd = rc ? (code_abs(t, end - 1 - i) ^ 2u) : code_abs(t, start + i);
+                acc |= cd << (2 * (i & 15));
+                if ((i & 15) == 15 || i + 1 == len) {
+                    dst[i >> 4] = acc;
+                    acc = 0;
+                }
+            }
+        }
+        if (A.n_colors) {
+            uint32_t lo = 0, hi = A.n_colors;  // last color range starting <= start
+            while (hi - lo > 1) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (A.color_starts[mid] <= (uint64_t)start) lo = mid; else hi = mid;
+            }
+            A.slot_color[slot] = A.colors[lo];
+        }
+    }
+}
+
+// Exclusive prefix sum of counts -> offsets (n+1 entries), one workgroup.
+__global__ __launch_bounds__(1024) void scan_kernel(const uint32_t* counts, uint64_t* off, uint32_t n) {
+    __shared__ uint64_t part[1024];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t per = (n + 1023) / 1024;
+    uint32_t s0 = tid * per, s1 = s0 + per;
+    if (s1 > n) s1 = n;
+    uint64_t sum = 0;
+    for (uint32_t i = s0; i < s1; i++) sum += counts[i];
+    part[tid] = sum;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024; d <<= 1) {
+        uint64_t v = (tid >= d) ? part[tid - d] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    uint64_t run = (tid == 0) ? 0 : part[tid - 1];
+    for (uint32_t i = s0; i < s1; i++) {
+        off[i] = run;
+        run += counts[i];
+    }
+    if (tid == 1023) off[n] = part[1023];
+}
+
+}  // namespace gg

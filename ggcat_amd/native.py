@@ -1,0 +1,219 @@
+"""ctypes binding of the C ABI in include/ggcat_gpu.h and of the synthetic
+workload generator.
+
+There is no CPU fallback: if the HIP library is missing or a call fails, an
+exception is raised.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD = os.path.join(HERE, "_build")
+GPU_LIB = os.path.join(BUILD, "libggcat_gpu.so")
+SYNTH_LIB = os.path.join(BUILD, "libggcat_synth.so")
+
+OK, EINVAL, EHIP, ENOMEM, ESTATE = 0, 1, 2, 3, 4
+HASH_SEQ, HASH_RK128 = 0, 1
+
+_u8p = C.POINTER(C.c_uint8)
+_u16p = C.POINTER(C.c_uint16)
+_u32p = C.POINTER(C.c_uint32)
+_u64p = C.POINTER(C.c_uint64)
+_f64p = C.POINTER(C.c_double)
+
+
+class GGCATGpuError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("k", C.c_uint32),
+        ("m", C.c_uint32),
+        ("buckets_log", C.c_uint32),
+        ("second_buckets_log", C.c_uint32),
+        ("forward_only", C.c_uint32),
+        ("hash_kind", C.c_uint32),
+        ("min_multiplicity", C.c_uint32),
+    ]
+
+
+_gpu = None
+_synth = None
+
+
+def _ptr(a: np.ndarray | None, t):
+    if a is None:
+        return C.cast(None, t)
+    return a.ctypes.data_as(t)
+
+
+def gpu_lib() -> C.CDLL:
+    """Loads libggcat_gpu.so (built by ggcat_amd.build); raises if absent."""
+    global _gpu
+    if _gpu is not None:
+        return _gpu
+    if not os.path.exists(GPU_LIB):
+        raise GGCATGpuError(f"{GPU_LIB} is missing: run `python -m ggcat_amd.build` (no CPU fallback exists)")
+    lib = C.CDLL(GPU_LIB)
+    V = C.c_void_p
+    lib.ggcat_gpu_create.argtypes = [C.c_int, C.POINTER(V)]
+    lib.ggcat_gpu_destroy.argtypes = [V]
+    lib.ggcat_gpu_destroy.restype = None
+    lib.ggcat_gpu_last_error.argtypes = [V]
+    lib.ggcat_gpu_last_error.restype = C.c_char_p
+    lib.ggcat_gpu_bucketing.argtypes = [V, V, C.c_uint64, C.c_int, _u64p, _u32p, C.c_uint32, C.POINTER(Params)]
+    lib.ggcat_gpu_records_info.argtypes = [V, _u64p, _u32p, _u64p]
+    lib.ggcat_gpu_records_copy.argtypes = [V, _u32p, _u32p, _u8p, _u16p, _u32p, _u64p, _u8p]
+    lib.ggcat_gpu_kmers_merge.argtypes = [V]
+    lib.ggcat_gpu_solid_info.argtypes = [V, _u64p, _u32p]
+    lib.ggcat_gpu_solid_copy.argtypes = [V, _u64p, _u32p, _u8p, _u32p]
+    lib.ggcat_gpu_timings.argtypes = [V, _f64p, C.c_uint32]
+    lib.ggcat_gpu_counters.argtypes = [V, _u64p, C.c_uint32]
+    lib.ggcat_gpu_device_alloc.argtypes = [V, C.c_uint64, C.POINTER(V)]
+    lib.ggcat_gpu_device_free.argtypes = [V, V]
+    lib.ggcat_gpu_copy_to_device.argtypes = [V, V, V, C.c_uint64]
+    for name in ("ggcat_gpu_device_alloc", "ggcat_gpu_device_free", "ggcat_gpu_copy_to_device",
+                 "ggcat_gpu_create", "ggcat_gpu_bucketing", "ggcat_gpu_records_info", "ggcat_gpu_records_copy",
+                 "ggcat_gpu_kmers_merge", "ggcat_gpu_solid_info", "ggcat_gpu_solid_copy", "ggcat_gpu_timings",
+                 "ggcat_gpu_counters"):
+        getattr(lib, name).restype = C.c_int
+    _gpu = lib
+    return lib
+
+
+def synth_lib() -> C.CDLL:
+    global _synth
+    if _synth is not None:
+        return _synth
+    if not os.path.exists(SYNTH_LIB):
+        raise GGCATGpuError(f"{SYNTH_LIB} is missing: run `python -m ggcat_amd.build`")
+    lib = C.CDLL(SYNTH_LIB)
+    lib.synth_genome.argtypes = [C.c_uint64, C.c_uint64, C.c_double, C.c_uint32, C.c_uint32, C.c_double, _u8p, C.c_int]
+    lib.synth_genome.restype = None
+    lib.synth_reads.argtypes = [_u8p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint32, C.c_double, _u8p, C.c_int]
+    lib.synth_reads.restype = None
+    lib.synth_descendant.argtypes = [_u8p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_uint64,
+                                     _u8p, C.c_uint64]
+    lib.synth_descendant.restype = C.c_uint64
+    lib.synth_lognormal_len.argtypes = [C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_double]
+    lib.synth_lognormal_len.restype = C.c_uint64
+    lib.synth_queries.argtypes = [_u8p, _u64p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32,
+                                  C.c_double, _u8p, _u32p]
+    lib.synth_queries.restype = C.c_uint64
+    _synth = lib
+    return lib
+
+
+@dataclass
+class Records:
+    sub: np.ndarray
+    length: np.ndarray
+    flags: np.ndarray
+    minpos: np.ndarray
+    color: np.ndarray
+    off: np.ndarray
+    bases: np.ndarray  # ASCII
+
+
+@dataclass
+class Solid:
+    keys: np.ndarray  # (n, key_words) uint64
+    mult: np.ndarray
+    flags: np.ndarray
+    sub: np.ndarray
+
+
+class GpuContext:
+    """One device context (one HIP stream, its buffers)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = gpu_lib()
+        h = C.c_void_p()
+        rc = self.lib.ggcat_gpu_create(device, C.byref(h))
+        if rc != OK:
+            raise GGCATGpuError(f"ggcat_gpu_create failed with status {rc}")
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.ggcat_gpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc != OK:
+            msg = self.lib.ggcat_gpu_last_error(self.h).decode(errors="replace")
+            raise GGCATGpuError(f"{what} failed (status {rc}): {msg}")
+
+    def bucketing(self, data, params: Params, colors: tuple[np.ndarray, np.ndarray] | None = None,
+                  device_ptr: int | None = None, n_bytes: int | None = None) -> None:
+        """data: uint8 numpy array (host) or None with device_ptr/n_bytes (device-resident)."""
+        if device_ptr is not None:
+            ptr, n, on_dev = C.c_void_p(device_ptr), int(n_bytes), 1
+        else:
+            data = np.ascontiguousarray(data, dtype=np.uint8)
+            ptr, n, on_dev = data.ctypes.data_as(C.c_void_p), data.size, 0
+        cs = cv = None
+        nc = 0
+        if colors is not None:
+            cs = np.ascontiguousarray(colors[0], dtype=np.uint64)
+            cv = np.ascontiguousarray(colors[1], dtype=np.uint32)
+            nc = cs.size
+        self._check(self.lib.ggcat_gpu_bucketing(self.h, ptr, n, on_dev, _ptr(cs, _u64p), _ptr(cv, _u32p), nc,
+                                                 C.byref(params)), "ggcat_gpu_bucketing")
+
+    def records(self) -> Records:
+        nr, ns, nb = C.c_uint64(), C.c_uint32(), C.c_uint64()
+        self._check(self.lib.ggcat_gpu_records_info(self.h, C.byref(nr), C.byref(ns), C.byref(nb)), "records_info")
+        n = nr.value
+        r = Records(np.empty(n, np.uint32), np.empty(n, np.uint32), np.empty(n, np.uint8), np.empty(n, np.uint16),
+                    np.empty(n, np.uint32), np.empty(n, np.uint64), np.empty(nb.value, np.uint8))
+        self._check(self.lib.ggcat_gpu_records_copy(self.h, _ptr(r.sub, _u32p), _ptr(r.length, _u32p),
+                                                    _ptr(r.flags, _u8p), _ptr(r.minpos, _u16p), _ptr(r.color, _u32p),
+                                                    _ptr(r.off, _u64p), _ptr(r.bases, _u8p)), "records_copy")
+        return r
+
+    def kmers_merge(self) -> None:
+        self._check(self.lib.ggcat_gpu_kmers_merge(self.h), "ggcat_gpu_kmers_merge")
+
+    def solid(self) -> Solid:
+        n, kw = C.c_uint64(), C.c_uint32()
+        self._check(self.lib.ggcat_gpu_solid_info(self.h, C.byref(n), C.byref(kw)), "solid_info")
+        s = Solid(np.empty((n.value, kw.value), np.uint64), np.empty(n.value, np.uint32), np.empty(n.value, np.uint8),
+                  np.empty(n.value, np.uint32))
+        self._check(self.lib.ggcat_gpu_solid_copy(self.h, _ptr(s.keys, _u64p), _ptr(s.mult, _u32p),
+                                                  _ptr(s.flags, _u8p), _ptr(s.sub, _u32p)), "solid_copy")
+        return s
+
+    def upload(self, data: np.ndarray) -> tuple[int, int]:
+        """Copies a host array into a new device allocation; returns (ptr, nbytes)."""
+        data = np.ascontiguousarray(data)
+        p = C.c_void_p()
+        self._check(self.lib.ggcat_gpu_device_alloc(self.h, data.nbytes, C.byref(p)), "device_alloc")
+        self._check(self.lib.ggcat_gpu_copy_to_device(self.h, p, data.ctypes.data_as(C.c_void_p), data.nbytes),
+                    "copy_to_device")
+        return p.value, data.nbytes
+
+    def free(self, ptr: int) -> None:
+        self._check(self.lib.ggcat_gpu_device_free(self.h, C.c_void_p(ptr)), "device_free")
+
+    def timings(self) -> np.ndarray:
+        out = np.zeros(7, np.float64)
+        self._check(self.lib.ggcat_gpu_timings(self.h, _ptr(out, _f64p), 7), "timings")
+        return out
+
+    def counters(self) -> np.ndarray:
+        out = np.zeros(7, np.uint64)
+        self._check(self.lib.ggcat_gpu_counters(self.h, _ptr(out, _u64p), 7), "counters")
+        return out

@@ -1,0 +1,138 @@
+/* ggcat_gpu.h -- C ABI of the MI355X (gfx950) implementation of GGCAT's
+ * two data-parallel front stages: minimizer bucketing and per-sub-bucket
+ * k-mer merging.
+ *
+ * Plain C types only; no function unwinds across this boundary.  Every call
+ * returns a GGCAT_GPU_* status; on failure ggcat_gpu_last_error() describes
+ * it.  One context owns one HIP stream and its device buffers; calls on one
+ * context must not overlap (the reference's API is not re-entrant either,
+ * crates/api/src/lib.rs:118-141).
+ *
+ * Input layout ("joined"): all records concatenated, each followed by at
+ * least one byte that is not A/C/G/T (e.g. '\n').  Non-ACGT bytes end an
+ * ACGT run exactly like 'N' after the reference's normalisation
+ * (crates/io/src/sequences_reader.rs:26-54,
+ *  crates/minimizer_bucketing/src/sequences_splitter.rs:15-40).
+ */
+#ifndef GGCAT_GPU_H
+#define GGCAT_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ggcat_gpu_ctx ggcat_gpu_ctx;
+
+enum {
+    GGCAT_GPU_OK = 0,
+    GGCAT_GPU_EINVAL = 1, /* bad argument */
+    GGCAT_GPU_EHIP = 2,   /* HIP runtime error */
+    GGCAT_GPU_ENOMEM = 3, /* device allocation failed */
+    GGCAT_GPU_ESTATE = 4  /* call out of order (e.g. merge before bucketing) */
+};
+
+enum { GGCAT_GPU_HASH_SEQ = 0, GGCAT_GPU_HASH_RK128 = 1 };
+
+/* Parameters of one build.  Mirrors the arguments run_assembler derives
+ * (crates/assembler/src/lib.rs:82-155): k, m (--minimizer-length), the
+ * bucket count log b, the second-bucket count log sb
+ * (crates/io/src/lib.rs:113-128), forward_only (-f), the merging hash
+ * (-w, crates/api/src/utils.rs:10-68) and min_multiplicity (-s). */
+typedef struct {
+    uint32_t k;
+    uint32_t m;
+    uint32_t buckets_log;
+    uint32_t second_buckets_log;
+    uint32_t forward_only;
+    uint32_t hash_kind; /* GGCAT_GPU_HASH_SEQ or GGCAT_GPU_HASH_RK128 */
+    uint32_t min_multiplicity;
+} ggcat_gpu_params;
+
+/* Creates a context on HIP device `device`.
+ * Replaces GGCATInstance::create (crates/api/src/lib.rs:141-215) for the
+ * device side of the two stages; returns GGCAT_GPU_OK and *out, or an error
+ * with *out == NULL (the reference's ggcat_create returns null,
+ * crates/capi/src/lib.rs:17-55). */
+int ggcat_gpu_create(int device, ggcat_gpu_ctx** out);
+
+/* Frees the context and every device buffer it owns. */
+void ggcat_gpu_destroy(ggcat_gpu_ctx* ctx);
+
+/* Description of the last failure on this context (never NULL). */
+const char* ggcat_gpu_last_error(const ggcat_gpu_ctx* ctx);
+
+/* Minimizer bucketing of one joined input.
+ * Replaces minimizer_bucketing
+ * (crates/assembler_minimizer_bucketing/src/lib.rs:279-340), called by
+ * run_assembler at crates/assembler/src/lib.rs:157-173.  Produces, in device
+ * memory owned by the context, every super-k-mer record grouped by
+ * sub-bucket (bucket << sb | second_bucket; bucket 2^b is the duplicates
+ * bucket, crates/assembler/src/lib.rs:147-155).
+ *   data, n_bytes   joined input; a host pointer, or a device pointer when
+ *                   input_on_device != 0 (then it must stay valid until the
+ *                   call returns and be 16-byte aligned)
+ *   color_starts, colors, n_colors
+ *                   optional per-record colors (colored builds): record
+ *                   color ranges start at the sorted byte offsets
+ *                   color_starts[i] (color_starts[0] == 0); NULL/0 when
+ *                   uncolored.  Host pointers. */
+int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_bytes, int input_on_device,
+                        const uint64_t* color_starts, const uint32_t* colors, uint32_t n_colors,
+                        const ggcat_gpu_params* params);
+
+/* Sizes of the last bucketing result: records, sub-buckets, and the bytes
+ * needed by ggcat_gpu_records_copy for the bases buffer. */
+int ggcat_gpu_records_info(ggcat_gpu_ctx* ctx, uint64_t* n_records, uint32_t* n_sub, uint64_t* n_bases);
+
+/* Decoded copy of the last bucketing result (test/inspection path):
+ * per record its sub-bucket, length, flags (READ_FLAG_INCL_BEGIN=1,
+ * READ_FLAG_INCL_END=2 as stored), minimizer position, color, offset into
+ * `bases`, and the bases as ASCII (already reverse-complemented when the
+ * reference stores them reverse-complemented,
+ * crates/assembler_minimizer_bucketing/src/lib.rs:228-263).
+ * Any output pointer may be NULL. */
+int ggcat_gpu_records_copy(ggcat_gpu_ctx* ctx, uint32_t* sub, uint32_t* len, uint8_t* flags, uint16_t* minpos,
+                           uint32_t* color, uint64_t* off, uint8_t* bases);
+
+/* Per-sub-bucket canonical k-mer counting of the last bucketing result.
+ * Replaces kmers_merge (crates/assembler_kmers_merge/src/lib.rs:159-284),
+ * called at crates/assembler/src/lib.rs:269-319, for its k-mer table: every
+ * k-mer whose multiplicity (crates/structs/src/map_entry.rs:79-83) is
+ * >= params.min_multiplicity is kept, with the MapEntry flags. */
+int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx);
+
+/* Number of solid (sub-bucket, k-mer) entries of the last merge. */
+int ggcat_gpu_solid_info(ggcat_gpu_ctx* ctx, uint64_t* n_solid, uint32_t* key_words);
+
+/* Copy of the solid entries: keys (key_words uint64 per entry, low word
+ * first), multiplicity, flags and sub-bucket.  Order is unspecified. */
+int ggcat_gpu_solid_copy(ggcat_gpu_ctx* ctx, uint64_t* keys, uint32_t* multiplicity, uint8_t* flags,
+                         uint32_t* sub);
+
+/* Device-side timings of the last calls, in milliseconds:
+ * [0] bucketing pass A, [1] scan, [2] bucketing pass B,
+ * [3] merge (LDS kernel), [4] merge (HBM-table kernel),
+ * [5] whole bucketing call, [6] whole merge call. */
+int ggcat_gpu_timings(ggcat_gpu_ctx* ctx, double* ms, uint32_t n);
+
+/* Counters of the last calls: [0] input bytes, [1] records,
+ * [2] sum of record lengths (bases), [3] k-mer occurrences,
+ * [4] sub-buckets merged in HBM, [5] solid entries, [6] long records. */
+int ggcat_gpu_counters(ggcat_gpu_ctx* ctx, uint64_t* out, uint32_t n);
+
+/* Device-memory plumbing for callers that keep inputs resident in HBM (no
+ * counterpart in the reference, which has no device memory).  The pointer
+ * returned by ggcat_gpu_device_alloc is 256-byte aligned and can be passed to
+ * ggcat_gpu_bucketing with input_on_device = 1. */
+int ggcat_gpu_device_alloc(ggcat_gpu_ctx* ctx, uint64_t bytes, void** out);
+int ggcat_gpu_device_free(ggcat_gpu_ctx* ctx, void* p);
+int ggcat_gpu_copy_to_device(ggcat_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GGCAT_GPU_H */

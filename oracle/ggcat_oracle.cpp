@@ -1,0 +1,653 @@
+// ggcat_oracle.cpp -- CPU restatement of GGCAT's minimizer bucketing and
+// per-sub-bucket k-mer merge, used ONLY as a test checker and as the
+// measured CPU baseline ("port").  Nothing in the product library links,
+// loads or calls this file; the HIP path must stand on its own.
+//
+// Each function cites the reference file:line whose behaviour it restates
+// (paths relative to the algbio/ggcat repository root).
+//
+// Parity status of each part (how it is pinned; see also tests/):
+//   ntHash m-mer hash (nthash_m)          pinned: closed form + RC symmetry + roll==direct
+//   BatchMinQueue restatement (minqueue)  pinned: brute-force closed-form window minimum
+//   process_sequence / splits (bucket_run) pinned: super-k-mer cover property + brute force
+//   seq-hash u64/u128, rabin-karp128      pinned: invertibility / roll==direct / RC symmetry
+//   MapEntry merge (merge_subbucket)      pinned: union of solid sets == exact k-mer counts
+//   compaction (compact)                  pinned: multiset expansion equality
+//   sorting extender (normal buckets)     parity unpinned (not restated; see DESIGN)
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+typedef unsigned __int128 u128;
+
+namespace {
+
+// ---------------------------------------------------------------- ntHash ---
+// crates/hashes/src/nthash_base.rs:49-61  h(c)/rc(c) for ASCII input:
+// mult = (c & 6) + 1 (forward), ((c & 6) ^ 4) + 1 (complement).
+constexpr uint64_t NT_MULT = 0x397f178c6ae330f9ULL;
+inline uint64_t rotl64(uint64_t x, unsigned r) {
+    r &= 63;
+    return r ? (x << r) | (x >> (64 - r)) : x;
+}
+inline uint64_t rotr64(uint64_t x, unsigned r) {
+    r &= 63;
+    return r ? (x >> r) | (x << (64 - r)) : x;
+}
+inline uint64_t nt_h(uint8_t c) { return (uint64_t)((c & 6) + 1) * NT_MULT; }
+inline uint64_t nt_rc(uint8_t c) { return (uint64_t)(((c & 6) ^ 4) + 1) * NT_MULT; }
+
+// crates/hashes/src/cn_nthash.rs:19-58 CanonicalNtHashIterator::{new,roll_hash}
+// Produces, for every m-mer of `seq`, the (fw, rc) pair exactly as the
+// rolling iterator yields it.
+void nthash_m(const uint8_t* seq, size_t len, unsigned m, uint64_t* fw_out, uint64_t* rc_out) {
+    if (len < m) return;
+    uint64_t fh = 0, bw = 0;
+    for (unsigned i = 0; i + 1 < m; i++) {
+        fh ^= rotl64(nt_h(seq[i]), m - i - 2);
+        bw ^= rotl64(nt_rc(seq[i]), i);
+    }
+    const unsigned km1 = m - 1;
+    for (size_t i = 0; i + km1 < len; i++) {
+        uint8_t bi = seq[i], bk = seq[i + km1];
+        uint64_t res = rotl64(fh, 1) ^ nt_h(bk);
+        fh = res ^ rotl64(nt_h(bi), km1);
+        uint64_t res_rc = bw ^ rotl64(nt_rc(bk), km1);
+        bw = rotr64(res_rc ^ nt_rc(bi), 1);
+        fw_out[i] = res;
+        rc_out[i] = res_rc;
+    }
+}
+
+// ------------------------------------------------------- BatchMinQueue ---
+// MinimizerExtraData (crates/assembler_minimizer_bucketing/src/lib.rs:31-35)
+struct MinX {
+    uint32_t index = 0;
+    bool is_forward = false;
+    bool operator!=(const MinX& o) const { return index != o.index || is_forward != o.is_forward; }
+};
+struct Item {
+    uint64_t v = 0;
+    MinX x;
+};
+// std::cmp::min_by_key: returns the first argument unless the second is
+// strictly smaller.
+inline Item min_by_key(const Item& a, const Item& b) { return (b.v < a.v) ? b : a; }
+
+// crates/hashes/src/rolling/batch_minqueue.rs:8-189 (ENABLE_DUPLICATE_CHECKING = true)
+struct BatchMinQueue {
+    size_t size;
+    std::vector<Item> backward;
+    struct Split {
+        Item value;
+        size_t position;
+    };
+    std::vector<Split> splits;
+    explicit BatchMinQueue(size_t w) : size(w), backward(w - 1), splits(w + 2) {}
+
+    // batch_minqueue.rs:37-124
+    template <class CB, class FL>
+    void get_minimizers(const Item* it, size_t it_len, size_t skip_ending_count, CB&& cb, FL&& flush) {
+        size_t pos = 0;
+        size_t sz = it_len - skip_ending_count;
+        if (sz < backward.size()) return;
+        for (auto& b : backward) b = it[pos++];
+        size_t offset = 0;
+        sz -= backward.size();
+        const size_t bl = backward.size();
+        while (offset < sz) {
+            size_t remaining = std::min(sz - offset, bl);
+            for (size_t ii = bl - 1; ii-- > 0;) {  // (0..bl-1).rev()
+                Item cur = backward[ii];
+                Item nxt = backward[ii + 1];
+                if (cur.v == nxt.v) cur.v &= ~1ULL;
+                backward[ii] = min_by_key(cur, nxt);
+            }
+            Item new_item = it[pos++];
+            Item first_min = min_by_key(new_item, backward[0]);
+            if (new_item.v == backward[0].v) first_min.v &= ~1ULL;
+            cb(first_min, offset);
+            Item last_forward = new_item;
+            backward[0] = new_item;
+            for (size_t i = 1; i < remaining; i++) {
+                Item ni = it[pos++];
+                Item& cb_ = backward[i];
+                if (ni.v == last_forward.v) last_forward.v &= ~1ULL;
+                last_forward = min_by_key(last_forward, ni);
+                Item cur_min = min_by_key(last_forward, cb_);
+                if (last_forward.v == cb_.v) cur_min.v &= ~1ULL;
+                cb(cur_min, offset + i);
+                cb_ = ni;
+            }
+            offset += remaining;
+            flush(offset == sz);
+        }
+    }
+
+    // batch_minqueue.rs:127-189
+    template <class SCB>
+    void get_minimizer_splits(const Item* it, size_t it_len, size_t skip_beginning_count,
+                              size_t skip_ending_count, SCB&& splits_cb) {
+        Item last_value;
+        bool is_first = true;
+        size_t last_index = it_len - size;
+        size_t ptr = 0;
+        get_minimizers(
+            it + skip_beginning_count, it_len - skip_beginning_count, skip_ending_count,
+            [&](const Item& mm, size_t index) {
+                bool has_diff = !is_first && (last_value.v != mm.v ||
+                                              (last_value.x != mm.x && (mm.v & 1) == 1));
+                splits[ptr].value = last_value;
+                splits[ptr].position = index + skip_beginning_count;
+                ptr += has_diff ? 1 : 0;
+                is_first = false;
+                last_value = mm;
+            },
+            [&](bool is_last) {
+                if (is_last) {
+                    splits[ptr].value = last_value;
+                    splits[ptr].position = last_index;
+                    ptr++;
+                }
+                for (size_t c = 0; c < ptr; c++) {
+                    bool last = (c + 1 == ptr);
+                    splits_cb(splits[c].position, splits[c].value, last && is_last);
+                }
+                ptr = 0;
+            });
+    }
+};
+
+// ------------------------------------------------------------ records ---
+struct Params {
+    uint32_t k, m, b, sb;
+    int canonical;
+};
+
+struct RecordSet {
+    // One entry per super-k-mer record; bases are ASCII (uppercase ACGT),
+    // already reverse-complemented when rc was chosen.
+    std::vector<uint32_t> sub;  // bucket * 2^sb + second_bucket
+    std::vector<uint32_t> len;
+    std::vector<uint8_t> flags;
+    std::vector<uint16_t> minpos;
+    std::vector<uint32_t> mult;
+    std::vector<uint32_t> color;
+    std::vector<uint64_t> off;  // into bases
+    std::vector<uint8_t> bases;
+    void clear() {
+        sub.clear(); len.clear(); flags.clear(); minpos.clear(); mult.clear(); color.clear();
+        off.clear(); bases.clear();
+    }
+    void append(const RecordSet& o) {
+        uint64_t base = bases.size();
+        sub.insert(sub.end(), o.sub.begin(), o.sub.end());
+        len.insert(len.end(), o.len.begin(), o.len.end());
+        flags.insert(flags.end(), o.flags.begin(), o.flags.end());
+        minpos.insert(minpos.end(), o.minpos.begin(), o.minpos.end());
+        mult.insert(mult.end(), o.mult.begin(), o.mult.end());
+        color.insert(color.end(), o.color.begin(), o.color.end());
+        for (auto x : o.off) off.push_back(base + x);
+        bases.insert(bases.end(), o.bases.begin(), o.bases.end());
+    }
+};
+
+inline uint8_t comp_ascii(uint8_t c) {
+    switch (c) {
+        case 'A': return 'T';
+        case 'C': return 'G';
+        case 'G': return 'C';
+        case 'T': return 'A';
+    }
+    return 'N';
+}
+
+// crates/io/src/sequences_reader.rs:26-37 SEQ_LETTERS_MAPPING
+inline uint8_t normalize(uint8_t c) {
+    switch (c) {
+        case 'A': case 'a': return 'A';
+        case 'C': case 'c': return 'C';
+        case 'G': case 'g': return 'G';
+        case 'T': case 't': return 'T';
+    }
+    return 'N';
+}
+
+// crates/assembler_minimizer_bucketing/src/lib.rs:171-270 process_sequence,
+// applied to one N-free run (include_first = include_last = true, as set by
+// preprocess_dna_sequence :149-150).
+void bucket_run(const uint8_t* seq, size_t L, const Params& P, uint32_t color, RecordSet& out,
+                std::vector<uint64_t>& fw, std::vector<uint64_t>& rcv, std::vector<Item>& items,
+                BatchMinQueue& q) {
+    const uint32_t k = P.k, m = P.m;
+    size_t n = L - m + 1;
+    fw.resize(n);
+    rcv.resize(n);
+    items.resize(n);
+    nthash_m(seq, L, m, fw.data(), rcv.data());
+    for (size_t i = 0; i < n; i++) {
+        // to_unextendable() | !is_rc_symmetric()   (cn_nthash.rs:93-109, lib.rs:200-209)
+        uint64_t mn = std::min(fw[i], rcv[i]);
+        items[i].v = (mn << 1) | (fw[i] != rcv[i] ? 1ULL : 0ULL);
+        items[i].x.index = (uint32_t)i;
+        items[i].x.is_forward = fw[i] < rcv[i];
+    }
+    size_t last_index = 1;
+    bool include_first = true;
+    const bool include_last_pre = true;
+    const uint32_t dup_bucket = 1u << P.b;
+    q.get_minimizer_splits(items.data(), n, 0, 0, [&](size_t index, const Item& mh, bool is_last) {
+        bool include_last = include_last_pre && is_last;
+        uint32_t bucket;
+        bool rc;
+        uint16_t minimizer_pos;
+        if ((mh.v & 1) == 0) {
+            bucket = dup_bucket;
+            rc = false;
+            minimizer_pos = 0;
+        } else {
+            rc = P.canonical && !mh.x.is_forward;
+            // MNHFactory::get_bucket(0, b, h) = (h >> 1) % 2^b   (cn_nthash.rs:135-142)
+            bucket = (uint32_t)((mh.v >> 1) & ((1ULL << P.b) - 1));
+            minimizer_pos = rc ? (uint16_t)(index + k - 1 - mh.x.index - m)
+                               : (uint16_t)(mh.x.index - (last_index - 1));
+        }
+        uint32_t second = (uint32_t)((mh.v >> (P.b + 1)) & ((1ULL << P.sb) - 1));
+        size_t s0 = last_index - 1, s1 = index + k - 1;
+        uint8_t flags = (uint8_t)(((include_first ? 1 : 0) << (rc ? 1 : 0)) |
+                                  ((include_last ? 1 : 0) << (rc ? 0 : 1)));
+        out.sub.push_back((bucket << P.sb) | second);
+        out.len.push_back((uint32_t)(s1 - s0));
+        out.flags.push_back(flags);
+        out.minpos.push_back(minimizer_pos);
+        out.mult.push_back(1);
+        out.color.push_back(color);
+        out.off.push_back(out.bases.size());
+        if (rc) {
+            for (size_t i = s1; i-- > s0;) out.bases.push_back(comp_ascii(seq[i]));
+        } else {
+            out.bases.insert(out.bases.end(), seq + s0, seq + s1);
+        }
+        last_index = index;
+        include_first = false;
+    });
+}
+
+// crates/minimizer_bucketing/src/sequences_splitter.rs:15-40 (runs >= k of
+// non-N bases) applied to normalised records, then bucket_run on each run.
+void bucket_records(const uint8_t* seq, const uint64_t* rec_off, size_t r0, size_t r1,
+                    const uint32_t* rec_color, const Params& P, RecordSet& out) {
+    std::vector<uint64_t> fw, rcv;
+    std::vector<Item> items;
+    std::vector<uint8_t> norm;
+    BatchMinQueue q(P.k - P.m);
+    for (size_t r = r0; r < r1; r++) {
+        const uint8_t* s = seq + rec_off[r];
+        size_t len = rec_off[r + 1] - rec_off[r];
+        norm.resize(len);
+        for (size_t i = 0; i < len; i++) norm[i] = normalize(s[i]);
+        size_t end = 0;
+        uint32_t color = rec_color ? rec_color[r] : 0;
+        while (end < len) {
+            size_t start = end;
+            while (start < len && ((norm[start] ^ 'N') & 7) == 0) start++;
+            end = start;
+            while (end < len && ((norm[end] ^ 'N') & 7) != 0) end++;
+            if (end - start >= P.k) bucket_run(norm.data() + start, end - start, P, color, out, fw, rcv, items, q);
+        }
+    }
+}
+
+// ------------------------------------------------------- k-mer hashes ---
+inline uint8_t cbase(uint8_t ascii) { return (ascii >> 1) & 3; }  // utils/src/lib.rs:44-47
+
+// crates/hashes/src/base/cn_seqhash_base.rs:22-72 (u64 for k <= 32, u128 otherwise)
+struct SeqHash {
+    u128 fw, rc;
+};
+inline u128 seq_mask(unsigned k) { return (k >= 64) ? ~(u128)0 : (((u128)1 << (2 * k)) - 1); }
+
+// crates/hashes/src/cn_rkhash.rs:61-78 (u128 constants), base/cn_rkhash_base.rs:10-110
+inline u128 mk128(uint64_t hi, uint64_t lo) { return ((u128)hi << 64) | lo; }
+const u128 RK_MULT = mk128(0x3eb9402f3e733993ULL, 0xadd64d3ca00e1b6bULL);
+const u128 RK_MULT_INV = mk128(0x09cb6ff6f1b1a6d7ULL, 0x33e0952e899c3943ULL);
+const u128 RK_A = mk128(0x4751137d01d863c5ULL, 0xb8c36de2b7d399dfULL);
+const u128 RK_C = mk128(0x37ea3a13226503fbULL, 0x783f5cb69f4552bdULL);
+const u128 RK_G = mk128(0x50796b285343f09aULL, 0x0c53113ae736572bULL);
+const u128 RK_T = mk128(0x1e62d96a5e1f5adeULL, 0x2d4e68d8f88110b7ULL);
+// FWD_LOOKUP / BKW_LOOKUP on compressed codes A=0,C=1,T=2,G=3
+inline u128 rk_fwd(uint8_t c) {
+    switch (c & 3) { case 0: return RK_A; case 1: return RK_C; case 2: return RK_T; default: return RK_G; }
+}
+inline u128 rk_bkw(uint8_t c) {
+    switch (c & 3) { case 0: return RK_T; case 1: return RK_G; case 2: return RK_A; default: return RK_C; }
+}
+u128 pow128(u128 b, size_t e) {  // crates/hashes/src/lib.rs:169-183 fastexp
+    u128 r = 1;
+    while (e) {
+        if (e & 1) r *= b;
+        e >>= 1;
+        b *= b;
+    }
+    return r;
+}
+
+// hash kind: 0 = canonical seq-hash, 1 = canonical rabin-karp128
+// Iterates the k-mers of a 2-bit sequence calling cb(idx, key, is_forward).
+template <class CB>
+void kmer_hashes(const uint8_t* codes, size_t len, unsigned k, int kind, CB&& cb) {
+    if (len < k) return;
+    if (kind == 0) {
+        const u128 mask = seq_mask(k);
+        u128 fh = 0, bw = 0;
+        for (unsigned i = 0; i + 1 < k; i++) {  // cn_seqhash_base.rs:22-51
+            fh |= (u128)codes[i] << (2 * i);
+            bw = (bw << 2) | (u128)(codes[i] ^ 2);
+        }
+        fh <<= 2;
+        bw &= mask;
+        for (size_t idx = k - 1; idx < len; idx++) {  // roll_hash :54-72
+            u128 base = codes[idx];
+            fh = (fh >> 2) | (base << ((k - 1) * 2));
+            bw = ((bw << 2) | (base ^ 2)) & mask;
+            cb(idx - (k - 1), fh < bw ? fh : bw, fh < bw);
+        }
+    } else {
+        u128 fh = 0, bw = 0;  // cn_rkhash_base.rs:65-110
+        for (unsigned i = 0; i + 1 < k; i++) fh = fh * RK_MULT + rk_fwd(codes[i]);
+        for (unsigned i = k - 1; i-- > 0;) bw = bw * RK_MULT + rk_bkw(codes[i]);
+        bw *= RK_MULT;
+        const u128 rmmult = pow128(RK_MULT, k - 1);
+        for (size_t idx = k - 1; idx < len; idx++) {
+            uint8_t in = codes[idx], out = codes[idx - (k - 1)];
+            u128 cur_fh = fh * RK_MULT + rk_fwd(in);
+            fh = cur_fh - rk_fwd(out) * rmmult;
+            u128 cur_bk = bw * RK_MULT_INV + rk_bkw(in) * rmmult;
+            bw = cur_bk - rk_bkw(out);
+            cb(idx - (k - 1), cur_fh < cur_bk ? cur_fh : cur_bk, cur_fh < cur_bk);
+        }
+    }
+}
+
+struct U128Hash {
+    size_t operator()(const u128& x) const {
+        uint64_t a = (uint64_t)x, b = (uint64_t)(x >> 64);
+        uint64_t h = a * 0x9E3779B97F4A7C15ULL ^ (b + 0x632BE59BD9B4E019ULL + (a << 6) + (a >> 2));
+        return (size_t)(h ^ (h >> 31));
+    }
+};
+
+// structs/src/map_entry.rs:6-85 (count 61 bits | used | 2 flag bits)
+struct Entry {
+    uint64_t count = 0;
+    uint8_t flags = 0;
+};
+
+struct SolidSet {
+    std::vector<uint32_t> sub;
+    std::vector<uint64_t> key_lo, key_hi;
+    std::vector<uint32_t> count;  // get_kmer_multiplicity()
+    std::vector<uint8_t> flags;
+    std::vector<uint32_t> colors_off;  // color-set offsets (colored runs)
+    std::vector<uint32_t> colors;
+    void append(const SolidSet& o) {
+        sub.insert(sub.end(), o.sub.begin(), o.sub.end());
+        key_lo.insert(key_lo.end(), o.key_lo.begin(), o.key_lo.end());
+        key_hi.insert(key_hi.end(), o.key_hi.begin(), o.key_hi.end());
+        count.insert(count.end(), o.count.begin(), o.count.end());
+        flags.insert(flags.end(), o.flags.begin(), o.flags.end());
+    }
+};
+
+// crates/assembler_kmers_merge/src/unitigs_extender/hashmap.rs:361-431
+// add_sequence over every record of one sub-bucket, then the solid filter
+// of hashmap.rs:134-137 / map_entry.rs:79-83.
+void merge_subbucket(const RecordSet& R, const std::vector<size_t>& idx, uint32_t sub, unsigned k,
+                     int kind, uint32_t min_mult, SolidSet& out,
+                     std::unordered_map<u128, Entry, U128Hash>& map, std::vector<uint8_t>& codes) {
+    map.clear();
+    for (size_t r : idx) {
+        uint32_t L = R.len[r];
+        const uint8_t* b = R.bases.data() + R.off[r];
+        codes.resize(L);
+        for (uint32_t i = 0; i < L; i++) codes[i] = cbase(b[i]);
+        const uint8_t fl = R.flags[r];
+        const uint32_t mult = R.mult[r];
+        const size_t last = L - k;
+        kmer_hashes(codes.data(), L, k, kind, [&](size_t i, u128 key, bool is_fwd) {
+            bool begin_ignored = (fl & 1) == 0 && i == 0;
+            bool end_ignored = (fl & 2) == 0 && i == last;
+            Entry& e = map[key];
+            e.flags |= (uint8_t)(((begin_ignored ? 1 : 0) << (is_fwd ? 0 : 1)) |
+                                 ((end_ignored ? 1 : 0) << (is_fwd ? 1 : 0)));
+            e.count += mult;
+        });
+    }
+    for (auto& kv : map) {
+        uint64_t mult = kv.second.count >> (kv.second.flags == 3 ? 1 : 0);
+        if (mult < min_mult) continue;
+        out.sub.push_back(sub);
+        out.key_lo.push_back((uint64_t)kv.first);
+        out.key_hi.push_back((uint64_t)(kv.first >> 64));
+        out.count.push_back((uint32_t)mult);
+        out.flags.push_back(kv.second.flags);
+    }
+}
+
+struct Handle {
+    RecordSet rec;
+    SolidSet solid;
+    double t_bucket = 0, t_merge = 0;
+};
+
+std::vector<std::vector<size_t>> group_by_sub(const RecordSet& R, uint32_t n_sub) {
+    std::vector<std::vector<size_t>> g(n_sub);
+    for (size_t i = 0; i < R.sub.size(); i++) g[R.sub[i]].push_back(i);
+    return g;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- C API ----
+extern "C" {
+
+// ntHash over an ASCII sequence: fw/rc arrays of len-m+1 entries.
+void orc_nthash(const uint8_t* seq, size_t len, unsigned m, uint64_t* fw, uint64_t* rc) {
+    nthash_m(seq, len, m, fw, rc);
+}
+
+// BatchMinQueue::get_minimizers over explicit (v, index, is_forward) items.
+// Writes one (v, index, is_forward) per window; returns the window count.
+size_t orc_minqueue(const uint64_t* v, const uint32_t* index, const uint8_t* fwd, size_t n, size_t w,
+                    uint64_t* out_v, uint32_t* out_index, uint8_t* out_fwd) {
+    std::vector<Item> it(n);
+    for (size_t i = 0; i < n; i++) {
+        it[i].v = v[i];
+        it[i].x.index = index[i];
+        it[i].x.is_forward = fwd[i] != 0;
+    }
+    BatchMinQueue q(w);
+    size_t cnt = 0;
+    q.get_minimizers(it.data(), n, 0, [&](const Item& mm, size_t off) {
+        out_v[off] = mm.v;
+        out_index[off] = mm.x.index;
+        out_fwd[off] = mm.x.is_forward;
+        cnt = std::max(cnt, off + 1);
+    }, [](bool) {});
+    return cnt;
+}
+
+// Full bucketing of a record collection with `threads` workers.
+void* orc_bucketing(const uint8_t* seq, const uint64_t* rec_off, size_t n_rec, const uint32_t* rec_color,
+                    uint32_t k, uint32_t m, uint32_t b, uint32_t sb, int canonical, int threads) {
+    Handle* h = new Handle();
+    Params P{k, m, b, sb, canonical};
+    if (threads < 1) threads = 1;
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<RecordSet> parts(threads);
+    std::vector<std::thread> th;
+    for (int t = 0; t < threads; t++) {
+        size_t r0 = n_rec * t / threads, r1 = n_rec * (t + 1) / threads;
+        th.emplace_back([&, t, r0, r1] { bucket_records(seq, rec_off, r0, r1, rec_color, P, parts[t]); });
+    }
+    for (auto& x : th) x.join();
+    for (auto& p : parts) h->rec.append(p);
+    h->t_bucket = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return h;
+}
+
+// crates/minimizer_bucketing/src/compactor.rs:128-196: identical records
+// (same sub-bucket, length, packed bases and flags) merged, multiplicities
+// summed.  Produces the fully compacted form of a record set.
+void orc_compact(void* hv) {
+    Handle* h = (Handle*)hv;
+    RecordSet& R = h->rec;
+    size_t n = R.sub.size();
+    std::vector<size_t> order(n);
+    for (size_t i = 0; i < n; i++) order[i] = i;
+    auto cmp = [&](size_t a, size_t b) {
+        if (R.sub[a] != R.sub[b]) return R.sub[a] < R.sub[b];
+        if (R.len[a] != R.len[b]) return R.len[a] < R.len[b];
+        if (R.flags[a] != R.flags[b]) return R.flags[a] < R.flags[b];
+        int c = memcmp(R.bases.data() + R.off[a], R.bases.data() + R.off[b], R.len[a]);
+        if (c != 0) return c < 0;
+        return R.color[a] < R.color[b];
+    };
+    std::sort(order.begin(), order.end(), cmp);
+    RecordSet O;
+    for (size_t i = 0; i < n;) {
+        size_t j = i;
+        uint32_t mult = 0;
+        size_t a = order[i];
+        while (j < n) {
+            size_t c = order[j];
+            if (R.sub[c] != R.sub[a] || R.len[c] != R.len[a] || R.flags[c] != R.flags[a] ||
+                memcmp(R.bases.data() + R.off[c], R.bases.data() + R.off[a], R.len[a]) != 0)
+                break;
+            mult += R.mult[c];
+            j++;
+        }
+        O.sub.push_back(R.sub[a]);
+        O.len.push_back(R.len[a]);
+        O.flags.push_back(R.flags[a]);
+        O.minpos.push_back(R.minpos[a]);
+        O.mult.push_back(mult);
+        O.color.push_back(R.color[a]);
+        O.off.push_back(O.bases.size());
+        O.bases.insert(O.bases.end(), R.bases.begin() + R.off[a], R.bases.begin() + R.off[a] + R.len[a]);
+        i = j;
+    }
+    R = std::move(O);
+}
+
+size_t orc_rec_count(void* hv) { return ((Handle*)hv)->rec.sub.size(); }
+size_t orc_rec_bases(void* hv) { return ((Handle*)hv)->rec.bases.size(); }
+void orc_rec_get(void* hv, uint32_t* sub, uint32_t* len, uint8_t* flags, uint16_t* minpos, uint32_t* mult,
+                 uint64_t* off, uint8_t* bases) {
+    RecordSet& R = ((Handle*)hv)->rec;
+    size_t n = R.sub.size();
+    memcpy(sub, R.sub.data(), n * 4);
+    memcpy(len, R.len.data(), n * 4);
+    memcpy(flags, R.flags.data(), n);
+    memcpy(minpos, R.minpos.data(), n * 2);
+    memcpy(mult, R.mult.data(), n * 4);
+    memcpy(off, R.off.data(), n * 8);
+    memcpy(bases, R.bases.data(), R.bases.size());
+}
+
+// Per-sub-bucket MapEntry merge over the handle's records, `threads` workers.
+void orc_merge(void* hv, uint32_t k, uint32_t b, uint32_t sb, int kind, uint32_t min_mult, int threads) {
+    Handle* h = (Handle*)hv;
+    auto t0 = std::chrono::steady_clock::now();
+    uint32_t n_sub = ((1u << b) + 1) << sb;
+    auto groups = group_by_sub(h->rec, n_sub);
+    if (threads < 1) threads = 1;
+    std::vector<SolidSet> parts(threads);
+    std::atomic<uint32_t> next{0};
+    std::vector<std::thread> th;
+    for (int t = 0; t < threads; t++) {
+        th.emplace_back([&, t] {
+            std::unordered_map<u128, Entry, U128Hash> map;
+            std::vector<uint8_t> codes;
+            for (;;) {
+                uint32_t s = next.fetch_add(1);
+                if (s >= n_sub) break;
+                if (groups[s].empty()) continue;
+                merge_subbucket(h->rec, groups[s], s, k, kind, min_mult, parts[t], map, codes);
+            }
+        });
+    }
+    for (auto& x : th) x.join();
+    h->solid = SolidSet();
+    for (auto& p : parts) h->solid.append(p);
+    h->t_merge = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+size_t orc_solid_count(void* hv) { return ((Handle*)hv)->solid.sub.size(); }
+void orc_solid_get(void* hv, uint32_t* sub, uint64_t* key_lo, uint64_t* key_hi, uint32_t* count, uint8_t* flags) {
+    SolidSet& S = ((Handle*)hv)->solid;
+    size_t n = S.sub.size();
+    memcpy(sub, S.sub.data(), n * 4);
+    memcpy(key_lo, S.key_lo.data(), n * 8);
+    memcpy(key_hi, S.key_hi.data(), n * 8);
+    memcpy(count, S.count.data(), n * 4);
+    memcpy(flags, S.flags.data(), n);
+}
+void orc_times(void* hv, double* t_bucket, double* t_merge) {
+    *t_bucket = ((Handle*)hv)->t_bucket;
+    *t_merge = ((Handle*)hv)->t_merge;
+}
+void orc_free(void* hv) { delete (Handle*)hv; }
+
+// Canonical k-mer keys of one ASCII sequence (all k-mers, no N handling).
+void orc_kmer_keys(const uint8_t* ascii, size_t len, unsigned k, int kind, uint64_t* key_lo, uint64_t* key_hi,
+                   uint8_t* fwd) {
+    std::vector<uint8_t> codes(len);
+    for (size_t i = 0; i < len; i++) codes[i] = cbase(ascii[i]);
+    kmer_hashes(codes.data(), len, k, kind, [&](size_t i, u128 key, bool f) {
+        key_lo[i] = (uint64_t)key;
+        key_hi[i] = (uint64_t)(key >> 64);
+        fwd[i] = f;
+    });
+}
+
+// Brute force: exact canonical (2-bit) k-mer occurrence counts over the
+// N-free runs of every record; writes keys with count >= min_mult.
+void* orc_bruteforce_counts(const uint8_t* seq, const uint64_t* rec_off, size_t n_rec, unsigned k,
+                            uint32_t min_mult) {
+    Handle* h = new Handle();
+    std::unordered_map<u128, Entry, U128Hash> map;
+    std::vector<uint8_t> codes;
+    for (size_t r = 0; r < n_rec; r++) {
+        const uint8_t* s = seq + rec_off[r];
+        size_t len = rec_off[r + 1] - rec_off[r];
+        size_t i = 0;
+        while (i < len) {
+            while (i < len && normalize(s[i]) == 'N') i++;
+            size_t st = i;
+            while (i < len && normalize(s[i]) != 'N') i++;
+            if (i - st < k) continue;
+            codes.resize(i - st);
+            for (size_t j = st; j < i; j++) codes[j - st] = cbase(normalize(s[j]));
+            kmer_hashes(codes.data(), i - st, k, 0, [&](size_t, u128 key, bool) { map[key].count++; });
+        }
+    }
+    for (auto& kv : map) {
+        if (kv.second.count < min_mult) continue;
+        h->solid.sub.push_back(0);
+        h->solid.key_lo.push_back((uint64_t)kv.first);
+        h->solid.key_hi.push_back((uint64_t)(kv.first >> 64));
+        h->solid.count.push_back((uint32_t)kv.second.count);
+        h->solid.flags.push_back(0);
+    }
+    return h;
+}
+
+}  // extern "C"

@@ -1,0 +1,108 @@
+"""Shared helpers for parity tests: canonical forms of record multisets and
+solid tables, and adversarial inputs."""
+from __future__ import annotations
+
+import numpy as np
+
+SEP = ord("\n")
+
+
+def record_tuples(r) -> list:
+    """Sorted list of (sub, len, flags, minpos, bases) for small record sets."""
+    get = (lambda x, n: x[n]) if isinstance(r, dict) else (lambda x, n: getattr(x, n))
+    sub, ln, fl, mp, off, bases = (get(r, n) for n in ("sub", "length", "flags", "minpos", "off", "bases"))
+    out = []
+    for i in range(sub.size):
+        o = int(off[i])
+        out.append((int(sub[i]), int(ln[i]), int(fl[i]), int(mp[i]), bytes(bases[o:o + int(ln[i])])))
+    out.sort()
+    return out
+
+
+def record_digests(r) -> np.ndarray:
+    """Sorted 2x64-bit digests of every record (sub, len, flags, minpos, bases);
+    used to compare multisets of millions of records."""
+    get = (lambda x, n: x[n]) if isinstance(r, dict) else (lambda x, n: getattr(x, n))
+    sub, ln, fl, mp, off, bases = (get(r, n) for n in ("sub", "length", "flags", "minpos", "off", "bases"))
+    n = sub.size
+    if n == 0:
+        return np.zeros((0, 2), np.uint64)
+    ln64 = ln.astype(np.uint64)
+    total = int(ln64.sum())
+    # records are contiguous in `bases` in record order with off == running sum
+    assert np.array_equal(off.astype(np.uint64), np.concatenate(([0], np.cumsum(ln64)[:-1])).astype(np.uint64))
+    pos = np.arange(total, dtype=np.uint64) - np.repeat(off.astype(np.uint64), ln64)
+    b = bases[:total].astype(np.uint64)
+    out = np.empty((n, 2), np.uint64)
+    with np.errstate(over="ignore"):
+        for j, (p1, p2) in enumerate(((0x9E3779B97F4A7C15, 0xC2B2AE3D27D4EB4F), (0x165667B19E3779F9, 0xD6E8FEB86659FD93))):
+            w = (pos + np.uint64(p1)) * np.uint64(p2)
+            w ^= w >> np.uint64(29)
+            w *= np.uint64(0xBF58476D1CE4E5B9)
+            h = np.add.reduceat(b * w, np.concatenate(([0], np.cumsum(ln64)[:-1])).astype(np.int64)) if total else np.zeros(n, np.uint64)
+            h = h.astype(np.uint64)
+            h ^= (sub.astype(np.uint64) * np.uint64(p1)) ^ (ln64 << np.uint64(20)) ^ (fl.astype(np.uint64) << np.uint64(40)) ^ (mp.astype(np.uint64) << np.uint64(48))
+            h *= np.uint64(p2)
+            h ^= h >> np.uint64(31)
+            out[:, j] = h
+    idx = np.lexsort((out[:, 1], out[:, 0]))
+    return out[idx]
+
+
+def solid_tuples(s) -> list:
+    if isinstance(s, dict):
+        lo, hi, mult, fl, sub = s["key_lo"], s["key_hi"], s["mult"], s["flags"], s["sub"]
+    else:
+        lo = s.keys[:, 0]
+        hi = s.keys[:, 1] if s.keys.shape[1] > 1 else np.zeros_like(lo)
+        mult, fl, sub = s.mult, s.flags, s.sub
+    return sorted(zip(sub.tolist(), lo.tolist(), hi.tolist(), mult.tolist(), fl.tolist()))
+
+
+def solid_array(s) -> np.ndarray:
+    """(n, 5) uint64 array sorted lexicographically: sub, key_lo, key_hi, mult, flags."""
+    if isinstance(s, dict):
+        cols = [s["sub"], s["key_lo"], s["key_hi"], s["mult"], s["flags"]]
+    else:
+        lo = s.keys[:, 0]
+        hi = s.keys[:, 1] if s.keys.shape[1] > 1 else np.zeros_like(lo)
+        cols = [s.sub, lo, hi, s.mult, s.flags]
+    a = np.stack([c.astype(np.uint64) for c in cols], axis=1)
+    idx = np.lexsort(tuple(a[:, j] for j in range(4, -1, -1)))
+    return a[idx]
+
+
+def joined(records: list[bytes]) -> np.ndarray:
+    return np.frombuffer(b"".join(r + b"\n" for r in records), np.uint8).copy()
+
+
+def adversarial_records(rng: np.random.Generator) -> list[bytes]:
+    """Edge cases: N runs, lowercase, short/exact-k records, homopolymers,
+    tandem repeats, a long random contig and a long poly-A run."""
+    def rnd(n):
+        return bytes(rng.choice(np.frombuffer(b"ACGT", np.uint8), n))
+
+    recs = [
+        b"",
+        b"ACGT",
+        rnd(30),  # k-1
+        rnd(31),  # exactly k
+        rnd(32),
+        rnd(60) + b"N" + rnd(40),
+        rnd(10) + b"NNNN" + rnd(31) + b"N" + rnd(29) + b"NN" + rnd(80),
+        rnd(70).lower(),
+        rnd(20) + rnd(40).lower() + rnd(20),
+        b"A" * 200,
+        b"C" * 31,
+        b"AC" * 120,
+        b"ACG" * 90,
+        b"AAAAAAAAAAAAAAAACAAAAAAAAAAAAAAAAAAAAA" * 4,
+        rnd(40) + b"A" * 90 + rnd(40),
+        rnd(100) + b"X" + rnd(100) + b"-" + rnd(50),
+        rnd(5000),
+        b"A" * 5000,
+        rnd(300) + (b"GATTACA" * 200) + rnd(300),
+    ]
+    for _ in range(40):
+        recs.append(rnd(int(rng.integers(20, 400))))
+    return recs

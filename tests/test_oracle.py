@@ -1,0 +1,277 @@
+"""CPU tests that pin the oracle (oracle/ggcat_oracle.cpp) independently of
+its own code: closed forms, invariants from the reference's tests, literal
+golden vectors from the reference's tests, and brute force on small inputs."""
+import numpy as np
+import pytest
+
+from oracle import binding as O
+from tests.helpers import adversarial_records, joined, record_tuples
+
+M64 = (1 << 64) - 1
+NT_MULT = 0x397F178C6AE330F9
+COMP = bytes.maketrans(b"ACGT", b"TGCA")
+
+
+def rotl(x, r):
+    r %= 64
+    return ((x << r) | (x >> (64 - r))) & M64 if r else x
+
+
+def revcomp(s: bytes) -> bytes:
+    return s.translate(COMP)[::-1]
+
+
+def nt_direct(s: bytes, m: int):
+    """Closed form of crates/hashes/src/cn_nthash.rs:19-58:
+    fw = XOR_j rotl(h(s_j), m-1-j), rc = XOR_j rotl(rc(s_j), j)."""
+    h = lambda c: (((c & 6) + 1) * NT_MULT) & M64
+    r = lambda c: ((((c & 6) ^ 4) + 1) * NT_MULT) & M64
+    fw = rc = 0
+    for j, c in enumerate(s[:m]):
+        fw ^= rotl(h(c), m - 1 - j)
+        rc ^= rotl(r(c), j)
+    return fw, rc
+
+
+# ---------------------------------------------------------------- ntHash ---
+@pytest.mark.parametrize("m", [2, 5, 15, 21, 33, 63])
+def test_nthash_roll_equals_closed_form(m):
+    rng = np.random.default_rng(m)
+    s = bytes(rng.choice(np.frombuffer(b"ACGT", np.uint8), 300))
+    fw, rc = O.nthash(s, m)
+    for i in range(0, len(s) - m + 1, 7):
+        assert (int(fw[i]), int(rc[i])) == nt_direct(s[i:i + m], m)
+
+
+@pytest.mark.parametrize("m", [15, 21])
+def test_nthash_rc_symmetry(m):
+    # test_hash_function's canonical check (crates/hashes/src/lib.rs:265-455):
+    # the hash of a sequence's reverse complement swaps fw and rc
+    rng = np.random.default_rng(3 * m)
+    s = bytes(rng.choice(np.frombuffer(b"ACGT", np.uint8), 200))
+    fw, rc = O.nthash(s, m)
+    fw2, rc2 = O.nthash(revcomp(s), m)
+    n = len(s) - m + 1
+    assert np.array_equal(fw, rc2[::-1]) and np.array_equal(rc, fw2[::-1])
+    assert n == fw.size
+
+
+def test_nthash_lowercase_same_as_uppercase():
+    s = b"ACGTTGCAACGGTTAC"
+    assert all(np.array_equal(a, b) for a, b in zip(O.nthash(s, 7), O.nthash(s.lower(), 7)))
+
+
+# --------------------------------------------------------- BatchMinQueue ---
+def brute_window(v, idx, fwd, w):
+    """Closed-form window minimum: the minimum value V over the window; it
+    keeps its unique bit only if V is odd and occurs once (the semantics the
+    reference's minqueue_test asserts, crates/hashes/src/rolling/minqueue_testing.rs:345-365)."""
+    out = []
+    for s in range(len(v) - w + 1):
+        win = v[s:s + w]
+        V = min(win)
+        cnt = win.count(V)
+        uniq = (V & 1) == 1 and cnt == 1
+        val = V if uniq else V & ~1
+        if uniq:
+            j = s + win.index(V)
+            out.append((val, idx[j], fwd[j]))
+        else:
+            out.append((val, None, None))
+    return out
+
+
+@pytest.mark.parametrize("w,vrange", [(2, 8), (3, 16), (13, 64), (16, 40), (16, 1 << 40), (42, 200)])
+def test_minqueue_matches_closed_form(w, vrange):
+    rng = np.random.default_rng(w * 1000 + vrange % 997)
+    n = 3000
+    v = (rng.integers(0, vrange, n, dtype=np.uint64) << np.uint64(1)) | rng.integers(0, 2, n, dtype=np.uint64)
+    idx = np.arange(n, dtype=np.uint32)
+    fwd = rng.integers(0, 2, n, dtype=np.uint8)
+    ov, oi, of = O.minqueue(v, idx, fwd, w)
+    bf = brute_window(v.tolist(), idx.tolist(), fwd.tolist(), w)
+    assert ov.size == len(bf)
+    for i, (val, j, f) in enumerate(bf):
+        assert int(ov[i]) == val, (i, hex(int(ov[i])), hex(val))
+        if j is not None:
+            assert int(oi[i]) == j and int(of[i]) == f
+
+
+# ------------------------------------------------------- super-k-mers -----
+def bf_bucketing(data: bytes, k: int, m: int, b: int, sb: int):
+    """Independent restatement of the group semantics: window minima over
+    (k-1)-mers, groups of equal minimizer (value, and position when unique),
+    super-k-mer = k-mers touching the group, clipped to the ACGT run."""
+    recs = []
+    s = data.upper()
+    runs = []
+    i = 0
+    while i < len(s):
+        while i < len(s) and s[i:i + 1] not in (b"A", b"C", b"G", b"T"):
+            i += 1
+        st = i
+        while i < len(s) and s[i:i + 1] in (b"A", b"C", b"G", b"T"):
+            i += 1
+        if i - st >= k:
+            runs.append(s[st:i])
+    w = k - m
+    for run in runs:
+        L = len(run)
+        items = []
+        for p in range(L - m + 1):
+            fw, rc = nt_direct(run[p:p + m], m)
+            items.append((((min(fw, rc) << 1) & M64) | (fw != rc), p, fw < rc))
+        v = [x[0] for x in items]
+        win = brute_window(v, [x[1] for x in items], [x[2] for x in items], w)
+        nwin = len(win)  # = L - k + 2
+        starts = [0] + [j for j in range(1, nwin) if win[j][0] != win[j - 1][0] or
+                        (win[j][0] & 1 and win[j][1] != win[j - 1][1])]
+        for gi, a in enumerate(starts):
+            bb = (starts[gi + 1] - 1) if gi + 1 < len(starts) else nwin - 1
+            val, pos, f = win[bb]
+            first, last = a == 0, bb == nwin - 1
+            start = a if first else a - 1
+            end = bb + k - 1 if last else bb + k
+            uniq = val & 1
+            rc = bool(uniq) and not f
+            bucket = ((val >> 1) & ((1 << b) - 1)) if uniq else (1 << b)
+            second = (val >> (b + 1)) & ((1 << sb) - 1)
+            minpos = 0 if not uniq else (end - pos - m if rc else pos - start)
+            flags = (int(first) << int(rc)) | (int(last) << int(not rc))
+            seq = run[start:end]
+            if rc:
+                seq = revcomp(seq)
+            recs.append(((bucket << sb) | second, end - start, flags, minpos, seq))
+    return sorted(recs)
+
+
+@pytest.mark.parametrize("k,m", [(31, 15), (11, 5), (7, 3), (21, 9)])
+def test_bucketing_matches_bruteforce(k, m):
+    rng = np.random.default_rng(k * 31 + m)
+    recs = adversarial_records(rng)
+    recs = [r for r in recs if len(r) <= 600]  # keep the pure-Python brute force quick
+    data = joined(recs)
+    got = record_tuples(O.bucketing(data, k, m, 4, 2).records())
+    exp = bf_bucketing(bytes(data), k, m, 4, 2)
+    assert got == exp
+
+
+def test_superkmer_cover_property():
+    """Every k-mer of every run is in exactly one record as an interior k-mer;
+    boundary k-mers of consecutive groups are in exactly two records."""
+    rng = np.random.default_rng(5)
+    g = bytes(rng.choice(np.frombuffer(b"ACGT", np.uint8), 4000))
+    data = joined([g])
+    k, m = 31, 15
+    r = O.bucketing(data, k, m, 5, 2).records()
+    seen = {}
+    for i in range(r["sub"].size):
+        o, L = int(r["off"][i]), int(r["length"][i])
+        s = bytes(r["bases"][o:o + L])
+        for p in range(L - k + 1):
+            km = s[p:p + k]
+            c = min(km, revcomp(km))
+            seen[c] = seen.get(c, 0) + 1
+    kmers = {}
+    for p in range(len(g) - k + 1):
+        km = g[p:p + k]
+        c = min(km, revcomp(km))
+        kmers[c] = kmers.get(c, 0) + 1
+    assert set(seen) == set(kmers)
+    n_rec = r["sub"].size
+    # occurrences = k-mers + one extra per group boundary
+    assert sum(seen.values()) == sum(kmers.values()) + (n_rec - 1)
+
+
+# ------------------------------------------------------------ k-mer keys ---
+def test_seqhash_is_packed_kmer_and_invertible():
+    # crates/hashes/src/base/cn_seqhash_base.rs: fw = sum c_i << 2i (A0 C1 T2 G3)
+    s = b"ACGTTGCAACGGTTACGATCGATCGGGATCC"
+    lo, hi, fw = O.kmer_keys(s, 31, 0)
+    code = {ord("A"): 0, ord("C"): 1, ord("T"): 2, ord("G"): 3}
+    f = sum(code[c] << (2 * i) for i, c in enumerate(s))
+    r = sum((code[c] ^ 2) << (2 * (30 - i)) for i, c in enumerate(s))
+    assert int(lo[0]) == min(f, r) and bool(fw[0]) == (f < r)
+    # invert: decode the key back to a k-mer equal to s or its reverse complement
+    key = int(lo[0])
+    letters = "ACTG"
+    dec = "".join(letters[(key >> (2 * i)) & 3] for i in range(31)).encode()
+    assert dec in (s, revcomp(s))
+
+
+def test_rk128_constants_and_symmetry():
+    M = 0x3EB9402F3E733993ADD64D3CA00E1B6B
+    MI = 0x9CB6FF6F1B1A6D733E0952E899C3943
+    assert (M * MI) % (1 << 128) == 1  # MULT_INV is the inverse (crates/hashes/src/cn_rkhash.rs:66-67)
+    rng = np.random.default_rng(9)
+    s = bytes(rng.choice(np.frombuffer(b"ACGT", np.uint8), 120))
+    k = 63
+    lo, hi, fw = O.kmer_keys(s, k, 1)
+    lo2, hi2, fw2 = O.kmer_keys(revcomp(s), k, 1)
+    assert np.array_equal(lo, lo2[::-1]) and np.array_equal(hi, hi2[::-1])
+    assert np.array_equal(fw, 1 - fw2[::-1])
+    # direct polynomial definition of the forward hash
+    F = {ord("A"): 0x4751137D01D863C5B8C36DE2B7D399DF, ord("C"): 0x37EA3A13226503FB783F5CB69F4552BD,
+         ord("G"): 0x50796B285343F09A0C53113AE736572B, ord("T"): 0x1E62D96A5E1F5ADE2D4E68D8F88110B7}
+    comp = {ord("A"): ord("T"), ord("C"): ord("G"), ord("G"): ord("C"), ord("T"): ord("A")}
+    for p in (0, 17, len(s) - k):
+        km = s[p:p + k]
+        f = sum(F[c] * pow(M, k - 1 - i, 1 << 128) for i, c in enumerate(km)) % (1 << 128)
+        r = sum(F[comp[c]] * pow(M, i, 1 << 128) for i, c in enumerate(km)) % (1 << 128)
+        key = int(lo[p]) | (int(hi[p]) << 64)
+        assert key == min(f, r) and bool(fw[p]) == (f < r)
+
+
+# ---------------------------------------------------- golden vectors -------
+def test_rc_compression_golden():
+    # crates/io/src/compressed_read.rs:1012-1023 test_rc_compression literal
+    bases = b"ACGTACGCGGTAGCTAAGCATCGATGCCGATCGTGTTTAACCATG"
+    expected = b"CATGGTTAAACACGATCGGCATCGATGCTTAGCTACCGCGTACGT"
+    assert revcomp(bases) == expected
+    # a run forced to be stored reverse-complemented reproduces it: find a
+    # record stored rc in the oracle's output and check its bases
+    data = joined([bases])
+    r = O.bucketing(data, 31, 15, 4, 2).records()
+    for i in range(r["sub"].size):
+        o, L = int(r["off"][i]), int(r["length"][i])
+        s = bytes(r["bases"][o:o + L])
+        assert s in bases or revcomp(s) in bases
+
+
+# ------------------------------------------------------------- merge ------
+@pytest.mark.parametrize("k,m,s", [(31, 15, 2), (21, 9, 1), (31, 15, 3)])
+def test_merge_union_equals_exact_counts(k, m, s):
+    from ggcat_amd import synth
+    g = synth.genome(11, 40_000)
+    data = synth.reads(g, 12, 8000, 150, 0.01)
+    r = O.bucketing(data, k, m, 6, 2, threads=2)
+    r.merge(k, 6, 2, 0, s, 2)
+    sol = r.solid()
+    union = {}
+    for lo, mult in zip(sol["key_lo"].tolist(), sol["mult"].tolist()):
+        assert union.get(lo, mult) == mult
+        union[lo] = mult
+    bf = O.bruteforce_counts(data, k, s)
+    assert union == dict(zip(bf["key_lo"].tolist(), bf["mult"].tolist()))
+    # a k-mer kept in two sub-buckets is a group boundary: one flag bit each
+    from collections import Counter
+    c = Counter(sol["key_lo"].tolist())
+    for i in range(sol["sub"].size):
+        if c[int(sol["key_lo"][i])] > 1:
+            assert int(sol["flags"][i]) in (1, 2)
+
+
+def test_compaction_preserves_solid_table():
+    from ggcat_amd import synth
+    from tests.helpers import solid_array
+    g = synth.genome(13, 30_000)
+    data = synth.reads(g, 14, 6000, 150, 0.005)
+    r = O.bucketing(data, 31, 15, 6, 2, threads=2)
+    r.merge(31, 6, 2, 0, 2, 2)
+    a = solid_array(r.solid())
+    n_before = int(r.records()["sub"].size)
+    r.compact()
+    rec = r.records()
+    assert int(rec["mult"].sum()) == n_before and rec["sub"].size < n_before
+    r.merge(31, 6, 2, 0, 2, 2)
+    assert np.array_equal(a, solid_array(r.solid()))
