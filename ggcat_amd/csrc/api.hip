@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <tuple>
 #include <vector>
 
 using namespace gg;
@@ -23,19 +24,21 @@ struct DevBuf {
 
 struct ggcat_gpu_ctx {
     int device = 0;
+    int n_cu = 256;
     hipStream_t stream = nullptr;
     std::string err;
     hipEvent_t ev[16];
-    // buffers
-    DevBuf input, seg_bits, sub_count, sub_cursor, sub_off, sub_kmers, slots, pool, pool_cursor, slot_color,
-        color_starts, colors;
-    DevBuf work, big_list, big_count, out_key, out_val, out_sub, out_cursor, g_klo, g_khi, g_st, g_val, g_off,
-        g_cap;
+    // bucketing buffers
+    DevBuf input, seg_bits, unit_count, unit_cursor, unit_off, unit_kmers, scan_part, slots, pool, pool_cursor,
+        slot_color, color_starts, colors, error_flag;
+    // merge buffers
+    DevBuf items, n_items, work, big_list, big_count, out_key, out_val, out_sub, out_cursor, fix_key, fix_val,
+        fix_sub, fix_cursor, g_klo, g_khi, g_st, g_val, g_off, g_cap;
     // state
     ggcat_gpu_params P{};
     bool have_records = false, have_solid = false;
-    uint32_t n_sub = 0, slot_words = 0, key_words = 1;
-    uint64_t n_records = 0, pool_words = 0, n_solid = 0, n_input = 0, kmer_occ = 0, n_big = 0, n_long = 0;
+    uint32_t n_sub = 0, n_units = 0, xb = 0, slot_words = 0, key_words = 1;
+    uint64_t n_records = 0, pool_words = 0, n_solid = 0, n_input = 0, kmer_occ = 0, n_big = 0, n_fix = 0;
     bool colored = false;
     double ms[8] = {0};
 };
@@ -47,11 +50,11 @@ int fail(ggcat_gpu_ctx* c, int code, const std::string& msg) {
     return code;
 }
 
-#define HIPCHK(expr)                                                                                 \
-    do {                                                                                             \
-        hipError_t e_ = (expr);                                                                      \
-        if (e_ != hipSuccess)                                                                        \
-            return fail(ctx, GGCAT_GPU_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+#define HIPCHK(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail(ctx, GGCAT_GPU_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
 int ensure(ggcat_gpu_ctx* ctx, DevBuf& b, size_t bytes) {
@@ -73,10 +76,10 @@ int ensure(ggcat_gpu_ctx* ctx, DevBuf& b, size_t bytes) {
     return GGCAT_GPU_OK;
 }
 
-#define ENSURE(buf, bytes)                                  \
-    do {                                                    \
-        int r_ = ensure(ctx, (buf), (bytes));               \
-        if (r_ != GGCAT_GPU_OK) return r_;                  \
+#define ENSURE(buf, bytes)                        \
+    do {                                          \
+        int r_ = ensure(ctx, (buf), (bytes));     \
+        if (r_ != GGCAT_GPU_OK) return r_;        \
     } while (0)
 
 template <class T>
@@ -109,6 +112,18 @@ u128_t mk128(uint64_t hi, uint64_t lo) { return ((u128_t)hi << 64) | lo; }
 
 const uint32_t TILE_T = 1024;
 
+// Device-wide exclusive scan counts[0..n) -> off[0..n].
+int device_scan(ggcat_gpu_ctx* ctx, const uint32_t* counts, uint64_t* off, uint32_t n) {
+    const uint32_t nb = std::max<uint32_t>(1, (n + SCAN_CHUNK - 1) / SCAN_CHUNK);
+    ENSURE(ctx->scan_part, ((size_t)nb + 1) * 8);
+    uint64_t* part = P_<uint64_t>(ctx->scan_part);
+    hipLaunchKernelGGL(scan_reduce_kernel, dim3(nb), dim3(1024), 0, ctx->stream, counts, part, n);
+    hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(1024), 0, ctx->stream, part, nb);
+    hipLaunchKernelGGL(scan_apply_kernel, dim3(nb), dim3(1024), 0, ctx->stream, counts, part, off, n, nb);
+    HIPCHK(hipGetLastError());
+    return GGCAT_GPU_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -121,6 +136,8 @@ int ggcat_gpu_create(int device, ggcat_gpu_ctx** out) {
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     for (int i = 0; i < 16 && e == hipSuccess; i++) e = hipEventCreate(&ctx->ev[i]);
+    hipDeviceProp_t prop;
+    if (e == hipSuccess && hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->n_cu = prop.multiProcessorCount;
     if (e != hipSuccess) {
         delete ctx;
         return GGCAT_GPU_EHIP;
@@ -133,11 +150,12 @@ void ggcat_gpu_destroy(ggcat_gpu_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    DevBuf* bufs[] = {&ctx->input, &ctx->seg_bits, &ctx->sub_count, &ctx->sub_cursor, &ctx->sub_off,
-                      &ctx->sub_kmers, &ctx->slots, &ctx->pool, &ctx->pool_cursor, &ctx->slot_color,
-                      &ctx->color_starts, &ctx->colors, &ctx->work, &ctx->big_list, &ctx->big_count,
-                      &ctx->out_key, &ctx->out_val, &ctx->out_sub, &ctx->out_cursor, &ctx->g_klo,
-                      &ctx->g_khi, &ctx->g_st, &ctx->g_val, &ctx->g_off, &ctx->g_cap};
+    DevBuf* bufs[] = {&ctx->input, &ctx->seg_bits, &ctx->unit_count, &ctx->unit_cursor, &ctx->unit_off,
+                      &ctx->unit_kmers, &ctx->scan_part, &ctx->slots, &ctx->pool, &ctx->pool_cursor,
+                      &ctx->slot_color, &ctx->color_starts, &ctx->colors, &ctx->error_flag, &ctx->items,
+                      &ctx->n_items, &ctx->work, &ctx->big_list, &ctx->big_count, &ctx->out_key, &ctx->out_val,
+                      &ctx->out_sub, &ctx->out_cursor, &ctx->fix_key, &ctx->fix_val, &ctx->fix_sub,
+                      &ctx->fix_cursor, &ctx->g_klo, &ctx->g_khi, &ctx->g_st, &ctx->g_val, &ctx->g_off, &ctx->g_cap};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (int i = 0; i < 16; i++) (void)hipEventDestroy(ctx->ev[i]);
@@ -158,7 +176,6 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     if (p.buckets_log < 1 || p.buckets_log > 13) return fail(ctx, GGCAT_GPU_EINVAL, "buckets_log must be in [1, 13]");
     if (p.second_buckets_log > 8) return fail(ctx, GGCAT_GPU_EINVAL, "second_buckets_log must be <= 8");
     if (p.hash_kind > 1) return fail(ctx, GGCAT_GPU_EINVAL, "unknown hash kind");
-    if (p.hash_kind == GGCAT_GPU_HASH_SEQ && p.k > 64) return fail(ctx, GGCAT_GPU_EINVAL, "seq-hash needs k <= 64");
     if (n_colors && (!color_starts || !colors)) return fail(ctx, GGCAT_GPU_EINVAL, "null color arrays");
     if (input_on_device && (reinterpret_cast<uintptr_t>(data) & 15))
         return fail(ctx, GGCAT_GPU_EINVAL, "device input must be 16-byte aligned");
@@ -167,11 +184,17 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     ctx->P = p;
     const uint32_t w = p.k - p.m;
     const uint32_t n_sub = ((1u << p.buckets_log) + 1) << p.second_buckets_log;
+    // physical units: about 4000 input bytes per unit, at most 2^23 units
+    uint32_t xb = 0;
+    while (xb < 8 && (n_bytes >> xb) / n_sub > 4000 && ((uint64_t)n_sub << (xb + 1)) <= (1u << 23)) xb++;
+    const uint32_t n_units = n_sub << xb;
     // slot: meta word + inline bases for the longest unique-minimizer record (2k - m)
     uint32_t sw = 1 + (2 * p.k - p.m + 15) / 16;
     uint32_t slot_words = 4;
     while (slot_words < sw) slot_words <<= 1;
     ctx->n_sub = n_sub;
+    ctx->xb = xb;
+    ctx->n_units = n_units;
     ctx->slot_words = slot_words;
     ctx->n_input = n_bytes;
     ctx->colored = n_colors > 0;
@@ -179,22 +202,23 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     const uint8_t* d_in = data;
     if (!input_on_device) {
         ENSURE(ctx->input, n_bytes + 16);
-        HIPCHK(hipMemcpyAsync(ctx->input.p, data, n_bytes, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpy(ctx->input.p, data, n_bytes, hipMemcpyHostToDevice));
         d_in = P_<uint8_t>(ctx->input);
     }
     const uint64_t n_tiles = (n_bytes + TILE_T - 1) / TILE_T;
     const uint64_t seg_words = n_tiles * TILE_T / 32 + 2;
     ENSURE(ctx->seg_bits, seg_words * 4);
-    ENSURE(ctx->sub_count, (size_t)n_sub * 4);
-    ENSURE(ctx->sub_cursor, (size_t)n_sub * 4);
-    ENSURE(ctx->sub_kmers, (size_t)n_sub * 4);
-    ENSURE(ctx->sub_off, ((size_t)n_sub + 1) * 8);
+    ENSURE(ctx->unit_count, (size_t)n_units * 4);
+    ENSURE(ctx->unit_cursor, (size_t)n_units * 4);
+    ENSURE(ctx->unit_kmers, (size_t)n_units * 4);
+    ENSURE(ctx->unit_off, ((size_t)n_units + 1) * 8);
     ENSURE(ctx->pool_cursor, 8);
+    ENSURE(ctx->error_flag, 8);
     if (n_colors) {
         ENSURE(ctx->color_starts, (size_t)n_colors * 8);
         ENSURE(ctx->colors, (size_t)n_colors * 4);
-        HIPCHK(hipMemcpyAsync(ctx->color_starts.p, color_starts, (size_t)n_colors * 8, hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(hipMemcpyAsync(ctx->colors.p, colors, (size_t)n_colors * 4, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpy(ctx->color_starts.p, color_starts, (size_t)n_colors * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->colors.p, colors, (size_t)n_colors * 4, hipMemcpyHostToDevice));
     }
 
     BucketArgs A{};
@@ -205,25 +229,25 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     A.w = w;
     A.b = p.buckets_log;
     A.sb = p.second_buckets_log;
+    A.xb = xb;
     A.canonical = p.forward_only ? 0 : 1;
     A.T = TILE_T;
     A.n_tiles = n_tiles;
     A.seg_bits = P_<uint32_t>(ctx->seg_bits);
-    A.sub_count = P_<uint32_t>(ctx->sub_count);
-    A.sub_cursor = P_<uint32_t>(ctx->sub_cursor);
-    A.sub_off = P_<uint64_t>(ctx->sub_off);
-    A.sub_kmers = P_<uint32_t>(ctx->sub_kmers);
+    A.sub_count = P_<uint32_t>(ctx->unit_count);
+    A.sub_cursor = P_<uint32_t>(ctx->unit_cursor);
+    A.sub_off = P_<uint64_t>(ctx->unit_off);
+    A.sub_kmers = P_<uint32_t>(ctx->unit_kmers);
     A.slot_words = slot_words;
     A.pool_cursor = P_<unsigned long long>(ctx->pool_cursor);
     A.color_starts = n_colors ? P_<uint64_t>(ctx->color_starts) : nullptr;
     A.colors = n_colors ? P_<uint32_t>(ctx->colors) : nullptr;
     A.n_colors = n_colors;
+    A.error_flag = P_<uint32_t>(ctx->error_flag);
     nthash_tables(p.m, A.gtab);
     const TileGeom G = tile_geom(TILE_T, w, p.k, p.m);
 
-    HIPCHK(hipMemsetAsync(ctx->sub_count.p, 0, (size_t)n_sub * 4, ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->sub_cursor.p, 0, (size_t)n_sub * 4, ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->sub_kmers.p, 0, (size_t)n_sub * 4, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->unit_count.p, 0, (size_t)n_units * 4, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->seg_bits.p, 0, seg_words * 4, ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
     if (n_tiles) {
@@ -231,11 +255,13 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
-    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, ctx->stream, A.sub_count, P_<uint64_t>(ctx->sub_off), n_sub);
-    HIPCHK(hipGetLastError());
+    {
+        int r = device_scan(ctx, A.sub_count, P_<uint64_t>(ctx->unit_off), n_units);
+        if (r != GGCAT_GPU_OK) return r;
+    }
     HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
     uint64_t total = 0;
-    HIPCHK(hipMemcpyAsync(&total, P_<uint64_t>(ctx->sub_off) + n_sub, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(&total, P_<uint64_t>(ctx->unit_off) + n_units, 8, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->n_records = total;
     ENSURE(ctx->slots, total * slot_words * 4);
@@ -248,9 +274,10 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
         ENSURE(ctx->pool, pool_cap * 4);
         A.pool = P_<uint32_t>(ctx->pool);
         A.pool_cap = pool_cap;
-        HIPCHK(hipMemsetAsync(ctx->sub_cursor.p, 0, (size_t)n_sub * 4, ctx->stream));
-        HIPCHK(hipMemsetAsync(ctx->sub_kmers.p, 0, (size_t)n_sub * 4, ctx->stream));
+        HIPCHK(hipMemsetAsync(ctx->unit_cursor.p, 0, (size_t)n_units * 4, ctx->stream));
+        HIPCHK(hipMemsetAsync(ctx->unit_kmers.p, 0, (size_t)n_units * 4, ctx->stream));
         HIPCHK(hipMemsetAsync(ctx->pool_cursor.p, 0, 8, ctx->stream));
+        HIPCHK(hipMemsetAsync(ctx->error_flag.p, 0, 8, ctx->stream));
         HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
         if (n_tiles) {
             hipLaunchKernelGGL(bucket_emit_kernel, dim3((uint32_t)n_tiles), dim3(64), G.bytes, ctx->stream, A);
@@ -258,8 +285,11 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
         }
         HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
         uint64_t used = 0;
+        uint32_t errf = 0;
         HIPCHK(hipMemcpyAsync(&used, ctx->pool_cursor.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(&errf, ctx->error_flag.p, 4, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (errf) return fail(ctx, GGCAT_GPU_EHIP, "bucketing passes disagree on unit sizes");
         ctx->pool_words = used;
         if (used <= pool_cap) break;
         if (attempt == 1) return fail(ctx, GGCAT_GPU_ENOMEM, "long-record pool overflow");
@@ -275,8 +305,8 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     HIPCHK(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[4]));
     ctx->ms[5] = t;
     // k-mer occurrence total (sizes the merge output)
-    std::vector<uint32_t> occ(n_sub);
-    HIPCHK(hipMemcpy(occ.data(), ctx->sub_kmers.p, (size_t)n_sub * 4, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> occ(n_units);
+    HIPCHK(hipMemcpy(occ.data(), ctx->unit_kmers.p, (size_t)n_units * 4, hipMemcpyDeviceToHost));
     uint64_t s = 0;
     for (uint32_t v : occ) s += v;
     ctx->kmer_occ = s;
@@ -300,19 +330,19 @@ int ggcat_gpu_records_copy(ggcat_gpu_ctx* ctx, uint32_t* sub, uint32_t* len, uin
     HIPCHK(hipSetDevice(ctx->device));
     const uint32_t SW = ctx->slot_words;
     std::vector<uint32_t> slots(ctx->n_records * SW), pool(ctx->pool_words), cols;
-    std::vector<uint64_t> soff(ctx->n_sub + 1);
-    HIPCHK(hipMemcpy(slots.data(), ctx->slots.p, slots.size() * 4, hipMemcpyDeviceToHost));
+    std::vector<uint64_t> uoff(ctx->n_units + 1);
+    if (!slots.empty()) HIPCHK(hipMemcpy(slots.data(), ctx->slots.p, slots.size() * 4, hipMemcpyDeviceToHost));
     if (!pool.empty()) HIPCHK(hipMemcpy(pool.data(), ctx->pool.p, pool.size() * 4, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(soff.data(), ctx->sub_off.p, soff.size() * 8, hipMemcpyDeviceToHost));
-    if (ctx->colored) {
+    HIPCHK(hipMemcpy(uoff.data(), ctx->unit_off.p, uoff.size() * 8, hipMemcpyDeviceToHost));
+    if (ctx->colored && ctx->n_records) {
         cols.resize(ctx->n_records);
         HIPCHK(hipMemcpy(cols.data(), ctx->slot_color.p, cols.size() * 4, hipMemcpyDeviceToHost));
     }
     static const char LET[4] = {'A', 'C', 'T', 'G'};
     uint64_t bpos = 0;
-    uint32_t s = 0;
+    uint32_t u = 0;
     for (uint64_t r = 0; r < ctx->n_records; r++) {
-        while (s + 1 < soff.size() && soff[s + 1] <= r) s++;
+        while (u + 1 < uoff.size() && uoff[u + 1] <= r) u++;
         const uint32_t* sl = slots.data() + r * SW;
         uint32_t meta = sl[0];
         uint32_t L;
@@ -324,7 +354,7 @@ int ggcat_gpu_records_copy(ggcat_gpu_ctx* ctx, uint32_t* sub, uint32_t* len, uin
             L = meta & 0xFF;
             bw = sl + 1;
         }
-        if (sub) sub[r] = s;
+        if (sub) sub[r] = u >> ctx->xb;
         if (len) len[r] = L;
         if (flags) flags[r] = (meta >> 8) & 3;
         if (minpos) minpos[r] = (meta >> 10) & 0xFF;
@@ -348,16 +378,15 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     M.slots = P_<uint32_t>(ctx->slots);
     M.slot_words = ctx->slot_words;
     M.pool = P_<uint32_t>(ctx->pool);
-    M.sub_off = P_<uint64_t>(ctx->sub_off);
-    M.sub_kmers = P_<uint32_t>(ctx->sub_kmers);
-    M.n_sub = ctx->n_sub;
+    M.unit_off = P_<uint64_t>(ctx->unit_off);
     M.k = p.k;
     M.min_mult = p.min_multiplicity;
     M.kind = p.hash_kind;
     M.key_words = key_words;
     M.lds_cap = key_words == 1 ? 4096 : 2048;
-    M.big_thresh = M.lds_cap / 2;
-    const size_t lds_bytes = (size_t)M.lds_cap * (8 * key_words + 4 + (key_words == 2 ? 4 : 0));
+    M.lds_try = M.lds_cap * 8;
+    const size_t lds_bytes = (size_t)M.lds_cap * (8 * key_words + 4 + (key_words == 2 ? 4 : 0)) + 64;
+    M.pool_words = ctx->pool_words;
     // rabin-karp128 constants (crates/hashes/src/cn_rkhash.rs:65-78)
     M.rk_mult = mk128(0x3eb9402f3e733993ULL, 0xadd64d3ca00e1b6bULL);
     M.rk_mult_inv = mk128(0x09cb6ff6f1b1a6d7ULL, 0x33e0952e899c3943ULL);
@@ -372,15 +401,37 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
         if (e & 1) r *= b;
     M.rk_rmmult = r;
 
+    // work items: runs of units of one sub-bucket
+    ENSURE(ctx->items, (size_t)ctx->n_units * sizeof(WorkItem));
+    ENSURE(ctx->n_items, 8);
     ENSURE(ctx->work, 8);
     ENSURE(ctx->big_count, 8);
-    ENSURE(ctx->big_list, (size_t)ctx->n_sub * 4);
+    ENSURE(ctx->big_list, (size_t)ctx->n_units * 4);
     ENSURE(ctx->out_cursor, 8);
+    ENSURE(ctx->fix_cursor, 8);
+    ENSURE(ctx->error_flag, 8);
+    M.error_flag = P_<uint32_t>(ctx->error_flag);
+    HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->n_items.p, 0, 8, ctx->stream));
+    hipLaunchKernelGGL(make_items_kernel, dim3((ctx->n_sub + 255) / 256), dim3(256), 0, ctx->stream,
+                       P_<uint32_t>(ctx->unit_kmers), ctx->n_sub, ctx->xb, M.lds_cap / 2,
+                       P_<WorkItem>(ctx->items), P_<unsigned int>(ctx->n_items));
+    HIPCHK(hipGetLastError());
+    uint32_t n_items = 0;
+    HIPCHK(hipMemcpyAsync(&n_items, ctx->n_items.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    M.items = P_<WorkItem>(ctx->items);
+    M.n_items = n_items;
+
     uint64_t out_cap = std::max<uint64_t>(ctx->kmer_occ / 6 + (1u << 20), ctx->out_val.cap / 4);
+    uint64_t fix_cap = std::max<uint64_t>(ctx->n_records / 64 + 4096, ctx->fix_val.cap / 4);
     for (int attempt = 0; attempt < 2; attempt++) {
         ENSURE(ctx->out_key, out_cap * 8 * key_words);
         ENSURE(ctx->out_val, out_cap * 4);
         ENSURE(ctx->out_sub, out_cap * 4);
+        ENSURE(ctx->fix_key, fix_cap * 8 * key_words);
+        ENSURE(ctx->fix_val, fix_cap * 4);
+        ENSURE(ctx->fix_sub, fix_cap * 4);
         M.work = P_<unsigned int>(ctx->work);
         M.big_list = P_<uint32_t>(ctx->big_list);
         M.big_count = P_<unsigned int>(ctx->big_count);
@@ -389,30 +440,38 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
         M.out_sub = P_<uint32_t>(ctx->out_sub);
         M.out_cursor = P_<unsigned long long>(ctx->out_cursor);
         M.out_cap = out_cap;
+        M.fix_key = P_<uint64_t>(ctx->fix_key);
+        M.fix_val = P_<uint32_t>(ctx->fix_val);
+        M.fix_sub = P_<uint32_t>(ctx->fix_sub);
+        M.fix_cursor = P_<unsigned long long>(ctx->fix_cursor);
+        M.fix_cap = fix_cap;
         HIPCHK(hipMemsetAsync(ctx->work.p, 0, 8, ctx->stream));
         HIPCHK(hipMemsetAsync(ctx->big_count.p, 0, 8, ctx->stream));
         HIPCHK(hipMemsetAsync(ctx->out_cursor.p, 0, 8, ctx->stream));
-        HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
-        int n_cu = 256;
-        hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) n_cu = prop.multiProcessorCount;
-        hipLaunchKernelGGL(merge_lds_kernel, dim3(n_cu * 2), dim3(256), lds_bytes, ctx->stream, M);
-        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemsetAsync(ctx->fix_cursor.p, 0, 8, ctx->stream));
+        HIPCHK(hipMemsetAsync(ctx->error_flag.p, 0, 8, ctx->stream));
         HIPCHK(hipEventRecord(ctx->ev[6], ctx->stream));
+        if (n_items) {
+            hipLaunchKernelGGL(merge_lds_kernel, dim3(ctx->n_cu * 2), dim3(256), lds_bytes, ctx->stream, M);
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipEventRecord(ctx->ev[7], ctx->stream));
         uint32_t n_big = 0;
         HIPCHK(hipMemcpyAsync(&n_big, ctx->big_count.p, 4, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
         ctx->n_big = n_big;
+        HIPCHK(hipEventRecord(ctx->ev[8], ctx->stream));
         if (n_big) {
-            std::vector<uint32_t> list(n_big), occ(ctx->n_sub);
-            HIPCHK(hipMemcpy(list.data(), ctx->big_list.p, n_big * 4, hipMemcpyDeviceToHost));
-            HIPCHK(hipMemcpy(occ.data(), ctx->sub_kmers.p, (size_t)ctx->n_sub * 4, hipMemcpyDeviceToHost));
+            std::vector<uint32_t> list(n_big);
+            std::vector<WorkItem> its(n_items);
+            HIPCHK(hipMemcpy(list.data(), ctx->big_list.p, (size_t)n_big * 4, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(its.data(), ctx->items.p, (size_t)n_items * sizeof(WorkItem), hipMemcpyDeviceToHost));
             std::vector<uint64_t> goff(n_big);
             std::vector<uint32_t> gcap(n_big);
             uint64_t tot = 0;
             for (uint32_t i = 0; i < n_big; i++) {
                 uint64_t c = 1024;
-                while (c < 2ull * occ[list[i]]) c <<= 1;
+                while (c < 2ull * its[list[i]].occ) c <<= 1;
                 goff[i] = tot;
                 gcap[i] = (uint32_t)c;
                 tot += c;
@@ -425,8 +484,8 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
                 ENSURE(ctx->g_khi, tot * 8);
                 ENSURE(ctx->g_st, tot * 4);
             }
-            HIPCHK(hipMemcpyAsync(ctx->g_off.p, goff.data(), n_big * 8, hipMemcpyHostToDevice, ctx->stream));
-            HIPCHK(hipMemcpyAsync(ctx->g_cap.p, gcap.data(), n_big * 4, hipMemcpyHostToDevice, ctx->stream));
+            HIPCHK(hipMemcpy(ctx->g_off.p, goff.data(), (size_t)n_big * 8, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(ctx->g_cap.p, gcap.data(), (size_t)n_big * 4, hipMemcpyHostToDevice));
             hipLaunchKernelGGL(fill_u64_kernel, dim3(1024), dim3(256), 0, ctx->stream, P_<uint64_t>(ctx->g_klo), EMPTY64, tot);
             hipLaunchKernelGGL(fill_u32_kernel, dim3(1024), dim3(256), 0, ctx->stream, P_<uint32_t>(ctx->g_val), 0u, tot);
             if (key_words == 2)
@@ -437,33 +496,76 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
             M.g_val = P_<uint32_t>(ctx->g_val);
             M.g_off = P_<uint64_t>(ctx->g_off);
             M.g_cap = P_<uint32_t>(ctx->g_cap);
-            HIPCHK(hipEventRecord(ctx->ev[7], ctx->stream));
             hipLaunchKernelGGL(merge_global_kernel, dim3(std::min<uint32_t>(n_big, 4096)), dim3(256), 0, ctx->stream, M, n_big);
             HIPCHK(hipGetLastError());
-            HIPCHK(hipEventRecord(ctx->ev[8], ctx->stream));
         }
-        uint64_t n_out = 0;
+        HIPCHK(hipEventRecord(ctx->ev[9], ctx->stream));
+        uint64_t n_out = 0, n_fix = 0;
+        uint32_t errf = 0;
         HIPCHK(hipMemcpyAsync(&n_out, ctx->out_cursor.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(&n_fix, ctx->fix_cursor.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(&errf, ctx->error_flag.p, 4, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (errf & 1) return fail(ctx, GGCAT_GPU_EHIP, "k-mer table overflow");
+        if (errf & 2) return fail(ctx, GGCAT_GPU_EHIP, "malformed record slot");
         float t;
-        HIPCHK(hipEventElapsedTime(&t, ctx->ev[5], ctx->ev[6]));
+        HIPCHK(hipEventElapsedTime(&t, ctx->ev[6], ctx->ev[7]));
         ctx->ms[3] = t;
-        ctx->ms[4] = 0;
-        if (n_big) {
-            HIPCHK(hipEventElapsedTime(&t, ctx->ev[7], ctx->ev[8]));
-            ctx->ms[4] = t;
-            HIPCHK(hipEventElapsedTime(&t, ctx->ev[5], ctx->ev[8]));
-            ctx->ms[6] = t;
-        } else {
-            ctx->ms[6] = ctx->ms[3];
+        HIPCHK(hipEventElapsedTime(&t, ctx->ev[8], ctx->ev[9]));
+        ctx->ms[4] = n_big ? t : 0.0;
+        if (n_out > out_cap || n_fix > fix_cap) {
+            if (attempt == 1) return fail(ctx, GGCAT_GPU_ENOMEM, "solid output overflow");
+            out_cap = std::max(out_cap, n_out + 1024);
+            fix_cap = std::max(fix_cap, n_fix + 1024);
+            continue;
         }
+        // combine the two halves of cross-unit boundary k-mers
+        if (n_fix) {
+            const uint32_t kw = key_words;
+            std::vector<uint64_t> fk(n_fix * kw);
+            std::vector<uint32_t> fv(n_fix), fs(n_fix);
+            HIPCHK(hipMemcpy(fk.data(), ctx->fix_key.p, fk.size() * 8, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(fv.data(), ctx->fix_val.p, n_fix * 4, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(fs.data(), ctx->fix_sub.p, n_fix * 4, hipMemcpyDeviceToHost));
+            std::vector<uint32_t> ord(n_fix);
+            for (uint32_t i = 0; i < n_fix; i++) ord[i] = i;
+            auto key_of = [&](uint32_t i) {
+                return std::make_tuple(fs[i], kw == 2 ? fk[2 * i + 1] : 0ull, fk[kw * i]);
+            };
+            std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b2) { return key_of(a) < key_of(b2); });
+            std::vector<uint64_t> ok_;
+            std::vector<uint32_t> ov, os;
+            for (uint64_t i = 0; i < n_fix;) {
+                uint64_t j = i + 1;
+                uint32_t v = fv[ord[i]];
+                while (j < n_fix && key_of(ord[j]) == key_of(ord[i])) v |= fv[ord[j++]] & 0xC0000000u;
+                uint32_t a = ord[i];
+                for (uint32_t q = 0; q < kw; q++) ok_.push_back(fk[kw * a + q]);
+                ov.push_back(v);
+                os.push_back(fs[a]);
+                i = j;
+            }
+            const uint64_t n_add = ov.size();
+            if (n_out + n_add > out_cap) {
+                if (attempt == 1) return fail(ctx, GGCAT_GPU_ENOMEM, "solid output overflow");
+                out_cap = n_out + n_add + 1024;
+                continue;
+            }
+            HIPCHK(hipMemcpy(P_<uint64_t>(ctx->out_key) + n_out * kw, ok_.data(), n_add * kw * 8, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(P_<uint32_t>(ctx->out_val) + n_out, ov.data(), n_add * 4, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(P_<uint32_t>(ctx->out_sub) + n_out, os.data(), n_add * 4, hipMemcpyHostToDevice));
+            n_out += n_add;
+        }
+        ctx->n_fix = n_fix;
         ctx->n_solid = n_out;
-        if (n_out <= out_cap) break;
-        if (attempt == 1) return fail(ctx, GGCAT_GPU_ENOMEM, "solid output overflow");
-        out_cap = n_out + 1024;
+        HIPCHK(hipEventRecord(ctx->ev[10], ctx->stream));
+        HIPCHK(hipEventSynchronize(ctx->ev[10]));
+        HIPCHK(hipEventElapsedTime(&t, ctx->ev[5], ctx->ev[10]));
+        ctx->ms[6] = t;
+        ctx->have_solid = true;
+        return GGCAT_GPU_OK;
     }
-    ctx->have_solid = true;
-    return GGCAT_GPU_OK;
+    return fail(ctx, GGCAT_GPU_ENOMEM, "solid output overflow");
 }
 
 int ggcat_gpu_solid_info(ggcat_gpu_ctx* ctx, uint64_t* n_solid, uint32_t* key_words) {
@@ -501,7 +603,7 @@ int ggcat_gpu_timings(ggcat_gpu_ctx* ctx, double* ms, uint32_t n) {
 int ggcat_gpu_counters(ggcat_gpu_ctx* ctx, uint64_t* out, uint32_t n) {
     if (!ctx || !out) return GGCAT_GPU_EINVAL;
     uint64_t v[7] = {ctx->n_input, ctx->n_records, ctx->kmer_occ + ctx->n_records * (ctx->P.k - 1), ctx->kmer_occ,
-                     ctx->n_big, ctx->n_solid, 0};
+                     ctx->n_big, ctx->n_solid, ctx->n_fix};
     for (uint32_t i = 0; i < n && i < 7; i++) out[i] = v[i];
     return GGCAT_GPU_OK;
 }

@@ -31,29 +31,34 @@
 
 namespace gg {
 
+// Records are scattered by physical unit: the sub-bucket (the semantic unit,
+// one kmers_transform job) refined by `xb` further bits of the minimizer hash
+// for unique minimizers.  Units of one sub-bucket are contiguous, so the merge
+// can process any run of them as one work item.
 struct BucketArgs {
     const uint8_t* seq;
     uint64_t n;  // bytes in seq
     uint32_t k, m, w;
-    uint32_t b, sb;
+    uint32_t b, sb, xb;
     uint32_t canonical;
     uint32_t T;  // windows per tile (multiple of 64)
     uint64_t n_tiles;
     uint32_t* seg_bits;  // [ceil(n/32)+2] group-start bitmap (pass A writes)
-    uint32_t* sub_count;  // [n_sub] records per sub-bucket (pass A)
+    uint32_t* sub_count;  // [n_units] records per unit (pass A)
     // pass B
-    uint32_t* sub_cursor;  // [n_sub]
-    const uint64_t* sub_off;  // [n_sub+1] slot offsets
+    uint32_t* sub_cursor;  // [n_units]
+    const uint64_t* sub_off;  // [n_units+1] slot offsets
     uint32_t* slots;
     uint32_t slot_words;
     uint32_t* pool;
     unsigned long long* pool_cursor;
     uint64_t pool_cap;  // words
-    uint32_t* sub_kmers;  // [n_sub] k-mer occurrences
+    uint32_t* sub_kmers;  // [n_units] k-mer occurrences
     const uint64_t* color_starts;
     const uint32_t* colors;
     uint32_t n_colors;
     uint32_t* slot_color;
+    uint32_t* error_flag;  // set when a consistency check fails
     uint64_t gtab[40];  // interleaved (fw, rc) rolling tables, index out*4+in
 };
 
@@ -265,10 +270,21 @@ GG_DEV bool tile_is_end(const Tile& t, uint32_t x) {
     return t.bit(t.last, x) || t.bit(t.seg, x + 1);
 }
 
+// Sub-bucket of a window minimum: bucket = (h >> 1) mod 2^b for a unique
+// minimizer, the duplicates bucket 2^b otherwise; second bucket = next sb bits
+// (crates/assembler_minimizer_bucketing/src/lib.rs:218-255,
+//  crates/hashes/src/cn_nthash.rs:135-142).
 GG_DEV uint32_t sub_of(uint64_t v, uint32_t b, uint32_t sb) {
     uint32_t bucket = (v & 1) ? (uint32_t)((v >> 1) & ((1ULL << b) - 1)) : (1u << b);
     uint32_t second = (uint32_t)((v >> (b + 1)) & ((1ULL << sb) - 1));
     return (bucket << sb) | second;
+}
+
+// Physical unit: the sub-bucket refined by xb more minimizer bits (unique
+// minimizers only; duplicates-bucket records keep one unit per sub-bucket).
+GG_DEV uint32_t unit_of(uint64_t v, uint32_t b, uint32_t sb, uint32_t xb) {
+    uint32_t extra = (v & 1) ? (uint32_t)((v >> (b + 1 + sb)) & ((1ULL << xb) - 1)) : 0u;
+    return (sub_of(v, b, sb) << xb) | extra;
 }
 
 GG_DEV void tile_init(Tile& t, const BucketArgs* A, uint64_t tile, uint8_t* lds) {
@@ -310,8 +326,8 @@ __global__ __launch_bounds__(64) void bucket_count_kernel(BucketArgs A) {
             A.seg_bits[wi + 1] = (uint32_t)(bs >> 32);
         }
         if (j < (int64_t)A.n && tile_is_end(t, x)) {
-            uint32_t sub = sub_of(t.wv[x], A.b, A.sb);
-            atomicAdd(&A.sub_count[sub], 1u);
+            uint32_t unit = unit_of(t.wv[x], A.b, A.sb, A.xb);
+            atomicAdd(&A.sub_count[unit], 1u);
         }
     }
 }
@@ -399,12 +415,32 @@ __global__ __launch_bounds__(64) void bucket_emit_kernel(BucketArgs A) {
         if (unique) minpos = rc ? (uint32_t)(end - mmabs - m) : (uint32_t)(mmabs - start);
         const uint32_t flags = ((include_first ? 1u : 0u) << (rc ? 1 : 0)) | ((include_last ? 1u : 0u) << (rc ? 0 : 1));
         const uint32_t sub = sub_of(v, A.b, A.sb);
-        const uint64_t slot = A.sub_off[sub] + atomicAdd(&A.sub_cursor[sub], 1u);
-        atomicAdd(&A.sub_kmers[sub], len - k + 1);
+        const uint32_t unit = unit_of(v, A.b, A.sb, A.xb);
+        // Boundary k-mers shared with a neighbouring group of the same
+        // sub-bucket that was scattered to another unit: the merge must combine
+        // the two halves (cross marks, in stored orientation).
+        uint32_t cross = 0;
+        if (unique && A.xb) {
+            if (!include_first) {
+                uint64_t vl = t.wv[a - t.wlo - 1];
+                if (sub_of(vl, A.b, A.sb) == sub && unit_of(vl, A.b, A.sb, A.xb) != unit) cross |= rc ? META_XEND : META_XBEGIN;
+            }
+            if (!include_last) {
+                uint64_t vr = t.wv[x + 1];
+                if (sub_of(vr, A.b, A.sb) == sub && unit_of(vr, A.b, A.sb, A.xb) != unit) cross |= rc ? META_XBEGIN : META_XEND;
+            }
+        }
+        const uint64_t slot = A.sub_off[unit] + atomicAdd(&A.sub_cursor[unit], 1u);
+        if (slot >= A.sub_off[unit + 1]) {
+            // pass A and pass B disagree on this unit: never write past it
+            atomicOr(A.error_flag, 1u);
+            continue;
+        }
+        atomicAdd(&A.sub_kmers[unit], len - k + 1);
         uint32_t* out = A.slots + slot * SW;
         const int64_t rs = start - t.borg;
         if (len <= inline_cap && rs >= 0) {
-            out[0] = len | (flags << 8) | (minpos << 10) | (rc ? META_RC : 0u);
+            out[0] = len | (flags << 8) | (minpos << 10) | (rc ? META_RC : 0u) | cross;
             const uint32_t nw = (len + 15) / 16;
             for (uint32_t q = 1; q < SW; q++) {
                 uint32_t wq = 0;
@@ -435,7 +471,7 @@ __global__ __launch_bounds__(64) void bucket_emit_kernel(BucketArgs A) {
                 if (off + words > A.pool_cap) continue;  // overflow: host retries with a larger pool
                 dst = A.pool + off;
             } else {
-                out[0] = len | (flags << 8) | (minpos << 10) | (rc ? META_RC : 0u);
+                out[0] = len | (flags << 8) | (minpos << 10) | (rc ? META_RC : 0u) | cross;
                 dst = out + 1;
                 for (uint32_t q = 1; q < SW; q++) out[q] = 0;
             }
@@ -458,6 +494,83 @@ __global__ __launch_bounds__(64) void bucket_emit_kernel(BucketArgs A) {
             A.slot_color[slot] = A.colors[lo];
         }
     }
+}
+
+// Device-wide exclusive scan of uint32 counts into uint64 offsets (n+1
+// entries): per-chunk sums, one-block scan of the sums, per-chunk scan.
+constexpr uint32_t SCAN_CHUNK = 4096;  // 1024 threads x 4 elements
+
+GG_DEV uint64_t block_exclusive_scan(uint64_t v, uint64_t* tmp, uint64_t& total) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    uint64_t x = v;
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        uint64_t y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) tmp[wave] = x;
+    __syncthreads();
+    if (wave == 0) {
+        uint64_t s = lane < nw ? tmp[lane] : 0;
+        for (uint32_t d = 1; d < 64; d <<= 1) {
+            uint64_t y = __shfl_up(s, d);
+            if (lane >= d) s += y;
+        }
+        if (lane < nw) tmp[lane] = s;
+    }
+    __syncthreads();
+    uint64_t base = wave ? tmp[wave - 1] : 0;
+    total = tmp[nw - 1];
+    __syncthreads();
+    return base + x - v;
+}
+
+__global__ __launch_bounds__(1024) void scan_reduce_kernel(const uint32_t* counts, uint64_t* partial, uint32_t n) {
+    __shared__ uint64_t tmp[16];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_CHUNK;
+    uint64_t s = 0;
+    for (uint32_t q = 0; q < 4; q++) {
+        uint64_t i = base + (uint64_t)q * 1024 + threadIdx.x;
+        if (i < n) s += counts[i];
+    }
+    uint64_t total;
+    block_exclusive_scan(s, tmp, total);
+    if (threadIdx.x == 0) partial[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void scan_partials_kernel(uint64_t* partial, uint32_t nb) {
+    __shared__ uint64_t tmp[16];
+    uint64_t carry = 0;
+    for (uint32_t c = 0; c < nb; c += 1024) {
+        uint32_t i = c + threadIdx.x;
+        uint64_t v = i < nb ? partial[i] : 0;
+        uint64_t total;
+        uint64_t ex = block_exclusive_scan(v, tmp, total);
+        if (i < nb) partial[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) partial[nb] = carry;
+}
+
+__global__ __launch_bounds__(1024) void scan_apply_kernel(const uint32_t* counts, const uint64_t* partial,
+                                                          uint64_t* off, uint32_t n, uint32_t nb) {
+    __shared__ uint64_t tmp[16];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_CHUNK;
+    // thread t owns elements base + 4t .. base + 4t + 3
+    uint32_t c[4];
+    uint64_t s = 0;
+    for (uint32_t q = 0; q < 4; q++) {
+        uint64_t i = base + 4ull * threadIdx.x + q;
+        c[q] = i < n ? counts[i] : 0;
+        s += c[q];
+    }
+    uint64_t total;
+    uint64_t run = partial[blockIdx.x] + block_exclusive_scan(s, tmp, total);
+    for (uint32_t q = 0; q < 4; q++) {
+        uint64_t i = base + 4ull * threadIdx.x + q;
+        if (i < n) off[i] = run;
+        run += c[q];
+    }
+    if (blockIdx.x == nb - 1 && threadIdx.x == 0) off[n] = partial[nb];
 }
 
 // Exclusive prefix sum of counts -> offsets (n+1 entries), one workgroup.

@@ -15,8 +15,13 @@ typedef unsigned __int128 u128_t;
 //   bit  18      long record (bases live in the pool: word1 = length,
 //                words 2..3 = 64-bit pool word offset)
 //   bit  19      record was reverse-complemented when stored
+//   bit  20      first k-mer (stored orientation) is shared with a group of
+//                the same sub-bucket that lives in another physical unit
+//   bit  21      same for the last k-mer
 constexpr uint32_t META_LONG = 1u << 18;
 constexpr uint32_t META_RC = 1u << 19;
+constexpr uint32_t META_XBEGIN = 1u << 20;
+constexpr uint32_t META_XEND = 1u << 21;
 
 GG_DEV uint32_t meta_len(uint32_t m) { return m & 0xFF; }
 GG_DEV uint32_t meta_flags(uint32_t m) { return (m >> 8) & 3; }

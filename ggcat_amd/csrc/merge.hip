@@ -12,42 +12,64 @@
 // 2-bit seq-hash, u64 for k <= 32, u128 for k <= 64) or
 // crates/hashes/src/base/cn_rkhash_base.rs:65-110 (rabin-karp128).
 //
-// Table entry value: bits 0..29 counter, bits 30..31 the two MapEntry flags,
-// so one u32 atomicAdd + one atomicOr update an entry.
+// Work items are runs of physical units of one sub-bucket.  A k-mer shared by
+// two groups of the same sub-bucket stored in different work items is seen
+// half in each: both halves carry the same counter (every occurrence of such
+// a k-mer is a group boundary, so it appears once on each side) and one flag
+// bit each.  Those entries carry a cross mark and are combined afterwards
+// (flags ORed), which yields exactly the sub-bucket's MapEntry.
+//
+// Table entry value: bits 0..28 counter, bit 29 cross mark, bits 30..31 the
+// two MapEntry flags; one u32 atomicAdd + atomicOr update an entry.
 #include "common.h"
 
 namespace gg {
 
 constexpr uint64_t EMPTY64 = ~0ULL;
+constexpr uint32_t CNT_MASK = 0x1FFFFFFFu;
+constexpr uint32_t VAL_CROSS = 1u << 29;
+
+struct WorkItem {
+    uint32_t u0, u1;  // physical units [u0, u1)
+    uint32_t occ;  // k-mer occurrences
+    uint32_t sub;  // sub-bucket
+};
 
 struct MergeArgs {
     const uint32_t* slots;
     uint32_t slot_words;
     const uint32_t* pool;
-    const uint64_t* sub_off;
-    const uint32_t* sub_kmers;
-    uint32_t n_sub;
+    const uint64_t* unit_off;  // [n_units+1] slot offsets
     uint32_t k;
     uint32_t min_mult;
     uint32_t kind;  // 0 = canonical seq-hash, 1 = canonical rabin-karp128
     uint32_t key_words;  // 1 (u64 keys) or 2 (u128 keys)
-    uint32_t lds_cap;  // max table entries in LDS (power of two)
-    uint32_t big_thresh;  // occurrences above which the global path is used
+    uint32_t lds_cap;  // table entries in LDS (power of two)
+    uint32_t lds_try;  // items with more occurrences go straight to HBM tables
+    const WorkItem* items;
+    uint32_t n_items;
     unsigned int* work;
-    uint32_t* big_list;
+    uint32_t* big_list;  // indices into items
     unsigned int* big_count;
     uint64_t* out_key;  // key_words per entry
     uint32_t* out_val;  // multiplicity | flags << 30
     uint32_t* out_sub;
     unsigned long long* out_cursor;
     uint64_t out_cap;
-    // global-memory tables for oversize sub-buckets
+    uint64_t* fix_key;  // cross-marked entries awaiting their other half
+    uint32_t* fix_val;
+    uint32_t* fix_sub;
+    unsigned long long* fix_cursor;
+    uint64_t fix_cap;
+    // HBM tables for big items
     uint64_t* g_klo;
     uint64_t* g_khi;
     uint32_t* g_st;
     uint32_t* g_val;
     const uint64_t* g_off;  // per big-list entry: table offset
     const uint32_t* g_cap;  // per big-list entry: table capacity (power of two)
+    uint64_t pool_words;  // valid words in the long-record pool
+    uint32_t* error_flag;  // bit 0: table full, bit 1: malformed slot
     // rabin-karp128 constants (crates/hashes/src/cn_rkhash.rs:65-78)
     u128_t rk_mult, rk_mult_inv, rk_rmmult;
     u128_t rk_fwd[4], rk_bkw[4];
@@ -55,30 +77,41 @@ struct MergeArgs {
 
 GG_DEV uint32_t hash128(uint64_t lo, uint64_t hi) { return tab_hash64(lo ^ (hi * 0x9E3779B97F4A7C15ULL)); }
 
+// Insert one occurrence.  Returns 1 when this call claimed a new slot, 0 when
+// the key was present, 2 when no slot was found within mask+1 probes.
 template <class KP, class VP>
-GG_DEV void tab_insert64(KP keys, VP vals, uint32_t mask, uint64_t key, uint32_t add, uint32_t fl) {
+GG_DEV uint32_t tab_insert64(KP keys, VP vals, uint32_t mask, uint64_t key, uint32_t bits) {
     uint32_t i = tab_hash64(key) & mask;
-    while (true) {
+    uint32_t claimed = 0;
+    for (uint32_t probes = 0;; probes++) {
+        if (probes > mask) return 2;
         uint64_t cur = keys[i];
         if (cur == key) break;
         if (cur == EMPTY64) {
             uint64_t old = atomicCAS((unsigned long long*)&keys[i], (unsigned long long)EMPTY64,
                                      (unsigned long long)key);
-            if (old == EMPTY64 || old == key) break;
+            if (old == EMPTY64) {
+                claimed = 1;
+                break;
+            }
+            if (old == key) break;
         }
         i = (i + 1) & mask;
     }
-    atomicAdd((uint32_t*)&vals[i], add);
-    if (fl) atomicOr((uint32_t*)&vals[i], fl << 30);
+    atomicAdd((uint32_t*)&vals[i], bits & CNT_MASK);
+    if (bits & ~CNT_MASK) atomicOr((uint32_t*)&vals[i], bits & ~CNT_MASK);
+    return claimed;
 }
 
 // 128-bit keys: a slot is claimed through its state word (0 free, 1 being
 // written, 2 ready).  The loop body runs the writer and the waiters in the
-// same iteration, so lanes of one wavefront cannot wait on each other.
+// same iteration, so lanes of one wavefront cannot wait on each other; the
+// wait on a slot in state 1 is bounded.
 template <class KP, class SP, class VP>
-GG_DEV void tab_insert128(KP klo, KP khi, SP st, VP vals, uint32_t mask, uint64_t lo, uint64_t hi, uint32_t add,
-                          uint32_t fl) {
+GG_DEV uint32_t tab_insert128(KP klo, KP khi, SP st, VP vals, uint32_t mask, uint64_t lo, uint64_t hi,
+                              uint32_t bits) {
     uint32_t i = hash128(lo, hi) & mask;
+    uint32_t claimed = 0, probes = 0, spins = 0;
     bool done = false;
     while (!done) {
         uint32_t s = __hip_atomic_load((uint32_t*)&st[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -88,17 +121,25 @@ GG_DEV void tab_insert128(KP klo, KP khi, SP st, VP vals, uint32_t mask, uint64_
                 khi[i] = hi;
                 __threadfence();
                 atomicExch((uint32_t*)&st[i], 2u);
+                claimed = 1;
                 done = true;
             }
         } else if (s == 2) {
             uint64_t a = __hip_atomic_load((uint64_t*)&klo[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             uint64_t b = __hip_atomic_load((uint64_t*)&khi[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (a == lo && b == hi) done = true;
-            else i = (i + 1) & mask;
+            if (a == lo && b == hi) {
+                done = true;
+            } else {
+                i = (i + 1) & mask;
+                if (++probes > mask) return 2;
+            }
+        } else if (++spins > (1u << 20)) {
+            return 2;
         }
     }
-    atomicAdd((uint32_t*)&vals[i], add);
-    if (fl) atomicOr((uint32_t*)&vals[i], fl << 30);
+    atomicAdd((uint32_t*)&vals[i], bits & CNT_MASK);
+    if (bits & ~CNT_MASK) atomicOr((uint32_t*)&vals[i], bits & ~CNT_MASK);
+    return claimed;
 }
 
 // Iterate the k-mers of one slot; cb(lo, hi, is_forward, idx, last_idx).
@@ -110,9 +151,17 @@ GG_DEV void for_each_kmer(const MergeArgs& A, const uint32_t* slot, CB&& cb) {
     if (meta & META_LONG) {
         len = slot[1];
         uint64_t off = (uint64_t)slot[2] | ((uint64_t)slot[3] << 32);
+        if (off + ((uint64_t)len + 15) / 16 > A.pool_words) {
+            atomicOr(A.error_flag, 2u);
+            return;
+        }
         bw = A.pool + off;
     } else {
         len = meta_len(meta);
+        if (len > (A.slot_words - 1) * 16) {
+            atomicOr(A.error_flag, 2u);
+            return;
+        }
         bw = slot + 1;
     }
     const uint32_t k = A.k;
@@ -162,37 +211,49 @@ GG_DEV void for_each_kmer(const MergeArgs& A, const uint32_t* slot, CB&& cb) {
     }
 }
 
-GG_DEV uint32_t occurrence_flags(uint32_t rflags, bool is_fwd, uint32_t idx, uint32_t last) {
+// Value bits contributed by one occurrence (count 1, flags, cross mark).
+GG_DEV uint32_t occurrence_bits(uint32_t meta, bool is_fwd, uint32_t idx, uint32_t last) {
+    const uint32_t rflags = meta_flags(meta);
     bool begin_ignored = (rflags & 1) == 0 && idx == 0;
     bool end_ignored = (rflags & 2) == 0 && idx == last;
-    return ((begin_ignored ? 1u : 0u) << (is_fwd ? 0 : 1)) | ((end_ignored ? 1u : 0u) << (is_fwd ? 1 : 0));
+    uint32_t fl = ((begin_ignored ? 1u : 0u) << (is_fwd ? 0 : 1)) | ((end_ignored ? 1u : 0u) << (is_fwd ? 1 : 0));
+    bool cross = (idx == 0 && (meta & META_XBEGIN)) || (idx == last && (meta & META_XEND));
+    return 1u | (cross ? VAL_CROSS : 0u) | (fl << 30);
 }
 
-// Emit solid entries of table positions [i0, i1) (stride 256 per thread).
+// Emit solid entries of a table of `cap` slots; cross-marked entries holding
+// only one flag bit go to the fixup list.
 template <class KP, class SP, class VP>
 GG_DEV void emit_solid(const MergeArgs& A, uint32_t sub, KP klo, KP khi, SP st, VP vals, uint32_t cap) {
     const uint32_t lane = threadIdx.x & 63;
     for (uint32_t i0 = 0; i0 < cap; i0 += blockDim.x) {
         uint32_t i = i0 + threadIdx.x;
-        bool solid = false;
+        bool solid = false, fix = false;
         uint32_t outv = 0;
         if (i < cap) {
             bool occupied = (A.key_words == 1) ? (klo[i] != EMPTY64) : (st[i] == 2);
             if (occupied) {
                 uint32_t v = vals[i];
-                uint32_t cnt = v & 0x3FFFFFFFu, fl = v >> 30;
+                uint32_t cnt = v & CNT_MASK, fl = v >> 30;
                 uint32_t mult = cnt >> (fl == 3 ? 1 : 0);
                 solid = mult >= A.min_mult;
+                fix = solid && (v & VAL_CROSS) && (fl == 1 || fl == 2);
                 outv = mult | (fl << 30);
             }
         }
-        uint64_t bal = __ballot(solid);
-        if (bal == 0) continue;
-        unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(A.out_cursor, (unsigned long long)__popcll(bal));
+        const bool main = solid && !fix;
+        uint64_t bal = __ballot(main), balf = __ballot(fix);
+        if ((bal | balf) == 0) continue;
+        unsigned long long base = 0, basef = 0;
+        if (lane == 0) {
+            if (bal) base = atomicAdd(A.out_cursor, (unsigned long long)__popcll(bal));
+            if (balf) basef = atomicAdd(A.fix_cursor, (unsigned long long)__popcll(balf));
+        }
         base = __shfl(base, 0);
-        if (solid) {
-            uint64_t pos = base + __popcll(bal & ((1ULL << lane) - 1));
+        basef = __shfl(basef, 0);
+        const uint64_t below = (1ULL << lane) - 1;
+        if (main) {
+            uint64_t pos = base + __popcll(bal & below);
             if (pos < A.out_cap) {
                 if (A.key_words == 1) {
                     A.out_key[pos] = klo[i];
@@ -204,46 +265,71 @@ GG_DEV void emit_solid(const MergeArgs& A, uint32_t sub, KP klo, KP khi, SP st, 
                 A.out_sub[pos] = sub;
             }
         }
+        if (fix) {
+            uint64_t pos = basef + __popcll(balf & below);
+            if (pos < A.fix_cap) {
+                if (A.key_words == 1) {
+                    A.fix_key[pos] = klo[i];
+                } else {
+                    A.fix_key[2 * pos] = klo[i];
+                    A.fix_key[2 * pos + 1] = khi[i];
+                }
+                A.fix_val[pos] = outv;
+                A.fix_sub[pos] = sub;
+            }
+        }
     }
 }
 
+// Inserts every occurrence of the item's records.  Stops inserting new keys
+// once `limit` keys were claimed (sets *overflow); returns nothing.
 template <class KP, class SP, class VP>
-GG_DEV void insert_records(const MergeArgs& A, uint32_t sub, KP klo, KP khi, SP st, VP vals, uint32_t mask) {
-    const uint64_t r0 = A.sub_off[sub], r1 = A.sub_off[sub + 1];
+GG_DEV void insert_item(const MergeArgs& A, const WorkItem& it, KP klo, KP khi, SP st, VP vals, uint32_t mask,
+                        uint32_t* claims, uint32_t limit, uint32_t* overflow) {
+    const uint64_t r0 = A.unit_off[it.u0], r1 = A.unit_off[it.u1];
     const uint32_t SW = A.slot_words;
     for (uint64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
         const uint32_t* slot = A.slots + r * SW;
-        const uint32_t rflags = meta_flags(slot[0]);
+        const uint32_t meta = slot[0];
         for_each_kmer(A, slot, [&](uint64_t lo, uint64_t hi, bool is_fwd, uint32_t idx, uint32_t last) {
-            uint32_t fl = occurrence_flags(rflags, is_fwd, idx, last);
-            if (A.key_words == 1) tab_insert64(klo, vals, mask, lo, 1u, fl);
-            else tab_insert128(klo, khi, st, vals, mask, lo, hi, 1u, fl);
+            if (*(volatile uint32_t*)overflow) return;
+            uint32_t bits = occurrence_bits(meta, is_fwd, idx, last);
+            uint32_t res = (A.key_words == 1) ? tab_insert64(klo, vals, mask, lo, bits)
+                                              : tab_insert128(klo, khi, st, vals, mask, lo, hi, bits);
+            if (res == 1) {
+                if (atomicAdd(claims, 1u) + 1 > limit) atomicOr(overflow, 1u);
+            } else if (res == 2) {
+                atomicOr(overflow, 1u);
+            }
         });
     }
 }
 
 __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
     extern __shared__ __align__(16) uint8_t lds[];
-    __shared__ uint32_t s_item;
-    const uint32_t cap_max = A.lds_cap;
+    __shared__ uint32_t s_item, s_claims, s_overflow;
+    const uint32_t cap_max = A.lds_cap;  // the host allocates this many entries plus a margin
     uint64_t* klo = reinterpret_cast<uint64_t*>(lds);
     uint64_t* khi = klo + cap_max;  // only used for 128-bit keys
     uint32_t* vals = reinterpret_cast<uint32_t*>(lds + (size_t)cap_max * 8 * A.key_words);
     uint32_t* st = vals + cap_max;
     while (true) {
-        if (threadIdx.x == 0) s_item = atomicAdd(A.work, 1u);
+        if (threadIdx.x == 0) {
+            s_item = atomicAdd(A.work, 1u);
+            s_claims = 0;
+            s_overflow = 0;
+        }
         __syncthreads();
-        const uint32_t sub = s_item;
+        const uint32_t e = s_item;
         __syncthreads();
-        if (sub >= A.n_sub) break;
-        const uint32_t occ = A.sub_kmers[sub];
-        if (occ == 0) continue;
-        if (occ > A.big_thresh) {
-            if (threadIdx.x == 0) A.big_list[atomicAdd(A.big_count, 1u)] = sub;
+        if (e >= A.n_items) break;
+        const WorkItem it = A.items[e];
+        if (it.occ > A.lds_try) {
+            if (threadIdx.x == 0) A.big_list[atomicAdd(A.big_count, 1u)] = e;
             continue;
         }
         uint32_t cap = 256;
-        while (cap < 2 * occ && cap < cap_max) cap <<= 1;
+        while (cap < 2 * it.occ && cap < cap_max) cap <<= 1;
         const uint32_t mask = cap - 1;
         for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) {
             klo[i] = EMPTY64;
@@ -251,30 +337,69 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
             if (A.key_words == 2) st[i] = 0;
         }
         __syncthreads();
-        insert_records(A, sub, klo, khi, st, vals, mask);
+        insert_item(A, it, klo, khi, st, vals, mask, &s_claims, cap - cap / 4, &s_overflow);
         __syncthreads();
-        emit_solid(A, sub, klo, khi, st, vals, cap);
+        if (s_overflow) {
+            // more distinct k-mers than the LDS table holds: redo in HBM
+            if (threadIdx.x == 0) A.big_list[atomicAdd(A.big_count, 1u)] = e;
+        } else {
+            emit_solid(A, it.sub, klo, khi, st, vals, cap);
+        }
         __syncthreads();
     }
 }
 
-// Oversize sub-buckets: same algorithm with the table in HBM (one workgroup
-// per sub-bucket, tables pre-cleared by the host).
+// Big items: same algorithm with the table in HBM, sized to twice the
+// occurrences (one workgroup per item, tables pre-cleared by the host).
 __global__ __launch_bounds__(256) void merge_global_kernel(MergeArgs A, uint32_t n_big) {
+    __shared__ uint32_t s_claims, s_overflow;
     for (uint32_t e = blockIdx.x; e < n_big; e += gridDim.x) {
-        const uint32_t sub = A.big_list[e];
+        const WorkItem it = A.items[A.big_list[e]];
         const uint64_t off = A.g_off[e];
         const uint32_t cap = A.g_cap[e];
         uint64_t* klo = A.g_klo + off;
         uint64_t* khi = A.g_khi ? A.g_khi + off : nullptr;
         uint32_t* st = A.g_st ? A.g_st + off : nullptr;
         uint32_t* vals = A.g_val + off;
-        insert_records(A, sub, klo, khi, st, vals, cap - 1);
+        if (threadIdx.x == 0) {
+            s_claims = 0;
+            s_overflow = 0;
+        }
+        __syncthreads();
+        insert_item(A, it, klo, khi, st, vals, cap - 1, &s_claims, cap, &s_overflow);
         __syncthreads();
         __threadfence();
-        emit_solid(A, sub, klo, khi, st, vals, cap);
+        if (s_overflow) {
+            if (threadIdx.x == 0) atomicOr(A.error_flag, 1u);
+        } else {
+            emit_solid(A, it.sub, klo, khi, st, vals, cap);
+        }
         __syncthreads();
     }
+}
+
+// One thread per sub-bucket: runs of its units become work items of at most
+// `group_occ` occurrences (a single unit above that is an item of its own).
+__global__ void make_items_kernel(const uint32_t* unit_kmers, uint32_t n_sub, uint32_t xb, uint32_t group_occ,
+                                  WorkItem* items, unsigned int* n_items) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_sub) return;
+    const uint32_t u_begin = s << xb, u_end = (s + 1) << xb;
+    uint32_t start = u_begin, acc = 0;
+    for (uint32_t u = u_begin; u < u_end; u++) {
+        const uint32_t o = unit_kmers[u];
+        if (acc == 0 && o == 0) {
+            start = u + 1;
+            continue;
+        }
+        if (acc > 0 && acc + o > group_occ) {
+            items[atomicAdd(n_items, 1u)] = WorkItem{start, u, acc, s};
+            start = u;
+            acc = 0;
+        }
+        acc += o;
+    }
+    if (acc > 0) items[atomicAdd(n_items, 1u)] = WorkItem{start, u_end, acc, s};
 }
 
 __global__ void fill_u64_kernel(uint64_t* p, uint64_t v, uint64_t n) {

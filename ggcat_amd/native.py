@@ -43,7 +43,7 @@ class Params(C.Structure):
     ]
 
 
-_gpu = None
+_gpu: dict = {}
 _synth = None
 
 
@@ -53,14 +53,13 @@ def _ptr(a: np.ndarray | None, t):
     return a.ctypes.data_as(t)
 
 
-def gpu_lib() -> C.CDLL:
+def gpu_lib(path: str = GPU_LIB) -> C.CDLL:
     """Loads libggcat_gpu.so (built by ggcat_amd.build); raises if absent."""
-    global _gpu
-    if _gpu is not None:
-        return _gpu
-    if not os.path.exists(GPU_LIB):
-        raise GGCATGpuError(f"{GPU_LIB} is missing: run `python -m ggcat_amd.build` (no CPU fallback exists)")
-    lib = C.CDLL(GPU_LIB)
+    if path in _gpu:
+        return _gpu[path]
+    if not os.path.exists(path):
+        raise GGCATGpuError(f"{path} is missing: run `python -m ggcat_amd.build` (no CPU fallback exists)")
+    lib = C.CDLL(path)
     V = C.c_void_p
     lib.ggcat_gpu_create.argtypes = [C.c_int, C.POINTER(V)]
     lib.ggcat_gpu_destroy.argtypes = [V]
@@ -83,7 +82,7 @@ def gpu_lib() -> C.CDLL:
                  "ggcat_gpu_kmers_merge", "ggcat_gpu_solid_info", "ggcat_gpu_solid_copy", "ggcat_gpu_timings",
                  "ggcat_gpu_counters"):
         getattr(lib, name).restype = C.c_int
-    _gpu = lib
+    _gpu[path] = lib
     return lib
 
 
@@ -132,8 +131,8 @@ class Solid:
 class GpuContext:
     """One device context (one HIP stream, its buffers)."""
 
-    def __init__(self, device: int = 0):
-        self.lib = gpu_lib()
+    def __init__(self, device: int = 0, lib_path: str = GPU_LIB):
+        self.lib = gpu_lib(lib_path)
         h = C.c_void_p()
         rc = self.lib.ggcat_gpu_create(device, C.byref(h))
         if rc != OK:
