@@ -42,7 +42,7 @@ def _run(cmd: list[str]) -> None:
 def build_gpu() -> str:
     os.makedirs(OUT, exist_ok=True)
     tmp = GPU_LIB + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, "api.hip"), "-o", tmp])
+    _run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, "api.hip"), "-o", tmp, "-lz"])
     os.replace(tmp, GPU_LIB)
     return GPU_LIB
 

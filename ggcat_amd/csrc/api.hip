@@ -3,6 +3,7 @@
 #include "../../include/ggcat_gpu.h"
 #include "bucketing.hip"
 #include "merge.hip"
+#include "fasta.cpp"
 
 #include <algorithm>
 #include <cstdio>
@@ -32,21 +33,32 @@ struct ggcat_gpu_ctx {
     DevBuf input, seg_bits, unit_count, unit_cursor, unit_off, unit_kmers, scan_part, slots, pool, pool_cursor,
         slot_color, color_starts, colors, error_flag;
     // merge buffers
-    DevBuf items, n_items, work, big_list, big_count, out_key, out_val, out_sub, out_cursor, fix_key, fix_val,
-        fix_sub, fix_cursor, g_klo, g_khi, g_st, g_val, g_off, g_cap;
+    DevBuf items, n_items, work, big_list, big_count, out_key, out_val, out_sub, chunk_cursor, chunk_fill_dev, fix_key,
+        fix_val, fix_sub, fix_cursor, g_klo, g_khi, g_st, g_val, g_off, g_cap;
     // state
     ggcat_gpu_params P{};
     bool have_records = false, have_solid = false;
     uint32_t n_sub = 0, n_units = 0, xb = 0, slot_words = 0, key_words = 1;
     uint64_t n_records = 0, pool_words = 0, n_solid = 0, n_input = 0, kmer_occ = 0, n_big = 0, n_fix = 0;
+    uint32_t n_chunks_used = 0;  // output chunks of the last merge
+    std::vector<uint32_t> chunk_fill;  // host copy of their fill counts
+    uint64_t n_tail = 0;  // entries appended after the chunks (combined fixups)
     bool colored = false;
     double ms[8] = {0};
+    ggcat_gpu_tuning tune{};  // zero fields select the defaults
 };
 
 namespace {
 
+// Error text of calls made without a context (ggcat_gpu_read_files accepts
+// ctx == NULL so that inputs can be read on a machine without a GPU).
+thread_local std::string g_ctxless_err;
+
 int fail(ggcat_gpu_ctx* c, int code, const std::string& msg) {
-    if (c) c->err = msg;
+    if (c)
+        c->err = msg;
+    else
+        g_ctxless_err = msg;
     return code;
 }
 
@@ -154,7 +166,7 @@ void ggcat_gpu_destroy(ggcat_gpu_ctx* ctx) {
                       &ctx->unit_kmers, &ctx->scan_part, &ctx->slots, &ctx->pool, &ctx->pool_cursor,
                       &ctx->slot_color, &ctx->color_starts, &ctx->colors, &ctx->error_flag, &ctx->items,
                       &ctx->n_items, &ctx->work, &ctx->big_list, &ctx->big_count, &ctx->out_key, &ctx->out_val,
-                      &ctx->out_sub, &ctx->out_cursor, &ctx->fix_key, &ctx->fix_val, &ctx->fix_sub,
+                      &ctx->out_sub, &ctx->chunk_cursor, &ctx->chunk_fill_dev, &ctx->fix_key, &ctx->fix_val, &ctx->fix_sub,
                       &ctx->fix_cursor, &ctx->g_klo, &ctx->g_khi, &ctx->g_st, &ctx->g_val, &ctx->g_off, &ctx->g_cap};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
@@ -163,7 +175,10 @@ void ggcat_gpu_destroy(ggcat_gpu_ctx* ctx) {
     delete ctx;
 }
 
-const char* ggcat_gpu_last_error(const ggcat_gpu_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char* ggcat_gpu_last_error(const ggcat_gpu_ctx* ctx) {
+    if (ctx) return ctx->err.c_str();
+    return g_ctxless_err.empty() ? "null context" : g_ctxless_err.c_str();
+}
 
 int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_bytes, int input_on_device,
                         const uint64_t* color_starts, const uint32_t* colors, uint32_t n_colors,
@@ -383,8 +398,9 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     M.min_mult = p.min_multiplicity;
     M.kind = p.hash_kind;
     M.key_words = key_words;
-    M.lds_cap = key_words == 1 ? 4096 : 2048;
-    M.lds_try = M.lds_cap * 8;
+    M.lds_cap = key_words == 1 ? 2048 : 1024;
+    M.lds_try = ctx->tune.lds_max_occ ? ctx->tune.lds_max_occ : M.lds_cap * 16;
+    const uint32_t item_occ = ctx->tune.item_occ ? ctx->tune.item_occ : M.lds_cap / 2;
     const size_t lds_bytes = (size_t)M.lds_cap * (8 * key_words + 4 + (key_words == 2 ? 4 : 0)) + 64;
     M.pool_words = ctx->pool_words;
     // rabin-karp128 constants (crates/hashes/src/cn_rkhash.rs:65-78)
@@ -407,14 +423,14 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     ENSURE(ctx->work, 8);
     ENSURE(ctx->big_count, 8);
     ENSURE(ctx->big_list, (size_t)ctx->n_units * 4);
-    ENSURE(ctx->out_cursor, 8);
+    ENSURE(ctx->chunk_cursor, 8);
     ENSURE(ctx->fix_cursor, 8);
     ENSURE(ctx->error_flag, 8);
     M.error_flag = P_<uint32_t>(ctx->error_flag);
     HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->n_items.p, 0, 8, ctx->stream));
     hipLaunchKernelGGL(make_items_kernel, dim3((ctx->n_sub + 255) / 256), dim3(256), 0, ctx->stream,
-                       P_<uint32_t>(ctx->unit_kmers), ctx->n_sub, ctx->xb, M.lds_cap / 2,
+                       P_<uint32_t>(ctx->unit_kmers), ctx->n_sub, ctx->xb, item_occ,
                        P_<WorkItem>(ctx->items), P_<unsigned int>(ctx->n_items));
     HIPCHK(hipGetLastError());
     uint32_t n_items = 0;
@@ -423,12 +439,17 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     M.items = P_<WorkItem>(ctx->items);
     M.n_items = n_items;
 
-    uint64_t out_cap = std::max<uint64_t>(ctx->kmer_occ / 6 + (1u << 20), ctx->out_val.cap / 4);
+    const uint32_t lds_wgs = ctx->n_cu * (ctx->tune.merge_wgs_per_cu ? ctx->tune.merge_wgs_per_cu : 6);
+    uint64_t n_chunks = std::max<uint64_t>((ctx->kmer_occ / 6) / OUT_CHUNK + 2 * lds_wgs + 4096 + 64,
+                                           ctx->out_val.cap / 4 / OUT_CHUNK);
+    uint64_t out_cap = n_chunks * OUT_CHUNK;
     uint64_t fix_cap = std::max<uint64_t>(ctx->n_records / 64 + 4096, ctx->fix_val.cap / 4);
     for (int attempt = 0; attempt < 2; attempt++) {
+        out_cap = n_chunks * OUT_CHUNK;
         ENSURE(ctx->out_key, out_cap * 8 * key_words);
         ENSURE(ctx->out_val, out_cap * 4);
         ENSURE(ctx->out_sub, out_cap * 4);
+        ENSURE(ctx->chunk_fill_dev, n_chunks * 4);
         ENSURE(ctx->fix_key, fix_cap * 8 * key_words);
         ENSURE(ctx->fix_val, fix_cap * 4);
         ENSURE(ctx->fix_sub, fix_cap * 4);
@@ -438,8 +459,9 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
         M.out_key = P_<uint64_t>(ctx->out_key);
         M.out_val = P_<uint32_t>(ctx->out_val);
         M.out_sub = P_<uint32_t>(ctx->out_sub);
-        M.out_cursor = P_<unsigned long long>(ctx->out_cursor);
-        M.out_cap = out_cap;
+        M.chunk_cursor = P_<unsigned int>(ctx->chunk_cursor);
+        M.chunk_fill = P_<uint32_t>(ctx->chunk_fill_dev);
+        M.n_chunks = (uint32_t)n_chunks;
         M.fix_key = P_<uint64_t>(ctx->fix_key);
         M.fix_val = P_<uint32_t>(ctx->fix_val);
         M.fix_sub = P_<uint32_t>(ctx->fix_sub);
@@ -447,12 +469,12 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
         M.fix_cap = fix_cap;
         HIPCHK(hipMemsetAsync(ctx->work.p, 0, 8, ctx->stream));
         HIPCHK(hipMemsetAsync(ctx->big_count.p, 0, 8, ctx->stream));
-        HIPCHK(hipMemsetAsync(ctx->out_cursor.p, 0, 8, ctx->stream));
+        HIPCHK(hipMemsetAsync(ctx->chunk_cursor.p, 0, 8, ctx->stream));
         HIPCHK(hipMemsetAsync(ctx->fix_cursor.p, 0, 8, ctx->stream));
         HIPCHK(hipMemsetAsync(ctx->error_flag.p, 0, 8, ctx->stream));
         HIPCHK(hipEventRecord(ctx->ev[6], ctx->stream));
         if (n_items) {
-            hipLaunchKernelGGL(merge_lds_kernel, dim3(ctx->n_cu * 2), dim3(256), lds_bytes, ctx->stream, M);
+            hipLaunchKernelGGL(merge_lds_kernel, dim3(lds_wgs), dim3(256), lds_bytes, ctx->stream, M);
             HIPCHK(hipGetLastError());
         }
         HIPCHK(hipEventRecord(ctx->ev[7], ctx->stream));
@@ -500,9 +522,9 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
             HIPCHK(hipGetLastError());
         }
         HIPCHK(hipEventRecord(ctx->ev[9], ctx->stream));
-        uint64_t n_out = 0, n_fix = 0;
-        uint32_t errf = 0;
-        HIPCHK(hipMemcpyAsync(&n_out, ctx->out_cursor.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+        uint64_t n_fix = 0;
+        uint32_t used_chunks = 0, errf = 0;
+        HIPCHK(hipMemcpyAsync(&used_chunks, ctx->chunk_cursor.p, 4, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipMemcpyAsync(&n_fix, ctx->fix_cursor.p, 8, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipMemcpyAsync(&errf, ctx->error_flag.p, 4, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -513,12 +535,20 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
         ctx->ms[3] = t;
         HIPCHK(hipEventElapsedTime(&t, ctx->ev[8], ctx->ev[9]));
         ctx->ms[4] = n_big ? t : 0.0;
-        if (n_out > out_cap || n_fix > fix_cap) {
+        if (used_chunks > n_chunks || n_fix > fix_cap) {
             if (attempt == 1) return fail(ctx, GGCAT_GPU_ENOMEM, "solid output overflow");
-            out_cap = std::max(out_cap, n_out + 1024);
+            n_chunks = std::max<uint64_t>(n_chunks, (uint64_t)used_chunks + 64);
             fix_cap = std::max(fix_cap, n_fix + 1024);
             continue;
         }
+        ctx->chunk_fill.resize(used_chunks);
+        if (used_chunks)
+            HIPCHK(hipMemcpy(ctx->chunk_fill.data(), ctx->chunk_fill_dev.p, (size_t)used_chunks * 4, hipMemcpyDeviceToHost));
+        uint64_t n_out = 0;
+        for (uint32_t f : ctx->chunk_fill) n_out += f;
+        // combined fixups are appended after the last used chunk
+        const uint64_t tail0 = (uint64_t)used_chunks * OUT_CHUNK;
+        uint64_t n_add = 0;
         // combine the two halves of cross-unit boundary k-mers
         if (n_fix) {
             const uint32_t kw = key_words;
@@ -532,30 +562,32 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
             auto key_of = [&](uint32_t i) {
                 return std::make_tuple(fs[i], kw == 2 ? fk[2 * i + 1] : 0ull, fk[kw * i]);
             };
-            std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b2) { return key_of(a) < key_of(b2); });
+            std::sort(ord.begin(), ord.end(), [&](uint32_t a2, uint32_t b2) { return key_of(a2) < key_of(b2); });
             std::vector<uint64_t> ok_;
             std::vector<uint32_t> ov, os;
             for (uint64_t i = 0; i < n_fix;) {
                 uint64_t j = i + 1;
                 uint32_t v = fv[ord[i]];
                 while (j < n_fix && key_of(ord[j]) == key_of(ord[i])) v |= fv[ord[j++]] & 0xC0000000u;
-                uint32_t a = ord[i];
-                for (uint32_t q = 0; q < kw; q++) ok_.push_back(fk[kw * a + q]);
+                uint32_t a2 = ord[i];
+                for (uint32_t q = 0; q < kw; q++) ok_.push_back(fk[kw * a2 + q]);
                 ov.push_back(v);
-                os.push_back(fs[a]);
+                os.push_back(fs[a2]);
                 i = j;
             }
-            const uint64_t n_add = ov.size();
-            if (n_out + n_add > out_cap) {
+            n_add = ov.size();
+            if (tail0 + n_add > out_cap) {
                 if (attempt == 1) return fail(ctx, GGCAT_GPU_ENOMEM, "solid output overflow");
-                out_cap = n_out + n_add + 1024;
+                n_chunks = (tail0 + n_add) / OUT_CHUNK + 64;
                 continue;
             }
-            HIPCHK(hipMemcpy(P_<uint64_t>(ctx->out_key) + n_out * kw, ok_.data(), n_add * kw * 8, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(P_<uint32_t>(ctx->out_val) + n_out, ov.data(), n_add * 4, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(P_<uint32_t>(ctx->out_sub) + n_out, os.data(), n_add * 4, hipMemcpyHostToDevice));
-            n_out += n_add;
+            HIPCHK(hipMemcpy(P_<uint64_t>(ctx->out_key) + tail0 * kw, ok_.data(), n_add * kw * 8, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(P_<uint32_t>(ctx->out_val) + tail0, ov.data(), n_add * 4, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(P_<uint32_t>(ctx->out_sub) + tail0, os.data(), n_add * 4, hipMemcpyHostToDevice));
         }
+        ctx->n_chunks_used = used_chunks;
+        ctx->n_tail = n_add;
+        n_out += n_add;
         ctx->n_fix = n_fix;
         ctx->n_solid = n_out;
         HIPCHK(hipEventRecord(ctx->ev[10], ctx->stream));
@@ -580,17 +612,37 @@ int ggcat_gpu_solid_copy(ggcat_gpu_ctx* ctx, uint64_t* keys, uint32_t* multiplic
     if (!ctx) return GGCAT_GPU_EINVAL;
     if (!ctx->have_solid) return fail(ctx, GGCAT_GPU_ESTATE, "no merge result");
     HIPCHK(hipSetDevice(ctx->device));
-    const uint64_t n = ctx->n_solid;
-    if (keys && n) HIPCHK(hipMemcpy(keys, ctx->out_key.p, n * 8 * ctx->key_words, hipMemcpyDeviceToHost));
-    if (sub && n) HIPCHK(hipMemcpy(sub, ctx->out_sub.p, n * 4, hipMemcpyDeviceToHost));
-    if ((multiplicity || flags) && n) {
-        std::vector<uint32_t> v(n);
-        HIPCHK(hipMemcpy(v.data(), ctx->out_val.p, n * 4, hipMemcpyDeviceToHost));
-        for (uint64_t i = 0; i < n; i++) {
-            if (multiplicity) multiplicity[i] = v[i] & 0x3FFFFFFFu;
-            if (flags) flags[i] = (uint8_t)(v[i] >> 30);
+    const uint32_t kw = ctx->key_words;
+    // the output is the valid prefix of every used chunk, then the tail of
+    // combined cross-unit entries
+    std::vector<std::pair<uint64_t, uint64_t>> runs;  // (first entry, count)
+    for (uint32_t c = 0; c < ctx->n_chunks_used; c++)
+        if (ctx->chunk_fill[c]) runs.emplace_back((uint64_t)c * OUT_CHUNK, ctx->chunk_fill[c]);
+    if (ctx->n_tail) runs.emplace_back((uint64_t)ctx->n_chunks_used * OUT_CHUNK, ctx->n_tail);
+    std::vector<uint32_t> v;
+    uint64_t o = 0;
+    for (auto& r : runs) {
+        if (keys)
+            HIPCHK(hipMemcpy(keys + o * kw, P_<uint64_t>(ctx->out_key) + r.first * kw, r.second * 8 * kw,
+                             hipMemcpyDeviceToHost));
+        if (sub) HIPCHK(hipMemcpy(sub + o, P_<uint32_t>(ctx->out_sub) + r.first, r.second * 4, hipMemcpyDeviceToHost));
+        if (multiplicity || flags) {
+            v.resize(r.second);
+            HIPCHK(hipMemcpy(v.data(), P_<uint32_t>(ctx->out_val) + r.first, r.second * 4, hipMemcpyDeviceToHost));
+            for (uint64_t i = 0; i < r.second; i++) {
+                if (multiplicity) multiplicity[o + i] = v[i] & 0x3FFFFFFFu;
+                if (flags) flags[o + i] = (uint8_t)(v[i] >> 30);
+            }
         }
+        o += r.second;
     }
+    return GGCAT_GPU_OK;
+}
+
+int ggcat_gpu_set_tuning(ggcat_gpu_ctx* ctx, const ggcat_gpu_tuning* t) {
+    if (!ctx || !t) return GGCAT_GPU_EINVAL;
+    if (t->merge_wgs_per_cu > 64) return fail(ctx, GGCAT_GPU_EINVAL, "merge_wgs_per_cu must be <= 64");
+    ctx->tune = *t;
     return GGCAT_GPU_OK;
 }
 
@@ -633,6 +685,73 @@ int ggcat_gpu_copy_to_device(ggcat_gpu_ctx* ctx, void* dst, const void* src, uin
     HIPCHK(hipSetDevice(ctx->device));
     HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
     return GGCAT_GPU_OK;
+}
+
+int ggcat_gpu_read_files(ggcat_gpu_ctx* ctx, const char* const* paths, uint32_t n_paths, uint32_t k, int colors,
+                         uint8_t** joined, uint64_t* n_bytes, uint64_t** color_starts, uint32_t** color_ids,
+                         uint32_t* n_colors, uint64_t* estimated_bases) {
+    if ((!paths && n_paths) || !joined || !n_bytes) return fail(ctx, GGCAT_GPU_EINVAL, "null argument");
+    std::vector<std::string> ps;
+    for (uint32_t i = 0; i < n_paths; i++) {
+        if (!paths[i]) return fail(ctx, GGCAT_GPU_EINVAL, "null path");
+        ps.emplace_back(paths[i]);
+    }
+    gg_host::ReadResult R;
+    if (!gg_host::read_files(ps, k, colors != 0, R)) return fail(ctx, GGCAT_GPU_EINVAL, R.error);
+    *joined = (uint8_t*)malloc(R.joined.size() + 16);
+    if (!*joined) return fail(ctx, GGCAT_GPU_ENOMEM, "host allocation failed");
+    memcpy(*joined, R.joined.data(), R.joined.size());
+    *n_bytes = R.joined.size();
+    if (estimated_bases) *estimated_bases = R.estimated_bases;
+    if (n_colors) *n_colors = (uint32_t)R.colors.size();
+    if (color_starts) {
+        *color_starts = (uint64_t*)malloc(R.color_starts.size() * 8 + 8);
+        memcpy(*color_starts, R.color_starts.data(), R.color_starts.size() * 8);
+    }
+    if (color_ids) {
+        *color_ids = (uint32_t*)malloc(R.colors.size() * 4 + 4);
+        memcpy(*color_ids, R.colors.data(), R.colors.size() * 4);
+    }
+    return GGCAT_GPU_OK;
+}
+
+void ggcat_gpu_free_host(void* p) { free(p); }
+
+int ggcat_gpu_plan_build(const ggcat_gpu_build_options* o, uint64_t estimated_bases, ggcat_gpu_params* out) {
+    if (!o || !out || o->k < 2 || (o->m == 0 && o->k < 4)) return GGCAT_GPU_EINVAL;
+    ggcat_gpu_params p{};
+    p.k = o->k;
+    p.m = o->m ? o->m : gg_host::compute_best_m(o->k);
+    uint32_t b_auto = 0, sb_auto = 0;
+    gg_host::auto_bucket_logs(estimated_bases, &b_auto, &sb_auto);
+    p.buckets_log = o->buckets_log ? o->buckets_log : b_auto;
+    p.second_buckets_log = sb_auto;
+    p.forward_only = o->forward_only;
+    p.hash_kind = o->hash_kind;
+    p.min_multiplicity = o->min_multiplicity;
+    *out = p;
+    return GGCAT_GPU_OK;
+}
+
+int ggcat_gpu_build_from_files(ggcat_gpu_ctx* ctx, const char* const* paths, uint32_t n_paths,
+                               const ggcat_gpu_build_options* options, ggcat_gpu_params* used) {
+    if (!ctx || !options) return GGCAT_GPU_EINVAL;
+    const ggcat_gpu_build_options o = *options;
+    std::vector<std::string> ps;
+    for (uint32_t i = 0; i < n_paths; i++) {
+        if (!paths || !paths[i]) return fail(ctx, GGCAT_GPU_EINVAL, "null path");
+        ps.emplace_back(paths[i]);
+    }
+    gg_host::ReadResult R;
+    if (!gg_host::read_files(ps, o.k, o.colors != 0, R)) return fail(ctx, GGCAT_GPU_EINVAL, R.error);
+    ggcat_gpu_params p{};
+    int r = ggcat_gpu_plan_build(&o, R.estimated_bases, &p);
+    if (r != GGCAT_GPU_OK) return fail(ctx, r, "invalid build options");
+    if (used) *used = p;
+    r = ggcat_gpu_bucketing(ctx, R.joined.data(), R.joined.size(), 0, o.colors ? R.color_starts.data() : nullptr,
+                                o.colors ? R.colors.data() : nullptr, o.colors ? (uint32_t)R.colors.size() : 0, &p);
+    if (r != GGCAT_GPU_OK) return r;
+    return ggcat_gpu_kmers_merge(ctx);
 }
 
 }  // extern "C"

@@ -51,11 +51,12 @@ struct MergeArgs {
     unsigned int* work;
     uint32_t* big_list;  // indices into items
     unsigned int* big_count;
-    uint64_t* out_key;  // key_words per entry
+    uint64_t* out_key;  // key_words per entry, chunked (see OUT_CHUNK)
     uint32_t* out_val;  // multiplicity | flags << 30
     uint32_t* out_sub;
-    unsigned long long* out_cursor;
-    uint64_t out_cap;
+    unsigned int* chunk_cursor;
+    uint32_t* chunk_fill;
+    uint32_t n_chunks;
     uint64_t* fix_key;  // cross-marked entries awaiting their other half
     uint32_t* fix_val;
     uint32_t* fix_sub;
@@ -78,13 +79,14 @@ struct MergeArgs {
 GG_DEV uint32_t hash128(uint64_t lo, uint64_t hi) { return tab_hash64(lo ^ (hi * 0x9E3779B97F4A7C15ULL)); }
 
 // Insert one occurrence.  Returns 1 when this call claimed a new slot, 0 when
-// the key was present, 2 when no slot was found within mask+1 probes.
+// the key was present, 2 when no slot was found within min(mask+1,
+// probe_limit) probes.
 template <class KP, class VP>
-GG_DEV uint32_t tab_insert64(KP keys, VP vals, uint32_t mask, uint64_t key, uint32_t bits) {
+GG_DEV uint32_t tab_insert64(KP keys, VP vals, uint32_t mask, uint64_t key, uint32_t bits, uint32_t probe_limit) {
     uint32_t i = tab_hash64(key) & mask;
     uint32_t claimed = 0;
     for (uint32_t probes = 0;; probes++) {
-        if (probes > mask) return 2;
+        if (probes > mask || probes >= probe_limit) return 2;
         uint64_t cur = keys[i];
         if (cur == key) break;
         if (cur == EMPTY64) {
@@ -109,7 +111,7 @@ GG_DEV uint32_t tab_insert64(KP keys, VP vals, uint32_t mask, uint64_t key, uint
 // wait on a slot in state 1 is bounded.
 template <class KP, class SP, class VP>
 GG_DEV uint32_t tab_insert128(KP klo, KP khi, SP st, VP vals, uint32_t mask, uint64_t lo, uint64_t hi,
-                              uint32_t bits) {
+                              uint32_t bits, uint32_t probe_limit) {
     uint32_t i = hash128(lo, hi) & mask;
     uint32_t claimed = 0, probes = 0, spins = 0;
     bool done = false;
@@ -131,7 +133,7 @@ GG_DEV uint32_t tab_insert128(KP klo, KP khi, SP st, VP vals, uint32_t mask, uin
                 done = true;
             } else {
                 i = (i + 1) & mask;
-                if (++probes > mask) return 2;
+                if (++probes > mask || probes >= probe_limit) return 2;
             }
         } else if (++spins > (1u << 20)) {
             return 2;
@@ -221,11 +223,31 @@ GG_DEV uint32_t occurrence_bits(uint32_t meta, bool is_fwd, uint32_t idx, uint32
     return 1u | (cross ? VAL_CROSS : 0u) | (fl << 30);
 }
 
+// Output is written in chunks of OUT_CHUNK entries.  A workgroup owns one
+// open chunk at a time (reserved with one atomic), fills it locally and
+// records its fill count when it moves on, so solid entries cost no global
+// atomics.  Entries of chunk c live at [c * OUT_CHUNK, c * OUT_CHUNK + fill[c]).
+constexpr uint32_t OUT_CHUNK = 8192;
+
+struct EmitState {
+    uint32_t chunk;  // open chunk, ~0u when none
+    uint32_t fill;
+    uint32_t wave_cnt[4];
+    uint32_t wave_base[4];
+};
+
+GG_DEV void close_chunk(const MergeArgs& A, EmitState& es) {
+    if (es.chunk != ~0u && es.chunk < A.n_chunks) A.chunk_fill[es.chunk] = es.fill;
+    es.chunk = ~0u;
+    es.fill = 0;
+}
+
 // Emit solid entries of a table of `cap` slots; cross-marked entries holding
 // only one flag bit go to the fixup list.
 template <class KP, class SP, class VP>
-GG_DEV void emit_solid(const MergeArgs& A, uint32_t sub, KP klo, KP khi, SP st, VP vals, uint32_t cap) {
-    const uint32_t lane = threadIdx.x & 63;
+GG_DEV void emit_solid(const MergeArgs& A, uint32_t sub, KP klo, KP khi, SP st, VP vals, uint32_t cap,
+                       EmitState& es) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
     for (uint32_t i0 = 0; i0 < cap; i0 += blockDim.x) {
         uint32_t i = i0 + threadIdx.x;
         bool solid = false, fix = false;
@@ -242,50 +264,66 @@ GG_DEV void emit_solid(const MergeArgs& A, uint32_t sub, KP klo, KP khi, SP st, 
             }
         }
         const bool main = solid && !fix;
-        uint64_t bal = __ballot(main), balf = __ballot(fix);
-        if ((bal | balf) == 0) continue;
-        unsigned long long base = 0, basef = 0;
-        if (lane == 0) {
-            if (bal) base = atomicAdd(A.out_cursor, (unsigned long long)__popcll(bal));
-            if (balf) basef = atomicAdd(A.fix_cursor, (unsigned long long)__popcll(balf));
-        }
-        base = __shfl(base, 0);
-        basef = __shfl(basef, 0);
+        const uint64_t bal = __ballot(main), balf = __ballot(fix);
         const uint64_t below = (1ULL << lane) - 1;
-        if (main) {
-            uint64_t pos = base + __popcll(bal & below);
-            if (pos < A.out_cap) {
-                if (A.key_words == 1) {
-                    A.out_key[pos] = klo[i];
-                } else {
-                    A.out_key[2 * pos] = klo[i];
-                    A.out_key[2 * pos + 1] = khi[i];
+        if (lane == 0) es.wave_cnt[wave] = (uint32_t)__popcll(bal);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t tot = 0;
+            for (uint32_t q = 0; q < nwaves; q++) tot += es.wave_cnt[q];
+            if (tot) {
+                if (es.chunk == ~0u || es.fill + tot > OUT_CHUNK) {
+                    close_chunk(A, es);
+                    es.chunk = atomicAdd(A.chunk_cursor, 1u);
                 }
-                A.out_val[pos] = outv;
-                A.out_sub[pos] = sub;
+                uint32_t run = es.fill;
+                for (uint32_t q = 0; q < nwaves; q++) {
+                    es.wave_base[q] = run;
+                    run += es.wave_cnt[q];
+                }
+                es.fill = run;
             }
         }
-        if (fix) {
-            uint64_t pos = basef + __popcll(balf & below);
-            if (pos < A.fix_cap) {
-                if (A.key_words == 1) {
-                    A.fix_key[pos] = klo[i];
-                } else {
-                    A.fix_key[2 * pos] = klo[i];
-                    A.fix_key[2 * pos + 1] = khi[i];
+        __syncthreads();
+        if (main && es.chunk < A.n_chunks) {
+            uint64_t pos = (uint64_t)es.chunk * OUT_CHUNK + es.wave_base[wave] + __popcll(bal & below);
+            if (A.key_words == 1) {
+                A.out_key[pos] = klo[i];
+            } else {
+                A.out_key[2 * pos] = klo[i];
+                A.out_key[2 * pos + 1] = khi[i];
+            }
+            A.out_val[pos] = outv;
+            A.out_sub[pos] = sub;
+        }
+        if (balf) {
+            unsigned long long basef = 0;
+            if (lane == 0) basef = atomicAdd(A.fix_cursor, (unsigned long long)__popcll(balf));
+            basef = __shfl(basef, 0);
+            if (fix) {
+                uint64_t pos = basef + __popcll(balf & below);
+                if (pos < A.fix_cap) {
+                    if (A.key_words == 1) {
+                        A.fix_key[pos] = klo[i];
+                    } else {
+                        A.fix_key[2 * pos] = klo[i];
+                        A.fix_key[2 * pos + 1] = khi[i];
+                    }
+                    A.fix_val[pos] = outv;
+                    A.fix_sub[pos] = sub;
                 }
-                A.fix_val[pos] = outv;
-                A.fix_sub[pos] = sub;
             }
         }
+        __syncthreads();
     }
 }
 
-// Inserts every occurrence of the item's records.  Stops inserting new keys
-// once `limit` keys were claimed (sets *overflow); returns nothing.
+// Inserts every occurrence of the item's records.  An insert that needs more
+// than `probe_limit` probes sets *overflow and insertion stops (the item is
+// then redone with a larger table).
 template <class KP, class SP, class VP>
 GG_DEV void insert_item(const MergeArgs& A, const WorkItem& it, KP klo, KP khi, SP st, VP vals, uint32_t mask,
-                        uint32_t* claims, uint32_t limit, uint32_t* overflow) {
+                        uint32_t probe_limit, uint32_t* overflow) {
     const uint64_t r0 = A.unit_off[it.u0], r1 = A.unit_off[it.u1];
     const uint32_t SW = A.slot_words;
     for (uint64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
@@ -294,29 +332,29 @@ GG_DEV void insert_item(const MergeArgs& A, const WorkItem& it, KP klo, KP khi, 
         for_each_kmer(A, slot, [&](uint64_t lo, uint64_t hi, bool is_fwd, uint32_t idx, uint32_t last) {
             if (*(volatile uint32_t*)overflow) return;
             uint32_t bits = occurrence_bits(meta, is_fwd, idx, last);
-            uint32_t res = (A.key_words == 1) ? tab_insert64(klo, vals, mask, lo, bits)
-                                              : tab_insert128(klo, khi, st, vals, mask, lo, hi, bits);
-            if (res == 1) {
-                if (atomicAdd(claims, 1u) + 1 > limit) atomicOr(overflow, 1u);
-            } else if (res == 2) {
-                atomicOr(overflow, 1u);
-            }
+            uint32_t res = (A.key_words == 1) ? tab_insert64(klo, vals, mask, lo, bits, probe_limit)
+                                              : tab_insert128(klo, khi, st, vals, mask, lo, hi, bits, probe_limit);
+            if (res == 2) atomicOr(overflow, 1u);
         });
     }
 }
 
 __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
     extern __shared__ __align__(16) uint8_t lds[];
-    __shared__ uint32_t s_item, s_claims, s_overflow;
+    __shared__ uint32_t s_item, s_overflow;
+    __shared__ EmitState es;
     const uint32_t cap_max = A.lds_cap;  // the host allocates this many entries plus a margin
     uint64_t* klo = reinterpret_cast<uint64_t*>(lds);
     uint64_t* khi = klo + cap_max;  // only used for 128-bit keys
     uint32_t* vals = reinterpret_cast<uint32_t*>(lds + (size_t)cap_max * 8 * A.key_words);
     uint32_t* st = vals + cap_max;
+    if (threadIdx.x == 0) {
+        es.chunk = ~0u;
+        es.fill = 0;
+    }
     while (true) {
         if (threadIdx.x == 0) {
             s_item = atomicAdd(A.work, 1u);
-            s_claims = 0;
             s_overflow = 0;
         }
         __syncthreads();
@@ -337,22 +375,28 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
             if (A.key_words == 2) st[i] = 0;
         }
         __syncthreads();
-        insert_item(A, it, klo, khi, st, vals, mask, &s_claims, cap - cap / 4, &s_overflow);
+        insert_item(A, it, klo, khi, st, vals, mask, 96u, &s_overflow);
         __syncthreads();
         if (s_overflow) {
             // more distinct k-mers than the LDS table holds: redo in HBM
             if (threadIdx.x == 0) A.big_list[atomicAdd(A.big_count, 1u)] = e;
         } else {
-            emit_solid(A, it.sub, klo, khi, st, vals, cap);
+            emit_solid(A, it.sub, klo, khi, st, vals, cap, es);
         }
         __syncthreads();
     }
+    if (threadIdx.x == 0) close_chunk(A, es);
 }
 
 // Big items: same algorithm with the table in HBM, sized to twice the
 // occurrences (one workgroup per item, tables pre-cleared by the host).
 __global__ __launch_bounds__(256) void merge_global_kernel(MergeArgs A, uint32_t n_big) {
-    __shared__ uint32_t s_claims, s_overflow;
+    __shared__ uint32_t s_overflow;
+    __shared__ EmitState es;
+    if (threadIdx.x == 0) {
+        es.chunk = ~0u;
+        es.fill = 0;
+    }
     for (uint32_t e = blockIdx.x; e < n_big; e += gridDim.x) {
         const WorkItem it = A.items[A.big_list[e]];
         const uint64_t off = A.g_off[e];
@@ -361,21 +405,19 @@ __global__ __launch_bounds__(256) void merge_global_kernel(MergeArgs A, uint32_t
         uint64_t* khi = A.g_khi ? A.g_khi + off : nullptr;
         uint32_t* st = A.g_st ? A.g_st + off : nullptr;
         uint32_t* vals = A.g_val + off;
-        if (threadIdx.x == 0) {
-            s_claims = 0;
-            s_overflow = 0;
-        }
+        if (threadIdx.x == 0) s_overflow = 0;
         __syncthreads();
-        insert_item(A, it, klo, khi, st, vals, cap - 1, &s_claims, cap, &s_overflow);
+        insert_item(A, it, klo, khi, st, vals, cap - 1, cap, &s_overflow);
         __syncthreads();
         __threadfence();
         if (s_overflow) {
             if (threadIdx.x == 0) atomicOr(A.error_flag, 1u);
         } else {
-            emit_solid(A, it.sub, klo, khi, st, vals, cap);
+            emit_solid(A, it.sub, klo, khi, st, vals, cap, es);
         }
         __syncthreads();
     }
+    if (threadIdx.x == 0) close_chunk(A, es);
 }
 
 // One thread per sub-bucket: runs of its units become work items of at most

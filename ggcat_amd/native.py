@@ -43,6 +43,28 @@ class Params(C.Structure):
     ]
 
 
+class BuildOptions(C.Structure):
+    """ggcat_gpu_build_options: m = 0 and buckets_log = 0 select the automatic values."""
+    _fields_ = [
+        ("k", C.c_uint32),
+        ("m", C.c_uint32),
+        ("min_multiplicity", C.c_uint32),
+        ("buckets_log", C.c_uint32),
+        ("forward_only", C.c_uint32),
+        ("hash_kind", C.c_uint32),
+        ("colors", C.c_uint32),
+    ]
+
+
+class Tuning(C.Structure):
+    """ggcat_gpu_tuning: merge schedule knobs (zero = default); never change results."""
+    _fields_ = [
+        ("item_occ", C.c_uint32),
+        ("lds_max_occ", C.c_uint32),
+        ("merge_wgs_per_cu", C.c_uint32),
+    ]
+
+
 _gpu: dict = {}
 _synth = None
 
@@ -77,10 +99,19 @@ def gpu_lib(path: str = GPU_LIB) -> C.CDLL:
     lib.ggcat_gpu_device_alloc.argtypes = [V, C.c_uint64, C.POINTER(V)]
     lib.ggcat_gpu_device_free.argtypes = [V, V]
     lib.ggcat_gpu_copy_to_device.argtypes = [V, V, V, C.c_uint64]
+    lib.ggcat_gpu_read_files.argtypes = [V, C.POINTER(C.c_char_p), C.c_uint32, C.c_uint32, C.c_int,
+                                         C.POINTER(_u8p), _u64p, C.POINTER(_u64p), C.POINTER(_u32p), _u32p, _u64p]
+    lib.ggcat_gpu_free_host.argtypes = [V]
+    lib.ggcat_gpu_free_host.restype = None
+    lib.ggcat_gpu_set_tuning.argtypes = [V, C.POINTER(Tuning)]
+    lib.ggcat_gpu_plan_build.argtypes = [C.POINTER(BuildOptions), C.c_uint64, C.POINTER(Params)]
+    lib.ggcat_gpu_build_from_files.argtypes = [V, C.POINTER(C.c_char_p), C.c_uint32, C.POINTER(BuildOptions),
+                                               C.POINTER(Params)]
     for name in ("ggcat_gpu_device_alloc", "ggcat_gpu_device_free", "ggcat_gpu_copy_to_device",
                  "ggcat_gpu_create", "ggcat_gpu_bucketing", "ggcat_gpu_records_info", "ggcat_gpu_records_copy",
                  "ggcat_gpu_kmers_merge", "ggcat_gpu_solid_info", "ggcat_gpu_solid_copy", "ggcat_gpu_timings",
-                 "ggcat_gpu_counters"):
+                 "ggcat_gpu_counters", "ggcat_gpu_read_files", "ggcat_gpu_build_from_files",
+                 "ggcat_gpu_plan_build", "ggcat_gpu_set_tuning"):
         getattr(lib, name).restype = C.c_int
     _gpu[path] = lib
     return lib
@@ -107,6 +138,51 @@ def synth_lib() -> C.CDLL:
     lib.synth_queries.restype = C.c_uint64
     _synth = lib
     return lib
+
+
+def _paths(paths) -> tuple:
+    enc = [os.fsencode(p) for p in paths]
+    arr = (C.c_char_p * max(1, len(enc)))(*enc)
+    return arr, len(enc)
+
+
+@dataclass
+class ReadInput:
+    joined: np.ndarray  # uint8, records each followed by '\n'
+    color_starts: np.ndarray  # uint64 byte offsets (empty when uncolored)
+    colors: np.ndarray  # uint32
+    estimated_bases: int
+
+
+def read_files(paths, k: int, colors: bool = False, lib_path: str = GPU_LIB) -> ReadInput:
+    """FASTA/FASTQ(.gz) files -> joined layout (host code; needs no GPU)."""
+    lib = gpu_lib(lib_path)
+    arr, n = _paths(paths)
+    jp, nb = _u8p(), C.c_uint64()
+    csp, cvp = _u64p(), _u32p()
+    nc, est = C.c_uint32(), C.c_uint64()
+    rc = lib.ggcat_gpu_read_files(None, arr, n, k, int(colors), C.byref(jp), C.byref(nb), C.byref(csp),
+                                  C.byref(cvp), C.byref(nc), C.byref(est))
+    if rc != OK:
+        raise GGCATGpuError(f"ggcat_gpu_read_files failed (status {rc}): "
+                            f"{lib.ggcat_gpu_last_error(None).decode(errors='replace')}")
+    try:
+        joined = np.ctypeslib.as_array(jp, shape=(nb.value,)).copy() if nb.value else np.zeros(0, np.uint8)
+        cs = np.ctypeslib.as_array(csp, shape=(nc.value,)).copy() if nc.value else np.zeros(0, np.uint64)
+        cv = np.ctypeslib.as_array(cvp, shape=(nc.value,)).copy() if nc.value else np.zeros(0, np.uint32)
+    finally:
+        for p in (jp, csp, cvp):
+            lib.ggcat_gpu_free_host(C.cast(p, C.c_void_p))
+    return ReadInput(joined, cs, cv, est.value)
+
+
+def plan_build(options: BuildOptions, estimated_bases: int, lib_path: str = GPU_LIB) -> Params:
+    """Parameters the reference derives for inputs of this size (host code)."""
+    out = Params()
+    rc = gpu_lib(lib_path).ggcat_gpu_plan_build(C.byref(options), estimated_bases, C.byref(out))
+    if rc != OK:
+        raise GGCATGpuError(f"ggcat_gpu_plan_build failed (status {rc})")
+    return out
 
 
 @dataclass
@@ -171,6 +247,18 @@ class GpuContext:
             nc = cs.size
         self._check(self.lib.ggcat_gpu_bucketing(self.h, ptr, n, on_dev, _ptr(cs, _u64p), _ptr(cv, _u32p), nc,
                                                  C.byref(params)), "ggcat_gpu_bucketing")
+
+    def build_from_files(self, paths, options: BuildOptions) -> Params:
+        """Reads the files, buckets and merges; returns the parameters used."""
+        arr, n = _paths(paths)
+        used = Params()
+        self._check(self.lib.ggcat_gpu_build_from_files(self.h, arr, n, C.byref(options), C.byref(used)),
+                    "ggcat_gpu_build_from_files")
+        return used
+
+    def set_tuning(self, item_occ: int = 0, lds_max_occ: int = 0, merge_wgs_per_cu: int = 0) -> None:
+        t = Tuning(item_occ, lds_max_occ, merge_wgs_per_cu)
+        self._check(self.lib.ggcat_gpu_set_tuning(self.h, C.byref(t)), "ggcat_gpu_set_tuning")
 
     def records(self) -> Records:
         nr, ns, nb = C.c_uint64(), C.c_uint32(), C.c_uint64()

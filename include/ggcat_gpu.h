@@ -112,6 +112,22 @@ int ggcat_gpu_solid_info(ggcat_gpu_ctx* ctx, uint64_t* n_solid, uint32_t* key_wo
 int ggcat_gpu_solid_copy(ggcat_gpu_ctx* ctx, uint64_t* keys, uint32_t* multiplicity, uint8_t* flags,
                          uint32_t* sub);
 
+/* Schedule knobs of the merge (no counterpart in the reference, whose
+ * schedule is its thread pool).  They change which kernel and table size
+ * handle a work item, never the result.  Zero selects the default.
+ *   item_occ          target k-mer occurrences per work item (runs of
+ *                     physical units of one sub-bucket); default: half the
+ *                     LDS table (1024 for 64-bit keys, 512 for 128-bit)
+ *   lds_max_occ       items with more occurrences skip the LDS table and
+ *                     use an HBM table; default 16x the LDS table
+ *   merge_wgs_per_cu  persistent merge workgroups per CU; default 6 */
+typedef struct {
+    uint32_t item_occ;
+    uint32_t lds_max_occ;
+    uint32_t merge_wgs_per_cu;
+} ggcat_gpu_tuning;
+int ggcat_gpu_set_tuning(ggcat_gpu_ctx* ctx, const ggcat_gpu_tuning* tuning);
+
 /* Device-side timings of the last calls, in milliseconds:
  * [0] bucketing pass A, [1] scan, [2] bucketing pass B,
  * [3] merge (LDS kernel), [4] merge (HBM-table kernel),
@@ -120,8 +136,51 @@ int ggcat_gpu_timings(ggcat_gpu_ctx* ctx, double* ms, uint32_t n);
 
 /* Counters of the last calls: [0] input bytes, [1] records,
  * [2] sum of record lengths (bases), [3] k-mer occurrences,
- * [4] sub-buckets merged in HBM, [5] solid entries, [6] long records. */
+ * [4] work items merged with an HBM table, [5] solid entries,
+ * [6] k-mers shared between two units of a sub-bucket (combined on the
+ * host after the merge). */
 int ggcat_gpu_counters(ggcat_gpu_ctx* ctx, uint64_t* out, uint32_t n);
+
+/* Reads FASTA/FASTQ files (optionally gzip-compressed) into the joined layout.
+ * Restates crates/io/src/sequences_reader.rs:56-241 including long-record
+ * chunking with a (k-1)-base copy-back.  With colors != 0, each file gets the
+ * color equal to its index and *color_starts / *colors describe the ranges.
+ * Buffers are allocated by the library; release them with
+ * ggcat_gpu_free_host.  *estimated_bases is the reference's input size
+ * estimate (crates/io/src/sequences_stream/fasta.rs:12-30).  Host-only: ctx
+ * may be NULL (then ggcat_gpu_last_error(NULL) describes a failure). */
+int ggcat_gpu_read_files(ggcat_gpu_ctx* ctx, const char* const* paths, uint32_t n_paths, uint32_t k, int colors,
+                         uint8_t** joined, uint64_t* n_bytes, uint64_t** color_starts, uint32_t** color_ids,
+                         uint32_t* n_colors, uint64_t* estimated_bases);
+void ggcat_gpu_free_host(void* p);
+
+/* Build options, mirroring ggcat_build_from_files
+ * (crates/capi/src/lib.rs:567-599): m == 0 selects compute_best_m(k)
+ * (crates/utils/src/lib.rs:29-40); buckets_log == 0 selects the automatic
+ * bucket count (crates/io/src/lib.rs:67-140); the second-bucket count is
+ * always automatic, as in run_assembler. */
+typedef struct {
+    uint32_t k;
+    uint32_t m;
+    uint32_t min_multiplicity;
+    uint32_t buckets_log;
+    uint32_t forward_only;
+    uint32_t hash_kind;
+    uint32_t colors;
+} ggcat_gpu_build_options;
+
+/* Parameters a build uses for inputs of `estimated_bases` bases (the sum of
+ * the files' estimates): run_assembler's derivation,
+ * crates/assembler/src/lib.rs:110-113 with crates/io/src/lib.rs:67-140, and
+ * the m default of crates/api/src/lib.rs:291.  Host-only; needs no GPU. */
+int ggcat_gpu_plan_build(const ggcat_gpu_build_options* options, uint64_t estimated_bases, ggcat_gpu_params* out);
+
+/* Reads the files and runs bucketing and the k-mer merge: the work of
+ * GGCATInstance::build_graph (crates/api/src/lib.rs:217-322) up to the end of
+ * kmers_merge.  Results stay in the context (ggcat_gpu_records_* /
+ * ggcat_gpu_solid_*); *used (optional) receives the derived parameters. */
+int ggcat_gpu_build_from_files(ggcat_gpu_ctx* ctx, const char* const* paths, uint32_t n_paths,
+                               const ggcat_gpu_build_options* options, ggcat_gpu_params* used);
 
 /* Device-memory plumbing for callers that keep inputs resident in HBM (no
  * counterpart in the reference, which has no device memory).  The pointer

@@ -31,7 +31,7 @@ def record_digests(r) -> np.ndarray:
     total = int(ln64.sum())
     # records are contiguous in `bases` in record order with off == running sum
     assert np.array_equal(off.astype(np.uint64), np.concatenate(([0], np.cumsum(ln64)[:-1])).astype(np.uint64))
-    pos = np.arange(total, dtype=np.uint64) - np.repeat(off.astype(np.uint64), ln64)
+    pos = np.arange(total, dtype=np.uint64) - np.repeat(off.astype(np.uint64), ln64.astype(np.int64))
     b = bases[:total].astype(np.uint64)
     out = np.empty((n, 2), np.uint64)
     with np.errstate(over="ignore"):

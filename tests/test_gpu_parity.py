@@ -105,3 +105,61 @@ def test_config1_medium_digest(gpu_ctx):
     wl = synth.config1(0.1)
     _say(f"case {wl.name} bytes={wl.data.size}")
     _run_both(gpu_ctx, wl.data, 31, 15, wl.buckets_log, wl.second_buckets_log, exact=False)
+
+
+@pytest.mark.parametrize("b,sb,s", [(3, 1, 2), (2, 1, 1)])
+def test_cross_unit_fixups(gpu_ctx, b, sb, s):
+    # few sub-buckets for the input size: several physical units per
+    # sub-bucket, so boundary k-mers are split across units and recombined
+    g = synth.genome(5, 60_000)
+    data = synth.reads(g, 6, 4000, 150, 0.01)
+    _say(f"case cross-unit b={b} sb={sb} s={s} bytes={data.size}")
+    _run_both(gpu_ctx, data, 31, 15, b, sb, s=s)
+    assert gpu_ctx.counters()[6] > 0  # fixup entries were produced
+
+
+@pytest.mark.parametrize("k,m,kind", [(31, 15, HASH_SEQ), (63, 21, HASH_RK128)])
+def test_hbm_table_path(gpu_ctx, k, m, kind):
+    # every work item skips the LDS table: the HBM-table kernel handles all
+    g = synth.genome(8, 200_000)
+    data = synth.reads(g, 9, 20_000, 150, 0.01)
+    _say(f"case hbm-table k={k} bytes={data.size}")
+    gpu_ctx.set_tuning(lds_max_occ=1)
+    try:
+        _run_both(gpu_ctx, data, k, m, 5, 2, s=2, kind=kind)
+        assert gpu_ctx.counters()[4] > 0
+    finally:
+        gpu_ctx.set_tuning()
+
+
+@pytest.mark.parametrize("tune", [(64, 0, 1), (4096, 0, 12)])
+def test_schedule_knobs_do_not_change_results(gpu_ctx, tune):
+    wl = synth.config1(0.01)
+    _say(f"case tuning {tune} bytes={wl.data.size}")
+    gpu_ctx.set_tuning(*tune)
+    try:
+        _run_both(gpu_ctx, wl.data, 31, 15, wl.buckets_log, wl.second_buckets_log, exact=False)
+    finally:
+        gpu_ctx.set_tuning()
+
+
+def test_build_from_files_fastq(gpu_ctx, tmp_path):
+    import gzip
+
+    from ggcat_amd.native import BuildOptions, read_files
+    g = synth.genome(12, 100_000)
+    data = synth.reads(g, 13, 20_000, 150, 0.005)
+    reads = data.tobytes().split(b"\n")[:-1]
+    path = tmp_path / "reads.fq.gz"
+    with gzip.open(path, "wb") as f:
+        for i, r in enumerate(reads):
+            f.write(b"@r%d\n%s\n+\n%s\n" % (i, r, b"I" * len(r)))
+    _say(f"case build_from_files fastq.gz reads={len(reads)}")
+    used = gpu_ctx.build_from_files([str(path)], BuildOptions(k=31, m=15, min_multiplicity=2))
+    inp = read_files([str(path)], 31)
+    assert inp.joined.tobytes() == data.tobytes()
+    assert (used.k, used.m) == (31, 15)
+    gs = gpu_ctx.solid()
+    o = O.bucketing(inp.joined, 31, 15, used.buckets_log, used.second_buckets_log, threads=4)
+    o.merge(31, used.buckets_log, used.second_buckets_log, HASH_SEQ, 2, 4)
+    assert np.array_equal(solid_array(gs), solid_array(o.solid()))

@@ -275,3 +275,23 @@ def test_compaction_preserves_solid_table():
     assert int(rec["mult"].sum()) == n_before and rec["sub"].size < n_before
     r.merge(31, 6, 2, 0, 2, 2)
     assert np.array_equal(a, solid_array(r.solid()))
+
+
+def test_record_digests_order_independent():
+    """The digest canonical form used for large parity checks is a function of
+    the record multiset only (record order does not matter, content does)."""
+    from ggcat_amd import synth
+    from tests.helpers import record_digests
+    g = synth.genome(21, 20_000)
+    data = synth.reads(g, 22, 2000, 150, 0.01)
+    r = O.bucketing(data, 31, 15, 5, 2).records()
+    d1 = record_digests(r)
+    perm = np.random.default_rng(0).permutation(r["sub"].size)
+    ln = r["length"][perm]
+    off = np.concatenate(([0], np.cumsum(ln.astype(np.uint64))[:-1])).astype(np.uint64)
+    bases = np.concatenate([r["bases"][int(r["off"][i]):int(r["off"][i]) + int(r["length"][i])] for i in perm])
+    shuf = dict(sub=r["sub"][perm], length=ln, flags=r["flags"][perm], minpos=r["minpos"][perm], off=off, bases=bases)
+    assert np.array_equal(d1, record_digests(shuf))
+    shuf["bases"] = bases.copy()
+    shuf["bases"][5] = ord("A") if shuf["bases"][5] != ord("A") else ord("C")
+    assert not np.array_equal(d1, record_digests(shuf))
