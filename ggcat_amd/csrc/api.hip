@@ -33,14 +33,15 @@ struct ggcat_gpu_ctx {
     DevBuf input, seg_bits, unit_count, unit_cursor, unit_off, unit_kmers, scan_part, slots, pool, pool_cursor,
         slot_color, color_starts, colors, error_flag;
     // merge buffers
-    DevBuf items, n_items, work, big_list, big_count, out_key, out_val, out_sub, chunk_cursor, chunk_fill_dev, fix_key,
-        fix_val, fix_sub, fix_cursor, g_klo, g_khi, g_st, g_val, g_off, g_cap;
+    DevBuf items, n_items, work, parts, redo, redo_count, out_key, out_val, out_sub, chunk_cursor, chunk_fill_dev,
+        fix_key, fix_val, fix_sub, fix_cursor;
     // state
     ggcat_gpu_params P{};
     bool have_records = false, have_solid = false;
     uint32_t n_sub = 0, n_units = 0, xb = 0, slot_words = 0, key_words = 1;
     uint64_t n_records = 0, pool_words = 0, n_solid = 0, n_input = 0, kmer_occ = 0, n_big = 0, n_fix = 0;
     uint32_t n_chunks_used = 0;  // output chunks of the last merge
+    uint32_t n_passes = 0;  // kernel passes of the last merge
     std::vector<uint32_t> chunk_fill;  // host copy of their fill counts
     uint64_t n_tail = 0;  // entries appended after the chunks (combined fixups)
     bool colored = false;
@@ -165,9 +166,9 @@ void ggcat_gpu_destroy(ggcat_gpu_ctx* ctx) {
     DevBuf* bufs[] = {&ctx->input, &ctx->seg_bits, &ctx->unit_count, &ctx->unit_cursor, &ctx->unit_off,
                       &ctx->unit_kmers, &ctx->scan_part, &ctx->slots, &ctx->pool, &ctx->pool_cursor,
                       &ctx->slot_color, &ctx->color_starts, &ctx->colors, &ctx->error_flag, &ctx->items,
-                      &ctx->n_items, &ctx->work, &ctx->big_list, &ctx->big_count, &ctx->out_key, &ctx->out_val,
-                      &ctx->out_sub, &ctx->chunk_cursor, &ctx->chunk_fill_dev, &ctx->fix_key, &ctx->fix_val, &ctx->fix_sub,
-                      &ctx->fix_cursor, &ctx->g_klo, &ctx->g_khi, &ctx->g_st, &ctx->g_val, &ctx->g_off, &ctx->g_cap};
+                      &ctx->n_items, &ctx->work, &ctx->parts, &ctx->redo, &ctx->redo_count, &ctx->out_key,
+                      &ctx->out_val, &ctx->out_sub, &ctx->chunk_cursor, &ctx->chunk_fill_dev, &ctx->fix_key,
+                      &ctx->fix_val, &ctx->fix_sub, &ctx->fix_cursor};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (int i = 0; i < 16; i++) (void)hipEventDestroy(ctx->ev[i]);
@@ -421,8 +422,7 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     ENSURE(ctx->items, (size_t)ctx->n_units * sizeof(WorkItem));
     ENSURE(ctx->n_items, 8);
     ENSURE(ctx->work, 8);
-    ENSURE(ctx->big_count, 8);
-    ENSURE(ctx->big_list, (size_t)ctx->n_units * 4);
+    ENSURE(ctx->redo_count, 8);
     ENSURE(ctx->chunk_cursor, 8);
     ENSURE(ctx->fix_cursor, 8);
     ENSURE(ctx->error_flag, 8);
@@ -437,7 +437,6 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     HIPCHK(hipMemcpyAsync(&n_items, ctx->n_items.p, 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     M.items = P_<WorkItem>(ctx->items);
-    M.n_items = n_items;
 
     const uint32_t lds_wgs = ctx->n_cu * (ctx->tune.merge_wgs_per_cu ? ctx->tune.merge_wgs_per_cu : 6);
     uint64_t n_chunks = std::max<uint64_t>((ctx->kmer_occ / 6) / OUT_CHUNK + 2 * lds_wgs + 4096 + 64,
@@ -454,8 +453,7 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
         ENSURE(ctx->fix_val, fix_cap * 4);
         ENSURE(ctx->fix_sub, fix_cap * 4);
         M.work = P_<unsigned int>(ctx->work);
-        M.big_list = P_<uint32_t>(ctx->big_list);
-        M.big_count = P_<unsigned int>(ctx->big_count);
+        M.redo_count = P_<unsigned int>(ctx->redo_count);
         M.out_key = P_<uint64_t>(ctx->out_key);
         M.out_val = P_<uint32_t>(ctx->out_val);
         M.out_sub = P_<uint32_t>(ctx->out_sub);
@@ -467,61 +465,74 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
         M.fix_sub = P_<uint32_t>(ctx->fix_sub);
         M.fix_cursor = P_<unsigned long long>(ctx->fix_cursor);
         M.fix_cap = fix_cap;
-        HIPCHK(hipMemsetAsync(ctx->work.p, 0, 8, ctx->stream));
-        HIPCHK(hipMemsetAsync(ctx->big_count.p, 0, 8, ctx->stream));
         HIPCHK(hipMemsetAsync(ctx->chunk_cursor.p, 0, 8, ctx->stream));
         HIPCHK(hipMemsetAsync(ctx->fix_cursor.p, 0, 8, ctx->stream));
         HIPCHK(hipMemsetAsync(ctx->error_flag.p, 0, 8, ctx->stream));
         HIPCHK(hipEventRecord(ctx->ev[6], ctx->stream));
-        if (n_items) {
+        HIPCHK(hipEventRecord(ctx->ev[7], ctx->stream));
+        // Pass 0: one work entry per item.  Entries whose table overflows (or
+        // items above lds_try) come back in `redo` and are processed again as
+        // 2^L hash partitions, refined x4 while a partition still overflows.
+        std::vector<PartWork> next;
+        std::vector<WorkItem> its;
+        uint32_t n_work = n_items, n_split_items = 0, n_passes_run = 0;
+        double ms_parts = 0;
+        for (uint32_t pass = 0; n_work; pass++) {
+            ENSURE(ctx->redo, (size_t)n_work * sizeof(PartWork));
+            M.redo = P_<PartWork>(ctx->redo);
+            M.n_work = n_work;
+            M.parts = pass ? P_<PartWork>(ctx->parts) : nullptr;
+            HIPCHK(hipMemsetAsync(ctx->work.p, 0, 8, ctx->stream));
+            HIPCHK(hipMemsetAsync(ctx->redo_count.p, 0, 8, ctx->stream));
+            if (pass) HIPCHK(hipEventRecord(ctx->ev[8], ctx->stream));
             hipLaunchKernelGGL(merge_lds_kernel, dim3(lds_wgs), dim3(256), lds_bytes, ctx->stream, M);
             HIPCHK(hipGetLastError());
-        }
-        HIPCHK(hipEventRecord(ctx->ev[7], ctx->stream));
-        uint32_t n_big = 0;
-        HIPCHK(hipMemcpyAsync(&n_big, ctx->big_count.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        ctx->n_big = n_big;
-        HIPCHK(hipEventRecord(ctx->ev[8], ctx->stream));
-        if (n_big) {
-            std::vector<uint32_t> list(n_big);
-            std::vector<WorkItem> its(n_items);
-            HIPCHK(hipMemcpy(list.data(), ctx->big_list.p, (size_t)n_big * 4, hipMemcpyDeviceToHost));
-            HIPCHK(hipMemcpy(its.data(), ctx->items.p, (size_t)n_items * sizeof(WorkItem), hipMemcpyDeviceToHost));
-            std::vector<uint64_t> goff(n_big);
-            std::vector<uint32_t> gcap(n_big);
-            uint64_t tot = 0;
-            for (uint32_t i = 0; i < n_big; i++) {
-                uint64_t c = 1024;
-                while (c < 2ull * its[list[i]].occ) c <<= 1;
-                goff[i] = tot;
-                gcap[i] = (uint32_t)c;
-                tot += c;
+            n_passes_run++;
+            HIPCHK(hipEventRecord(ctx->ev[pass ? 9 : 7], ctx->stream));
+            uint32_t n_redo = 0;
+            HIPCHK(hipMemcpyAsync(&n_redo, ctx->redo_count.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            if (pass) {
+                float tp;
+                HIPCHK(hipEventElapsedTime(&tp, ctx->ev[8], ctx->ev[9]));
+                ms_parts += tp;
             }
-            ENSURE(ctx->g_klo, tot * 8);
-            ENSURE(ctx->g_val, tot * 4);
-            ENSURE(ctx->g_off, (size_t)n_big * 8);
-            ENSURE(ctx->g_cap, (size_t)n_big * 4);
-            if (key_words == 2) {
-                ENSURE(ctx->g_khi, tot * 8);
-                ENSURE(ctx->g_st, tot * 4);
+            if (n_redo == 0) break;
+            std::vector<PartWork> redo(n_redo);
+            HIPCHK(hipMemcpy(redo.data(), ctx->redo.p, (size_t)n_redo * sizeof(PartWork), hipMemcpyDeviceToHost));
+            if (its.empty()) {
+                its.resize(n_items);
+                HIPCHK(hipMemcpy(its.data(), ctx->items.p, (size_t)n_items * sizeof(WorkItem), hipMemcpyDeviceToHost));
             }
-            HIPCHK(hipMemcpy(ctx->g_off.p, goff.data(), (size_t)n_big * 8, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(ctx->g_cap.p, gcap.data(), (size_t)n_big * 4, hipMemcpyHostToDevice));
-            hipLaunchKernelGGL(fill_u64_kernel, dim3(1024), dim3(256), 0, ctx->stream, P_<uint64_t>(ctx->g_klo), EMPTY64, tot);
-            hipLaunchKernelGGL(fill_u32_kernel, dim3(1024), dim3(256), 0, ctx->stream, P_<uint32_t>(ctx->g_val), 0u, tot);
-            if (key_words == 2)
-                hipLaunchKernelGGL(fill_u32_kernel, dim3(1024), dim3(256), 0, ctx->stream, P_<uint32_t>(ctx->g_st), 0u, tot);
-            M.g_klo = P_<uint64_t>(ctx->g_klo);
-            M.g_khi = key_words == 2 ? P_<uint64_t>(ctx->g_khi) : nullptr;
-            M.g_st = key_words == 2 ? P_<uint32_t>(ctx->g_st) : nullptr;
-            M.g_val = P_<uint32_t>(ctx->g_val);
-            M.g_off = P_<uint64_t>(ctx->g_off);
-            M.g_cap = P_<uint32_t>(ctx->g_cap);
-            hipLaunchKernelGGL(merge_global_kernel, dim3(std::min<uint32_t>(n_big, 4096)), dim3(256), 0, ctx->stream, M, n_big);
-            HIPCHK(hipGetLastError());
+            next.clear();
+            for (const PartWork& w : redo) {
+                if (w.item >= n_items) return fail(ctx, GGCAT_GPU_EHIP, "corrupt merge work list");
+                uint32_t L, base, stride;
+                if (w.log_parts == 0) {
+                    // about half a table of occurrences per part
+                    const uint64_t per = ctx->tune.part_occ ? ctx->tune.part_occ : M.lds_cap / 2;
+                    const uint64_t want = (its[w.item].occ + per - 1) / per;
+                    L = 1;
+                    while ((1ull << L) < want) L++;
+                    base = 0;
+                    stride = 1;
+                    n_split_items++;
+                } else {
+                    L = w.log_parts + 2;
+                    base = w.part;
+                    stride = 1u << w.log_parts;
+                }
+                if (L > MAX_LOG_PARTS)
+                    return fail(ctx, GGCAT_GPU_EHIP, "k-mer table overflow: a work item needs more than 2^16 parts");
+                const uint32_t n_new = w.log_parts == 0 ? (1u << L) : 4u;
+                for (uint32_t j = 0; j < n_new; j++) next.push_back(PartWork{w.item, base + j * stride, L, 0});
+            }
+            n_work = (uint32_t)next.size();
+            ENSURE(ctx->parts, (size_t)n_work * sizeof(PartWork));
+            HIPCHK(hipMemcpy(ctx->parts.p, next.data(), (size_t)n_work * sizeof(PartWork), hipMemcpyHostToDevice));
         }
-        HIPCHK(hipEventRecord(ctx->ev[9], ctx->stream));
+        ctx->n_big = n_split_items;
+        ctx->n_passes = n_passes_run;
         uint64_t n_fix = 0;
         uint32_t used_chunks = 0, errf = 0;
         HIPCHK(hipMemcpyAsync(&used_chunks, ctx->chunk_cursor.p, 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -533,11 +544,10 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
         float t;
         HIPCHK(hipEventElapsedTime(&t, ctx->ev[6], ctx->ev[7]));
         ctx->ms[3] = t;
-        HIPCHK(hipEventElapsedTime(&t, ctx->ev[8], ctx->ev[9]));
-        ctx->ms[4] = n_big ? t : 0.0;
+        ctx->ms[4] = ms_parts;
         if (used_chunks > n_chunks || n_fix > fix_cap) {
             if (attempt == 1) return fail(ctx, GGCAT_GPU_ENOMEM, "solid output overflow");
-            n_chunks = std::max<uint64_t>(n_chunks, (uint64_t)used_chunks + 64);
+            n_chunks = std::max<uint64_t>(n_chunks, (uint64_t)used_chunks + 2 * lds_wgs);
             fix_cap = std::max(fix_cap, n_fix + 1024);
             continue;
         }
@@ -654,9 +664,9 @@ int ggcat_gpu_timings(ggcat_gpu_ctx* ctx, double* ms, uint32_t n) {
 
 int ggcat_gpu_counters(ggcat_gpu_ctx* ctx, uint64_t* out, uint32_t n) {
     if (!ctx || !out) return GGCAT_GPU_EINVAL;
-    uint64_t v[7] = {ctx->n_input, ctx->n_records, ctx->kmer_occ + ctx->n_records * (ctx->P.k - 1), ctx->kmer_occ,
-                     ctx->n_big, ctx->n_solid, ctx->n_fix};
-    for (uint32_t i = 0; i < n && i < 7; i++) out[i] = v[i];
+    uint64_t v[8] = {ctx->n_input, ctx->n_records, ctx->kmer_occ + ctx->n_records * (ctx->P.k - 1), ctx->kmer_occ,
+                     ctx->n_big, ctx->n_solid, ctx->n_fix, ctx->n_passes};
+    for (uint32_t i = 0; i < n && i < 8; i++) out[i] = v[i];
     return GGCAT_GPU_OK;
 }
 

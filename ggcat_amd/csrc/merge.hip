@@ -35,6 +35,17 @@ struct WorkItem {
     uint32_t sub;  // sub-bucket
 };
 
+// One pass over a work item restricted to the k-mers whose partition hash
+// falls in `part` of 2^log_parts.  Items whose distinct k-mers do not fit one
+// LDS table are processed as several such passes.
+struct PartWork {
+    uint32_t item;
+    uint32_t part;
+    uint32_t log_parts;
+    uint32_t pad;
+};
+constexpr uint32_t MAX_LOG_PARTS = 16;
+
 struct MergeArgs {
     const uint32_t* slots;
     uint32_t slot_words;
@@ -45,12 +56,13 @@ struct MergeArgs {
     uint32_t kind;  // 0 = canonical seq-hash, 1 = canonical rabin-karp128
     uint32_t key_words;  // 1 (u64 keys) or 2 (u128 keys)
     uint32_t lds_cap;  // table entries in LDS (power of two)
-    uint32_t lds_try;  // items with more occurrences go straight to HBM tables
+    uint32_t lds_try;  // first pass: items with more occurrences go straight to partitioned passes
     const WorkItem* items;
-    uint32_t n_items;
-    unsigned int* work;
-    uint32_t* big_list;  // indices into items
-    unsigned int* big_count;
+    const PartWork* parts;  // null: first pass, one work entry per item
+    uint32_t n_work;  // entries of this pass
+    unsigned int* work;  // work-stealing cursor
+    PartWork* redo;  // work entries whose table overflowed (at most n_work)
+    unsigned int* redo_count;
     uint64_t* out_key;  // key_words per entry, chunked (see OUT_CHUNK)
     uint32_t* out_val;  // multiplicity | flags << 30
     uint32_t* out_sub;
@@ -62,15 +74,8 @@ struct MergeArgs {
     uint32_t* fix_sub;
     unsigned long long* fix_cursor;
     uint64_t fix_cap;
-    // HBM tables for big items
-    uint64_t* g_klo;
-    uint64_t* g_khi;
-    uint32_t* g_st;
-    uint32_t* g_val;
-    const uint64_t* g_off;  // per big-list entry: table offset
-    const uint32_t* g_cap;  // per big-list entry: table capacity (power of two)
     uint64_t pool_words;  // valid words in the long-record pool
-    uint32_t* error_flag;  // bit 0: table full, bit 1: malformed slot
+    uint32_t* error_flag;  // bit 1: malformed slot
     // rabin-karp128 constants (crates/hashes/src/cn_rkhash.rs:65-78)
     u128_t rk_mult, rk_mult_inv, rk_rmmult;
     u128_t rk_fwd[4], rk_bkw[4];
@@ -318,19 +323,25 @@ GG_DEV void emit_solid(const MergeArgs& A, uint32_t sub, KP klo, KP khi, SP st, 
     }
 }
 
-// Inserts every occurrence of the item's records.  An insert that needs more
-// than `probe_limit` probes sets *overflow and insertion stops (the item is
-// then redone with a larger table).
+// Inserts every occurrence of the item's records whose k-mer lies in the
+// work entry's partition (bits 16.. of the table hash; the slot index uses
+// the low bits).  An insert that needs more than `probe_limit` probes sets
+// *overflow and insertion stops (the entry is then redone as finer parts).
 template <class KP, class SP, class VP>
-GG_DEV void insert_item(const MergeArgs& A, const WorkItem& it, KP klo, KP khi, SP st, VP vals, uint32_t mask,
-                        uint32_t probe_limit, uint32_t* overflow) {
+GG_DEV void insert_item(const MergeArgs& A, const WorkItem& it, const PartWork& w, KP klo, KP khi, SP st, VP vals,
+                        uint32_t mask, uint32_t probe_limit, uint32_t* overflow) {
     const uint64_t r0 = A.unit_off[it.u0], r1 = A.unit_off[it.u1];
     const uint32_t SW = A.slot_words;
+    const uint32_t part_mask = (1u << w.log_parts) - 1;
     for (uint64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
         const uint32_t* slot = A.slots + r * SW;
         const uint32_t meta = slot[0];
         for_each_kmer(A, slot, [&](uint64_t lo, uint64_t hi, bool is_fwd, uint32_t idx, uint32_t last) {
             if (*(volatile uint32_t*)overflow) return;
+            if (part_mask) {
+                const uint32_t h = (A.key_words == 1) ? tab_hash64(lo) : hash128(lo, hi);
+                if (((h >> 16) & part_mask) != w.part) return;
+            }
             uint32_t bits = occurrence_bits(meta, is_fwd, idx, last);
             uint32_t res = (A.key_words == 1) ? tab_insert64(klo, vals, mask, lo, bits, probe_limit)
                                               : tab_insert128(klo, khi, st, vals, mask, lo, hi, bits, probe_limit);
@@ -360,14 +371,16 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
         __syncthreads();
         const uint32_t e = s_item;
         __syncthreads();
-        if (e >= A.n_items) break;
-        const WorkItem it = A.items[e];
-        if (it.occ > A.lds_try) {
-            if (threadIdx.x == 0) A.big_list[atomicAdd(A.big_count, 1u)] = e;
+        if (e >= A.n_work) break;
+        const PartWork w = A.parts ? A.parts[e] : PartWork{e, 0u, 0u, 0u};
+        const WorkItem it = A.items[w.item];
+        if (!A.parts && it.occ > A.lds_try) {
+            if (threadIdx.x == 0) A.redo[atomicAdd(A.redo_count, 1u)] = w;
             continue;
         }
         uint32_t cap = 256;
         while (cap < 2 * it.occ && cap < cap_max) cap <<= 1;
+        if (w.log_parts) cap = cap_max;
         const uint32_t mask = cap - 1;
         for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) {
             klo[i] = EMPTY64;
@@ -375,43 +388,11 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
             if (A.key_words == 2) st[i] = 0;
         }
         __syncthreads();
-        insert_item(A, it, klo, khi, st, vals, mask, 96u, &s_overflow);
+        insert_item(A, it, w, klo, khi, st, vals, mask, 96u, &s_overflow);
         __syncthreads();
         if (s_overflow) {
-            // more distinct k-mers than the LDS table holds: redo in HBM
-            if (threadIdx.x == 0) A.big_list[atomicAdd(A.big_count, 1u)] = e;
-        } else {
-            emit_solid(A, it.sub, klo, khi, st, vals, cap, es);
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) close_chunk(A, es);
-}
-
-// Big items: same algorithm with the table in HBM, sized to twice the
-// occurrences (one workgroup per item, tables pre-cleared by the host).
-__global__ __launch_bounds__(256) void merge_global_kernel(MergeArgs A, uint32_t n_big) {
-    __shared__ uint32_t s_overflow;
-    __shared__ EmitState es;
-    if (threadIdx.x == 0) {
-        es.chunk = ~0u;
-        es.fill = 0;
-    }
-    for (uint32_t e = blockIdx.x; e < n_big; e += gridDim.x) {
-        const WorkItem it = A.items[A.big_list[e]];
-        const uint64_t off = A.g_off[e];
-        const uint32_t cap = A.g_cap[e];
-        uint64_t* klo = A.g_klo + off;
-        uint64_t* khi = A.g_khi ? A.g_khi + off : nullptr;
-        uint32_t* st = A.g_st ? A.g_st + off : nullptr;
-        uint32_t* vals = A.g_val + off;
-        if (threadIdx.x == 0) s_overflow = 0;
-        __syncthreads();
-        insert_item(A, it, klo, khi, st, vals, cap - 1, cap, &s_overflow);
-        __syncthreads();
-        __threadfence();
-        if (s_overflow) {
-            if (threadIdx.x == 0) atomicOr(A.error_flag, 1u);
+            // more distinct k-mers than the table holds: redone as finer parts
+            if (threadIdx.x == 0) A.redo[atomicAdd(A.redo_count, 1u)] = w;
         } else {
             emit_solid(A, it.sub, klo, khi, st, vals, cap, es);
         }
@@ -442,13 +423,6 @@ __global__ void make_items_kernel(const uint32_t* unit_kmers, uint32_t n_sub, ui
         acc += o;
     }
     if (acc > 0) items[atomicAdd(n_items, 1u)] = WorkItem{start, u_end, acc, s};
-}
-
-__global__ void fill_u64_kernel(uint64_t* p, uint64_t v, uint64_t n) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
-}
-__global__ void fill_u32_kernel(uint32_t* p, uint32_t v, uint64_t n) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
 }  // namespace gg

@@ -62,6 +62,7 @@ class Tuning(C.Structure):
         ("item_occ", C.c_uint32),
         ("lds_max_occ", C.c_uint32),
         ("merge_wgs_per_cu", C.c_uint32),
+        ("part_occ", C.c_uint32),
     ]
 
 
@@ -256,8 +257,9 @@ class GpuContext:
                     "ggcat_gpu_build_from_files")
         return used
 
-    def set_tuning(self, item_occ: int = 0, lds_max_occ: int = 0, merge_wgs_per_cu: int = 0) -> None:
-        t = Tuning(item_occ, lds_max_occ, merge_wgs_per_cu)
+    def set_tuning(self, item_occ: int = 0, lds_max_occ: int = 0, merge_wgs_per_cu: int = 0,
+                   part_occ: int = 0) -> None:
+        t = Tuning(item_occ, lds_max_occ, merge_wgs_per_cu, part_occ)
         self._check(self.lib.ggcat_gpu_set_tuning(self.h, C.byref(t)), "ggcat_gpu_set_tuning")
 
     def records(self) -> Records:
@@ -301,6 +303,6 @@ class GpuContext:
         return out
 
     def counters(self) -> np.ndarray:
-        out = np.zeros(7, np.uint64)
-        self._check(self.lib.ggcat_gpu_counters(self.h, _ptr(out, _u64p), 7), "counters")
+        out = np.zeros(8, np.uint64)
+        self._check(self.lib.ggcat_gpu_counters(self.h, _ptr(out, _u64p), 8), "counters")
         return out

@@ -118,27 +118,33 @@ int ggcat_gpu_solid_copy(ggcat_gpu_ctx* ctx, uint64_t* keys, uint32_t* multiplic
  *   item_occ          target k-mer occurrences per work item (runs of
  *                     physical units of one sub-bucket); default: half the
  *                     LDS table (1024 for 64-bit keys, 512 for 128-bit)
- *   lds_max_occ       items with more occurrences skip the LDS table and
- *                     use an HBM table; default 16x the LDS table
- *   merge_wgs_per_cu  persistent merge workgroups per CU; default 6 */
+ *   lds_max_occ       items with more occurrences skip the whole-item pass
+ *                     and go straight to hash-partitioned passes; default
+ *                     16x the LDS table
+ *   merge_wgs_per_cu  persistent merge workgroups per CU; default 6
+ *   part_occ          target occurrences per partition of an item that is
+ *                     split into hash-partitioned passes; default half the
+ *                     LDS table (a partition that still overflows is split
+ *                     4 ways again) */
 typedef struct {
     uint32_t item_occ;
     uint32_t lds_max_occ;
     uint32_t merge_wgs_per_cu;
+    uint32_t part_occ;
 } ggcat_gpu_tuning;
 int ggcat_gpu_set_tuning(ggcat_gpu_ctx* ctx, const ggcat_gpu_tuning* tuning);
 
 /* Device-side timings of the last calls, in milliseconds:
  * [0] bucketing pass A, [1] scan, [2] bucketing pass B,
- * [3] merge (LDS kernel), [4] merge (HBM-table kernel),
+ * [3] merge (whole-item pass), [4] merge (partitioned passes),
  * [5] whole bucketing call, [6] whole merge call. */
 int ggcat_gpu_timings(ggcat_gpu_ctx* ctx, double* ms, uint32_t n);
 
 /* Counters of the last calls: [0] input bytes, [1] records,
  * [2] sum of record lengths (bases), [3] k-mer occurrences,
- * [4] work items merged with an HBM table, [5] solid entries,
+ * [4] work items split into hash-partitioned passes, [5] solid entries,
  * [6] k-mers shared between two units of a sub-bucket (combined on the
- * host after the merge). */
+ * host after the merge), [7] merge kernel passes (1 + partition rounds). */
 int ggcat_gpu_counters(ggcat_gpu_ctx* ctx, uint64_t* out, uint32_t n);
 
 /* Reads FASTA/FASTQ files (optionally gzip-compressed) into the joined layout.
