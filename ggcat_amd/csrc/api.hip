@@ -34,7 +34,7 @@ struct ggcat_gpu_ctx {
         slot_color, color_starts, colors, error_flag;
     // merge buffers
     DevBuf items, n_items, work, parts, redo, redo_count, out_key, out_val, out_sub, chunk_cursor, chunk_fill_dev,
-        fix_key, fix_val, fix_sub, fix_cursor;
+        fix_key, fix_val, fix_sub, fix_cursor, out_colors, fix_colors;
     // state
     ggcat_gpu_params P{};
     bool have_records = false, have_solid = false;
@@ -45,6 +45,7 @@ struct ggcat_gpu_ctx {
     std::vector<uint32_t> chunk_fill;  // host copy of their fill counts
     uint64_t n_tail = 0;  // entries appended after the chunks (combined fixups)
     bool colored = false;
+    uint32_t color_words = 0;  // bitmap words per solid entry of a colored build
     double ms[8] = {0};
     ggcat_gpu_tuning tune{};  // zero fields select the defaults
 };
@@ -124,6 +125,8 @@ void nthash_tables(uint32_t m, uint64_t* gtab) {
 u128_t mk128(uint64_t hi, uint64_t lo) { return ((u128_t)hi << 64) | lo; }
 
 const uint32_t TILE_T = 1024;
+// Colored merges keep one bit per color in each table entry.
+const uint32_t MAX_COLORS = 1024;
 
 // Device-wide exclusive scan counts[0..n) -> off[0..n].
 int device_scan(ggcat_gpu_ctx* ctx, const uint32_t* counts, uint64_t* off, uint32_t n) {
@@ -168,7 +171,7 @@ void ggcat_gpu_destroy(ggcat_gpu_ctx* ctx) {
                       &ctx->slot_color, &ctx->color_starts, &ctx->colors, &ctx->error_flag, &ctx->items,
                       &ctx->n_items, &ctx->work, &ctx->parts, &ctx->redo, &ctx->redo_count, &ctx->out_key,
                       &ctx->out_val, &ctx->out_sub, &ctx->chunk_cursor, &ctx->chunk_fill_dev, &ctx->fix_key,
-                      &ctx->fix_val, &ctx->fix_sub, &ctx->fix_cursor};
+                      &ctx->fix_val, &ctx->fix_sub, &ctx->fix_cursor, &ctx->out_colors, &ctx->fix_colors};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (int i = 0; i < 16; i++) (void)hipEventDestroy(ctx->ev[i]);
@@ -214,6 +217,14 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     ctx->slot_words = slot_words;
     ctx->n_input = n_bytes;
     ctx->colored = n_colors > 0;
+    ctx->color_words = 0;
+    if (n_colors) {
+        uint32_t max_color = 0;
+        for (uint32_t i = 0; i < n_colors; i++) max_color = std::max(max_color, colors[i]);
+        if (max_color >= MAX_COLORS)
+            return fail(ctx, GGCAT_GPU_EINVAL, "color ids must be < " + std::to_string(MAX_COLORS));
+        ctx->color_words = max_color / 32 + 1;
+    }
 
     const uint8_t* d_in = data;
     if (!input_on_device) {
@@ -399,10 +410,15 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     M.min_mult = p.min_multiplicity;
     M.kind = p.hash_kind;
     M.key_words = key_words;
+    const uint32_t W = ctx->colored ? ctx->color_words : 0;
+    const size_t entry_bytes = 12 * key_words + 4 * W;
     M.lds_cap = key_words == 1 ? 2048 : 1024;
+    while (M.lds_cap > 256 && M.lds_cap * entry_bytes > 48 * 1024) M.lds_cap >>= 1;
+    M.color_words = W;
+    M.slot_color = W ? P_<uint32_t>(ctx->slot_color) : nullptr;
     M.lds_try = ctx->tune.lds_max_occ ? ctx->tune.lds_max_occ : M.lds_cap * 16;
     const uint32_t item_occ = ctx->tune.item_occ ? ctx->tune.item_occ : M.lds_cap / 2;
-    const size_t lds_bytes = (size_t)M.lds_cap * (8 * key_words + 4 + (key_words == 2 ? 4 : 0)) + 64;
+    const size_t lds_bytes = (size_t)M.lds_cap * entry_bytes + 64;
     M.pool_words = ctx->pool_words;
     // rabin-karp128 constants (crates/hashes/src/cn_rkhash.rs:65-78)
     M.rk_mult = mk128(0x3eb9402f3e733993ULL, 0xadd64d3ca00e1b6bULL);
@@ -452,6 +468,12 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
         ENSURE(ctx->fix_key, fix_cap * 8 * key_words);
         ENSURE(ctx->fix_val, fix_cap * 4);
         ENSURE(ctx->fix_sub, fix_cap * 4);
+        if (W) {
+            ENSURE(ctx->out_colors, out_cap * 4 * W);
+            ENSURE(ctx->fix_colors, fix_cap * 4 * W);
+        }
+        M.out_colors = W ? P_<uint32_t>(ctx->out_colors) : nullptr;
+        M.fix_colors = W ? P_<uint32_t>(ctx->fix_colors) : nullptr;
         M.work = P_<unsigned int>(ctx->work);
         M.redo_count = P_<unsigned int>(ctx->redo_count);
         M.out_key = P_<uint64_t>(ctx->out_key);
@@ -563,7 +585,8 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
         if (n_fix) {
             const uint32_t kw = key_words;
             std::vector<uint64_t> fk(n_fix * kw);
-            std::vector<uint32_t> fv(n_fix), fs(n_fix);
+            std::vector<uint32_t> fv(n_fix), fs(n_fix), fc(n_fix * W);
+            if (W) HIPCHK(hipMemcpy(fc.data(), ctx->fix_colors.p, fc.size() * 4, hipMemcpyDeviceToHost));
             HIPCHK(hipMemcpy(fk.data(), ctx->fix_key.p, fk.size() * 8, hipMemcpyDeviceToHost));
             HIPCHK(hipMemcpy(fv.data(), ctx->fix_val.p, n_fix * 4, hipMemcpyDeviceToHost));
             HIPCHK(hipMemcpy(fs.data(), ctx->fix_sub.p, n_fix * 4, hipMemcpyDeviceToHost));
@@ -574,12 +597,18 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
             };
             std::sort(ord.begin(), ord.end(), [&](uint32_t a2, uint32_t b2) { return key_of(a2) < key_of(b2); });
             std::vector<uint64_t> ok_;
-            std::vector<uint32_t> ov, os;
+            std::vector<uint32_t> ov, os, oc;
             for (uint64_t i = 0; i < n_fix;) {
                 uint64_t j = i + 1;
-                uint32_t v = fv[ord[i]];
-                while (j < n_fix && key_of(ord[j]) == key_of(ord[i])) v |= fv[ord[j++]] & 0xC0000000u;
-                uint32_t a2 = ord[i];
+                const uint32_t a2 = ord[i];
+                uint32_t v = fv[a2];
+                const size_t c0 = oc.size();
+                for (uint32_t q = 0; q < W; q++) oc.push_back(fc[(size_t)a2 * W + q]);
+                while (j < n_fix && key_of(ord[j]) == key_of(a2)) {
+                    const uint32_t b2 = ord[j++];
+                    v |= fv[b2] & 0xC0000000u;
+                    for (uint32_t q = 0; q < W; q++) oc[c0 + q] |= fc[(size_t)b2 * W + q];
+                }
                 for (uint32_t q = 0; q < kw; q++) ok_.push_back(fk[kw * a2 + q]);
                 ov.push_back(v);
                 os.push_back(fs[a2]);
@@ -594,6 +623,9 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
             HIPCHK(hipMemcpy(P_<uint64_t>(ctx->out_key) + tail0 * kw, ok_.data(), n_add * kw * 8, hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(P_<uint32_t>(ctx->out_val) + tail0, ov.data(), n_add * 4, hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(P_<uint32_t>(ctx->out_sub) + tail0, os.data(), n_add * 4, hipMemcpyHostToDevice));
+            if (W)
+                HIPCHK(hipMemcpy(P_<uint32_t>(ctx->out_colors) + tail0 * W, oc.data(), n_add * 4 * W,
+                                 hipMemcpyHostToDevice));
         }
         ctx->n_chunks_used = used_chunks;
         ctx->n_tail = n_add;
@@ -646,6 +678,28 @@ int ggcat_gpu_solid_copy(ggcat_gpu_ctx* ctx, uint64_t* keys, uint32_t* multiplic
         }
         o += r.second;
     }
+    return GGCAT_GPU_OK;
+}
+
+int ggcat_gpu_solid_colors(ggcat_gpu_ctx* ctx, uint32_t* color_words, uint32_t* bitmaps) {
+    if (!ctx) return GGCAT_GPU_EINVAL;
+    if (!ctx->have_solid) return fail(ctx, GGCAT_GPU_ESTATE, "no merge result");
+    if (!ctx->colored) return fail(ctx, GGCAT_GPU_ESTATE, "the last build was not colored");
+    const uint32_t W = ctx->color_words;
+    if (color_words) *color_words = W;
+    if (!bitmaps) return GGCAT_GPU_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    uint64_t o = 0;
+    auto copy = [&](uint64_t first, uint64_t count) -> int {
+        HIPCHK(hipMemcpy(bitmaps + o * W, P_<uint32_t>(ctx->out_colors) + first * W, count * 4 * W,
+                         hipMemcpyDeviceToHost));
+        o += count;
+        return GGCAT_GPU_OK;
+    };
+    for (uint32_t c = 0; c < ctx->n_chunks_used; c++)
+        if (ctx->chunk_fill[c] && copy((uint64_t)c * OUT_CHUNK, ctx->chunk_fill[c]) != GGCAT_GPU_OK) return GGCAT_GPU_EHIP;
+    if (ctx->n_tail && copy((uint64_t)ctx->n_chunks_used * OUT_CHUNK, ctx->n_tail) != GGCAT_GPU_OK)
+        return GGCAT_GPU_EHIP;
     return GGCAT_GPU_OK;
 }
 

@@ -76,6 +76,12 @@ struct MergeArgs {
     uint64_t fix_cap;
     uint64_t pool_words;  // valid words in the long-record pool
     uint32_t* error_flag;  // bit 1: malformed slot
+    // colored builds: per-slot color, color bitmap words per entry (0 when
+    // uncolored), and the bitmaps of the output and fixup entries
+    const uint32_t* slot_color;
+    uint32_t color_words;
+    uint32_t* out_colors;
+    uint32_t* fix_colors;
     // rabin-karp128 constants (crates/hashes/src/cn_rkhash.rs:65-78)
     u128_t rk_mult, rk_mult_inv, rk_rmmult;
     u128_t rk_fwd[4], rk_bkw[4];
@@ -87,7 +93,8 @@ GG_DEV uint32_t hash128(uint64_t lo, uint64_t hi) { return tab_hash64(lo ^ (hi *
 // the key was present, 2 when no slot was found within min(mask+1,
 // probe_limit) probes.
 template <class KP, class VP>
-GG_DEV uint32_t tab_insert64(KP keys, VP vals, uint32_t mask, uint64_t key, uint32_t bits, uint32_t probe_limit) {
+GG_DEV uint32_t tab_insert64(KP keys, VP vals, uint32_t mask, uint64_t key, uint32_t bits, uint32_t probe_limit,
+                             uint32_t& slot) {
     uint32_t i = tab_hash64(key) & mask;
     uint32_t claimed = 0;
     for (uint32_t probes = 0;; probes++) {
@@ -107,6 +114,7 @@ GG_DEV uint32_t tab_insert64(KP keys, VP vals, uint32_t mask, uint64_t key, uint
     }
     atomicAdd((uint32_t*)&vals[i], bits & CNT_MASK);
     if (bits & ~CNT_MASK) atomicOr((uint32_t*)&vals[i], bits & ~CNT_MASK);
+    slot = i;
     return claimed;
 }
 
@@ -116,7 +124,7 @@ GG_DEV uint32_t tab_insert64(KP keys, VP vals, uint32_t mask, uint64_t key, uint
 // wait on a slot in state 1 is bounded.
 template <class KP, class SP, class VP>
 GG_DEV uint32_t tab_insert128(KP klo, KP khi, SP st, VP vals, uint32_t mask, uint64_t lo, uint64_t hi,
-                              uint32_t bits, uint32_t probe_limit) {
+                              uint32_t bits, uint32_t probe_limit, uint32_t& slot) {
     uint32_t i = hash128(lo, hi) & mask;
     uint32_t claimed = 0, probes = 0, spins = 0;
     bool done = false;
@@ -146,6 +154,7 @@ GG_DEV uint32_t tab_insert128(KP klo, KP khi, SP st, VP vals, uint32_t mask, uin
     }
     atomicAdd((uint32_t*)&vals[i], bits & CNT_MASK);
     if (bits & ~CNT_MASK) atomicOr((uint32_t*)&vals[i], bits & ~CNT_MASK);
+    slot = i;
     return claimed;
 }
 
@@ -250,8 +259,9 @@ GG_DEV void close_chunk(const MergeArgs& A, EmitState& es) {
 // Emit solid entries of a table of `cap` slots; cross-marked entries holding
 // only one flag bit go to the fixup list.
 template <class KP, class SP, class VP>
-GG_DEV void emit_solid(const MergeArgs& A, uint32_t sub, KP klo, KP khi, SP st, VP vals, uint32_t cap,
-                       EmitState& es) {
+GG_DEV void emit_solid(const MergeArgs& A, uint32_t sub, KP klo, KP khi, SP st, VP vals, const uint32_t* cbits,
+                       uint32_t cap, EmitState& es) {
+    const uint32_t W = A.color_words;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
     for (uint32_t i0 = 0; i0 < cap; i0 += blockDim.x) {
         uint32_t i = i0 + threadIdx.x;
@@ -300,6 +310,7 @@ GG_DEV void emit_solid(const MergeArgs& A, uint32_t sub, KP klo, KP khi, SP st, 
             }
             A.out_val[pos] = outv;
             A.out_sub[pos] = sub;
+            for (uint32_t j = 0; j < W; j++) A.out_colors[pos * W + j] = cbits[i * W + j];
         }
         if (balf) {
             unsigned long long basef = 0;
@@ -316,6 +327,7 @@ GG_DEV void emit_solid(const MergeArgs& A, uint32_t sub, KP klo, KP khi, SP st, 
                     }
                     A.fix_val[pos] = outv;
                     A.fix_sub[pos] = sub;
+                    for (uint32_t j = 0; j < W; j++) A.fix_colors[pos * W + j] = cbits[i * W + j];
                 }
             }
         }
@@ -329,23 +341,29 @@ GG_DEV void emit_solid(const MergeArgs& A, uint32_t sub, KP klo, KP khi, SP st, 
 // *overflow and insertion stops (the entry is then redone as finer parts).
 template <class KP, class SP, class VP>
 GG_DEV void insert_item(const MergeArgs& A, const WorkItem& it, const PartWork& w, KP klo, KP khi, SP st, VP vals,
-                        uint32_t mask, uint32_t probe_limit, uint32_t* overflow) {
+                        uint32_t* cbits, uint32_t mask, uint32_t probe_limit, uint32_t* overflow) {
+    const uint32_t W = A.color_words;
     const uint64_t r0 = A.unit_off[it.u0], r1 = A.unit_off[it.u1];
     const uint32_t SW = A.slot_words;
     const uint32_t part_mask = (1u << w.log_parts) - 1;
     for (uint64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
         const uint32_t* slot = A.slots + r * SW;
         const uint32_t meta = slot[0];
+        const uint32_t color = W ? A.slot_color[r] : 0u;
         for_each_kmer(A, slot, [&](uint64_t lo, uint64_t hi, bool is_fwd, uint32_t idx, uint32_t last) {
             if (*(volatile uint32_t*)overflow) return;
             if (part_mask) {
                 const uint32_t h = (A.key_words == 1) ? tab_hash64(lo) : hash128(lo, hi);
                 if (((h >> 16) & part_mask) != w.part) return;
             }
-            uint32_t bits = occurrence_bits(meta, is_fwd, idx, last);
-            uint32_t res = (A.key_words == 1) ? tab_insert64(klo, vals, mask, lo, bits, probe_limit)
-                                              : tab_insert128(klo, khi, st, vals, mask, lo, hi, bits, probe_limit);
-            if (res == 2) atomicOr(overflow, 1u);
+            uint32_t bits = occurrence_bits(meta, is_fwd, idx, last), at = 0;
+            uint32_t res = (A.key_words == 1)
+                               ? tab_insert64(klo, vals, mask, lo, bits, probe_limit, at)
+                               : tab_insert128(klo, khi, st, vals, mask, lo, hi, bits, probe_limit, at);
+            if (res == 2)
+                atomicOr(overflow, 1u);
+            else if (W)
+                atomicOr(&cbits[at * W + (color >> 5)], 1u << (color & 31));
         });
     }
 }
@@ -358,7 +376,9 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
     uint64_t* klo = reinterpret_cast<uint64_t*>(lds);
     uint64_t* khi = klo + cap_max;  // only used for 128-bit keys
     uint32_t* vals = reinterpret_cast<uint32_t*>(lds + (size_t)cap_max * 8 * A.key_words);
-    uint32_t* st = vals + cap_max;
+    uint32_t* st = vals + cap_max;  // only used for 128-bit keys
+    uint32_t* cbits = vals + (size_t)cap_max * A.key_words;  // color_words per entry (colored builds)
+    const uint32_t W = A.color_words;
     if (threadIdx.x == 0) {
         es.chunk = ~0u;
         es.fill = 0;
@@ -387,14 +407,15 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
             vals[i] = 0;
             if (A.key_words == 2) st[i] = 0;
         }
+        for (uint32_t i = threadIdx.x; i < cap * W; i += blockDim.x) cbits[i] = 0;
         __syncthreads();
-        insert_item(A, it, w, klo, khi, st, vals, mask, 96u, &s_overflow);
+        insert_item(A, it, w, klo, khi, st, vals, cbits, mask, 96u, &s_overflow);
         __syncthreads();
         if (s_overflow) {
             // more distinct k-mers than the table holds: redone as finer parts
             if (threadIdx.x == 0) A.redo[atomicAdd(A.redo_count, 1u)] = w;
         } else {
-            emit_solid(A, it.sub, klo, khi, st, vals, cap, es);
+            emit_solid(A, it.sub, klo, khi, st, vals, cbits, cap, es);
         }
         __syncthreads();
     }

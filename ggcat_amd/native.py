@@ -105,6 +105,7 @@ def gpu_lib(path: str = GPU_LIB) -> C.CDLL:
     lib.ggcat_gpu_free_host.argtypes = [V]
     lib.ggcat_gpu_free_host.restype = None
     lib.ggcat_gpu_set_tuning.argtypes = [V, C.POINTER(Tuning)]
+    lib.ggcat_gpu_solid_colors.argtypes = [V, _u32p, _u32p]
     lib.ggcat_gpu_plan_build.argtypes = [C.POINTER(BuildOptions), C.c_uint64, C.POINTER(Params)]
     lib.ggcat_gpu_build_from_files.argtypes = [V, C.POINTER(C.c_char_p), C.c_uint32, C.POINTER(BuildOptions),
                                                C.POINTER(Params)]
@@ -112,7 +113,7 @@ def gpu_lib(path: str = GPU_LIB) -> C.CDLL:
                  "ggcat_gpu_create", "ggcat_gpu_bucketing", "ggcat_gpu_records_info", "ggcat_gpu_records_copy",
                  "ggcat_gpu_kmers_merge", "ggcat_gpu_solid_info", "ggcat_gpu_solid_copy", "ggcat_gpu_timings",
                  "ggcat_gpu_counters", "ggcat_gpu_read_files", "ggcat_gpu_build_from_files",
-                 "ggcat_gpu_plan_build", "ggcat_gpu_set_tuning"):
+                 "ggcat_gpu_plan_build", "ggcat_gpu_set_tuning", "ggcat_gpu_solid_colors"):
         getattr(lib, name).restype = C.c_int
     _gpu[path] = lib
     return lib
@@ -284,6 +285,15 @@ class GpuContext:
         self._check(self.lib.ggcat_gpu_solid_copy(self.h, _ptr(s.keys, _u64p), _ptr(s.mult, _u32p),
                                                   _ptr(s.flags, _u8p), _ptr(s.sub, _u32p)), "solid_copy")
         return s
+
+    def solid_colors(self) -> np.ndarray:
+        """(n_solid, color_words) uint32 bitmaps, solid() order."""
+        n, kw, w = C.c_uint64(), C.c_uint32(), C.c_uint32()
+        self._check(self.lib.ggcat_gpu_solid_info(self.h, C.byref(n), C.byref(kw)), "solid_info")
+        self._check(self.lib.ggcat_gpu_solid_colors(self.h, C.byref(w), C.cast(None, _u32p)), "solid_colors")
+        out = np.zeros((n.value, w.value), np.uint32)
+        self._check(self.lib.ggcat_gpu_solid_colors(self.h, C.byref(w), _ptr(out, _u32p)), "solid_colors")
+        return out
 
     def upload(self, data: np.ndarray) -> tuple[int, int]:
         """Copies a host array into a new device allocation; returns (ptr, nbytes)."""

@@ -27,6 +27,23 @@ def reads(g: np.ndarray, seed: int, n_reads: int, read_len: int, err: float) -> 
     return out
 
 
+def descendant(anc: np.ndarray, seed: int, snp: float, indel: float, target_len: int | None = None,
+               stream: int = 0) -> np.ndarray:
+    """Ancestor copy with substitutions (rate snp) and 1-50 bp insertions or
+    deletions (rate indel); a longer target appends mutated ancestor segments,
+    a shorter one keeps a random window of the mutated copy."""
+    target = anc.size if target_len is None else int(target_len)
+    cap = max(target, 1)
+    out = np.empty(cap, np.uint8)
+    n = synth_lib().synth_descendant(_ptr(anc, _u8p), anc.size, seed, stream, snp, indel, target, _ptr(out, _u8p),
+                                     cap)
+    return out[:n]
+
+
+def lognormal_len(seed: int, stream: int, median: float, sigma: float, lo: float, hi: float) -> int:
+    return int(synth_lib().synth_lognormal_len(seed, stream, median, sigma, lo, hi))
+
+
 @dataclass
 class Workload:
     name: str
@@ -77,6 +94,52 @@ def config2(scale: float = 1.0) -> Workload:
     d = reads(g, 202, n, 250, 0.01)
     b, sb = auto_bucket_logs(d.size + n * 10)
     return Workload("config2-100Mbp-250bp-40x" + ("" if scale == 1.0 else f"-x{scale}"), d, 31, 15, 2, n * 250, b, sb)
+
+
+def config3_genomes(scale: float = 1.0, n_genomes: int = 256) -> list[np.ndarray]:
+    """256 genomes from one 4 Mbp ancestor (seed 3): 1% substitutions and 0.1%
+    1-50 bp indels each; lengths log-normal (median 2 Mbp, sigma 0.8) clipped
+    to 0.5-10 Mbp -- longer genomes append mutated ancestor segments, shorter
+    ones are a random window of the mutated ancestor."""
+    from concurrent.futures import ThreadPoolExecutor
+    A = int(4_000_000 * scale)
+    anc = genome(3, A)
+    lens = [lognormal_len(3, i, 2_000_000 * scale, 0.8, 500_000 * scale, 10_000_000 * scale) for i in range(n_genomes)]
+    with ThreadPoolExecutor(THREADS) as ex:
+        return list(ex.map(lambda i: descendant(anc, 3, 0.01, 0.001, lens[i], stream=1 + i), range(n_genomes)))
+
+
+def contigs_joined(genomes: list[np.ndarray], seed: int, lo: int = 20_000, hi: int = 100_000):
+    """Each genome cut into error-free contigs (lengths uniform in [lo, hi]);
+    joined layout plus the byte offset where each genome (color) starts."""
+    rng = np.random.default_rng(seed)
+    parts, starts, pos = [], [], 0
+    for g in genomes:
+        starts.append(pos)
+        cuts = [0]
+        while cuts[-1] < g.size:
+            cuts.append(min(g.size, cuts[-1] + int(rng.integers(lo, hi + 1))))
+        buf = np.empty(g.size + len(cuts) - 1, np.uint8)
+        o = 0
+        for a, b in zip(cuts, cuts[1:]):
+            buf[o:o + b - a] = g[a:b]
+            buf[o + b - a] = 10
+            o += b - a + 1
+        parts.append(buf)
+        pos += buf.size
+    return np.concatenate(parts) if parts else np.zeros(0, np.uint8), np.array(starts, np.uint64)
+
+
+def config3(scale: float = 1.0, n_genomes: int = 256) -> Workload:
+    """Colored build: one color per genome file, k=31, m=15, s=2 (the CLI
+    default; the reference's C++ API defaults to 1)."""
+    gs = config3_genomes(scale, n_genomes)
+    data, starts = contigs_joined(gs, 303)
+    bases = int(sum(g.size for g in gs))
+    b, sb = auto_bucket_logs(int(data.size + (data == 10).sum() * 10))
+    colors = np.arange(len(gs), dtype=np.uint32)
+    return Workload(f"config3-{len(gs)}genomes-colored" + ("" if scale == 1.0 else f"-x{scale}"), data, 31, 15, 2,
+                    bases, b, sb, colors=(starts, colors))
 
 
 def config4(scale: float = 1.0) -> Workload:

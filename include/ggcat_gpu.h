@@ -112,6 +112,14 @@ int ggcat_gpu_solid_info(ggcat_gpu_ctx* ctx, uint64_t* n_solid, uint32_t* key_wo
 int ggcat_gpu_solid_copy(ggcat_gpu_ctx* ctx, uint64_t* keys, uint32_t* multiplicity, uint8_t* flags,
                          uint32_t* sub);
 
+/* Color sets of the solid entries of a colored build, in ggcat_gpu_solid_copy
+ * order: color_words uint32 per entry, bit c set when color c occurs among
+ * the k-mer's occurrences -- the sorted unique union of
+ * crates/colors/src/managers/multiple.rs:99-208 (add_temp_buffer_structure_el,
+ * process_colors) before its subset id is assigned.  Color ids are < 1024.
+ * bitmaps may be NULL to query *color_words. */
+int ggcat_gpu_solid_colors(ggcat_gpu_ctx* ctx, uint32_t* color_words, uint32_t* bitmaps);
+
 /* Schedule knobs of the merge (no counterpart in the reference, whose
  * schedule is its thread pool).  They change which kernel and table size
  * handle a work item, never the result.  Zero selects the default.

@@ -61,7 +61,13 @@ def lib() -> C.CDLL:
     L.orc_free.restype = None
     L.orc_kmer_keys.argtypes = [_u8p, C.c_size_t, C.c_uint, C.c_int, _u64p, _u64p, _u8p]
     L.orc_kmer_keys.restype = None
-    L.orc_bruteforce_counts.argtypes = [_u8p, _u64p, C.c_size_t, C.c_uint, C.c_uint32]
+    L.orc_bruteforce_counts.argtypes = [_u8p, _u64p, C.c_size_t, _u32p, C.c_uint, C.c_uint32]
+    L.orc_rec_colors.argtypes = [V, _u32p]
+    L.orc_rec_colors.restype = None
+    L.orc_solid_colors_total.argtypes = [V]
+    L.orc_solid_colors_total.restype = C.c_size_t
+    L.orc_solid_colors_get.argtypes = [V, _u32p, _u32p]
+    L.orc_solid_colors_get.restype = None
     L.orc_bruteforce_counts.restype = V
     _lib = L
     return L
@@ -113,6 +119,8 @@ class Result:
                  bases=np.empty(nb, np.uint8))
         L.orc_rec_get(self.h, _p(r["sub"], _u32p), _p(r["length"], _u32p), _p(r["flags"], _u8p),
                       _p(r["minpos"], _u16p), _p(r["mult"], _u32p), _p(r["off"], _u64p), _p(r["bases"], _u8p))
+        r["color"] = np.zeros(n, np.uint32)
+        L.orc_rec_colors(self.h, _p(r["color"], _u32p))
         return r
 
     def merge(self, k: int, b: int, sb: int, kind: int, min_mult: int, threads: int = 1) -> None:
@@ -126,6 +134,15 @@ class Result:
         L.orc_solid_get(self.h, _p(s["sub"], _u32p), _p(s["key_lo"], _u64p), _p(s["key_hi"], _u64p),
                         _p(s["mult"], _u32p), _p(s["flags"], _u8p))
         return s
+
+    def solid_colors(self) -> tuple[np.ndarray, np.ndarray]:
+        """Per solid entry (solid() order): offsets [n+1] into sorted color lists."""
+        L = lib()
+        n = L.orc_solid_count(self.h)
+        off = np.zeros(n + 1, np.uint32)
+        cols = np.zeros(L.orc_solid_colors_total(self.h), np.uint32)
+        L.orc_solid_colors_get(self.h, _p(off, _u32p), _p(cols, _u32p))
+        return off, cols
 
     def times(self) -> tuple[float, float]:
         a, b = C.c_double(), C.c_double()
@@ -163,8 +180,28 @@ def kmer_keys(seq: bytes, k: int, kind: int = 0):
     return lo, hi, fw
 
 
-def bruteforce_counts(data: np.ndarray, k: int, min_mult: int) -> dict:
+def color_records(n_bytes: int, color_starts, colors) -> tuple[np.ndarray, np.ndarray]:
+    """Record ranges and record colors for a joined input whose color ranges
+    start at the byte offsets color_starts (the C ABI's color layout)."""
+    cs = np.asarray(color_starts, np.uint64)
+    rec_off = np.concatenate([cs, np.array([n_bytes], np.uint64)])
+    return rec_off, np.ascontiguousarray(colors, np.uint32)
+
+
+def bucketing_colored(data: np.ndarray, k: int, m: int, b: int, sb: int, color_starts, colors,
+                      canonical: bool = True, threads: int = 1) -> Result:
+    rec_off, rec_color = color_records(np.asarray(data).size, color_starts, colors)
+    return bucketing(data, k, m, b, sb, canonical, threads, rec_off, rec_color)
+
+
+def bruteforce_counts(data: np.ndarray, k: int, min_mult: int, color_starts=None, colors=None):
+    """Exact canonical counts (and, with colors, color sets) over the input;
+    returns the solid dict, plus (offsets, colors) when colored."""
     data = np.ascontiguousarray(data, np.uint8)
-    off = np.array([0, data.size], np.uint64)
-    r = Result(lib().orc_bruteforce_counts(_p(data, _u8p), _p(off, _u64p), 1, k, min_mult))
-    return r.solid()
+    if color_starts is None:
+        off = np.array([0, data.size], np.uint64)
+        r = Result(lib().orc_bruteforce_counts(_p(data, _u8p), _p(off, _u64p), 1, None, k, min_mult))
+        return r.solid()
+    off, rc = color_records(data.size, color_starts, colors)
+    r = Result(lib().orc_bruteforce_counts(_p(data, _u8p), _p(off, _u64p), off.size - 1, _p(rc, _u32p), k, min_mult))
+    return r.solid(), r.solid_colors()

@@ -388,7 +388,19 @@ struct U128Hash {
 struct Entry {
     uint64_t count = 0;
     uint8_t flags = 0;
+    std::vector<uint32_t> cols;  // colors of the occurrences (colored builds)
 };
+
+// Sorted unique union of an entry's occurrence colors: the k-mer's color
+// subset (crates/colors/src/managers/multiple.rs:174-208, process_colors:
+// sort_unstable + dedup of the concatenated per-occurrence color lists).
+inline void append_color_set(std::vector<uint32_t>& cols, std::vector<uint32_t>& out_off,
+                             std::vector<uint32_t>& out) {
+    std::sort(cols.begin(), cols.end());
+    cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+    out.insert(out.end(), cols.begin(), cols.end());
+    out_off.push_back((uint32_t)out.size());
+}
 
 struct SolidSet {
     std::vector<uint32_t> sub;
@@ -398,6 +410,12 @@ struct SolidSet {
     std::vector<uint32_t> colors_off;  // color-set offsets (colored runs)
     std::vector<uint32_t> colors;
     void append(const SolidSet& o) {
+        if (!o.colors_off.empty()) {
+            if (colors_off.empty()) colors_off.push_back(0);
+            const uint32_t base = (uint32_t)colors.size();
+            for (size_t i = 1; i < o.colors_off.size(); i++) colors_off.push_back(base + o.colors_off[i]);
+            colors.insert(colors.end(), o.colors.begin(), o.colors.end());
+        }
         sub.insert(sub.end(), o.sub.begin(), o.sub.end());
         key_lo.insert(key_lo.end(), o.key_lo.begin(), o.key_lo.end());
         key_hi.insert(key_hi.end(), o.key_hi.begin(), o.key_hi.end());
@@ -410,9 +428,10 @@ struct SolidSet {
 // add_sequence over every record of one sub-bucket, then the solid filter
 // of hashmap.rs:134-137 / map_entry.rs:79-83.
 void merge_subbucket(const RecordSet& R, const std::vector<size_t>& idx, uint32_t sub, unsigned k,
-                     int kind, uint32_t min_mult, SolidSet& out,
+                     int kind, uint32_t min_mult, bool colored, SolidSet& out,
                      std::unordered_map<u128, Entry, U128Hash>& map, std::vector<uint8_t>& codes) {
     map.clear();
+    if (colored && out.colors_off.empty()) out.colors_off.push_back(0);
     for (size_t r : idx) {
         uint32_t L = R.len[r];
         const uint8_t* b = R.bases.data() + R.off[r];
@@ -428,6 +447,7 @@ void merge_subbucket(const RecordSet& R, const std::vector<size_t>& idx, uint32_
             e.flags |= (uint8_t)(((begin_ignored ? 1 : 0) << (is_fwd ? 0 : 1)) |
                                  ((end_ignored ? 1 : 0) << (is_fwd ? 1 : 0)));
             e.count += mult;
+            if (colored && (e.cols.empty() || e.cols.back() != R.color[r])) e.cols.push_back(R.color[r]);
         });
     }
     for (auto& kv : map) {
@@ -438,12 +458,14 @@ void merge_subbucket(const RecordSet& R, const std::vector<size_t>& idx, uint32_
         out.key_hi.push_back((uint64_t)(kv.first >> 64));
         out.count.push_back((uint32_t)mult);
         out.flags.push_back(kv.second.flags);
+        if (colored) append_color_set(kv.second.cols, out.colors_off, out.colors);
     }
 }
 
 struct Handle {
     RecordSet rec;
     SolidSet solid;
+    bool colored = false;
     double t_bucket = 0, t_merge = 0;
 };
 
@@ -499,6 +521,7 @@ void* orc_bucketing(const uint8_t* seq, const uint64_t* rec_off, size_t n_rec, c
     }
     for (auto& x : th) x.join();
     for (auto& p : parts) h->rec.append(p);
+    h->colored = rec_color != nullptr;
     h->t_bucket = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return h;
 }
@@ -529,7 +552,8 @@ void orc_compact(void* hv) {
         while (j < n) {
             size_t c = order[j];
             if (R.sub[c] != R.sub[a] || R.len[c] != R.len[a] || R.flags[c] != R.flags[a] ||
-                memcmp(R.bases.data() + R.off[c], R.bases.data() + R.off[a], R.len[a]) != 0)
+                memcmp(R.bases.data() + R.off[c], R.bases.data() + R.off[a], R.len[a]) != 0 ||
+                (h->colored && R.color[c] != R.color[a]))
                 break;
             mult += R.mult[c];
             j++;
@@ -562,6 +586,11 @@ void orc_rec_get(void* hv, uint32_t* sub, uint32_t* len, uint8_t* flags, uint16_
     memcpy(bases, R.bases.data(), R.bases.size());
 }
 
+void orc_rec_colors(void* hv, uint32_t* color) {
+    RecordSet& R = ((Handle*)hv)->rec;
+    memcpy(color, R.color.data(), R.color.size() * 4);
+}
+
 // Per-sub-bucket MapEntry merge over the handle's records, `threads` workers.
 void orc_merge(void* hv, uint32_t k, uint32_t b, uint32_t sb, int kind, uint32_t min_mult, int threads) {
     Handle* h = (Handle*)hv;
@@ -580,7 +609,7 @@ void orc_merge(void* hv, uint32_t k, uint32_t b, uint32_t sb, int kind, uint32_t
                 uint32_t s = next.fetch_add(1);
                 if (s >= n_sub) break;
                 if (groups[s].empty()) continue;
-                merge_subbucket(h->rec, groups[s], s, k, kind, min_mult, parts[t], map, codes);
+                merge_subbucket(h->rec, groups[s], s, k, kind, min_mult, h->colored, parts[t], map, codes);
             }
         });
     }
@@ -599,6 +628,18 @@ void orc_solid_get(void* hv, uint32_t* sub, uint64_t* key_lo, uint64_t* key_hi, 
     memcpy(key_hi, S.key_hi.data(), n * 8);
     memcpy(count, S.count.data(), n * 4);
     memcpy(flags, S.flags.data(), n);
+}
+// Color subsets of the solid entries (colored handles): offsets [n+1] into
+// the concatenated sorted color lists.
+size_t orc_solid_colors_total(void* hv) { return ((Handle*)hv)->solid.colors.size(); }
+void orc_solid_colors_get(void* hv, uint32_t* off, uint32_t* colors) {
+    SolidSet& S = ((Handle*)hv)->solid;
+    if (S.colors_off.empty()) {
+        for (size_t i = 0; i <= S.sub.size(); i++) off[i] = 0;
+        return;
+    }
+    memcpy(off, S.colors_off.data(), S.colors_off.size() * 4);
+    memcpy(colors, S.colors.data(), S.colors.size() * 4);
 }
 void orc_times(void* hv, double* t_bucket, double* t_merge) {
     *t_bucket = ((Handle*)hv)->t_bucket;
@@ -620,9 +661,12 @@ void orc_kmer_keys(const uint8_t* ascii, size_t len, unsigned k, int kind, uint6
 
 // Brute force: exact canonical (2-bit) k-mer occurrence counts over the
 // N-free runs of every record; writes keys with count >= min_mult.
-void* orc_bruteforce_counts(const uint8_t* seq, const uint64_t* rec_off, size_t n_rec, unsigned k,
-                            uint32_t min_mult) {
+// With rec_color, each solid key also gets the sorted set of the colors of
+// the records it occurs in.
+void* orc_bruteforce_counts(const uint8_t* seq, const uint64_t* rec_off, size_t n_rec, const uint32_t* rec_color,
+                            unsigned k, uint32_t min_mult) {
     Handle* h = new Handle();
+    h->colored = rec_color != nullptr;
     std::unordered_map<u128, Entry, U128Hash> map;
     std::vector<uint8_t> codes;
     for (size_t r = 0; r < n_rec; r++) {
@@ -636,9 +680,14 @@ void* orc_bruteforce_counts(const uint8_t* seq, const uint64_t* rec_off, size_t 
             if (i - st < k) continue;
             codes.resize(i - st);
             for (size_t j = st; j < i; j++) codes[j - st] = cbase(normalize(s[j]));
-            kmer_hashes(codes.data(), i - st, k, 0, [&](size_t, u128 key, bool) { map[key].count++; });
+            kmer_hashes(codes.data(), i - st, k, 0, [&](size_t, u128 key, bool) {
+                Entry& e = map[key];
+                e.count++;
+                if (rec_color) e.cols.push_back(rec_color[r]);
+            });
         }
     }
+    if (rec_color) h->solid.colors_off.push_back(0);
     for (auto& kv : map) {
         if (kv.second.count < min_mult) continue;
         h->solid.sub.push_back(0);
@@ -646,6 +695,7 @@ void* orc_bruteforce_counts(const uint8_t* seq, const uint64_t* rec_off, size_t 
         h->solid.key_hi.push_back((uint64_t)(kv.first >> 64));
         h->solid.count.push_back((uint32_t)kv.second.count);
         h->solid.flags.push_back(0);
+        if (rec_color) append_color_set(kv.second.cols, h->solid.colors_off, h->solid.colors);
     }
     return h;
 }

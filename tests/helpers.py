@@ -19,9 +19,10 @@ def record_tuples(r) -> list:
     return out
 
 
-def record_digests(r) -> np.ndarray:
-    """Sorted 2x64-bit digests of every record (sub, len, flags, minpos, bases);
-    used to compare multisets of millions of records."""
+def record_digests(r, with_color: bool = False) -> np.ndarray:
+    """Sorted 2x64-bit digests of every record (sub, len, flags, minpos, bases,
+    and the record color when with_color); used to compare multisets of
+    millions of records."""
     get = (lambda x, n: x[n]) if isinstance(r, dict) else (lambda x, n: getattr(x, n))
     sub, ln, fl, mp, off, bases = (get(r, n) for n in ("sub", "length", "flags", "minpos", "off", "bases"))
     n = sub.size
@@ -42,6 +43,8 @@ def record_digests(r) -> np.ndarray:
             h = np.add.reduceat(b * w, np.concatenate(([0], np.cumsum(ln64)[:-1])).astype(np.int64)) if total else np.zeros(n, np.uint64)
             h = h.astype(np.uint64)
             h ^= (sub.astype(np.uint64) * np.uint64(p1)) ^ (ln64 << np.uint64(20)) ^ (fl.astype(np.uint64) << np.uint64(40)) ^ (mp.astype(np.uint64) << np.uint64(48))
+            if with_color:
+                h ^= (get(r, "color").astype(np.uint64) + np.uint64(1)) * np.uint64(0x9FB21C651E98DF25)
             h *= np.uint64(p2)
             h ^= h >> np.uint64(31)
             out[:, j] = h
@@ -68,6 +71,30 @@ def solid_array(s) -> np.ndarray:
         hi = s.keys[:, 1] if s.keys.shape[1] > 1 else np.zeros_like(lo)
         cols = [s.sub, lo, hi, s.mult, s.flags]
     a = np.stack([c.astype(np.uint64) for c in cols], axis=1)
+    idx = np.lexsort(tuple(a[:, j] for j in range(4, -1, -1)))
+    return a[idx]
+
+
+def bitmaps_from_sets(off: np.ndarray, cols: np.ndarray, words: int) -> np.ndarray:
+    """Color lists (offsets [n+1] into sorted ids) -> (n, words) uint32 bitmaps."""
+    n = off.size - 1
+    out = np.zeros((n, words), np.uint32)
+    row = np.repeat(np.arange(n), np.diff(off.astype(np.int64)))
+    c = cols.astype(np.int64)
+    np.bitwise_or.at(out, (row, c >> 5), (np.uint32(1) << (c & 31).astype(np.uint32)))
+    return out
+
+
+def colored_solid_array(s, bitmaps: np.ndarray) -> np.ndarray:
+    """solid_array rows extended with the color bitmap words, same sort."""
+    if isinstance(s, dict):
+        cols = [s["sub"], s["key_lo"], s["key_hi"], s["mult"], s["flags"]]
+    else:
+        lo = s.keys[:, 0]
+        hi = s.keys[:, 1] if s.keys.shape[1] > 1 else np.zeros_like(lo)
+        cols = [s.sub, lo, hi, s.mult, s.flags]
+    a = np.stack([c.astype(np.uint64) for c in cols] + [bitmaps[:, j].astype(np.uint64)
+                                                         for j in range(bitmaps.shape[1])], axis=1)
     idx = np.lexsort(tuple(a[:, j] for j in range(4, -1, -1)))
     return a[idx]
 

@@ -13,7 +13,8 @@ from ggcat_amd import synth
 from ggcat_amd.native import HASH_RK128, HASH_SEQ, GpuContext, Params
 from oracle import binding as O
 from tests.emu import build as emu_build
-from tests.helpers import adversarial_records, joined, record_tuples, solid_array
+from tests.helpers import (adversarial_records, bitmaps_from_sets, colored_solid_array, joined, record_tuples,
+                           solid_array)
 
 
 @pytest.fixture(scope="module")
@@ -55,5 +56,24 @@ def test_long_records_emulated(emu_ctx):
     emu_ctx.set_tuning(merge_wgs_per_cu=2)
     try:
         _check(emu_ctx, data, 31, 15, 4, 2, 1)
+    finally:
+        emu_ctx.set_tuning()
+
+
+def test_colored_merge_emulated(emu_ctx):
+    gs = synth.config3_genomes(0.002, 12)
+    data, starts = synth.contigs_joined(gs, 5, 3000, 9000)
+    colors = (np.arange(12, dtype=np.uint32) * 29) % 60  # ids need not follow file order; two bitmap words
+    emu_ctx.set_tuning(merge_wgs_per_cu=2)
+    try:
+        emu_ctx.bucketing(data, Params(31, 15, 3, 2, 0, HASH_SEQ, 1), colors=(starts, colors))
+        o = O.bucketing_colored(data, 31, 15, 3, 2, starts, colors, threads=2)
+        assert record_tuples(emu_ctx.records()) == record_tuples(o.records())
+        emu_ctx.kmers_merge()
+        o.merge(31, 3, 2, HASH_SEQ, 1, 2)
+        got = colored_solid_array(emu_ctx.solid(), emu_ctx.solid_colors())
+        off, cols = o.solid_colors()
+        want = colored_solid_array(o.solid(), bitmaps_from_sets(off, cols, 2))
+        assert np.array_equal(got, want)
     finally:
         emu_ctx.set_tuning()

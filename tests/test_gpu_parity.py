@@ -173,3 +173,24 @@ def test_build_from_files_fastq(gpu_ctx, tmp_path):
     o = O.bucketing(inp.joined, 31, 15, used.buckets_log, used.second_buckets_log, threads=4)
     o.merge(31, used.buckets_log, used.second_buckets_log, HASH_SEQ, 2, 4)
     assert np.array_equal(solid_array(gs), solid_array(o.solid()))
+
+
+@pytest.mark.parametrize("n_genomes,scale", [(64, 0.02), (256, 0.005)])
+def test_colored_contigs(gpu_ctx, n_genomes, scale):
+    # one color per genome file of error-free contigs (the colored build's shape)
+    from tests.helpers import bitmaps_from_sets, colored_solid_array
+    wl = synth.config3(scale, n_genomes)
+    starts, colors = wl.colors
+    _say(f"case colored genomes={n_genomes} bytes={wl.data.size}")
+    gpu_ctx.bucketing(wl.data, _params(31, 15, wl.buckets_log, wl.second_buckets_log, wl.min_mult),
+                      colors=(starts, colors))
+    o = O.bucketing_colored(wl.data, 31, 15, wl.buckets_log, wl.second_buckets_log, starts, colors, threads=4)
+    assert np.array_equal(record_digests(gpu_ctx.records(), True), record_digests(o.records(), True))
+    gpu_ctx.kmers_merge()
+    o.merge(31, wl.buckets_log, wl.second_buckets_log, HASH_SEQ, wl.min_mult, 4)
+    words = gpu_ctx.solid_colors().shape[1]
+    got = colored_solid_array(gpu_ctx.solid(), gpu_ctx.solid_colors())
+    off, cols = o.solid_colors()
+    want = colored_solid_array(o.solid(), bitmaps_from_sets(off, cols, words))
+    assert got.shape == want.shape
+    assert np.array_equal(got, want)

@@ -261,6 +261,38 @@ def test_merge_union_equals_exact_counts(k, m, s):
             assert int(sol["flags"][i]) in (1, 2)
 
 
+@pytest.mark.parametrize("s", [1, 2])
+def test_colored_merge_color_sets_equal_bruteforce(s):
+    # eight "files" (colors) of contigs sharing most k-mers: a descendant
+    # genome each, so color sets range from single colors to all eight
+    from ggcat_amd import synth
+    anc = synth.genome(21, 20_000)
+    files = [synth.descendant(anc, 100 + i, 0.01, 0.001) for i in range(8)]
+    parts, starts, pos = [], [], 0
+    for f in files:
+        starts.append(pos)
+        rec = f.tobytes()
+        chunk = b"".join(rec[i:i + 3000] + b"\n" for i in range(0, len(rec), 2500))
+        parts.append(chunk)
+        pos += len(chunk)
+    data = np.frombuffer(b"".join(parts), np.uint8).copy()
+    colors = np.arange(8, dtype=np.uint32)[::-1].copy()  # color ids need not follow file order
+    r = O.bucketing_colored(data, 31, 15, 5, 2, starts, colors, threads=2)
+    r.merge(31, 5, 2, 0, s, 2)
+    sol = r.solid()
+    off, cols = r.solid_colors()
+    got = {}
+    for i, lo in enumerate(sol["key_lo"].tolist()):
+        cs = tuple(cols[off[i]:off[i + 1]].tolist())
+        assert got.get(lo, cs) == cs  # both halves of a boundary k-mer carry the same set
+        got[lo] = cs
+    bf, (boff, bcols) = O.bruteforce_counts(data, 31, s, starts, colors)
+    want = {lo: tuple(bcols[boff[i]:boff[i + 1]].tolist()) for i, lo in enumerate(bf["key_lo"].tolist())}
+    assert got == want
+    sizes = {len(v) for v in want.values()}
+    assert 1 in sizes and 8 in sizes
+
+
 def test_compaction_preserves_solid_table():
     from ggcat_amd import synth
     from tests.helpers import solid_array
