@@ -232,8 +232,12 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
         HIPCHK(hipMemcpy(ctx->input.p, data, n_bytes, hipMemcpyHostToDevice));
         d_in = P_<uint8_t>(ctx->input);
     }
-    const uint64_t n_tiles = (n_bytes + TILE_T - 1) / TILE_T;
-    const uint64_t seg_words = n_tiles * TILE_T / 32 + 2;
+    // w == 16 forms window minima in registers; its tile of 960 windows (plus
+    // the w + 1 halo) is exactly 62 blocks of 16, one per lane
+    const bool regs16 = (w == 16);
+    const uint32_t T = regs16 ? 960 : TILE_T;
+    const uint64_t n_tiles = (n_bytes + T - 1) / T;
+    const uint64_t seg_words = n_tiles * T / 32 + 2;
     ENSURE(ctx->seg_bits, seg_words * 4);
     ENSURE(ctx->unit_count, (size_t)n_units * 4);
     ENSURE(ctx->unit_cursor, (size_t)n_units * 4);
@@ -258,7 +262,7 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     A.sb = p.second_buckets_log;
     A.xb = xb;
     A.canonical = p.forward_only ? 0 : 1;
-    A.T = TILE_T;
+    A.T = T;
     A.n_tiles = n_tiles;
     A.seg_bits = P_<uint32_t>(ctx->seg_bits);
     A.sub_count = P_<uint32_t>(ctx->unit_count);
@@ -272,13 +276,16 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     A.n_colors = n_colors;
     A.error_flag = P_<uint32_t>(ctx->error_flag);
     nthash_tables(p.m, A.gtab);
-    const TileGeom G = tile_geom(TILE_T, w, p.k, p.m);
+    const TileGeom G = tile_geom(T, w, p.k, p.m, regs16);
 
     HIPCHK(hipMemsetAsync(ctx->unit_count.p, 0, (size_t)n_units * 4, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->seg_bits.p, 0, seg_words * 4, ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
     if (n_tiles) {
-        hipLaunchKernelGGL(bucket_count_kernel, dim3((uint32_t)n_tiles), dim3(64), G.bytes, ctx->stream, A);
+        if (regs16)
+            hipLaunchKernelGGL(bucket_count_kernel<16>, dim3((uint32_t)n_tiles), dim3(64), G.bytes, ctx->stream, A);
+        else
+            hipLaunchKernelGGL(bucket_count_kernel<0>, dim3((uint32_t)n_tiles), dim3(64), G.bytes, ctx->stream, A);
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
@@ -307,7 +314,10 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
         HIPCHK(hipMemsetAsync(ctx->error_flag.p, 0, 8, ctx->stream));
         HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
         if (n_tiles) {
-            hipLaunchKernelGGL(bucket_emit_kernel, dim3((uint32_t)n_tiles), dim3(64), G.bytes, ctx->stream, A);
+            if (regs16)
+                hipLaunchKernelGGL(bucket_emit_kernel<16>, dim3((uint32_t)n_tiles), dim3(64), G.bytes, ctx->stream, A);
+            else
+                hipLaunchKernelGGL(bucket_emit_kernel<0>, dim3((uint32_t)n_tiles), dim3(64), G.bytes, ctx->stream, A);
             HIPCHK(hipGetLastError());
         }
         HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));

@@ -69,7 +69,9 @@ struct TileGeom {
     uint32_t bytes;
 };
 
-__host__ __device__ inline TileGeom tile_geom(uint32_t T, uint32_t w, uint32_t k, uint32_t m) {
+// With `regs` the window minima are formed in registers and the per-m-mer
+// hash array is not needed.
+__host__ __device__ inline TileGeom tile_geom(uint32_t T, uint32_t w, uint32_t k, uint32_t m, bool regs) {
     TileGeom g;
     g.NW = T + w + 1;
     g.NWB = (g.NW + w - 1) / w;
@@ -80,7 +82,7 @@ __host__ __device__ inline TileGeom tile_geom(uint32_t T, uint32_t w, uint32_t k
     g.NB = (need + 31) / 32 * 32;
     uint32_t o = 0;
     g.off_g = o; o += 40 * 8;
-    g.off_mm = o; o += g.NM * 8;
+    g.off_mm = o; o += regs ? 0 : g.NM * 8;
     g.off_wv = o; o += g.NWB * w * 8;
     g.off_pk = o; o += g.NB / 16 * 4;
     g.off_inv = o; o += g.NB / 32 * 4;
@@ -136,12 +138,37 @@ struct Tile {
     GG_DEV bool bit(const uint32_t* a, uint32_t x) const { return (a[x >> 5] >> (x & 31)) & 1; }
 };
 
-// Phases 0-3: load, hash, window minima, per-window flags.
+// 16 codes of relative positions [r, r+16) packed (position r at bits 0..1);
+// positions below 0 read as zero.
+GG_DEV uint32_t extract16(const Tile& t, int64_t r) {
+    if (r >= 0) {
+        uint32_t wi = (uint32_t)(r >> 4), s = (uint32_t)(r & 15) * 2;
+        uint32_t lo = t.pk[wi];
+        if (s == 0) return lo;
+        return (lo >> s) | (t.pk[wi + 1] << (32 - s));
+    }
+    if (r <= -16) return 0;
+    uint32_t sh = (uint32_t)(-r) * 2;
+    return t.pk[0] << sh;
+}
+
+GG_DEV uint32_t rev_comp16(uint32_t x) {
+    uint32_t y = __brev(x);
+    y = ((y >> 1) & 0x55555555u) | ((y & 0x55555555u) << 1);
+    return y ^ 0xAAAAAAAAu;
+}
+
+GG_DEV void tile_minima_lds(Tile& t);
+GG_DEV void tile_minima_w16(Tile& t);
+GG_DEV void tile_flags(Tile& t);
+
+// Phases 0-3: load, hash, window minima, per-window flags.  WC == 16 selects
+// the register path for w == 16 (k = 31, m = 15 and other k - m == 16).
+template <int WC>
 GG_DEV void tile_compute(Tile& t) {
     const BucketArgs& A = *t.A;
     const TileGeom& G = t.G;
     const uint32_t lane = threadIdx.x;
-    const uint32_t k = A.k, m = A.m, w = A.w;
 
     for (uint32_t i = lane; i < 40; i += 64) t.gt[i] = A.gtab[i];
 
@@ -175,6 +202,22 @@ GG_DEV void tile_compute(Tile& t) {
         reinterpret_cast<uint16_t*>(t.inv)[c] = (uint16_t)invm;
     }
     __syncthreads();
+
+    if (WC == 16) {
+        tile_minima_w16(t);
+    } else {
+        tile_minima_lds(t);
+    }
+    __syncthreads();
+    tile_flags(t);
+}
+
+// Phase 1 + 2, any w: every m-mer hash goes to LDS, then van Herk blocks.
+GG_DEV void tile_minima_lds(Tile& t) {
+    const BucketArgs& A = *t.A;
+    const TileGeom& G = t.G;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t m = A.m, w = A.w;
 
     // ---- phase 1: canonical ntHash of every m-mer (rolling strips) -------
     // fw_i = rotl1(fw_{i-1}) ^ rotl(h(s_{i-1}), m) ^ h(s_{i+m-1})
@@ -235,9 +278,90 @@ GG_DEV void tile_compute(Tile& t) {
             t.wp[base + i] = (uint16_t)rp;
         }
     }
-    __syncthreads();
+}
 
-    // ---- phase 3: per-window flags ----------------------------------------
+// Phase 1 + 2 for w == 16 with the blocks in registers: the lane owning
+// window block q hashes m-mer blocks q and q+1 (rolling update, tables in
+// LDS), takes suffix minima of block q and running prefix minima of block
+// q+1; window q*16+i is the combination of the two (van Herk / Gil-Werman).
+GG_DEV void tile_minima_w16(Tile& t) {
+    const TileGeom& G = t.G;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t m = t.A->m;
+    const int64_t rb = (int64_t)t.d0;  // relative base of m-mer 0
+    const ulonglong2* gt2 = reinterpret_cast<const ulonglong2*>(t.gt);
+    for (uint32_t q = lane; q < G.NWB; q += 64) {
+        const uint32_t p0 = q * 16;
+        uint64_t fw = 0, rc = 0;
+        for (uint32_t j = 0; j + 1 < m; j++) {
+            const ulonglong2 g = gt2[16 + t.code_rel(rb + p0 + j)];
+            fw = rotl1_64(fw) ^ g.x;
+            rc = rotr1_64(rc) ^ g.y;
+        }
+        uint64_t sv[16];
+        uint32_t sp[16];
+        uint32_t fbits = 0;
+        {
+            const uint32_t inw = extract16(t, rb + p0 + m - 1);
+            const uint32_t outw = extract16(t, rb + p0 - 1);
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const uint32_t in = (inw >> (2 * i)) & 3u, out = (outw >> (2 * i)) & 3u;
+                const ulonglong2 g = gt2[(i == 0 ? 16u : out * 4u) + in];
+                fw = rotl1_64(fw) ^ g.x;
+                rc = rotr1_64(rc) ^ g.y;
+                const uint64_t mn = fw < rc ? fw : rc;
+                sv[i] = (mn << 1) | (fw != rc ? 1ULL : 0ULL);
+                sp[i] = p0 + i;
+                fbits |= (fw < rc ? 1u : 0u) << i;
+            }
+        }
+#pragma unroll
+        for (int i = 14; i >= 0; i--) mm_combine(sv[i], sp[i], sv[i + 1], sp[i + 1], sv[i], sp[i]);
+        uint32_t fbits1 = 0;
+        {
+            const uint32_t p1 = p0 + 16;
+            const uint32_t inw = extract16(t, rb + p1 + m - 1);
+            const uint32_t outw = extract16(t, rb + p1 - 1);
+            uint64_t pv = 0;
+            uint32_t pp = 0;
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const uint32_t in = (inw >> (2 * i)) & 3u, out = (outw >> (2 * i)) & 3u;
+                const ulonglong2 g = gt2[out * 4u + in];
+                fw = rotl1_64(fw) ^ g.x;
+                rc = rotr1_64(rc) ^ g.y;
+                const uint64_t mn = fw < rc ? fw : rc;
+                const uint64_t v = (mn << 1) | (fw != rc ? 1ULL : 0ULL);
+                fbits1 |= (fw < rc ? 1u : 0u) << i;
+                uint64_t rv = sv[i];
+                uint32_t rp = sp[i];
+                if (i > 0) mm_combine(sv[i], sp[i], pv, pp, rv, rp);
+                t.wv[p0 + i] = rv;
+                t.wp[p0 + i] = (uint16_t)rp;
+                if (i == 0) {
+                    pv = v;
+                    pp = p1;
+                } else {
+                    mm_combine(pv, pp, v, p1 + i, pv, pp);
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 16; i++) t.fwd[p0 + i] = (fbits >> i) & 1u;
+        if (q + 1 == G.NWB) {
+#pragma unroll
+            for (int i = 0; i < 16; i++) t.fwd[p0 + 16 + i] = (fbits1 >> i) & 1u;
+        }
+    }
+}
+
+// Phase 3: per-window flags.
+GG_DEV void tile_flags(Tile& t) {
+    const BucketArgs& A = *t.A;
+    const TileGeom& G = t.G;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t k = A.k;
     const uint32_t nwords64 = (G.NW + 63) / 64;
     for (uint32_t c = 0; c < nwords64; c++) {
         uint32_t x = c * 64 + lane;
@@ -287,9 +411,9 @@ GG_DEV uint32_t unit_of(uint64_t v, uint32_t b, uint32_t sb, uint32_t xb) {
     return (sub_of(v, b, sb) << xb) | extra;
 }
 
-GG_DEV void tile_init(Tile& t, const BucketArgs* A, uint64_t tile, uint8_t* lds) {
+GG_DEV void tile_init(Tile& t, const BucketArgs* A, uint64_t tile, uint8_t* lds, bool regs) {
     t.A = A;
-    t.G = tile_geom(A->T, A->w, A->k, A->m);
+    t.G = tile_geom(A->T, A->w, A->k, A->m, regs);
     t.t0 = (int64_t)tile * A->T;
     t.wlo = t.t0 - A->w;
     int64_t bo = t.wlo - 1;
@@ -308,11 +432,12 @@ GG_DEV void tile_init(Tile& t, const BucketArgs* A, uint64_t tile, uint8_t* lds)
     t.last = reinterpret_cast<uint32_t*>(lds + t.G.off_last);
 }
 
+template <int WC>
 __global__ __launch_bounds__(64) void bucket_count_kernel(BucketArgs A) {
     extern __shared__ __align__(16) uint8_t lds[];
     Tile t;
-    tile_init(t, &A, blockIdx.x, lds);
-    tile_compute(t);
+    tile_init(t, &A, blockIdx.x, lds, WC != 0);
+    tile_compute<WC>(t);
     const uint32_t lane = threadIdx.x;
     const uint32_t w = A.w;
     for (uint32_t c = 0; c < A.T; c += 64) {
@@ -345,31 +470,12 @@ GG_DEV bool invalid_abs(const Tile& t, int64_t p) {
     return !is_acgt(t.A->seq[p]);
 }
 
-// 16 codes of relative positions [r, r+16) packed (position r at bits 0..1);
-// positions below 0 read as zero.
-GG_DEV uint32_t extract16(const Tile& t, int64_t r) {
-    if (r >= 0) {
-        uint32_t wi = (uint32_t)(r >> 4), s = (uint32_t)(r & 15) * 2;
-        uint32_t lo = t.pk[wi];
-        if (s == 0) return lo;
-        return (lo >> s) | (t.pk[wi + 1] << (32 - s));
-    }
-    if (r <= -16) return 0;
-    uint32_t sh = (uint32_t)(-r) * 2;
-    return t.pk[0] << sh;
-}
-
-GG_DEV uint32_t rev_comp16(uint32_t x) {
-    uint32_t y = __brev(x);
-    y = ((y >> 1) & 0x55555555u) | ((y & 0x55555555u) << 1);
-    return y ^ 0xAAAAAAAAu;
-}
-
+template <int WC>
 __global__ __launch_bounds__(64) void bucket_emit_kernel(BucketArgs A) {
     extern __shared__ __align__(16) uint8_t lds[];
     Tile t;
-    tile_init(t, &A, blockIdx.x, lds);
-    tile_compute(t);
+    tile_init(t, &A, blockIdx.x, lds, WC != 0);
+    tile_compute<WC>(t);
     const uint32_t lane = threadIdx.x;
     const uint32_t w = A.w, k = A.k, m = A.m;
     const uint32_t SW = A.slot_words;

@@ -27,6 +27,7 @@ struct dim3 {
     dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
 };
 struct uint4 { unsigned x, y, z, w; };
+struct ulonglong2 { unsigned long long x, y; };
 
 namespace emu {
 extern thread_local dim3 threadIdx;
