@@ -480,10 +480,26 @@ __global__ __launch_bounds__(64) void bucket_emit_kernel(BucketArgs A) {
     const uint32_t w = A.w, k = A.k, m = A.m;
     const uint32_t SW = A.slot_words;
     const uint32_t inline_cap = (SW - 1) * 16;
+    // Compact the tile's group ends first, then emit them with every lane
+    // busy: the dependent slot atomics of a tile overlap instead of running
+    // one 64-window stripe at a time.
+    __shared__ uint16_t ends[1024];
+    __shared__ uint32_t n_ends;
+    if (lane == 0) n_ends = 0;
+    __syncthreads();
     for (uint32_t c = 0; c < A.T; c += 64) {
-        uint32_t x = w + c + lane;
-        int64_t bwin = t.t0 + c + lane;
-        if (bwin >= (int64_t)A.n || !tile_is_end(t, x)) continue;
+        const uint32_t x = w + c + lane;
+        const bool e = t.t0 + c + lane < (int64_t)A.n && tile_is_end(t, x);
+        const uint64_t be = __ballot(e);
+        if (e) ends[n_ends + __popcll(be & ((1ULL << lane) - 1))] = (uint16_t)x;
+        __syncthreads();
+        if (lane == 0) n_ends += (uint32_t)__popcll(be);
+        __syncthreads();
+    }
+    const uint32_t n_end = n_ends;
+    for (uint32_t ei = lane; ei < n_end; ei += 64) {
+        const uint32_t x = ends[ei];
+        const int64_t bwin = t.t0 + (int64_t)x - w;
         // ---- group start: latest group-start bit at or before x ----
         int64_t a = -1;
         {
