@@ -234,7 +234,7 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     }
     // w == 16 forms window minima in registers; its tile of 960 windows (plus
     // the w + 1 halo) is exactly 62 blocks of 16, one per lane
-    const bool regs16 = (w == 16);
+    const bool regs16 = (w == 16 && p.k <= 47);
     const uint32_t T = regs16 ? 960 : TILE_T;
     const uint64_t n_tiles = (n_bytes + T - 1) / T;
     const uint64_t seg_words = n_tiles * T / 32 + 2;

@@ -204,12 +204,12 @@ GG_DEV void tile_compute(Tile& t) {
     __syncthreads();
 
     if (WC == 16) {
-        tile_minima_w16(t);
+        tile_minima_w16(t);  // also writes the flag bitmaps
     } else {
         tile_minima_lds(t);
+        __syncthreads();
+        tile_flags(t);
     }
-    __syncthreads();
-    tile_flags(t);
 }
 
 // Phase 1 + 2, any w: every m-mer hash goes to LDS, then van Herk blocks.
@@ -280,18 +280,33 @@ GG_DEV void tile_minima_lds(Tile& t) {
     }
 }
 
-// Phase 1 + 2 for w == 16 with the blocks in registers: the lane owning
-// window block q hashes m-mer blocks q and q+1 (rolling update, tables in
-// LDS), takes suffix minima of block q and running prefix minima of block
-// q+1; window q*16+i is the combination of the two (van Herk / Gil-Werman).
+// 64 bits of a bit array starting at bit `r` (r >= 0).
+GG_DEV uint64_t bits64(const uint32_t* a, int64_t r) {
+    const uint32_t wi = (uint32_t)(r >> 5), sh = (uint32_t)(r & 31);
+    const uint64_t lo = (uint64_t)a[wi] | ((uint64_t)a[wi + 1] << 32);
+    if (sh == 0) return lo;
+    return (lo >> sh) | ((uint64_t)a[wi + 2] << (64 - sh));
+}
+
+// Phases 1-3 for w == 16 (k <= 47), one window block of 16 per lane (the
+// tile geometry makes NWB <= 64).  The lane owning window block q hashes
+// m-mer blocks q and q+1 (rolling update, tables in LDS), takes suffix minima
+// of block q and running prefix minima of block q+1; window q*16+i is the
+// combination of the two (van Herk / Gil-Werman in registers).  The window
+// flags of phase 3 are formed in the same pass from a 64-bit slice of the
+// invalid-base bitmap and written as 16-bit pieces.
 GG_DEV void tile_minima_w16(Tile& t) {
     const TileGeom& G = t.G;
     const uint32_t lane = threadIdx.x;
-    const uint32_t m = t.A->m;
-    const int64_t rb = (int64_t)t.d0;  // relative base of m-mer 0
+    const uint32_t m = t.A->m, k = t.A->k;
+    const int64_t rb = (int64_t)t.d0;  // relative base of m-mer 0 (and of window 0)
     const ulonglong2* gt2 = reinterpret_cast<const ulonglong2*>(t.gt);
-    for (uint32_t q = lane; q < G.NWB; q += 64) {
-        const uint32_t p0 = q * 16;
+    const uint32_t q = lane;
+    const bool have = q < G.NWB;
+    const uint32_t p0 = q * 16;
+    uint32_t segm = 0, okm = 0, lastm = 0;
+    bool need_prev = false;
+    if (have) {
         uint64_t fw = 0, rc = 0;
         for (uint32_t j = 0; j + 1 < m; j++) {
             const ulonglong2 g = gt2[16 + t.code_rel(rb + p0 + j)];
@@ -318,13 +333,16 @@ GG_DEV void tile_minima_w16(Tile& t) {
         }
 #pragma unroll
         for (int i = 14; i >= 0; i--) mm_combine(sv[i], sp[i], sv[i + 1], sp[i + 1], sv[i], sp[i]);
+        // invalid bits of relative bases rb + p0 - 1 .. rb + p0 + 62
+        const uint64_t im = bits64(t.inv, rb + p0 - 1);
+        const uint64_t kmask = (1ULL << (k - 1)) - 1;
         uint32_t fbits1 = 0;
         {
             const uint32_t p1 = p0 + 16;
             const uint32_t inw = extract16(t, rb + p1 + m - 1);
             const uint32_t outw = extract16(t, rb + p1 - 1);
-            uint64_t pv = 0;
-            uint32_t pp = 0;
+            uint64_t pv = 0, prv = 0;
+            uint32_t pp = 0, prp = 0;
 #pragma unroll
             for (int i = 0; i < 16; i++) {
                 const uint32_t in = (inw >> (2 * i)) & 3u, out = (outw >> (2 * i)) & 3u;
@@ -339,6 +357,27 @@ GG_DEV void tile_minima_w16(Tile& t) {
                 if (i > 0) mm_combine(sv[i], sp[i], pv, pp, rv, rp);
                 t.wv[p0 + i] = rv;
                 t.wp[p0 + i] = (uint16_t)rp;
+                // flags of window x = p0 + i (bases rb + x .. rb + x + k - 2)
+                const uint32_t x = p0 + i;
+                if (x >= 1 && x < G.NW) {
+                    const bool valid = ((im >> (i + 1)) & kmask) == 0;
+                    const bool first = (im >> i) & 1;
+                    const bool lst = (im >> (i + k)) & 1;
+                    const bool okf = valid && !(first && lst);
+                    if (okf) {
+                        okm |= 1u << i;
+                        if (lst) lastm |= 1u << i;
+                        if (first) {
+                            segm |= 1u << i;
+                        } else if (i == 0) {
+                            need_prev = true;
+                        } else if (rv != prv || ((rv & 1) && rp != prp)) {
+                            segm |= 1u << i;
+                        }
+                    }
+                }
+                prv = rv;
+                prp = rp;
                 if (i == 0) {
                     pv = v;
                     pp = p1;
@@ -354,6 +393,21 @@ GG_DEV void tile_minima_w16(Tile& t) {
             for (int i = 0; i < 16; i++) t.fwd[p0 + 16 + i] = (fbits1 >> i) & 1u;
         }
     }
+    __syncthreads();
+    if (need_prev) {
+        // window p0 starts a group when its minimum differs from window p0 - 1
+        const uint64_t v1 = t.wv[p0], v0 = t.wv[p0 - 1];
+        if (v0 != v1 || ((v1 & 1) && t.wp[p0] != t.wp[p0 - 1])) segm |= 1u;
+    }
+    // bitmap words past the last block stay zero
+    const uint32_t npieces = (G.NW / 64 + 2) * 4;
+    for (uint32_t pc = q; pc < npieces; pc += 64) {
+        const bool mine = pc == q && have;
+        reinterpret_cast<uint16_t*>(t.seg)[pc] = mine ? (uint16_t)segm : 0;
+        reinterpret_cast<uint16_t*>(t.ok)[pc] = mine ? (uint16_t)okm : 0;
+        reinterpret_cast<uint16_t*>(t.last)[pc] = mine ? (uint16_t)lastm : 0;
+    }
+    __syncthreads();
 }
 
 // Phase 3: per-window flags.
@@ -432,6 +486,25 @@ GG_DEV void tile_init(Tile& t, const BucketArgs* A, uint64_t tile, uint8_t* lds,
     t.last = reinterpret_cast<uint32_t*>(lds + t.G.off_last);
 }
 
+// w == 16 path: group ends among the 16 windows of the lane's block that the
+// tile owns (windows w .. w+T-1, absolute index below n).  Returns the mask
+// and sets *x0 / *j0 (first window, relative and absolute) and *segm.
+GG_DEV uint32_t lane_end_mask16(const Tile& t, uint32_t* x0, int64_t* j0, uint32_t* segm) {
+    const uint32_t q = threadIdx.x;
+    *x0 = q * 16;
+    *j0 = t.t0 + (int64_t)q * 16 - 16;
+    *segm = 0;
+    if (q < 1 || q > t.A->T / 16) return 0;
+    const uint32_t sg = reinterpret_cast<const uint16_t*>(t.seg)[q];
+    const uint32_t okm = reinterpret_cast<const uint16_t*>(t.ok)[q];
+    const uint32_t lastm = reinterpret_cast<const uint16_t*>(t.last)[q];
+    const uint32_t next = t.bit(t.seg, *x0 + 16) ? 1u : 0u;
+    const int64_t left = (int64_t)t.A->n - *j0;
+    const uint32_t vm = left >= 16 ? 0xFFFFu : (left <= 0 ? 0u : (1u << left) - 1u);
+    *segm = sg & vm;
+    return okm & (lastm | (sg >> 1) | (next << 15)) & vm;
+}
+
 template <int WC>
 __global__ __launch_bounds__(64) void bucket_count_kernel(BucketArgs A) {
     extern __shared__ __align__(16) uint8_t lds[];
@@ -440,6 +513,18 @@ __global__ __launch_bounds__(64) void bucket_count_kernel(BucketArgs A) {
     tile_compute<WC>(t);
     const uint32_t lane = threadIdx.x;
     const uint32_t w = A.w;
+    if (WC == 16) {
+        uint32_t x0, segm;
+        int64_t j0;
+        uint32_t endm = lane_end_mask16(t, &x0, &j0, &segm);
+        if (lane >= 1 && lane <= A.T / 16) reinterpret_cast<uint16_t*>(A.seg_bits)[j0 >> 4] = (uint16_t)segm;
+        while (endm) {
+            const uint32_t i = __ffs(endm) - 1;
+            endm &= endm - 1;
+            atomicAdd(&A.sub_count[unit_of(t.wv[x0 + i], A.b, A.sb, A.xb)], 1u);
+        }
+        return;
+    }
     for (uint32_t c = 0; c < A.T; c += 64) {
         uint32_t x = w + c + lane;
         int64_t j = t.t0 + c + lane;
@@ -485,16 +570,37 @@ __global__ __launch_bounds__(64) void bucket_emit_kernel(BucketArgs A) {
     // one 64-window stripe at a time.
     __shared__ uint16_t ends[1024];
     __shared__ uint32_t n_ends;
-    if (lane == 0) n_ends = 0;
-    __syncthreads();
-    for (uint32_t c = 0; c < A.T; c += 64) {
-        const uint32_t x = w + c + lane;
-        const bool e = t.t0 + c + lane < (int64_t)A.n && tile_is_end(t, x);
-        const uint64_t be = __ballot(e);
-        if (e) ends[n_ends + __popcll(be & ((1ULL << lane) - 1))] = (uint16_t)x;
+    if (WC == 16) {
+        uint32_t x0, segm;
+        int64_t j0;
+        uint32_t endm = lane_end_mask16(t, &x0, &j0, &segm);
+        // exclusive prefix of the lanes' end counts
+        const uint32_t cnt = __popc(endm);
+        uint32_t inc = cnt;
+        for (uint32_t d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(inc, d);
+            if (lane >= d) inc += y;
+        }
+        uint32_t o = inc - cnt;
+        while (endm) {
+            const uint32_t i = __ffs(endm) - 1;
+            endm &= endm - 1;
+            ends[o++] = (uint16_t)(x0 + i);
+        }
+        if (lane == 63) n_ends = inc;
         __syncthreads();
-        if (lane == 0) n_ends += (uint32_t)__popcll(be);
+    } else {
+        if (lane == 0) n_ends = 0;
         __syncthreads();
+        for (uint32_t c = 0; c < A.T; c += 64) {
+            const uint32_t x = w + c + lane;
+            const bool e = t.t0 + c + lane < (int64_t)A.n && tile_is_end(t, x);
+            const uint64_t be = __ballot(e);
+            if (e) ends[n_ends + __popcll(be & ((1ULL << lane) - 1))] = (uint16_t)x;
+            __syncthreads();
+            if (lane == 0) n_ends += (uint32_t)__popcll(be);
+            __syncthreads();
+        }
     }
     const uint32_t n_end = n_ends;
     for (uint32_t ei = lane; ei < n_end; ei += 64) {
