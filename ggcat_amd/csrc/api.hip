@@ -203,9 +203,9 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     ctx->P = p;
     const uint32_t w = p.k - p.m;
     const uint32_t n_sub = ((1u << p.buckets_log) + 1) << p.second_buckets_log;
-    // physical units: about 4000 input bytes per unit, at most 2^23 units
+    // physical units: about 2000 input bytes per unit, at most 2^23 units
     uint32_t xb = 0;
-    while (xb < 8 && (n_bytes >> xb) / n_sub > 4000 && ((uint64_t)n_sub << (xb + 1)) <= (1u << 23)) xb++;
+    while (xb < 8 && (n_bytes >> xb) / n_sub > 2000 && ((uint64_t)n_sub << (xb + 1)) <= (1u << 23)) xb++;
     const uint32_t n_units = n_sub << xb;
     // slot: meta word + inline bases for the longest unique-minimizer record (2k - m)
     uint32_t sw = 1 + (2 * p.k - p.m + 15) / 16;

@@ -340,10 +340,9 @@ GG_DEV void emit_solid(const MergeArgs& A, uint32_t sub, KP klo, KP khi, SP st, 
 // the low bits).  An insert that needs more than `probe_limit` probes sets
 // *overflow and insertion stops (the entry is then redone as finer parts).
 template <class KP, class SP, class VP>
-GG_DEV void insert_item(const MergeArgs& A, const WorkItem& it, const PartWork& w, KP klo, KP khi, SP st, VP vals,
-                        uint32_t* cbits, uint32_t mask, uint32_t probe_limit, uint32_t* overflow) {
+GG_DEV void insert_item(const MergeArgs& A, uint64_t r0, uint64_t r1, const PartWork& w, KP klo, KP khi, SP st,
+                        VP vals, uint32_t* cbits, uint32_t mask, uint32_t probe_limit, uint32_t* overflow) {
     const uint32_t W = A.color_words;
-    const uint64_t r0 = A.unit_off[it.u0], r1 = A.unit_off[it.u1];
     const uint32_t SW = A.slot_words;
     const uint32_t part_mask = (1u << w.log_parts) - 1;
     for (uint64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
@@ -368,9 +367,30 @@ GG_DEV void insert_item(const MergeArgs& A, const WorkItem& it, const PartWork& 
     }
 }
 
+// Work entry with the metadata a pass needs, fetched ahead by thread 0.
+struct Fetched {
+    uint32_t e;
+    PartWork w;
+    WorkItem it;
+    uint64_t r0, r1;
+};
+
+GG_DEV void fetch_meta(const MergeArgs& A, Fetched& f) {
+    if (f.e >= A.n_work) return;
+    f.w = A.parts ? A.parts[f.e] : PartWork{f.e, 0u, 0u, 0u};
+    f.it = A.items[f.w.item];
+    f.r0 = A.unit_off[f.it.u0];
+    f.r1 = A.unit_off[f.it.u1];
+}
+
+// Persistent work-stealing merge: each workgroup takes work entries from a
+// global cursor.  Thread 0 claims the next entry when the current one starts
+// and loads its metadata between insertion and emission, so the claim and the
+// metadata loads overlap the current entry's work.
 __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
     extern __shared__ __align__(16) uint8_t lds[];
-    __shared__ uint32_t s_item, s_overflow;
+    __shared__ uint32_t s_overflow;
+    __shared__ Fetched s_cur;
     __shared__ EmitState es;
     const uint32_t cap_max = A.lds_cap;  // the host allocates this many entries plus a margin
     uint64_t* klo = reinterpret_cast<uint64_t*>(lds);
@@ -379,23 +399,31 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
     uint32_t* st = vals + cap_max;  // only used for 128-bit keys
     uint32_t* cbits = vals + (size_t)cap_max * A.key_words;  // color_words per entry (colored builds)
     const uint32_t W = A.color_words;
-    if (threadIdx.x == 0) {
+    const bool leader = threadIdx.x == 0;
+    Fetched nxt{};
+    if (leader) {
         es.chunk = ~0u;
         es.fill = 0;
+        nxt.e = atomicAdd(A.work, 1u);
+        fetch_meta(A, nxt);
     }
     while (true) {
-        if (threadIdx.x == 0) {
-            s_item = atomicAdd(A.work, 1u);
+        if (leader) {
+            s_cur = nxt;
             s_overflow = 0;
+            nxt.e = (nxt.e < A.n_work) ? atomicAdd(A.work, 1u) : A.n_work;
         }
         __syncthreads();
-        const uint32_t e = s_item;
+        const Fetched cur = s_cur;
         __syncthreads();
-        if (e >= A.n_work) break;
-        const PartWork w = A.parts ? A.parts[e] : PartWork{e, 0u, 0u, 0u};
-        const WorkItem it = A.items[w.item];
+        if (cur.e >= A.n_work) break;
+        const PartWork w = cur.w;
+        const WorkItem it = cur.it;
         if (!A.parts && it.occ > A.lds_try) {
-            if (threadIdx.x == 0) A.redo[atomicAdd(A.redo_count, 1u)] = w;
+            if (leader) {
+                A.redo[atomicAdd(A.redo_count, 1u)] = w;
+                fetch_meta(A, nxt);
+            }
             continue;
         }
         uint32_t cap = 256;
@@ -409,17 +437,18 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
         }
         for (uint32_t i = threadIdx.x; i < cap * W; i += blockDim.x) cbits[i] = 0;
         __syncthreads();
-        insert_item(A, it, w, klo, khi, st, vals, cbits, mask, 96u, &s_overflow);
+        insert_item(A, cur.r0, cur.r1, w, klo, khi, st, vals, cbits, mask, 96u, &s_overflow);
+        if (leader) fetch_meta(A, nxt);
         __syncthreads();
         if (s_overflow) {
             // more distinct k-mers than the table holds: redone as finer parts
-            if (threadIdx.x == 0) A.redo[atomicAdd(A.redo_count, 1u)] = w;
+            if (leader) A.redo[atomicAdd(A.redo_count, 1u)] = w;
         } else {
             emit_solid(A, it.sub, klo, khi, st, vals, cbits, cap, es);
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) close_chunk(A, es);
+    if (leader) close_chunk(A, es);
 }
 
 // One thread per sub-bucket: runs of its units become work items of at most
