@@ -81,6 +81,8 @@ def main() -> None:
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run one 1 GiB device-to-device copy first (HBM counter calibration under rocprofv3 --pmc)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; multi-GPU sharding is a design "
@@ -95,6 +97,12 @@ def main() -> None:
         f"b={wl.buckets_log} sb={wl.second_buckets_log} ({time.perf_counter() - t:.1f} s to generate)")
     params = Params(wl.k, wl.m, wl.buckets_log, wl.second_buckets_log, 0, wl.hash_kind, wl.min_mult)
     ctx = GpuContext(0)
+    if a.calib_copy:
+        src, dst = ctx.alloc(1 << 30), ctx.alloc(1 << 30)
+        ctx.copy_device(dst, src, 1 << 30)
+        ctx.free(src)
+        ctx.free(dst)
+        log("[bench] calibration copy (copy16_kernel): 1073741824 bytes read, 1073741824 bytes written")
     t = time.perf_counter()
     dptr, nbytes = ctx.upload(wl.data)
     h2d_s = time.perf_counter() - t
@@ -139,7 +147,7 @@ def main() -> None:
         log(f"[bench] step {i + 1}/{a.steps} done")
     dt = time.perf_counter() - t0
     cnt = ctx.counters()
-    n, kw = ctx.solid().mult.size, (1 if (wl.k <= 32 and wl.hash_kind == 0) else 2)
+    n, kw = int(cnt[5]), (1 if (wl.k <= 32 and wl.hash_kind == 0) else 2)
     ms = np.mean(np.array(per_kernel), axis=0)
     kb = kernel_bytes(cnt, kw)
     kern_ms = {"bucket_count": ms[0], "bucket_emit": ms[2], "merge": ms[3] + ms[4]}

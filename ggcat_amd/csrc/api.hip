@@ -423,7 +423,9 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     const uint32_t W = ctx->colored ? ctx->color_words : 0;
     const size_t entry_bytes = 12 * key_words + 4 * W;
     M.lds_cap = key_words == 1 ? 2048 : 1024;
+    if (ctx->tune.lds_table_log) M.lds_cap = 1u << std::min<uint32_t>(std::max<uint32_t>(ctx->tune.lds_table_log, 8), 13);
     while (M.lds_cap > 256 && M.lds_cap * entry_bytes > 48 * 1024) M.lds_cap >>= 1;
+    M.table_pct = ctx->tune.table_pct ? ctx->tune.table_pct : 200;
     M.color_words = W;
     M.slot_color = W ? P_<uint32_t>(ctx->slot_color) : nullptr;
     M.lds_try = ctx->tune.lds_max_occ ? ctx->tune.lds_max_occ : M.lds_cap * 16;
@@ -464,7 +466,10 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     HIPCHK(hipStreamSynchronize(ctx->stream));
     M.items = P_<WorkItem>(ctx->items);
 
-    const uint32_t lds_wgs = ctx->n_cu * (ctx->tune.merge_wgs_per_cu ? ctx->tune.merge_wgs_per_cu : 6);
+    // resident workgroups per CU: what 150 KiB of the 160 KiB LDS holds, at
+    // most 8 (32 waves of 64)
+    const uint32_t fit = (uint32_t)std::max<size_t>(1, std::min<size_t>(8, (150 * 1024) / lds_bytes));
+    const uint32_t lds_wgs = ctx->n_cu * (ctx->tune.merge_wgs_per_cu ? ctx->tune.merge_wgs_per_cu : fit);
     uint64_t n_chunks = std::max<uint64_t>((ctx->kmer_occ / 6) / OUT_CHUNK + 2 * lds_wgs + 4096 + 64,
                                            ctx->out_val.cap / 4 / OUT_CHUNK);
     uint64_t out_cap = n_chunks * OUT_CHUNK;
@@ -751,6 +756,26 @@ int ggcat_gpu_device_free(ggcat_gpu_ctx* ctx, void* p) {
     if (!ctx) return GGCAT_GPU_EINVAL;
     HIPCHK(hipSetDevice(ctx->device));
     if (p) HIPCHK(hipFree(p));
+    return GGCAT_GPU_OK;
+}
+
+// Plain device-to-device copy (16-byte vectors, grid-stride).
+__global__ void copy16_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src, uint64_t n16) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+int ggcat_gpu_copy_device(ggcat_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return GGCAT_GPU_EINVAL;
+    if ((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src) | bytes) & 15)
+        return fail(ctx, GGCAT_GPU_EINVAL, "device copy needs 16-byte aligned pointers and size");
+    HIPCHK(hipSetDevice(ctx->device));
+    if (bytes) {
+        hipLaunchKernelGGL(copy16_kernel, dim3(ctx->n_cu * 8), dim3(256), 0, ctx->stream, reinterpret_cast<uint4*>(dst),
+                           reinterpret_cast<const uint4*>(src), bytes / 16);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     return GGCAT_GPU_OK;
 }
 

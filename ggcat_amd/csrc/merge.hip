@@ -57,6 +57,7 @@ struct MergeArgs {
     uint32_t key_words;  // 1 (u64 keys) or 2 (u128 keys)
     uint32_t lds_cap;  // table entries in LDS (power of two)
     uint32_t lds_try;  // first pass: items with more occurrences go straight to partitioned passes
+    uint32_t table_pct;  // table slots per 100 occurrences of a whole-item pass (power-of-two rounded)
     const WorkItem* items;
     const PartWork* parts;  // null: first pass, one work entry per item
     uint32_t n_work;  // entries of this pass
@@ -427,7 +428,8 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
             continue;
         }
         uint32_t cap = 256;
-        while (cap < 2 * it.occ && cap < cap_max) cap <<= 1;
+        const uint64_t want = ((uint64_t)it.occ * A.table_pct + 99) / 100;
+        while (cap < want && cap < cap_max) cap <<= 1;
         if (w.log_parts) cap = cap_max;
         const uint32_t mask = cap - 1;
         for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) {

@@ -63,6 +63,8 @@ class Tuning(C.Structure):
         ("lds_max_occ", C.c_uint32),
         ("merge_wgs_per_cu", C.c_uint32),
         ("part_occ", C.c_uint32),
+        ("lds_table_log", C.c_uint32),
+        ("table_pct", C.c_uint32),
     ]
 
 
@@ -100,6 +102,7 @@ def gpu_lib(path: str = GPU_LIB) -> C.CDLL:
     lib.ggcat_gpu_device_alloc.argtypes = [V, C.c_uint64, C.POINTER(V)]
     lib.ggcat_gpu_device_free.argtypes = [V, V]
     lib.ggcat_gpu_copy_to_device.argtypes = [V, V, V, C.c_uint64]
+    lib.ggcat_gpu_copy_device.argtypes = [V, V, V, C.c_uint64]
     lib.ggcat_gpu_read_files.argtypes = [V, C.POINTER(C.c_char_p), C.c_uint32, C.c_uint32, C.c_int,
                                          C.POINTER(_u8p), _u64p, C.POINTER(_u64p), C.POINTER(_u32p), _u32p, _u64p]
     lib.ggcat_gpu_free_host.argtypes = [V]
@@ -113,7 +116,8 @@ def gpu_lib(path: str = GPU_LIB) -> C.CDLL:
                  "ggcat_gpu_create", "ggcat_gpu_bucketing", "ggcat_gpu_records_info", "ggcat_gpu_records_copy",
                  "ggcat_gpu_kmers_merge", "ggcat_gpu_solid_info", "ggcat_gpu_solid_copy", "ggcat_gpu_timings",
                  "ggcat_gpu_counters", "ggcat_gpu_read_files", "ggcat_gpu_build_from_files",
-                 "ggcat_gpu_plan_build", "ggcat_gpu_set_tuning", "ggcat_gpu_solid_colors"):
+                 "ggcat_gpu_plan_build", "ggcat_gpu_set_tuning", "ggcat_gpu_solid_colors",
+                 "ggcat_gpu_copy_device"):
         getattr(lib, name).restype = C.c_int
     _gpu[path] = lib
     return lib
@@ -259,8 +263,8 @@ class GpuContext:
         return used
 
     def set_tuning(self, item_occ: int = 0, lds_max_occ: int = 0, merge_wgs_per_cu: int = 0,
-                   part_occ: int = 0) -> None:
-        t = Tuning(item_occ, lds_max_occ, merge_wgs_per_cu, part_occ)
+                   part_occ: int = 0, lds_table_log: int = 0, table_pct: int = 0) -> None:
+        t = Tuning(item_occ, lds_max_occ, merge_wgs_per_cu, part_occ, lds_table_log, table_pct)
         self._check(self.lib.ggcat_gpu_set_tuning(self.h, C.byref(t)), "ggcat_gpu_set_tuning")
 
     def records(self) -> Records:
@@ -303,6 +307,14 @@ class GpuContext:
         self._check(self.lib.ggcat_gpu_copy_to_device(self.h, p, data.ctypes.data_as(C.c_void_p), data.nbytes),
                     "copy_to_device")
         return p.value, data.nbytes
+
+    def alloc(self, nbytes: int) -> int:
+        p = C.c_void_p()
+        self._check(self.lib.ggcat_gpu_device_alloc(self.h, nbytes, C.byref(p)), "device_alloc")
+        return p.value
+
+    def copy_device(self, dst: int, src: int, nbytes: int) -> None:
+        self._check(self.lib.ggcat_gpu_copy_device(self.h, C.c_void_p(dst), C.c_void_p(src), nbytes), "copy_device")
 
     def free(self, ptr: int) -> None:
         self._check(self.lib.ggcat_gpu_device_free(self.h, C.c_void_p(ptr)), "device_free")

@@ -133,12 +133,21 @@ int ggcat_gpu_solid_colors(ggcat_gpu_ctx* ctx, uint32_t* color_words, uint32_t* 
  *   part_occ          target occurrences per partition of an item that is
  *                     split into hash-partitioned passes; default half the
  *                     LDS table (a partition that still overflows is split
- *                     4 ways again) */
+ *                     4 ways again)
+ *   lds_table_log     log2 of the LDS table entries per workgroup (8..13;
+ *                     default 11 for 64-bit keys, 10 for 128-bit; reduced
+ *                     to fit 48 KiB with colors)
+ *   table_pct         table slots per 100 occurrences of an item, rounded up
+ *                     to a power of two (default 200: never overflows when
+ *                     every k-mer is distinct; lower values trade overflow
+ *                     redos for smaller tables) */
 typedef struct {
     uint32_t item_occ;
     uint32_t lds_max_occ;
     uint32_t merge_wgs_per_cu;
     uint32_t part_occ;
+    uint32_t lds_table_log;
+    uint32_t table_pct;
 } ggcat_gpu_tuning;
 int ggcat_gpu_set_tuning(ggcat_gpu_ctx* ctx, const ggcat_gpu_tuning* tuning);
 
@@ -203,6 +212,9 @@ int ggcat_gpu_build_from_files(ggcat_gpu_ctx* ctx, const char* const* paths, uin
 int ggcat_gpu_device_alloc(ggcat_gpu_ctx* ctx, uint64_t bytes, void** out);
 int ggcat_gpu_device_free(ggcat_gpu_ctx* ctx, void* p);
 int ggcat_gpu_copy_to_device(ggcat_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+/* Device-to-device copy on the context's stream with a plain copy kernel
+ * (16-byte aligned pointers and size); returns when the copy is done. */
+int ggcat_gpu_copy_device(ggcat_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 
 #ifdef __cplusplus
 }
