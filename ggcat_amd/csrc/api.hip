@@ -3,10 +3,14 @@
 #include "../../include/ggcat_gpu.h"
 #include "bucketing.hip"
 #include "merge.hip"
+#include "query.hip"
 #include "fasta.cpp"
 
 #include <algorithm>
 #include <cstdio>
+#include <deque>
+#include <map>
+#include <memory>
 #include <cstring>
 #include <string>
 #include <tuple>
@@ -48,6 +52,13 @@ struct ggcat_gpu_ctx {
     uint32_t color_words = 0;  // bitmap words per solid entry of a colored build
     double ms[8] = {0};
     ggcat_gpu_tuning tune{};  // zero fields select the defaults
+    uint32_t n_color_ids = 0;  // largest color id + 1 of the colored build
+    // query index over the solid table
+    DevBuf idx_key, idx_pos, idx_boff, idx_bcount, idx_bcursor;
+    bool have_index = false;
+    uint32_t idx_log_buckets = 0;
+    DevBuf q_seq, q_off, q_matched, q_counts, q_work;
+    struct Server* server = nullptr;
 };
 
 namespace {
@@ -140,6 +151,193 @@ int device_scan(ggcat_gpu_ctx* ctx, const uint32_t* counts, uint64_t* off, uint3
     return GGCAT_GPU_OK;
 }
 
+// rabin-karp128 constants (crates/hashes/src/cn_rkhash.rs:65-78) and
+// rmmult = MULT^(k-1) (crates/hashes/src/lib.rs:167-191).
+template <class Args>
+void set_rk_constants(Args& M, uint32_t k) {
+    M.rk_mult = mk128(0x3eb9402f3e733993ULL, 0xadd64d3ca00e1b6bULL);
+    M.rk_mult_inv = mk128(0x09cb6ff6f1b1a6d7ULL, 0x33e0952e899c3943ULL);
+    const u128_t A_ = mk128(0x4751137d01d863c5ULL, 0xb8c36de2b7d399dfULL);
+    const u128_t C_ = mk128(0x37ea3a13226503fbULL, 0x783f5cb69f4552bdULL);
+    const u128_t G_ = mk128(0x50796b285343f09aULL, 0x0c53113ae736572bULL);
+    const u128_t T_ = mk128(0x1e62d96a5e1f5adeULL, 0x2d4e68d8f88110b7ULL);
+    M.rk_fwd[0] = A_; M.rk_fwd[1] = C_; M.rk_fwd[2] = T_; M.rk_fwd[3] = G_;
+    M.rk_bkw[0] = T_; M.rk_bkw[1] = G_; M.rk_bkw[2] = A_; M.rk_bkw[3] = C_;
+    u128_t r = 1, b = M.rk_mult;
+    for (uint32_t e = k - 1; e; e >>= 1, b *= b)
+        if (e & 1) r *= b;
+    M.rk_rmmult = r;
+}
+
+// Query records of a joined buffer: start offsets [n + 1] (entry q + 1 is one
+// past query q's separator).  False when the separator count differs from n
+// or the buffer does not end with a separator.
+bool query_offsets(const uint8_t* q, uint64_t n_bytes, uint32_t n, std::vector<uint64_t>& off) {
+    off.clear();
+    off.reserve((size_t)n + 1);
+    off.push_back(0);
+    for (uint64_t i = 0; i < n_bytes; i++)
+        if (q[i] == '\n') off.push_back(i + 1);
+    return off.size() == (size_t)n + 1 && (n_bytes == 0 || q[n_bytes - 1] == '\n');
+}
+
+void query_kmer_counts(const std::vector<uint64_t>& off, uint32_t k, uint64_t* out) {
+    for (size_t q = 0; q + 1 < off.size(); q++) {
+        const uint64_t len = off[q + 1] - off[q] - 1;
+        out[q] = len >= k ? len - (k - 1) : 0;
+    }
+}
+
+QueryArgs query_args(ggcat_gpu_ctx* ctx) {
+    QueryArgs Q{};
+    Q.k = ctx->P.k;
+    Q.kind = ctx->P.hash_kind;
+    Q.key_words = ctx->key_words;
+    Q.idx_key = P_<uint64_t>(ctx->idx_key);
+    Q.idx_pos = P_<uint32_t>(ctx->idx_pos);
+    Q.boff = P_<uint64_t>(ctx->idx_boff);
+    Q.log_buckets = ctx->idx_log_buckets;
+    Q.color_words = ctx->colored ? ctx->color_words : 0;
+    Q.out_colors = ctx->colored ? P_<uint32_t>(ctx->out_colors) : nullptr;
+    Q.n_colors = ctx->colored ? ctx->n_color_ids : 0;
+    set_rk_constants(Q, ctx->P.k);
+    return Q;
+}
+
+// First-fit allocator over one pre-allocated range (offsets 256-aligned).
+struct RangeAlloc {
+    std::map<uint64_t, uint64_t> free_;  // offset -> size
+    void init(uint64_t size) {
+        free_.clear();
+        free_[0] = size;
+    }
+    bool alloc(uint64_t want, uint64_t* off) {
+        want = (want + 255) & ~255ull;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < want) continue;
+            *off = it->first;
+            const uint64_t rest = it->second - want, at = it->first + want;
+            free_.erase(it);
+            if (rest) free_[at] = rest;
+            return true;
+        }
+        return false;
+    }
+    void release(uint64_t off, uint64_t size) {
+        size = (size + 255) & ~255ull;
+        auto it = free_.emplace(off, size).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+struct ServeReq {
+    std::vector<uint8_t> data;
+    std::vector<uint64_t> off;
+    uint32_t n_queries = 0;
+    bool admitted = false, done = false;
+    int stream = -1;
+    uint64_t dev_off = 0, pin_off = 0, bytes = 0;
+    // layout inside both ranges
+    uint64_t o_seq = 0, o_off = 0, o_work = 0, o_matched = 0, o_counts = 0;
+    hipEvent_t ev = nullptr;
+};
+
+}  // namespace
+
+struct Server {
+    std::vector<hipStream_t> streams;
+    std::vector<uint32_t> load;  // admitted requests per stream
+    uint8_t* dev = nullptr;
+    uint8_t* pin = nullptr;
+    uint64_t budget = 0;
+    RangeAlloc dalloc, palloc;
+    std::deque<uint64_t> waiting;
+    std::map<uint64_t, ServeReq> reqs;
+    std::vector<hipEvent_t> free_events;
+    uint64_t next_ticket = 1;
+    uint32_t n_admitted = 0;
+};
+
+namespace {
+
+// Admits waiting requests in arrival order while their memory fits.
+int serve_admit(ggcat_gpu_ctx* ctx) {
+    Server& S = *ctx->server;
+    while (!S.waiting.empty()) {
+        ServeReq& r = S.reqs[S.waiting.front()];
+        const uint32_t nq = r.n_queries, nc = ctx->colored ? ctx->n_color_ids : 0;
+        uint64_t o = 0;
+        auto place = [&](uint64_t bytes) {
+            const uint64_t at = o;
+            o = (o + bytes + 255) & ~255ull;
+            return at;
+        };
+        r.o_seq = place(r.data.size() + 16);
+        r.o_off = place(((uint64_t)nq + 1) * 8);
+        r.o_work = place(8);
+        r.o_matched = place((uint64_t)nq * 8);
+        r.o_counts = place((uint64_t)nq * nc * 4 + 4);
+        r.bytes = o;
+        if (r.bytes > S.budget) return fail(ctx, GGCAT_GPU_ENOMEM, "request larger than the serving memory budget");
+        uint64_t doff, poff;
+        if (!S.dalloc.alloc(r.bytes, &doff)) break;
+        if (!S.palloc.alloc(r.bytes, &poff)) {
+            S.dalloc.release(doff, r.bytes);
+            break;
+        }
+        r.dev_off = doff;
+        r.pin_off = poff;
+        uint8_t* hp = S.pin + poff;
+        memcpy(hp + r.o_seq, r.data.data(), r.data.size());
+        memcpy(hp + r.o_off, r.off.data(), r.off.size() * 8);
+        memset(hp + r.o_work, 0, 8);
+        int si = 0;
+        for (int i = 1; i < (int)S.streams.size(); i++)
+            if (S.load[i] < S.load[si]) si = i;
+        r.stream = si;
+        hipStream_t st = S.streams[si];
+        uint8_t* dp = S.dev + doff;
+        HIPCHK(hipMemcpyAsync(dp, hp, r.o_matched, hipMemcpyHostToDevice, st));  // seq, offsets, work counter
+        QueryArgs Q = query_args(ctx);
+        Q.seq = dp + r.o_seq;
+        Q.qoff = reinterpret_cast<const uint64_t*>(dp + r.o_off);
+        Q.n_queries = nq;
+        Q.matched = reinterpret_cast<uint64_t*>(dp + r.o_matched);
+        Q.color_counts = reinterpret_cast<uint32_t*>(dp + r.o_counts);
+        Q.work = reinterpret_cast<unsigned int*>(dp + r.o_work);
+        if (nq) {
+            const uint32_t grid = std::min<uint32_t>(nq, ctx->n_cu * 4);
+            hipLaunchKernelGGL(query_kernel, dim3(grid), dim3(256), (size_t)nc * 4 + 16, st, Q);
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipMemcpyAsync(hp + r.o_matched, dp + r.o_matched, r.bytes - r.o_matched, hipMemcpyDeviceToHost, st));
+        if (S.free_events.empty()) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            S.free_events.push_back(e);
+        }
+        r.ev = S.free_events.back();
+        S.free_events.pop_back();
+        HIPCHK(hipEventRecord(r.ev, st));
+        r.admitted = true;
+        S.load[si]++;
+        S.n_admitted++;
+        S.waiting.pop_front();
+    }
+    return GGCAT_GPU_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -171,7 +369,10 @@ void ggcat_gpu_destroy(ggcat_gpu_ctx* ctx) {
                       &ctx->slot_color, &ctx->color_starts, &ctx->colors, &ctx->error_flag, &ctx->items,
                       &ctx->n_items, &ctx->work, &ctx->parts, &ctx->redo, &ctx->redo_count, &ctx->out_key,
                       &ctx->out_val, &ctx->out_sub, &ctx->chunk_cursor, &ctx->chunk_fill_dev, &ctx->fix_key,
-                      &ctx->fix_val, &ctx->fix_sub, &ctx->fix_cursor, &ctx->out_colors, &ctx->fix_colors};
+                      &ctx->fix_val, &ctx->fix_sub, &ctx->fix_cursor, &ctx->out_colors, &ctx->fix_colors,
+                      &ctx->idx_key, &ctx->idx_pos, &ctx->idx_boff, &ctx->idx_bcount, &ctx->idx_bcursor,
+                      &ctx->q_seq, &ctx->q_off, &ctx->q_matched, &ctx->q_counts, &ctx->q_work};
+    if (ctx->server) (void)ggcat_gpu_serve_close(ctx);
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (int i = 0; i < 16; i++) (void)hipEventDestroy(ctx->ev[i]);
@@ -199,7 +400,8 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     if (input_on_device && (reinterpret_cast<uintptr_t>(data) & 15))
         return fail(ctx, GGCAT_GPU_EINVAL, "device input must be 16-byte aligned");
     HIPCHK(hipSetDevice(ctx->device));
-    ctx->have_records = ctx->have_solid = false;
+    if (ctx->server) return fail(ctx, GGCAT_GPU_ESTATE, "close the query server before a new build");
+    ctx->have_records = ctx->have_solid = ctx->have_index = false;
     ctx->P = p;
     const uint32_t w = p.k - p.m;
     const uint32_t n_sub = ((1u << p.buckets_log) + 1) << p.second_buckets_log;
@@ -218,12 +420,14 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     ctx->n_input = n_bytes;
     ctx->colored = n_colors > 0;
     ctx->color_words = 0;
+    ctx->n_color_ids = 0;
     if (n_colors) {
         uint32_t max_color = 0;
         for (uint32_t i = 0; i < n_colors; i++) max_color = std::max(max_color, colors[i]);
         if (max_color >= MAX_COLORS)
             return fail(ctx, GGCAT_GPU_EINVAL, "color ids must be < " + std::to_string(MAX_COLORS));
         ctx->color_words = max_color / 32 + 1;
+        ctx->n_color_ids = max_color + 1;
     }
 
     const uint8_t* d_in = data;
@@ -407,6 +611,8 @@ int ggcat_gpu_records_copy(ggcat_gpu_ctx* ctx, uint32_t* sub, uint32_t* len, uin
 int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     if (!ctx) return GGCAT_GPU_EINVAL;
     if (!ctx->have_records) return fail(ctx, GGCAT_GPU_ESTATE, "no bucketing result");
+    if (ctx->server) return fail(ctx, GGCAT_GPU_ESTATE, "close the query server before a new merge");
+    ctx->have_index = false;
     HIPCHK(hipSetDevice(ctx->device));
     const ggcat_gpu_params& p = ctx->P;
     const uint32_t key_words = (p.hash_kind == GGCAT_GPU_HASH_SEQ && p.k <= 32) ? 1 : 2;
@@ -851,6 +1057,233 @@ int ggcat_gpu_build_from_files(ggcat_gpu_ctx* ctx, const char* const* paths, uin
                                 o.colors ? R.colors.data() : nullptr, o.colors ? (uint32_t)R.colors.size() : 0, &p);
     if (r != GGCAT_GPU_OK) return r;
     return ggcat_gpu_kmers_merge(ctx);
+}
+
+int ggcat_gpu_index_build(ggcat_gpu_ctx* ctx) {
+    if (!ctx) return GGCAT_GPU_EINVAL;
+    if (!ctx->have_solid) return fail(ctx, GGCAT_GPU_ESTATE, "no merge result");
+    if (ctx->server) return fail(ctx, GGCAT_GPU_ESTATE, "close the query server before rebuilding the index");
+    HIPCHK(hipSetDevice(ctx->device));
+    const uint32_t kw = ctx->key_words;
+    const uint64_t n = ctx->n_solid;
+    uint32_t L = 0;
+    while (L < 30 && (1ull << (L + 2)) < n) L++;  // about 4 entries per bucket
+    const uint32_t nb = 1u << L;
+    ENSURE(ctx->idx_bcount, (size_t)nb * 4);
+    ENSURE(ctx->idx_bcursor, (size_t)nb * 4);
+    ENSURE(ctx->idx_boff, ((size_t)nb + 1) * 8);
+    ENSURE(ctx->idx_key, n * 8 * kw + 8);
+    ENSURE(ctx->idx_pos, n * 4 + 4);
+    if (ctx->n_chunks_used) {
+        ENSURE(ctx->chunk_fill_dev, (size_t)ctx->n_chunks_used * 4);
+        HIPCHK(hipMemcpy(ctx->chunk_fill_dev.p, ctx->chunk_fill.data(), (size_t)ctx->n_chunks_used * 4,
+                         hipMemcpyHostToDevice));
+    }
+    IndexArgs I{};
+    I.out_key = P_<uint64_t>(ctx->out_key);
+    I.chunk_fill = P_<uint32_t>(ctx->chunk_fill_dev);
+    I.n_chunks = ctx->n_chunks_used;
+    I.n_tail = ctx->n_tail;
+    I.chunk = OUT_CHUNK;
+    I.key_words = kw;
+    I.log_buckets = L;
+    I.bcount = P_<uint32_t>(ctx->idx_bcount);
+    I.boff = P_<uint64_t>(ctx->idx_boff);
+    I.bcursor = P_<uint32_t>(ctx->idx_bcursor);
+    I.idx_key = P_<uint64_t>(ctx->idx_key);
+    I.idx_pos = P_<uint32_t>(ctx->idx_pos);
+    const uint64_t n_pos = (uint64_t)ctx->n_chunks_used * OUT_CHUNK + ctx->n_tail;
+    if (n_pos >= 0xFFFFFFFFull) return fail(ctx, GGCAT_GPU_EINVAL, "solid table too large for the query index");
+    HIPCHK(hipMemsetAsync(ctx->idx_bcount.p, 0, (size_t)nb * 4, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->idx_bcursor.p, 0, (size_t)nb * 4, ctx->stream));
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n_pos + 255) / 256 + 1, (uint64_t)ctx->n_cu * 16);
+    if (n_pos) hipLaunchKernelGGL(index_count_kernel, dim3(grid), dim3(256), 0, ctx->stream, I, n_pos);
+    HIPCHK(hipGetLastError());
+    {
+        int r = device_scan(ctx, I.bcount, P_<uint64_t>(ctx->idx_boff), nb);
+        if (r != GGCAT_GPU_OK) return r;
+    }
+    if (n_pos) hipLaunchKernelGGL(index_fill_kernel, dim3(grid), dim3(256), 0, ctx->stream, I, n_pos);
+    HIPCHK(hipGetLastError());
+    uint64_t total = 0;
+    HIPCHK(hipMemcpyAsync(&total, P_<uint64_t>(ctx->idx_boff) + nb, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (total != n) return fail(ctx, GGCAT_GPU_EHIP, "query index holds a different number of entries than the solid table");
+    ctx->idx_log_buckets = L;
+    ctx->have_index = true;
+    return GGCAT_GPU_OK;
+}
+
+int ggcat_gpu_index_colors(ggcat_gpu_ctx* ctx, uint32_t* n_colors) {
+    if (!ctx || !n_colors) return GGCAT_GPU_EINVAL;
+    if (!ctx->have_index) return fail(ctx, GGCAT_GPU_ESTATE, "no query index");
+    *n_colors = ctx->colored ? ctx->n_color_ids : 0;
+    return GGCAT_GPU_OK;
+}
+
+int ggcat_gpu_query(ggcat_gpu_ctx* ctx, const uint8_t* queries, uint64_t n_bytes, uint32_t n_queries,
+                    uint64_t* query_kmers, uint64_t* matched, uint32_t* color_counts) {
+    if (!ctx || (!queries && n_bytes)) return GGCAT_GPU_EINVAL;
+    if (!ctx->have_index) return fail(ctx, GGCAT_GPU_ESTATE, "no query index (call ggcat_gpu_index_build)");
+    std::vector<uint64_t> off;
+    if (!query_offsets(queries, n_bytes, n_queries, off))
+        return fail(ctx, GGCAT_GPU_EINVAL, "queries must be n_queries records, each ending with '\\n'");
+    if (query_kmers) query_kmer_counts(off, ctx->P.k, query_kmers);
+    HIPCHK(hipSetDevice(ctx->device));
+    const uint32_t nc = ctx->colored ? ctx->n_color_ids : 0;
+    ENSURE(ctx->q_seq, n_bytes + 16);
+    ENSURE(ctx->q_off, ((size_t)n_queries + 1) * 8);
+    ENSURE(ctx->q_matched, (size_t)n_queries * 8 + 8);
+    ENSURE(ctx->q_counts, (size_t)n_queries * nc * 4 + 4);
+    ENSURE(ctx->q_work, 8);
+    if (n_bytes) HIPCHK(hipMemcpy(ctx->q_seq.p, queries, n_bytes, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->q_off.p, off.data(), off.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemsetAsync(ctx->q_work.p, 0, 8, ctx->stream));
+    QueryArgs Q = query_args(ctx);
+    Q.seq = P_<uint8_t>(ctx->q_seq);
+    Q.qoff = P_<uint64_t>(ctx->q_off);
+    Q.n_queries = n_queries;
+    Q.matched = P_<uint64_t>(ctx->q_matched);
+    Q.color_counts = P_<uint32_t>(ctx->q_counts);
+    Q.work = P_<unsigned int>(ctx->q_work);
+    if (n_queries) {
+        const uint32_t grid = std::min<uint32_t>(n_queries, ctx->n_cu * 4);
+        hipLaunchKernelGGL(query_kernel, dim3(grid), dim3(256), (size_t)nc * 4 + 16, ctx->stream, Q);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (matched && n_queries) HIPCHK(hipMemcpy(matched, ctx->q_matched.p, (size_t)n_queries * 8, hipMemcpyDeviceToHost));
+    if (color_counts && nc && n_queries)
+        HIPCHK(hipMemcpy(color_counts, ctx->q_counts.p, (size_t)n_queries * nc * 4, hipMemcpyDeviceToHost));
+    return GGCAT_GPU_OK;
+}
+
+int ggcat_gpu_serve_open(ggcat_gpu_ctx* ctx, uint32_t streams, uint64_t memory_budget) {
+    if (!ctx || streams == 0 || streams > 16 || memory_budget < 4096) return GGCAT_GPU_EINVAL;
+    if (!ctx->have_index) return fail(ctx, GGCAT_GPU_ESTATE, "no query index (call ggcat_gpu_index_build)");
+    if (ctx->server) return fail(ctx, GGCAT_GPU_ESTATE, "query server already open");
+    HIPCHK(hipSetDevice(ctx->device));
+    std::unique_ptr<Server> S(new Server());
+    S->budget = memory_budget;
+    hipError_t e = hipMalloc(&S->dev, memory_budget);
+    if (e != hipSuccess) return fail(ctx, GGCAT_GPU_ENOMEM, "serving device memory budget not available");
+    e = hipHostMalloc(&S->pin, memory_budget, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipFree(S->dev);
+        return fail(ctx, GGCAT_GPU_ENOMEM, "serving pinned staging not available");
+    }
+    for (uint32_t i = 0; i < streams; i++) {
+        hipStream_t st;
+        e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (e != hipSuccess) break;
+        S->streams.push_back(st);
+    }
+    S->load.assign(S->streams.size(), 0);
+    S->dalloc.init(memory_budget);
+    S->palloc.init(memory_budget);
+    ctx->server = S.release();
+    if (e != hipSuccess) {
+        (void)ggcat_gpu_serve_close(ctx);
+        return fail(ctx, GGCAT_GPU_EHIP, "stream creation failed");
+    }
+    return GGCAT_GPU_OK;
+}
+
+int ggcat_gpu_serve_submit(ggcat_gpu_ctx* ctx, const uint8_t* queries, uint64_t n_bytes, uint32_t n_queries,
+                           uint64_t* ticket) {
+    if (!ctx || !ticket || (!queries && n_bytes)) return GGCAT_GPU_EINVAL;
+    if (!ctx->server) return fail(ctx, GGCAT_GPU_ESTATE, "query server not open");
+    Server& S = *ctx->server;
+    const uint64_t t = S.next_ticket++;
+    ServeReq& r = S.reqs[t];
+    if (!query_offsets(queries, n_bytes, n_queries, r.off)) {
+        S.reqs.erase(t);
+        return fail(ctx, GGCAT_GPU_EINVAL, "queries must be n_queries records, each ending with '\\n'");
+    }
+    r.data.assign(queries, queries + n_bytes);
+    r.n_queries = n_queries;
+    S.waiting.push_back(t);
+    *ticket = t;
+    HIPCHK(hipSetDevice(ctx->device));
+    return serve_admit(ctx);
+}
+
+int ggcat_gpu_serve_poll(ggcat_gpu_ctx* ctx, int block, uint64_t* ticket) {
+    if (!ctx || !ticket) return GGCAT_GPU_EINVAL;
+    if (!ctx->server) return fail(ctx, GGCAT_GPU_ESTATE, "query server not open");
+    Server& S = *ctx->server;
+    HIPCHK(hipSetDevice(ctx->device));
+    int r0 = serve_admit(ctx);
+    if (r0 != GGCAT_GPU_OK) return r0;
+    uint64_t oldest = 0;
+    for (auto& kv : S.reqs) {
+        ServeReq& r = kv.second;
+        if (!r.admitted || r.done) continue;
+        if (!oldest) oldest = kv.first;
+        hipError_t q = hipEventQuery(r.ev);
+        if (q == hipSuccess) {
+            r.done = true;
+            *ticket = kv.first;
+            return GGCAT_GPU_OK;
+        }
+        if (q != hipErrorNotReady) return fail(ctx, GGCAT_GPU_EHIP, std::string("query request: ") + hipGetErrorString(q));
+    }
+    if (block && oldest) {
+        ServeReq& r = S.reqs[oldest];
+        HIPCHK(hipEventSynchronize(r.ev));
+        r.done = true;
+        *ticket = oldest;
+        return GGCAT_GPU_OK;
+    }
+    return GGCAT_GPU_EAGAIN;
+}
+
+int ggcat_gpu_serve_fetch(ggcat_gpu_ctx* ctx, uint64_t ticket, uint64_t* query_kmers, uint64_t* matched,
+                          uint32_t* color_counts) {
+    if (!ctx) return GGCAT_GPU_EINVAL;
+    if (!ctx->server) return fail(ctx, GGCAT_GPU_ESTATE, "query server not open");
+    Server& S = *ctx->server;
+    auto it = S.reqs.find(ticket);
+    if (it == S.reqs.end() || !it->second.done) return fail(ctx, GGCAT_GPU_ESTATE, "request not finished");
+    ServeReq& r = it->second;
+    const uint32_t nq = r.n_queries, nc = ctx->colored ? ctx->n_color_ids : 0;
+    const uint8_t* hp = S.pin + r.pin_off;
+    if (query_kmers) query_kmer_counts(r.off, ctx->P.k, query_kmers);
+    if (matched) memcpy(matched, hp + r.o_matched, (size_t)nq * 8);
+    if (color_counts && nc) memcpy(color_counts, hp + r.o_counts, (size_t)nq * nc * 4);
+    S.dalloc.release(r.dev_off, r.bytes);
+    S.palloc.release(r.pin_off, r.bytes);
+    S.free_events.push_back(r.ev);
+    S.load[r.stream]--;
+    S.n_admitted--;
+    S.reqs.erase(it);
+    HIPCHK(hipSetDevice(ctx->device));
+    return serve_admit(ctx);
+}
+
+int ggcat_gpu_serve_stats(ggcat_gpu_ctx* ctx, uint32_t* outstanding, uint32_t* admitted) {
+    if (!ctx) return GGCAT_GPU_EINVAL;
+    if (!ctx->server) return fail(ctx, GGCAT_GPU_ESTATE, "query server not open");
+    if (outstanding) *outstanding = (uint32_t)ctx->server->reqs.size();
+    if (admitted) *admitted = ctx->server->n_admitted;
+    return GGCAT_GPU_OK;
+}
+
+int ggcat_gpu_serve_close(ggcat_gpu_ctx* ctx) {
+    if (!ctx) return GGCAT_GPU_EINVAL;
+    Server* S = ctx->server;
+    if (!S) return GGCAT_GPU_OK;
+    (void)hipSetDevice(ctx->device);
+    for (hipStream_t st : S->streams) (void)hipStreamSynchronize(st);
+    for (auto& kv : S->reqs)
+        if (kv.second.ev) S->free_events.push_back(kv.second.ev);
+    for (hipEvent_t e : S->free_events) (void)hipEventDestroy(e);
+    for (hipStream_t st : S->streams) (void)hipStreamDestroy(st);
+    if (S->dev) (void)hipFree(S->dev);
+    if (S->pin) (void)hipHostFree(S->pin);
+    delete S;
+    ctx->server = nullptr;
+    return GGCAT_GPU_OK;
 }
 
 }  // extern "C"

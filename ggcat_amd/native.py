@@ -17,7 +17,7 @@ BUILD = os.path.join(HERE, "_build")
 GPU_LIB = os.path.join(BUILD, "libggcat_gpu.so")
 SYNTH_LIB = os.path.join(BUILD, "libggcat_synth.so")
 
-OK, EINVAL, EHIP, ENOMEM, ESTATE = 0, 1, 2, 3, 4
+OK, EINVAL, EHIP, ENOMEM, ESTATE, EAGAIN = 0, 1, 2, 3, 4, 5
 HASH_SEQ, HASH_RK128 = 0, 1
 
 _u8p = C.POINTER(C.c_uint8)
@@ -103,6 +103,15 @@ def gpu_lib(path: str = GPU_LIB) -> C.CDLL:
     lib.ggcat_gpu_device_free.argtypes = [V, V]
     lib.ggcat_gpu_copy_to_device.argtypes = [V, V, V, C.c_uint64]
     lib.ggcat_gpu_copy_device.argtypes = [V, V, V, C.c_uint64]
+    lib.ggcat_gpu_index_build.argtypes = [V]
+    lib.ggcat_gpu_index_colors.argtypes = [V, _u32p]
+    lib.ggcat_gpu_query.argtypes = [V, V, C.c_uint64, C.c_uint32, _u64p, _u64p, _u32p]
+    lib.ggcat_gpu_serve_open.argtypes = [V, C.c_uint32, C.c_uint64]
+    lib.ggcat_gpu_serve_submit.argtypes = [V, V, C.c_uint64, C.c_uint32, _u64p]
+    lib.ggcat_gpu_serve_poll.argtypes = [V, C.c_int, _u64p]
+    lib.ggcat_gpu_serve_fetch.argtypes = [V, C.c_uint64, _u64p, _u64p, _u32p]
+    lib.ggcat_gpu_serve_stats.argtypes = [V, _u32p, _u32p]
+    lib.ggcat_gpu_serve_close.argtypes = [V]
     lib.ggcat_gpu_read_files.argtypes = [V, C.POINTER(C.c_char_p), C.c_uint32, C.c_uint32, C.c_int,
                                          C.POINTER(_u8p), _u64p, C.POINTER(_u64p), C.POINTER(_u32p), _u32p, _u64p]
     lib.ggcat_gpu_free_host.argtypes = [V]
@@ -117,7 +126,9 @@ def gpu_lib(path: str = GPU_LIB) -> C.CDLL:
                  "ggcat_gpu_kmers_merge", "ggcat_gpu_solid_info", "ggcat_gpu_solid_copy", "ggcat_gpu_timings",
                  "ggcat_gpu_counters", "ggcat_gpu_read_files", "ggcat_gpu_build_from_files",
                  "ggcat_gpu_plan_build", "ggcat_gpu_set_tuning", "ggcat_gpu_solid_colors",
-                 "ggcat_gpu_copy_device"):
+                 "ggcat_gpu_copy_device", "ggcat_gpu_index_build", "ggcat_gpu_index_colors", "ggcat_gpu_query",
+                 "ggcat_gpu_serve_open", "ggcat_gpu_serve_submit", "ggcat_gpu_serve_poll", "ggcat_gpu_serve_fetch",
+                 "ggcat_gpu_serve_stats", "ggcat_gpu_serve_close"):
         getattr(lib, name).restype = C.c_int
     _gpu[path] = lib
     return lib
@@ -298,6 +309,63 @@ class GpuContext:
         out = np.zeros((n.value, w.value), np.uint32)
         self._check(self.lib.ggcat_gpu_solid_colors(self.h, C.byref(w), _ptr(out, _u32p)), "solid_colors")
         return out
+
+    # ---- queries -------------------------------------------------------
+    def index_build(self) -> None:
+        self._check(self.lib.ggcat_gpu_index_build(self.h), "ggcat_gpu_index_build")
+
+    def index_colors(self) -> int:
+        n = C.c_uint32()
+        self._check(self.lib.ggcat_gpu_index_colors(self.h, C.byref(n)), "ggcat_gpu_index_colors")
+        return n.value
+
+    def query(self, queries: np.ndarray):
+        """Joined queries -> (query_kmers, matched, color_counts (n, n_colors) or None)."""
+        q = np.ascontiguousarray(queries, np.uint8)
+        nq = int(np.count_nonzero(q == 10))
+        nc = self.index_colors()
+        qk, mt = np.zeros(nq, np.uint64), np.zeros(nq, np.uint64)
+        cc = np.zeros((nq, max(nc, 1)), np.uint32)
+        self._check(self.lib.ggcat_gpu_query(self.h, q.ctypes.data_as(C.c_void_p), q.size, nq, _ptr(qk, _u64p),
+                                             _ptr(mt, _u64p), _ptr(cc, _u32p) if nc else C.cast(None, _u32p)),
+                    "ggcat_gpu_query")
+        return qk, mt, (cc if nc else None)
+
+    def serve_open(self, streams: int = 4, memory_budget: int = 1 << 30) -> None:
+        self._check(self.lib.ggcat_gpu_serve_open(self.h, streams, memory_budget), "ggcat_gpu_serve_open")
+
+    def serve_submit(self, queries: np.ndarray) -> int:
+        q = np.ascontiguousarray(queries, np.uint8)
+        t = C.c_uint64()
+        self._check(self.lib.ggcat_gpu_serve_submit(self.h, q.ctypes.data_as(C.c_void_p), q.size,
+                                                    int(np.count_nonzero(q == 10)), C.byref(t)),
+                    "ggcat_gpu_serve_submit")
+        return t.value
+
+    def serve_poll(self, block: bool = False) -> int | None:
+        t = C.c_uint64()
+        rc = self.lib.ggcat_gpu_serve_poll(self.h, int(block), C.byref(t))
+        if rc == EAGAIN:
+            return None
+        self._check(rc, "ggcat_gpu_serve_poll")
+        return t.value
+
+    def serve_fetch(self, ticket: int, n_queries: int):
+        nc = self.index_colors()
+        qk, mt = np.zeros(n_queries, np.uint64), np.zeros(n_queries, np.uint64)
+        cc = np.zeros((n_queries, max(nc, 1)), np.uint32)
+        self._check(self.lib.ggcat_gpu_serve_fetch(self.h, ticket, _ptr(qk, _u64p), _ptr(mt, _u64p),
+                                                   _ptr(cc, _u32p) if nc else C.cast(None, _u32p)),
+                    "ggcat_gpu_serve_fetch")
+        return qk, mt, (cc if nc else None)
+
+    def serve_stats(self) -> tuple[int, int]:
+        a, b = C.c_uint32(), C.c_uint32()
+        self._check(self.lib.ggcat_gpu_serve_stats(self.h, C.byref(a), C.byref(b)), "ggcat_gpu_serve_stats")
+        return a.value, b.value
+
+    def serve_close(self) -> None:
+        self._check(self.lib.ggcat_gpu_serve_close(self.h), "ggcat_gpu_serve_close")
 
     def upload(self, data: np.ndarray) -> tuple[int, int]:
         """Copies a host array into a new device allocation; returns (ptr, nbytes)."""

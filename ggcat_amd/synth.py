@@ -142,6 +142,20 @@ def config3(scale: float = 1.0, n_genomes: int = 256) -> Workload:
                     bases, b, sb, colors=(starts, colors))
 
 
+def queries(genomes: list[np.ndarray], seed: int, n: int, lmin: int = 100, lmax: int = 10_000, mut: float = 0.02,
+            stream: int = 0) -> np.ndarray:
+    """n queries in the joined layout: lengths log-uniform in [lmin, lmax]; half
+    are substrings of the genomes with substitution rate mut, half i.i.d."""
+    cat = np.concatenate(genomes) if genomes else np.zeros(1, np.uint8)
+    off = np.zeros(len(genomes) + 1, np.uint64)
+    off[1:] = np.cumsum([g.size for g in genomes])
+    out = np.empty(n * (lmax + 1), np.uint8)
+    lens = np.empty(max(n, 1), np.uint32)
+    nb = synth_lib().synth_queries(_ptr(cat, _u8p), _ptr(off, _u64p), len(genomes), seed, stream, n, lmin, lmax, mut,
+                                   _ptr(out, _u8p), _ptr(lens, _u32p))
+    return out[:nb].copy()
+
+
 def config4(scale: float = 1.0) -> Workload:
     """500 Mbp genome with 30% repeats, 150 bp paired reads at 30x, 0.5%
     substitutions; k=63, m=21, s=2, rabin-karp128."""

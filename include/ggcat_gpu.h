@@ -31,7 +31,8 @@ enum {
     GGCAT_GPU_EINVAL = 1, /* bad argument */
     GGCAT_GPU_EHIP = 2,   /* HIP runtime error */
     GGCAT_GPU_ENOMEM = 3, /* device allocation failed */
-    GGCAT_GPU_ESTATE = 4  /* call out of order (e.g. merge before bucketing) */
+    GGCAT_GPU_ESTATE = 4, /* call out of order (e.g. merge before bucketing) */
+    GGCAT_GPU_EAGAIN = 5  /* nothing ready yet (serving poll) */
 };
 
 enum { GGCAT_GPU_HASH_SEQ = 0, GGCAT_GPU_HASH_RK128 = 1 };
@@ -204,6 +205,55 @@ int ggcat_gpu_plan_build(const ggcat_gpu_build_options* options, uint64_t estima
  * ggcat_gpu_solid_*); *used (optional) receives the derived parameters. */
 int ggcat_gpu_build_from_files(ggcat_gpu_ctx* ctx, const char* const* paths, uint32_t n_paths,
                                const ggcat_gpu_build_options* options, ggcat_gpu_params* used);
+
+/* ---- queries ------------------------------------------------------------
+ * Replaces the counting part of run_query (crates/querier/src/lib.rs:54-200):
+ * the graph is the solid k-mer table of the last merge (the k-mer set of the
+ * unitigs the reference would query), kept resident in HBM with its color
+ * sets.  Per query (records of a joined buffer): query_kmers = length - (k-1)
+ * saturating (crates/querier/src/lib.rs:138-150); matched = occurrences of
+ * the query's all-ACGT k-mers present in the graph
+ * (crates/querier/src/pipeline/parallel_kmers_query.rs:437-471); colored:
+ * color_counts[q * n_colors + c] = matched k-mers whose color set holds c
+ * (crates/querier/src/pipeline/colored_query_output.rs:200-320).  The CSV /
+ * JSONL formatting of those numbers is host code (ggcat_amd/query.py). */
+
+/* Builds the query index over the last merge's solid table (count, scan,
+ * scatter into hash buckets).  Invalidated by the next bucketing call. */
+int ggcat_gpu_index_build(ggcat_gpu_ctx* ctx);
+
+/* Colors per query of the current index (0 when uncolored): one more than the
+ * largest color id given to the colored build. */
+int ggcat_gpu_index_colors(ggcat_gpu_ctx* ctx, uint32_t* n_colors);
+
+/* Synchronous batch query.  queries: joined layout with exactly n_queries
+ * records (host memory).  query_kmers and matched hold n_queries entries;
+ * color_counts (colored index, may be NULL) n_queries * n_colors. */
+int ggcat_gpu_query(ggcat_gpu_ctx* ctx, const uint8_t* queries, uint64_t n_bytes, uint32_t n_queries,
+                    uint64_t* query_kmers, uint64_t* matched, uint32_t* color_counts);
+
+/* ---- serving: continuous batching of query requests ---------------------
+ * No counterpart in the reference (each ggcat query call re-buckets the whole
+ * graph).  Requests are admitted while their device and pinned staging
+ * memory fits in `memory_budget` bytes (allocated once by serve_open; no
+ * allocation happens per request), each on one of `streams` HIP streams, so
+ * admitted requests overlap on the device; a request that does not fit waits
+ * (first come, first admitted) until earlier ones are fetched. */
+int ggcat_gpu_serve_open(ggcat_gpu_ctx* ctx, uint32_t streams, uint64_t memory_budget);
+/* Queues one request (copies the query bytes); returns its ticket at once. */
+int ggcat_gpu_serve_submit(ggcat_gpu_ctx* ctx, const uint8_t* queries, uint64_t n_bytes, uint32_t n_queries,
+                           uint64_t* ticket);
+/* Admits waiting requests that now fit and reports one finished ticket;
+ * GGCAT_GPU_EAGAIN when none is finished (with block != 0 it waits for the
+ * oldest admitted request instead). */
+int ggcat_gpu_serve_poll(ggcat_gpu_ctx* ctx, int block, uint64_t* ticket);
+/* Copies a finished request's results (as ggcat_gpu_query) and releases its
+ * memory. */
+int ggcat_gpu_serve_fetch(ggcat_gpu_ctx* ctx, uint64_t ticket, uint64_t* query_kmers, uint64_t* matched,
+                          uint32_t* color_counts);
+/* Requests submitted and not yet fetched, and those admitted to the device. */
+int ggcat_gpu_serve_stats(ggcat_gpu_ctx* ctx, uint32_t* outstanding, uint32_t* admitted);
+int ggcat_gpu_serve_close(ggcat_gpu_ctx* ctx);
 
 /* Device-memory plumbing for callers that keep inputs resident in HBM (no
  * counterpart in the reference, which has no device memory).  The pointer

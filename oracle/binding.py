@@ -62,6 +62,8 @@ def lib() -> C.CDLL:
     L.orc_kmer_keys.argtypes = [_u8p, C.c_size_t, C.c_uint, C.c_int, _u64p, _u64p, _u8p]
     L.orc_kmer_keys.restype = None
     L.orc_bruteforce_counts.argtypes = [_u8p, _u64p, C.c_size_t, _u32p, C.c_uint, C.c_uint32]
+    L.orc_query.argtypes = [V, _u8p, C.c_uint64, C.c_uint, C.c_int, C.c_uint32, _u64p, _u64p, _u64p, C.c_uint64]
+    L.orc_query.restype = C.c_uint64
     L.orc_rec_colors.argtypes = [V, _u32p]
     L.orc_rec_colors.restype = None
     L.orc_solid_colors_total.argtypes = [V]
@@ -143,6 +145,18 @@ class Result:
         cols = np.zeros(L.orc_solid_colors_total(self.h), np.uint32)
         L.orc_solid_colors_get(self.h, _p(off, _u32p), _p(cols, _u32p))
         return off, cols
+
+    def query(self, queries: np.ndarray, k: int, kind: int = 0, n_colors: int = 0):
+        """Per query of a joined buffer: (query_kmers, matched, color_counts or None)
+        against this handle's solid k-mers (after merge())."""
+        qd = np.ascontiguousarray(queries, np.uint8)
+        nq = int(np.count_nonzero(qd == 10))
+        qk, mt = np.zeros(nq, np.uint64), np.zeros(nq, np.uint64)
+        cc = np.zeros(max(1, nq * n_colors), np.uint64)
+        got = lib().orc_query(self.h, _p(qd, _u8p), qd.size, k, kind, n_colors, _p(qk, _u64p), _p(mt, _u64p),
+                              _p(cc, _u64p), nq)
+        assert got == nq
+        return qk, mt, (cc[:nq * n_colors].reshape(nq, n_colors) if n_colors else None)
 
     def times(self) -> tuple[float, float]:
         a, b = C.c_double(), C.c_double()

@@ -647,6 +647,59 @@ void orc_times(void* hv, double* t_bucket, double* t_merge) {
 }
 void orc_free(void* hv) { delete (Handle*)hv; }
 
+// Query against the handle's solid k-mers (the graph's k-mer set).
+// crates/querier/src/lib.rs:138-150: query_kmers = len - (k-1), saturating,
+// over the raw record length; crates/querier/src/pipeline/
+// parallel_kmers_query.rs:372-402,437-471: every k-mer occurrence of an ACGT
+// run of the query (runs split at other bytes, as in minimizer bucketing)
+// whose canonical hash is in the graph counts once for the query and once
+// for the graph k-mer's color subset; colored_query_output.rs:200-320 turns
+// subset counts into per-color sums.  Queries: records of a joined buffer
+// (each followed by '\n').  Outputs per query: query_kmers[q], matched[q];
+// colored handles also fill color_counts[q * n_colors + c].
+uint64_t orc_query(void* hv, const uint8_t* joined, uint64_t n_bytes, unsigned k, int kind, uint32_t n_colors,
+                   uint64_t* query_kmers, uint64_t* matched, uint64_t* color_counts, uint64_t max_queries) {
+    Handle* h = (Handle*)hv;
+    const SolidSet& S = h->solid;
+    std::unordered_map<u128, size_t, U128Hash> graph;
+    graph.reserve(S.sub.size() * 2);
+    for (size_t i = 0; i < S.sub.size(); i++) graph.emplace(((u128)S.key_hi[i] << 64) | S.key_lo[i], i);
+    const bool colored = h->colored && !S.colors_off.empty();
+    uint64_t q = 0, start = 0;
+    std::vector<uint8_t> codes;
+    for (uint64_t p = 0; p < n_bytes && q < max_queries; p++) {
+        if (joined[p] != '\n') continue;
+        const uint8_t* s = joined + start;
+        const uint64_t len = p - start;
+        query_kmers[q] = len >= k ? len - (k - 1) : 0;
+        matched[q] = 0;
+        if (colored)
+            for (uint32_t c = 0; c < n_colors; c++) color_counts[q * n_colors + c] = 0;
+        uint64_t i = 0;
+        while (i < len) {
+            while (i < len && normalize(s[i]) == 'N') i++;
+            const uint64_t st = i;
+            while (i < len && normalize(s[i]) != 'N') i++;
+            if (i - st < k) continue;
+            codes.resize(i - st);
+            for (uint64_t j = st; j < i; j++) codes[j - st] = cbase(normalize(s[j]));
+            kmer_hashes(codes.data(), i - st, k, kind, [&](size_t, u128 key, bool) {
+                auto it = graph.find(key);
+                if (it == graph.end()) return;
+                matched[q]++;
+                if (colored) {
+                    const size_t e = it->second;
+                    for (uint32_t o = S.colors_off[e]; o < S.colors_off[e + 1]; o++)
+                        if (S.colors[o] < n_colors) color_counts[q * n_colors + S.colors[o]]++;
+                }
+            });
+        }
+        q++;
+        start = p + 1;
+    }
+    return q;
+}
+
 // Canonical k-mer keys of one ASCII sequence (all k-mers, no N handling).
 void orc_kmer_keys(const uint8_t* ascii, size_t len, unsigned k, int kind, uint64_t* key_lo, uint64_t* key_hi,
                    uint8_t* fwd) {

@@ -194,3 +194,66 @@ def test_colored_contigs(gpu_ctx, n_genomes, scale):
     want = colored_solid_array(o.solid(), bitmaps_from_sets(off, cols, words))
     assert got.shape == want.shape
     assert np.array_equal(got, want)
+
+
+def _query_parity(gpu_ctx, data, k, m, b, sb, s, kind, colors, queries):
+    from oracle import binding as O2
+    params = _params(k, m, b, sb, s, kind)
+    gpu_ctx.bucketing(data, params, colors=colors)
+    gpu_ctx.kmers_merge()
+    gpu_ctx.index_build()
+    qk, mt, cc = gpu_ctx.query(queries)
+    if colors is None:
+        o = O2.bucketing(data, k, m, b, sb, threads=4)
+    else:
+        o = O2.bucketing_colored(data, k, m, b, sb, colors[0], colors[1], threads=4)
+    o.merge(k, b, sb, kind, s, 4)
+    oqk, omt, occ = o.query(queries, k, kind, gpu_ctx.index_colors())
+    assert np.array_equal(qk, oqk)
+    assert np.array_equal(mt, omt)
+    if colors is not None:
+        assert np.array_equal(cc.astype(np.uint64), occ)
+    assert int(mt.sum()) > 0
+    return qk, mt, cc
+
+
+def test_query_uncolored_reads(gpu_ctx):
+    g = synth.genome(61, 300_000)
+    data = synth.reads(g, 62, 60_000, 150, 0.005)
+    q = synth.queries([g], 63, 400, 100, 5000, 0.02)
+    _say(f"case query uncolored k=31 bytes={data.size} queries={q.size}")
+    _query_parity(gpu_ctx, data, 31, 15, 6, 3, 2, HASH_SEQ, None, q)
+
+
+def test_query_rk128_k63(gpu_ctx):
+    g = synth.genome(64, 200_000)
+    data = synth.reads(g, 65, 40_000, 150, 0.005)
+    q = synth.queries([g], 66, 200, 100, 3000, 0.02)
+    _say(f"case query rk128 k=63 bytes={data.size} queries={q.size}")
+    _query_parity(gpu_ctx, data, 63, 21, 6, 3, 2, HASH_RK128, None, q)
+
+
+def test_query_colored_and_serving_mix(gpu_ctx):
+    from ggcat_amd.query import serve_poisson
+    gs = synth.config3_genomes(0.01, 64)
+    data, starts = synth.contigs_joined(gs, 303)
+    colors = np.arange(64, dtype=np.uint32)
+    q = synth.queries(gs, 67, 300, 100, 5000, 0.02)
+    _say(f"case query colored genomes=64 bytes={data.size} queries={q.size}")
+    _query_parity(gpu_ctx, data, 31, 15, 5, 3, 1, HASH_SEQ, (starts, colors), q)
+    # continuous batching: requests admitted as memory frees up, overlapping on
+    # four streams; every result equals the request run alone
+    reqs = [synth.queries(gs, 200 + i, int(n), 100, 5000, 0.02)
+            for i, n in enumerate(np.random.default_rng(3).integers(1, 300, 24))]
+    alone = [gpu_ctx.query(r) for r in reqs]
+    gpu_ctx.serve_open(4, 16 << 20)
+    try:
+        results = {}
+        rep = serve_poisson(gpu_ctx, reqs, rate_hz=2000.0, results=results)
+    finally:
+        gpu_ctx.serve_close()
+    _say(f"serving: max outstanding {rep.max_outstanding}, max admitted {rep.max_admitted}, "
+         f"p50 {rep.percentile(50):.2f} ms p99 {rep.percentile(99):.2f} ms")
+    for i in range(len(reqs)):
+        assert all(np.array_equal(a, b) for a, b in zip(results[i], alone[i]))
+    assert rep.max_admitted >= 2  # requests overlapped on the device
