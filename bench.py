@@ -72,6 +72,52 @@ def cpu_baseline(wl, threads: int) -> dict:
                       f"{threads} threads, {dt:.2f} s"}
 
 
+def serve_measurement(ctx, rate_hz: float, n_requests: int) -> dict:
+    """Colored query serving: the 256-genome colored graph (one color per
+    genome, k=31 m=15 s=2) built and indexed in HBM, then a Poisson stream of
+    requests of U{1..10000} queries with log-uniform lengths in 100..10000 bp,
+    half sampled from the genomes with 2% substitutions, half i.i.d.; latency
+    is submission to completion as seen by the polling loop."""
+    from ggcat_amd import synth
+    from ggcat_amd.native import Params
+    from ggcat_amd.query import serve_poisson
+
+    wl = synth.config3(1.0)
+    starts, colors = wl.colors
+    p = Params(wl.k, wl.m, wl.buckets_log, wl.second_buckets_log, 0, 0, wl.min_mult)
+    t = time.perf_counter()
+    ctx.bucketing(wl.data, p, colors=(starts, colors))
+    ctx.kmers_merge()
+    t_build = time.perf_counter() - t
+    t = time.perf_counter()
+    ctx.index_build()
+    t_index = time.perf_counter() - t
+    n_solid = int(ctx.counters()[5])
+    gs = synth.config3_genomes(1.0)
+    rng = np.random.default_rng(5)
+    reqs = [synth.queries(gs, 500 + i, int(n), 100, 10_000, 0.02) for i, n in
+            enumerate(rng.integers(1, 10_001, n_requests))]
+    log(f"[bench] serving: graph of {len(gs)} genomes ({wl.bases} bases) built in {t_build:.2f} s, "
+        f"{n_solid} solid k-mers indexed in {t_index * 1e3:.1f} ms; {n_requests} requests at {rate_hz}/s")
+    ctx.serve_open(4, 2 << 30)
+    results = {}
+    try:
+        rep = serve_poisson(ctx, reqs, rate_hz, results=results)
+    finally:
+        ctx.serve_close()
+    # each result equals the same request run alone (spot check of three)
+    for i in (0, n_requests // 2, n_requests - 1):
+        alone = ctx.query(reqs[i])
+        if not all(np.array_equal(x, y) for x, y in zip(results[i], alone)):
+            raise SystemExit("[bench] serving result differs from the request run alone")
+    return {"graph": f"{wl.name}: {len(gs)} genomes, {wl.bases} bases, colored, k=31 m=15 s=2",
+            "rate_hz": rate_hz, "requests": n_requests, "queries": rep.queries, "query_kmers": rep.kmers,
+            "p50_ms": round(rep.percentile(50), 2), "p99_ms": round(rep.percentile(99), 2),
+            "max_ms": round(max(rep.latencies_ms), 2), "wall_s": round(rep.wall_s, 2),
+            "kmers_per_s": round(rep.kmers / rep.wall_s, 1), "max_admitted": rep.max_admitted,
+            "graph_build_s": round(t_build, 3), "index_build_ms": round(t_index * 1e3, 1)}
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +127,9 @@ def main() -> None:
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the colored query serving measurement")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson request rate of the serving run (1/s)")
+    ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 1 GiB device-to-device copy first (HBM counter calibration under rocprofv3 --pmc)")
     a = ap.parse_args()
@@ -181,6 +230,8 @@ def main() -> None:
         "pcie_inclusive_gbases_s": round(wl.bases / (dt / a.steps + h2d_s) / 1e9, 4),
     }
     ctx.free(dptr)
+    if not a.no_serve:
+        out["query"] = serve_measurement(ctx, a.serve_rate, a.serve_requests)
     ctx.close()
     if not a.no_cpu:
         threads = min(os.cpu_count() or 1, 64)
