@@ -134,8 +134,8 @@ def main() -> None:
                     help="run one 1 GiB device-to-device copy first (HBM counter calibration under rocprofv3 --pmc)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on exactly one GPU; multi-GPU sharding is a design "
-                                   "note only (see the sharding paragraph of the design document)"}))
+        print(json.dumps({"error": "this build runs on exactly one GPU; sharding the path over several GPUs "
+                                   "is described but not implemented"}))
         raise SystemExit(2)
 
     from ggcat_amd.native import GpuContext, Params

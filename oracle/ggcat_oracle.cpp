@@ -13,7 +13,10 @@
 //   seq-hash u64/u128, rabin-karp128      pinned: invertibility / roll==direct / RC symmetry
 //   MapEntry merge (merge_subbucket)      pinned: union of solid sets == exact k-mer counts
 //   compaction (compact)                  pinned: multiset expansion equality
-//   sorting extender (normal buckets)     parity unpinned (not restated; see DESIGN)
+//   color sets (merge_subbucket, colored) pinned: brute-force colored k-mer counts
+//   query counts (orc_query)              pinned: pure-Python brute force over the graph k-mer set
+//   sorting extender (normal buckets)     parity unpinned: not restated; the reference intends it to
+//                                         give the hash-map rule's solid set, which is pinned above
 #include <algorithm>
 #include <atomic>
 #include <chrono>
