@@ -4,10 +4,12 @@
 #include "bucketing.hip"
 #include "merge.hip"
 #include "query.hip"
+#include "unitigs.hip"
 #include "fasta.cpp"
 
 #include <algorithm>
 #include <cstdio>
+#include <array>
 #include <deque>
 #include <map>
 #include <memory>
@@ -59,6 +61,13 @@ struct ggcat_gpu_ctx {
     uint32_t idx_log_buckets = 0;
     DevBuf q_seq, q_off, q_matched, q_counts, q_work;
     struct Server* server = nullptr;
+    // unitig chains
+    DevBuf u_nb, u_ext, u_link, u_hbytes, u_hcount, u_hoff, u_hidx, u_seq, u_sub, u_flags, u_visited, u_rest,
+        u_rest_n;
+    bool have_unitigs = false;
+    uint64_t u_n = 0, u_bytes = 0, u_npos = 0;
+    std::vector<uint8_t> cyc_seq, cyc_flags;
+    std::vector<uint32_t> cyc_sub;
 };
 
 namespace {
@@ -371,7 +380,10 @@ void ggcat_gpu_destroy(ggcat_gpu_ctx* ctx) {
                       &ctx->out_val, &ctx->out_sub, &ctx->chunk_cursor, &ctx->chunk_fill_dev, &ctx->fix_key,
                       &ctx->fix_val, &ctx->fix_sub, &ctx->fix_cursor, &ctx->out_colors, &ctx->fix_colors,
                       &ctx->idx_key, &ctx->idx_pos, &ctx->idx_boff, &ctx->idx_bcount, &ctx->idx_bcursor,
-                      &ctx->q_seq, &ctx->q_off, &ctx->q_matched, &ctx->q_counts, &ctx->q_work};
+                      &ctx->q_seq, &ctx->q_off, &ctx->q_matched, &ctx->q_counts, &ctx->q_work,
+                      &ctx->u_nb, &ctx->u_ext, &ctx->u_link, &ctx->u_hbytes, &ctx->u_hcount, &ctx->u_hoff,
+                      &ctx->u_hidx, &ctx->u_seq, &ctx->u_sub, &ctx->u_flags, &ctx->u_visited, &ctx->u_rest,
+                      &ctx->u_rest_n};
     if (ctx->server) (void)ggcat_gpu_serve_close(ctx);
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
@@ -401,7 +413,7 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
         return fail(ctx, GGCAT_GPU_EINVAL, "device input must be 16-byte aligned");
     HIPCHK(hipSetDevice(ctx->device));
     if (ctx->server) return fail(ctx, GGCAT_GPU_ESTATE, "close the query server before a new build");
-    ctx->have_records = ctx->have_solid = ctx->have_index = false;
+    ctx->have_records = ctx->have_solid = ctx->have_index = ctx->have_unitigs = false;
     ctx->P = p;
     const uint32_t w = p.k - p.m;
     const uint32_t n_sub = ((1u << p.buckets_log) + 1) << p.second_buckets_log;
@@ -612,7 +624,7 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     if (!ctx) return GGCAT_GPU_EINVAL;
     if (!ctx->have_records) return fail(ctx, GGCAT_GPU_ESTATE, "no bucketing result");
     if (ctx->server) return fail(ctx, GGCAT_GPU_ESTATE, "close the query server before a new merge");
-    ctx->have_index = false;
+    ctx->have_index = ctx->have_unitigs = false;
     HIPCHK(hipSetDevice(ctx->device));
     const ggcat_gpu_params& p = ctx->P;
     const uint32_t key_words = (p.hash_kind == GGCAT_GPU_HASH_SEQ && p.k <= 32) ? 1 : 2;
@@ -933,7 +945,7 @@ int ggcat_gpu_set_tuning(ggcat_gpu_ctx* ctx, const ggcat_gpu_tuning* t) {
 
 int ggcat_gpu_timings(ggcat_gpu_ctx* ctx, double* ms, uint32_t n) {
     if (!ctx || !ms) return GGCAT_GPU_EINVAL;
-    for (uint32_t i = 0; i < n && i < 7; i++) ms[i] = ctx->ms[i];
+    for (uint32_t i = 0; i < n && i < 8; i++) ms[i] = ctx->ms[i];
     return GGCAT_GPU_OK;
 }
 
@@ -963,6 +975,15 @@ int ggcat_gpu_device_free(ggcat_gpu_ctx* ctx, void* p) {
     HIPCHK(hipSetDevice(ctx->device));
     if (p) HIPCHK(hipFree(p));
     return GGCAT_GPU_OK;
+}
+
+// Valid solid positions that no path walk visited (nodes on cycles).
+__global__ void unvisited_kernel(UnitigArgs A, uint64_t n_pos, uint32_t* rest, unsigned int* n_rest, uint32_t cap) {
+    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < n_pos; p += (uint64_t)gridDim.x * blockDim.x)
+        if (unitig_valid(A, p) && !A.visited[p]) {
+            const uint32_t i = atomicAdd(n_rest, 1u);
+            if (i < cap) rest[i] = (uint32_t)p;
+        }
 }
 
 // Plain device-to-device copy (16-byte vectors, grid-stride).
@@ -1093,7 +1114,7 @@ int ggcat_gpu_index_build(ggcat_gpu_ctx* ctx) {
     I.idx_key = P_<uint64_t>(ctx->idx_key);
     I.idx_pos = P_<uint32_t>(ctx->idx_pos);
     const uint64_t n_pos = (uint64_t)ctx->n_chunks_used * OUT_CHUNK + ctx->n_tail;
-    if (n_pos >= 0xFFFFFFFFull) return fail(ctx, GGCAT_GPU_EINVAL, "solid table too large for the query index");
+    if (n_pos >= 0x7FFFFFFFull) return fail(ctx, GGCAT_GPU_EINVAL, "solid table too large for the query index");
     HIPCHK(hipMemsetAsync(ctx->idx_bcount.p, 0, (size_t)nb * 4, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->idx_bcursor.p, 0, (size_t)nb * 4, ctx->stream));
     const uint32_t grid = (uint32_t)std::min<uint64_t>((n_pos + 255) / 256 + 1, (uint64_t)ctx->n_cu * 16);
@@ -1283,6 +1304,187 @@ int ggcat_gpu_serve_close(ggcat_gpu_ctx* ctx) {
     if (S->pin) (void)hipHostFree(S->pin);
     delete S;
     ctx->server = nullptr;
+    return GGCAT_GPU_OK;
+}
+
+int ggcat_gpu_unitigs(ggcat_gpu_ctx* ctx, uint64_t* n_unitigs, uint64_t* n_bytes) {
+    if (!ctx) return GGCAT_GPU_EINVAL;
+    if (!ctx->have_solid) return fail(ctx, GGCAT_GPU_ESTATE, "no merge result");
+    if (ctx->P.hash_kind != GGCAT_GPU_HASH_SEQ)
+        return fail(ctx, GGCAT_GPU_EINVAL, "unitig extension needs seq-hash keys (the k-mer is the key)");
+    if (!ctx->have_index) {
+        int r = ggcat_gpu_index_build(ctx);
+        if (r != GGCAT_GPU_OK) return r;
+    }
+    HIPCHK(hipSetDevice(ctx->device));
+    const uint64_t n_pos = (uint64_t)ctx->n_chunks_used * OUT_CHUNK + ctx->n_tail;
+    const uint64_t ne = 2 * n_pos;
+    ENSURE(ctx->u_nb, ne * 4 + 8);
+    ENSURE(ctx->u_link, ne * 4 + 8);
+    ENSURE(ctx->u_ext, n_pos + 8);
+    ENSURE(ctx->u_hbytes, ne * 4 + 8);
+    ENSURE(ctx->u_hcount, ne * 4 + 8);
+    ENSURE(ctx->u_hoff, (ne + 1) * 8);
+    ENSURE(ctx->u_hidx, (ne + 1) * 8);
+    ENSURE(ctx->u_visited, n_pos + 8);
+    ENSURE(ctx->u_rest_n, 8);
+    UnitigArgs U{};
+    U.out_key = P_<uint64_t>(ctx->out_key);
+    U.out_val = P_<uint32_t>(ctx->out_val);
+    U.out_sub = P_<uint32_t>(ctx->out_sub);
+    U.chunk_fill = P_<uint32_t>(ctx->chunk_fill_dev);
+    U.n_chunks = ctx->n_chunks_used;
+    U.n_tail = ctx->n_tail;
+    U.chunk = OUT_CHUNK;
+    U.key_words = ctx->key_words;
+    U.k = ctx->P.k;
+    U.idx_key = P_<uint64_t>(ctx->idx_key);
+    U.idx_pos = P_<uint32_t>(ctx->idx_pos);
+    U.boff = P_<uint64_t>(ctx->idx_boff);
+    U.log_buckets = ctx->idx_log_buckets;
+    U.nb = P_<uint32_t>(ctx->u_nb);
+    U.ext = P_<uint8_t>(ctx->u_ext);
+    U.link = P_<uint32_t>(ctx->u_link);
+    U.head_bytes = P_<uint32_t>(ctx->u_hbytes);
+    U.head_count = P_<uint32_t>(ctx->u_hcount);
+    U.head_off = P_<uint64_t>(ctx->u_hoff);
+    U.head_idx = P_<uint64_t>(ctx->u_hidx);
+    U.visited = P_<uint8_t>(ctx->u_visited);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((ne + 255) / 256 + 1, (uint64_t)ctx->n_cu * 16);
+    HIPCHK(hipEventRecord(ctx->ev[11], ctx->stream));
+    if (n_pos) {
+        hipLaunchKernelGGL(ext_kernel, dim3(grid), dim3(256), 0, ctx->stream, U, n_pos);
+        hipLaunchKernelGGL(link_kernel, dim3(grid), dim3(256), 0, ctx->stream, U, n_pos);
+        hipLaunchKernelGGL(walk_kernel<false>, dim3(grid), dim3(256), 0, ctx->stream, U, n_pos);
+        HIPCHK(hipGetLastError());
+    }
+    int r = device_scan(ctx, U.head_bytes, P_<uint64_t>(ctx->u_hoff), (uint32_t)ne);
+    if (r != GGCAT_GPU_OK) return r;
+    r = device_scan(ctx, U.head_count, P_<uint64_t>(ctx->u_hidx), (uint32_t)ne);
+    if (r != GGCAT_GPU_OK) return r;
+    uint64_t tot_bytes = 0, tot_n = 0;
+    HIPCHK(hipMemcpyAsync(&tot_bytes, P_<uint64_t>(ctx->u_hoff) + ne, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(&tot_n, P_<uint64_t>(ctx->u_hidx) + ne, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ENSURE(ctx->u_seq, tot_bytes + 8);
+    ENSURE(ctx->u_sub, tot_n * 4 + 8);
+    ENSURE(ctx->u_flags, tot_n + 8);
+    U.seq = P_<uint8_t>(ctx->u_seq);
+    U.uni_sub = P_<uint32_t>(ctx->u_sub);
+    U.uni_flags = P_<uint8_t>(ctx->u_flags);
+    HIPCHK(hipMemsetAsync(ctx->u_visited.p, 0, n_pos + 8, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->u_rest_n.p, 0, 8, ctx->stream));
+    const uint32_t cap = 1u << 20;
+    ENSURE(ctx->u_rest, (size_t)cap * 4);
+    if (n_pos) {
+        hipLaunchKernelGGL(walk_kernel<true>, dim3(grid), dim3(256), 0, ctx->stream, U, n_pos);
+        hipLaunchKernelGGL(unvisited_kernel, dim3(grid), dim3(256), 0, ctx->stream, U, n_pos, P_<uint32_t>(ctx->u_rest),
+                           P_<unsigned int>(ctx->u_rest_n), cap);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipEventRecord(ctx->ev[12], ctx->stream));
+    uint32_t n_rest = 0;
+    HIPCHK(hipMemcpyAsync(&n_rest, ctx->u_rest_n.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (n_rest > cap) return fail(ctx, GGCAT_GPU_ENOMEM, "too many k-mers on cycles");
+    // Cycles: every remaining k-mer has both sides linked.  Walk each from its
+    // smallest key, forward (entering at side 0), on the host.
+    ctx->cyc_seq.clear();
+    ctx->cyc_sub.clear();
+    ctx->cyc_flags.clear();
+    if (n_rest) {
+        std::vector<uint32_t> rest(n_rest);
+        HIPCHK(hipMemcpy(rest.data(), ctx->u_rest.p, (size_t)n_rest * 4, hipMemcpyDeviceToHost));
+        const uint32_t kw = ctx->key_words, k = ctx->P.k;
+        std::map<uint32_t, std::pair<u128_t, std::array<uint32_t, 2>>> node;  // pos -> key, links
+        for (uint32_t p : rest) {
+            uint64_t kk[2] = {0, 0};
+            uint32_t lk[2];
+            HIPCHK(hipMemcpy(kk, P_<uint64_t>(ctx->out_key) + (uint64_t)p * kw, 8 * kw, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(lk, P_<uint32_t>(ctx->u_link) + 2 * (uint64_t)p, 8, hipMemcpyDeviceToHost));
+            node[p] = {((u128_t)kk[1] << 64) | kk[0], {lk[0], lk[1]}};
+        }
+        std::vector<uint32_t> order(rest);
+        std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return node[a].first < node[b].first; });
+        std::map<uint32_t, bool> seen;
+        const char B[4] = {'A', 'C', 'T', 'G'};
+        auto rc = [&](u128_t x) {
+            u128_t y = 0;
+            for (uint32_t i = 0; i < k; i++) y |= (u128_t)((((uint32_t)(x >> (2 * i))) & 3u) ^ 2u) << (2 * (k - 1 - i));
+            return y;
+        };
+        for (uint32_t p0 : order) {
+            if (seen[p0]) continue;
+            seen[p0] = true;
+            const u128_t x = node[p0].first;
+            for (uint32_t i = 0; i < k; i++) ctx->cyc_seq.push_back((uint8_t)B[(uint32_t)(x >> (2 * i)) & 3u]);
+            uint64_t cur = 2 * (uint64_t)p0;  // entered at side 0
+            uint32_t sub = 0;
+            HIPCHK(hipMemcpy(&sub, P_<uint32_t>(ctx->out_sub) + p0, 4, hipMemcpyDeviceToHost));
+            for (uint64_t guard = 0; guard <= n_rest; guard++) {
+                auto it = node.find((uint32_t)(cur >> 1));
+                if (it == node.end()) return fail(ctx, GGCAT_GPU_EHIP, "cycle walk left the cycle set");
+                const uint32_t nx = it->second.second[(cur ^ 1) & 1];
+                if (nx == NO_LINK) return fail(ctx, GGCAT_GPU_EHIP, "cycle walk met an open side");
+                const uint32_t q = nx >> 1;
+                if (seen[q]) break;
+                seen[q] = true;
+                auto jt = node.find(q);
+                if (jt == node.end()) return fail(ctx, GGCAT_GPU_EHIP, "cycle walk left the cycle set");
+                const u128_t y = (nx & 1) == 0 ? jt->second.first : rc(jt->second.first);
+                ctx->cyc_seq.push_back((uint8_t)B[(uint32_t)(y >> (2 * (k - 1))) & 3u]);
+                cur = nx;
+            }
+            ctx->cyc_seq.push_back('\n');
+            ctx->cyc_sub.push_back(sub);
+            ctx->cyc_flags.push_back(4);
+        }
+    }
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, ctx->ev[11], ctx->ev[12]));
+    ctx->ms[7] = t;
+    ctx->u_n = tot_n;
+    ctx->u_bytes = tot_bytes;
+    ctx->u_npos = n_pos;
+    ctx->have_unitigs = true;
+    if (n_unitigs) *n_unitigs = tot_n + ctx->cyc_sub.size();
+    if (n_bytes) *n_bytes = tot_bytes + ctx->cyc_seq.size();
+    return GGCAT_GPU_OK;
+}
+
+int ggcat_gpu_unitigs_copy(ggcat_gpu_ctx* ctx, uint8_t* seq, uint32_t* sub, uint8_t* flags) {
+    if (!ctx) return GGCAT_GPU_EINVAL;
+    if (!ctx->have_unitigs) return fail(ctx, GGCAT_GPU_ESTATE, "no unitig result");
+    HIPCHK(hipSetDevice(ctx->device));
+    if (seq) {
+        if (ctx->u_bytes) HIPCHK(hipMemcpy(seq, ctx->u_seq.p, ctx->u_bytes, hipMemcpyDeviceToHost));
+        memcpy(seq + ctx->u_bytes, ctx->cyc_seq.data(), ctx->cyc_seq.size());
+    }
+    if (sub) {
+        if (ctx->u_n) HIPCHK(hipMemcpy(sub, ctx->u_sub.p, ctx->u_n * 4, hipMemcpyDeviceToHost));
+        memcpy(sub + ctx->u_n, ctx->cyc_sub.data(), ctx->cyc_sub.size() * 4);
+    }
+    if (flags) {
+        if (ctx->u_n) HIPCHK(hipMemcpy(flags, ctx->u_flags.p, ctx->u_n, hipMemcpyDeviceToHost));
+        memcpy(flags + ctx->u_n, ctx->cyc_flags.data(), ctx->cyc_flags.size());
+    }
+    return GGCAT_GPU_OK;
+}
+
+int ggcat_gpu_extension_copy(ggcat_gpu_ctx* ctx, uint8_t* ext) {
+    if (!ctx || !ext) return GGCAT_GPU_EINVAL;
+    if (!ctx->have_unitigs) return fail(ctx, GGCAT_GPU_ESTATE, "no unitig result");
+    HIPCHK(hipSetDevice(ctx->device));
+    uint64_t o = 0;
+    for (uint32_t c = 0; c < ctx->n_chunks_used; c++) {
+        if (!ctx->chunk_fill[c]) continue;
+        HIPCHK(hipMemcpy(ext + o, P_<uint8_t>(ctx->u_ext) + (uint64_t)c * OUT_CHUNK, ctx->chunk_fill[c],
+                         hipMemcpyDeviceToHost));
+        o += ctx->chunk_fill[c];
+    }
+    if (ctx->n_tail)
+        HIPCHK(hipMemcpy(ext + o, P_<uint8_t>(ctx->u_ext) + (uint64_t)ctx->n_chunks_used * OUT_CHUNK, ctx->n_tail,
+                         hipMemcpyDeviceToHost));
     return GGCAT_GPU_OK;
 }
 

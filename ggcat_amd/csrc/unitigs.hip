@@ -1,0 +1,190 @@
+// unitigs.hip -- in/out extension flags and intra-sub-bucket unitig chains
+// over the solid seq-hash table of the last merge.
+//
+// Restates the extension of crates/assembler_kmers_merge/src/unitigs_extender/
+// hashmap.rs:162-297,442-601 as an order-free rule (the reference's greedy
+// loop gives the same chains except where a k-mer's flagged side has a
+// neighbour in its own sub-bucket, where its own result depends on the order
+// in which it visits k-mers):
+//   * in a k-mer's canonical orientation, side 0 is the predecessor side and
+//     side 1 the successor side; its degree on a side is the number of solid
+//     k-mers of the same sub-bucket one base away (manual_roll_forward /
+//     manual_roll_reverse, crates/hashes/src/base/cn_seqhash_base.rs:165-200);
+//   * a k-mer whose flags are exactly INCL_BEGIN (or INCL_END) has its side 0
+//     (or side 1) flagged: its continuation is in another bucket
+//     (contig_break, hashmap.rs:287-292; begin/end_ignored, :483-494);
+//   * sides (p,s) and (q,t) are linked when each is the other's only
+//     neighbour, p != q, and neither side is flagged;
+//   * a unitig is a maximal linked path, written from its end with the
+//     smaller (key, side); an end is open when its outer side is flagged.
+// Cycles (paths without ends) are rare and are walked on the host.
+#include "common.h"
+
+namespace gg {
+
+constexpr uint32_t NO_LINK = 0xFFFFFFFFu;
+
+struct UnitigArgs {
+    const uint64_t* out_key;
+    const uint32_t* out_val;
+    const uint32_t* out_sub;
+    const uint32_t* chunk_fill;
+    uint32_t n_chunks;
+    uint64_t n_tail;
+    uint32_t chunk;
+    uint32_t key_words;
+    uint32_t k;
+    const uint64_t* idx_key;
+    const uint32_t* idx_pos;
+    const uint64_t* boff;
+    uint32_t log_buckets;
+    uint32_t* nb;  // [2 n_pos] only neighbour (q << 1 | t) or NO_LINK
+    uint8_t* ext;  // [n_pos] degree side 0 (0, 1, 2 = more) | degree side 1 << 2
+    uint32_t* link;  // [2 n_pos]
+    uint32_t* head_bytes;  // [2 n_pos] bytes of the unitig written from this end (0 if not its start)
+    uint32_t* head_count;  // [2 n_pos] 1 when the unitig starts here
+    const uint64_t* head_off;  // [2 n_pos + 1] exclusive scan of head_bytes
+    const uint64_t* head_idx;  // [2 n_pos + 1] exclusive scan of head_count
+    uint8_t* seq;
+    uint32_t* uni_sub;
+    uint8_t* uni_flags;
+    uint8_t* visited;  // [n_pos]
+};
+
+GG_DEV bool unitig_valid(const UnitigArgs& A, uint64_t p) {
+    const uint64_t c = p / A.chunk;
+    if (c < A.n_chunks) return (p - c * A.chunk) < A.chunk_fill[c];
+    return p - (uint64_t)A.n_chunks * A.chunk < A.n_tail;
+}
+
+GG_DEV u128_t key_at(const UnitigArgs& A, uint64_t p) {
+    const uint64_t lo = A.out_key[p * A.key_words];
+    const uint64_t hi = A.key_words == 2 ? A.out_key[p * 2 + 1] : 0;
+    return ((u128_t)hi << 64) | lo;
+}
+
+GG_DEV u128_t kmer_rc128(u128_t x, uint32_t k) {
+    u128_t r = 0;
+    for (uint32_t i = 0; i < k; i++) r |= (u128_t)((((uint32_t)(x >> (2 * i))) & 3u) ^ 2u) << (2 * (k - 1 - i));
+    return r;
+}
+
+GG_DEV bool side_flagged(const UnitigArgs& A, uint64_t p, uint32_t s) {
+    const uint32_t fl = A.out_val[p] >> 30;
+    return (fl == 1 && s == 0) || (fl == 2 && s == 1);
+}
+
+// Position of key `c` inside sub-bucket `sub`, or NO_LINK.
+GG_DEV uint32_t lookup_in_sub(const UnitigArgs& A, u128_t c, uint32_t sub) {
+    const uint64_t lo = (uint64_t)c, hi = (uint64_t)(c >> 64);
+    const uint32_t b = A.log_buckets ? (uint32_t)(index_hash(lo, hi) >> (64 - A.log_buckets)) : 0u;
+    for (uint64_t e = A.boff[b]; e < A.boff[b + 1]; e++) {
+        const bool eq = A.key_words == 1 ? A.idx_key[e] == lo : (A.idx_key[2 * e] == lo && A.idx_key[2 * e + 1] == hi);
+        if (eq && A.out_sub[A.idx_pos[e]] == sub) return A.idx_pos[e];
+    }
+    return NO_LINK;
+}
+
+__global__ void ext_kernel(UnitigArgs A, uint64_t n_pos) {
+    const uint32_t k = A.k;
+    const u128_t mask = (k >= 64) ? ~(u128_t)0 : (((u128_t)1 << (2 * k)) - 1);
+    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < n_pos; p += (uint64_t)gridDim.x * blockDim.x) {
+        if (!unitig_valid(A, p)) {
+            A.nb[2 * p] = A.nb[2 * p + 1] = NO_LINK;
+            A.ext[p] = 0;
+            continue;
+        }
+        const u128_t x = key_at(A, p);
+        const uint32_t sub = A.out_sub[p];
+        uint32_t degs = 0;
+        for (uint32_t s = 0; s < 2; s++) {
+            uint32_t cnt = 0, only = NO_LINK;
+            for (uint32_t b = 0; b < 4; b++) {
+                const u128_t y = s ? ((x >> 2) | ((u128_t)b << (2 * (k - 1)))) : (((x << 2) | b) & mask);
+                const u128_t yr = kmer_rc128(y, k);
+                const u128_t c = y < yr ? y : yr;
+                const uint32_t q = lookup_in_sub(A, c, sub);
+                if (q == NO_LINK) continue;
+                cnt++;
+                const uint32_t t = (y == c) ? (s ? 0u : 1u) : (s ? 1u : 0u);
+                only = (q << 1) | t;
+            }
+            A.nb[2 * p + s] = cnt == 1 ? only : NO_LINK;
+            degs |= (cnt > 2 ? 2u : cnt) << (2 * s);
+        }
+        A.ext[p] = (uint8_t)degs;
+    }
+}
+
+__global__ void link_kernel(UnitigArgs A, uint64_t n_pos) {
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < 2 * n_pos;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t v = A.nb[e];
+        uint32_t out = NO_LINK;
+        if (v != NO_LINK) {
+            const uint64_t p = e >> 1, q = v >> 1;
+            const uint32_t s = (uint32_t)(e & 1), t = v & 1;
+            if (q != p && !side_flagged(A, p, s) && !side_flagged(A, q, t) && A.nb[v] == (uint32_t)e) out = v;
+        }
+        A.link[e] = out;
+    }
+}
+
+// Pass 1 (write == false): every end of a path walks to its other end; the
+// end with the smaller (key, side) records the unitig's bytes.  Pass 2
+// (write == true): those ends write the sequence, its sub-bucket and flags.
+template <bool WRITE>
+__global__ void walk_kernel(UnitigArgs A, uint64_t n_pos) {
+    const uint32_t k = A.k;
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < 2 * n_pos;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t p = e >> 1;
+        const uint32_t s = (uint32_t)(e & 1);
+        bool head = unitig_valid(A, p) && A.link[e] == NO_LINK;
+        if (!WRITE) {
+            uint32_t bytes = 0, cnt = 0;
+            if (head) {
+                uint64_t cur = e;
+                uint64_t steps = 1;
+                while (A.link[cur ^ 1] != NO_LINK && steps <= n_pos) {
+                    cur = A.link[cur ^ 1];
+                    steps++;
+                }
+                const uint64_t q = cur >> 1;
+                const uint32_t qs = (uint32_t)((cur ^ 1) & 1);
+                const u128_t kp = key_at(A, p), kq = key_at(A, q);
+                const bool canon = kp < kq || (kp == kq && s < qs);
+                if (canon) {
+                    bytes = (uint32_t)(steps + k);  // k-mer, one base per further k-mer, '\n'
+                    cnt = 1;
+                }
+            }
+            A.head_bytes[e] = bytes;
+            A.head_count[e] = cnt;
+            continue;
+        }
+        if (!head || A.head_off[e + 1] == A.head_off[e]) continue;
+        uint8_t* out = A.seq + A.head_off[e];
+        const uint64_t ui = A.head_idx[e];
+        const char B[4] = {'A', 'C', 'T', 'G'};
+        const u128_t x = s == 0 ? key_at(A, p) : kmer_rc128(key_at(A, p), k);
+        for (uint32_t i = 0; i < k; i++) out[i] = (uint8_t)B[(uint32_t)(x >> (2 * i)) & 3u];
+        uint64_t o = k;
+        A.visited[p] = 1;
+        uint64_t cur = e;
+        while (A.link[cur ^ 1] != NO_LINK) {
+            cur = A.link[cur ^ 1];
+            const uint64_t q = cur >> 1;
+            A.visited[q] = 1;
+            const u128_t y = (cur & 1) == 0 ? key_at(A, q) : kmer_rc128(key_at(A, q), k);
+            out[o++] = (uint8_t)B[(uint32_t)(y >> (2 * (k - 1))) & 3u];
+        }
+        out[o] = '\n';
+        const uint64_t qend = cur >> 1;
+        const uint32_t exit_side = (uint32_t)((cur ^ 1) & 1);
+        A.uni_sub[ui] = A.out_sub[p];
+        A.uni_flags[ui] = (uint8_t)((side_flagged(A, p, s) ? 1u : 0u) | (side_flagged(A, qend, exit_side) ? 2u : 0u));
+    }
+}
+
+}  // namespace gg

@@ -112,6 +112,9 @@ def gpu_lib(path: str = GPU_LIB) -> C.CDLL:
     lib.ggcat_gpu_serve_fetch.argtypes = [V, C.c_uint64, _u64p, _u64p, _u32p]
     lib.ggcat_gpu_serve_stats.argtypes = [V, _u32p, _u32p]
     lib.ggcat_gpu_serve_close.argtypes = [V]
+    lib.ggcat_gpu_unitigs.argtypes = [V, _u64p, _u64p]
+    lib.ggcat_gpu_unitigs_copy.argtypes = [V, _u8p, _u32p, _u8p]
+    lib.ggcat_gpu_extension_copy.argtypes = [V, _u8p]
     lib.ggcat_gpu_read_files.argtypes = [V, C.POINTER(C.c_char_p), C.c_uint32, C.c_uint32, C.c_int,
                                          C.POINTER(_u8p), _u64p, C.POINTER(_u64p), C.POINTER(_u32p), _u32p, _u64p]
     lib.ggcat_gpu_free_host.argtypes = [V]
@@ -128,7 +131,8 @@ def gpu_lib(path: str = GPU_LIB) -> C.CDLL:
                  "ggcat_gpu_plan_build", "ggcat_gpu_set_tuning", "ggcat_gpu_solid_colors",
                  "ggcat_gpu_copy_device", "ggcat_gpu_index_build", "ggcat_gpu_index_colors", "ggcat_gpu_query",
                  "ggcat_gpu_serve_open", "ggcat_gpu_serve_submit", "ggcat_gpu_serve_poll", "ggcat_gpu_serve_fetch",
-                 "ggcat_gpu_serve_stats", "ggcat_gpu_serve_close"):
+                 "ggcat_gpu_serve_stats", "ggcat_gpu_serve_close", "ggcat_gpu_unitigs", "ggcat_gpu_unitigs_copy",
+                 "ggcat_gpu_extension_copy"):
         getattr(lib, name).restype = C.c_int
     _gpu[path] = lib
     return lib
@@ -310,6 +314,25 @@ class GpuContext:
         self._check(self.lib.ggcat_gpu_solid_colors(self.h, C.byref(w), _ptr(out, _u32p)), "solid_colors")
         return out
 
+    # ---- unitigs -------------------------------------------------------
+    def unitigs(self):
+        """Intra-sub-bucket unitigs: (list of ASCII sequences, sub-bucket per
+        unitig, flags: bit0 open start, bit1 open end, bit2 circular)."""
+        n, nb = C.c_uint64(), C.c_uint64()
+        self._check(self.lib.ggcat_gpu_unitigs(self.h, C.byref(n), C.byref(nb)), "ggcat_gpu_unitigs")
+        seq = np.zeros(nb.value, np.uint8)
+        sub, fl = np.zeros(n.value, np.uint32), np.zeros(n.value, np.uint8)
+        self._check(self.lib.ggcat_gpu_unitigs_copy(self.h, _ptr(seq, _u8p), _ptr(sub, _u32p), _ptr(fl, _u8p)),
+                    "ggcat_gpu_unitigs_copy")
+        return seq.tobytes().split(b"\n")[:-1], sub, fl
+
+    def extension_flags(self) -> np.ndarray:
+        n, kw = C.c_uint64(), C.c_uint32()
+        self._check(self.lib.ggcat_gpu_solid_info(self.h, C.byref(n), C.byref(kw)), "solid_info")
+        out = np.zeros(n.value, np.uint8)
+        self._check(self.lib.ggcat_gpu_extension_copy(self.h, _ptr(out, _u8p)), "ggcat_gpu_extension_copy")
+        return out
+
     # ---- queries -------------------------------------------------------
     def index_build(self) -> None:
         self._check(self.lib.ggcat_gpu_index_build(self.h), "ggcat_gpu_index_build")
@@ -388,8 +411,8 @@ class GpuContext:
         self._check(self.lib.ggcat_gpu_device_free(self.h, C.c_void_p(ptr)), "device_free")
 
     def timings(self) -> np.ndarray:
-        out = np.zeros(7, np.float64)
-        self._check(self.lib.ggcat_gpu_timings(self.h, _ptr(out, _f64p), 7), "timings")
+        out = np.zeros(8, np.float64)
+        self._check(self.lib.ggcat_gpu_timings(self.h, _ptr(out, _f64p), 8), "timings")
         return out
 
     def counters(self) -> np.ndarray:

@@ -155,7 +155,7 @@ int ggcat_gpu_set_tuning(ggcat_gpu_ctx* ctx, const ggcat_gpu_tuning* tuning);
 /* Device-side timings of the last calls, in milliseconds:
  * [0] bucketing pass A, [1] scan, [2] bucketing pass B,
  * [3] merge (whole-item pass), [4] merge (partitioned passes),
- * [5] whole bucketing call, [6] whole merge call. */
+ * [5] whole bucketing call, [6] whole merge call, [7] unitig kernels. */
 int ggcat_gpu_timings(ggcat_gpu_ctx* ctx, double* ms, uint32_t n);
 
 /* Counters of the last calls: [0] input bytes, [1] records,
@@ -205,6 +205,25 @@ int ggcat_gpu_plan_build(const ggcat_gpu_build_options* options, uint64_t estima
  * ggcat_gpu_solid_*); *used (optional) receives the derived parameters. */
 int ggcat_gpu_build_from_files(ggcat_gpu_ctx* ctx, const char* const* paths, uint32_t n_paths,
                                const ggcat_gpu_build_options* options, ggcat_gpu_params* used);
+
+/* ---- unitig chains inside each sub-bucket -------------------------------
+ * Replaces compute_unitigs / try_extend_function
+ * (crates/assembler_kmers_merge/src/unitigs_extender/hashmap.rs:162-297,
+ * 442-601) on the last merge's solid table (seq-hash keys).  Every unitig is
+ * a maximal path of k-mers of one sub-bucket that are each other's only
+ * neighbour, not continuing through a k-mer whose flags are exactly
+ * INCL_BEGIN or INCL_END (a continuation into another bucket); output as
+ * ASCII, each unitig followed by '\n', with its sub-bucket and flags
+ * (bit 0: the start continues in another bucket, bit 1: the end does,
+ * bit 2: circular).  Unitigs are written from the end with the smaller
+ * (key, side); cycles from their smallest k-mer, forward. */
+int ggcat_gpu_unitigs(ggcat_gpu_ctx* ctx, uint64_t* n_unitigs, uint64_t* n_bytes);
+int ggcat_gpu_unitigs_copy(ggcat_gpu_ctx* ctx, uint8_t* seq, uint32_t* sub, uint8_t* flags);
+/* In/out extension flags of every solid entry (ggcat_gpu_solid_copy order):
+ * bits 0-1 = number of solid k-mers of the same sub-bucket that can precede
+ * it in its canonical orientation (2 = two or more), bits 2-3 = that can
+ * follow it.  Valid after ggcat_gpu_unitigs. */
+int ggcat_gpu_extension_copy(ggcat_gpu_ctx* ctx, uint8_t* ext);
 
 /* ---- queries ------------------------------------------------------------
  * Replaces the counting part of run_query (crates/querier/src/lib.rs:54-200):

@@ -64,6 +64,14 @@ def lib() -> C.CDLL:
     L.orc_bruteforce_counts.argtypes = [_u8p, _u64p, C.c_size_t, _u32p, C.c_uint, C.c_uint32]
     L.orc_query.argtypes = [V, _u8p, C.c_uint64, C.c_uint, C.c_int, C.c_uint32, _u64p, _u64p, _u64p, C.c_uint64]
     L.orc_query.restype = C.c_uint64
+    L.orc_extension.argtypes = [V, C.c_uint32, _u8p]
+    L.orc_extension.restype = None
+    L.orc_unitigs.argtypes = [V, C.c_uint32, C.c_int]
+    L.orc_unitigs.restype = C.c_size_t
+    L.orc_unitigs_bytes.argtypes = [V]
+    L.orc_unitigs_bytes.restype = C.c_size_t
+    L.orc_unitigs_get.argtypes = [V, _u8p, _u32p, _u8p]
+    L.orc_unitigs_get.restype = None
     L.orc_rec_colors.argtypes = [V, _u32p]
     L.orc_rec_colors.restype = None
     L.orc_solid_colors_total.argtypes = [V]
@@ -157,6 +165,23 @@ class Result:
                               _p(cc, _u64p), nq)
         assert got == nq
         return qk, mt, (cc[:nq * n_colors].reshape(nq, n_colors) if n_colors else None)
+
+    def extension(self, k: int) -> np.ndarray:
+        """In/out extension flags per solid entry (solid() order)."""
+        L = lib()
+        out = np.zeros(L.orc_solid_count(self.h), np.uint8)
+        L.orc_extension(self.h, k, _p(out, _u8p))
+        return out
+
+    def unitigs(self, k: int, sequential: bool = False):
+        """Intra-sub-bucket unitigs of the merged table: (list of ASCII sequences,
+        sub-bucket per unitig, flags: bit0 open start, bit1 open end, bit2 circular)."""
+        L = lib()
+        n = L.orc_unitigs(self.h, k, 1 if sequential else 0)
+        seq = np.zeros(L.orc_unitigs_bytes(self.h), np.uint8)
+        sub, fl = np.zeros(n, np.uint32), np.zeros(n, np.uint8)
+        L.orc_unitigs_get(self.h, _p(seq, _u8p), _p(sub, _u32p), _p(fl, _u8p))
+        return seq.tobytes().split(b"\n")[:-1], sub, fl
 
     def times(self) -> tuple[float, float]:
         a, b = C.c_double(), C.c_double()

@@ -23,6 +23,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -465,9 +466,197 @@ void merge_subbucket(const RecordSet& R, const std::vector<size_t>& idx, uint32_
     }
 }
 
+// ----------------------------------------------------- unitig chains ----
+// Intra-sub-bucket unitig extension over the solid seq-hash table
+// (crates/assembler_kmers_merge/src/unitigs_extender/hashmap.rs:162-297,
+// 442-601).  Two restatements:
+//  * unitig_chains: the order-free rule the device implements.  In a k-mer's
+//    canonical orientation side 0 is the predecessor side, side 1 the
+//    successor side; deg = number of solid k-mers of the same sub-bucket one
+//    base away on that side (manual_roll_forward / _reverse,
+//    crates/hashes/src/base/cn_seqhash_base.rs:165-200).  Sides (p,s) and
+//    (q,t) are linked when each is the other's only neighbour, p != q, and
+//    neither side is the flagged side of a k-mer whose flags are exactly
+//    INCL_BEGIN (side 0) or INCL_END (side 1): the reference stops after such
+//    a k-mer (contig_break, hashmap.rs:287-292) and never extends from its
+//    flagged side (begin_ignored / end_ignored, :483-494).  Unitigs are the
+//    maximal linked paths; an end is open when its outer side is flagged.
+//    Cycles start at their smallest key, forward.
+//  * unitig_sequential: the reference's greedy loop itself, started from
+//    every unused k-mer in a given order, with its used marks.  It agrees
+//    with the order-free rule except where a flagged side has a neighbour in
+//    the same sub-bucket, the case in which the reference's own result
+//    depends on which k-mer it starts from.
+struct UnitigOut {
+    std::vector<uint8_t> seq;  // ASCII, each unitig followed by '\n'
+    std::vector<uint32_t> sub;
+    std::vector<uint8_t> flags;  // bit 0 open start, bit 1 open end, bit 2 circular
+};
+
+inline u128 kmer_mask(unsigned k) { return k >= 64 ? ~(u128)0 : (((u128)1 << (2 * k)) - 1); }
+inline u128 kmer_rc(u128 x, unsigned k) {
+    u128 r = 0;
+    for (unsigned i = 0; i < k; i++) r |= (u128)(((x >> (2 * i)) & 3) ^ 2) << (2 * (k - 1 - i));
+    return r;
+}
+inline void put_kmer(std::vector<uint8_t>& out, u128 x, unsigned k) {
+    static const char B[4] = {'A', 'C', 'T', 'G'};
+    for (unsigned i = 0; i < k; i++) out.push_back((uint8_t)B[(x >> (2 * i)) & 3]);
+}
+
+struct SubGraph {
+    unsigned k;
+    std::vector<u128> key;
+    std::vector<uint8_t> flags;
+    std::unordered_map<u128, uint32_t, U128Hash> at;
+    // neighbour of side s of node p: (q, t) with the number of candidates
+    uint32_t neighbours(uint32_t p, int s, uint32_t* q, int* t) const {
+        const u128 x = key[p], mask = kmer_mask(k);
+        uint32_t cnt = 0;
+        for (uint32_t b = 0; b < 4; b++) {
+            const u128 y = s ? ((x >> 2) | ((u128)b << (2 * (k - 1)))) : (((x << 2) | b) & mask);
+            const u128 yr = kmer_rc(y, k);
+            const u128 c = y < yr ? y : yr;
+            auto it = at.find(c);
+            if (it == at.end()) continue;
+            cnt++;
+            *q = it->second;
+            // moving forward we enter q at its predecessor side when y is q's
+            // canonical orientation; moving backward, at its successor side
+            *t = (y == c) ? (s ? 0 : 1) : (s ? 1 : 0);
+        }
+        return cnt;
+    }
+    bool flagged(uint32_t p, int s) const { return (flags[p] == 1 && s == 0) || (flags[p] == 2 && s == 1); }
+};
+
+void unitig_chains(const SubGraph& G, uint32_t sub, UnitigOut& out) {
+    const uint32_t n = (uint32_t)G.key.size();
+    const unsigned k = G.k;
+    std::vector<int64_t> link(2 * (size_t)n, -1);
+    for (uint32_t p = 0; p < n; p++)
+        for (int s = 0; s < 2; s++) {
+            uint32_t q = 0;
+            int t = 0;
+            if (G.neighbours(p, s, &q, &t) != 1 || q == p || G.flagged(p, s) || G.flagged(q, t)) continue;
+            uint32_t p2 = 0;
+            int s2 = 0;
+            if (G.neighbours(q, t, &p2, &s2) != 1 || p2 != p || s2 != s) continue;
+            link[2 * p + s] = 2 * (int64_t)q + t;
+        }
+    std::vector<uint8_t> seen(n, 0);
+    auto walk = [&](uint32_t p, int s_in, bool circular) {
+        // enter p at side s_in; forward orientation when entering at side 0
+        const size_t start = out.seq.size();
+        u128 x = s_in == 0 ? G.key[p] : kmer_rc(G.key[p], k);
+        put_kmer(out.seq, x, k);
+        const bool open_start = G.flagged(p, s_in);
+        seen[p] = 1;
+        uint32_t cur = p;
+        int side_out = s_in ^ 1;
+        uint32_t steps = 1;
+        while (link[2 * cur + side_out] >= 0) {
+            const int64_t e = link[2 * cur + side_out];
+            const uint32_t q = (uint32_t)(e >> 1);
+            const int t = (int)(e & 1);
+            if (seen[q]) break;  // back at the start of a cycle
+            seen[q] = 1;
+            const u128 y = t == 0 ? G.key[q] : kmer_rc(G.key[q], k);
+            static const char B[4] = {'A', 'C', 'T', 'G'};
+            out.seq.push_back((uint8_t)B[(y >> (2 * (k - 1))) & 3]);
+            cur = q;
+            side_out = t ^ 1;
+            steps++;
+        }
+        (void)start;
+        const bool open_end = !circular && G.flagged(cur, side_out);
+        out.seq.push_back('\n');
+        out.sub.push_back(sub);
+        out.flags.push_back((uint8_t)((open_start && !circular ? 1 : 0) | (open_end ? 2 : 0) | (circular ? 4 : 0)));
+    };
+    // linear chains from the end with the smaller (key, side)
+    for (uint32_t p = 0; p < n; p++) {
+        if (seen[p]) continue;
+        const bool end0 = link[2 * p] < 0, end1 = link[2 * p + 1] < 0;
+        if (!end0 && !end1) continue;
+        // find the other end of this chain
+        int s_in = end0 ? 0 : 1;
+        uint32_t cur = p;
+        int side_out = s_in ^ 1;
+        while (link[2 * cur + side_out] >= 0) {
+            const int64_t e = link[2 * cur + side_out];
+            cur = (uint32_t)(e >> 1);
+            side_out = (int)(e & 1) ^ 1;
+        }
+        // chain ends: (p, s_in) and (cur, side_out); start from the smaller
+        const u128 kp = G.key[p], kc = G.key[cur];
+        if (kc < kp || (kc == kp && side_out < s_in))
+            walk(cur, side_out, false);
+        else
+            walk(p, s_in, false);
+    }
+    // cycles: smallest key, forward
+    std::vector<uint32_t> rest;
+    for (uint32_t p = 0; p < n; p++)
+        if (!seen[p]) rest.push_back(p);
+    std::sort(rest.begin(), rest.end(), [&](uint32_t a, uint32_t b) { return G.key[a] < G.key[b]; });
+    for (uint32_t p : rest)
+        if (!seen[p]) walk(p, 0, true);
+}
+
+void unitig_sequential(const SubGraph& G, uint32_t sub, const std::vector<uint32_t>& order, UnitigOut& out) {
+    const unsigned k = G.k;
+    std::vector<uint8_t> used(G.key.size(), 0);
+    static const char B[4] = {'A', 'C', 'T', 'G'};
+    for (uint32_t p : order) {
+        if (used[p]) continue;
+        used[p] = 1;
+        // extend from side s of p (forward string grows at its end)
+        auto extend = [&](int s, std::vector<uint8_t>& bases) -> bool {
+            uint32_t cur = p;
+            int side_out = s;
+            while (true) {
+                uint32_t q = 0;
+                int t = 0;
+                if (G.neighbours(cur, side_out, &q, &t) != 1) return false;
+                uint32_t back = 0;
+                int bs = 0;
+                if (G.neighbours(q, t, &back, &bs) != 1) return false;
+                if (used[q]) return false;
+                used[q] = 1;
+                const u128 y = t == 0 ? G.key[q] : kmer_rc(G.key[q], k);
+                bases.push_back((uint8_t)B[(y >> (2 * (k - 1))) & 3]);
+                cur = q;
+                side_out = t ^ 1;
+                if (G.flags[q] == 1 || G.flags[q] == 2) return true;
+            }
+        };
+        std::vector<uint8_t> fw, bw;
+        const bool fw_open = G.flagged(p, 1) ? true : extend(1, fw);
+        // backward: extend the reverse complement forward
+        bool bw_open;
+        if (G.flagged(p, 0)) {
+            bw_open = true;
+        } else {
+            bw_open = extend(0, bw);  // bases of the reverse strand
+        }
+        // sequence: rc(bw bases) + kmer + fw bases
+        for (size_t i = bw.size(); i-- > 0;) {
+            const uint8_t c = bw[i];
+            out.seq.push_back(c == 'A' ? 'T' : c == 'C' ? 'G' : c == 'G' ? 'C' : 'A');
+        }
+        put_kmer(out.seq, G.key[p], k);
+        out.seq.insert(out.seq.end(), fw.begin(), fw.end());
+        out.seq.push_back('\n');
+        out.sub.push_back(sub);
+        out.flags.push_back((uint8_t)((bw_open ? 1 : 0) | (fw_open ? 2 : 0)));
+    }
+}
+
 struct Handle {
     RecordSet rec;
     SolidSet solid;
+    UnitigOut unitigs;
     bool colored = false;
     double t_bucket = 0, t_merge = 0;
 };
@@ -620,6 +809,70 @@ void orc_merge(void* hv, uint32_t k, uint32_t b, uint32_t sb, int kind, uint32_t
     h->solid = SolidSet();
     for (auto& p : parts) h->solid.append(p);
     h->t_merge = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Unitig chains over the merged solid table (seq-hash keys).  mode 0: the
+// order-free rule; mode 1: the reference's greedy loop, starting k-mers in
+// ascending key order per sub-bucket.  Returns the number of unitigs; the
+// data stays in the handle (orc_unitigs_get).
+size_t orc_unitigs(void* hv, uint32_t k, int mode) {
+    Handle* h = (Handle*)hv;
+    const SolidSet& S = h->solid;
+    std::map<uint32_t, std::vector<size_t>> by_sub;
+    for (size_t i = 0; i < S.sub.size(); i++) by_sub[S.sub[i]].push_back(i);
+    h->unitigs = UnitigOut();
+    for (auto& kv : by_sub) {
+        SubGraph G;
+        G.k = k;
+        for (size_t i : kv.second) {
+            const u128 key = ((u128)S.key_hi[i] << 64) | S.key_lo[i];
+            G.at.emplace(key, (uint32_t)G.key.size());
+            G.key.push_back(key);
+            G.flags.push_back(S.flags[i]);
+        }
+        if (mode == 0) {
+            unitig_chains(G, kv.first, h->unitigs);
+        } else {
+            std::vector<uint32_t> order(G.key.size());
+            for (uint32_t i = 0; i < order.size(); i++) order[i] = i;
+            std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return G.key[a] < G.key[b]; });
+            unitig_sequential(G, kv.first, order, h->unitigs);
+        }
+    }
+    return h->unitigs.sub.size();
+}
+// In/out extension flags of every solid entry (solid order): degree of side
+// 0 in bits 0-1 and of side 1 in bits 2-3 (2 = two or more), counted among
+// the solid k-mers of the same sub-bucket.
+void orc_extension(void* hv, uint32_t k, uint8_t* ext) {
+    Handle* h = (Handle*)hv;
+    const SolidSet& S = h->solid;
+    std::map<uint32_t, std::vector<size_t>> by_sub;
+    for (size_t i = 0; i < S.sub.size(); i++) by_sub[S.sub[i]].push_back(i);
+    for (auto& kv : by_sub) {
+        SubGraph G;
+        G.k = k;
+        for (size_t i : kv.second) {
+            const u128 key = ((u128)S.key_hi[i] << 64) | S.key_lo[i];
+            G.at.emplace(key, (uint32_t)G.key.size());
+            G.key.push_back(key);
+            G.flags.push_back(S.flags[i]);
+        }
+        for (uint32_t j = 0; j < G.key.size(); j++) {
+            uint32_t q = 0;
+            int t = 0;
+            const uint32_t d0 = G.neighbours(j, 0, &q, &t), d1 = G.neighbours(j, 1, &q, &t);
+            ext[kv.second[j]] = (uint8_t)((d0 > 2 ? 2 : d0) | ((d1 > 2 ? 2 : d1) << 2));
+        }
+    }
+}
+
+size_t orc_unitigs_bytes(void* hv) { return ((Handle*)hv)->unitigs.seq.size(); }
+void orc_unitigs_get(void* hv, uint8_t* seq, uint32_t* sub, uint8_t* flags) {
+    UnitigOut& U = ((Handle*)hv)->unitigs;
+    memcpy(seq, U.seq.data(), U.seq.size());
+    memcpy(sub, U.sub.data(), U.sub.size() * 4);
+    memcpy(flags, U.flags.data(), U.flags.size());
 }
 
 size_t orc_solid_count(void* hv) { return ((Handle*)hv)->solid.sub.size(); }

@@ -257,3 +257,24 @@ def test_query_colored_and_serving_mix(gpu_ctx):
     for i in range(len(reqs)):
         assert all(np.array_equal(a, b) for a, b in zip(results[i], alone[i]))
     assert rep.max_admitted >= 2  # requests overlapped on the device
+
+
+@pytest.mark.parametrize("k,m,rep,err", [(31, 15, 0.2, 0.005), (21, 9, 0.0, 0.01), (45, 21, 0.2, 0.005)])
+def test_unitigs_and_extension_flags(gpu_ctx, k, m, rep, err):
+    from tests.test_unitigs import canon_unitigs
+    g = synth.genome(77, 200_000, repeat_frac=rep, rep_min=300, rep_max=2000)
+    data = synth.reads(g, 78, 30_000, 150, err)
+    _say(f"case unitigs k={k} m={m} bytes={data.size}")
+    gpu_ctx.bucketing(data, _params(k, m, 5, 2, 2))
+    gpu_ctx.kmers_merge()
+    gu = gpu_ctx.unitigs()
+    ext = gpu_ctx.extension_flags()
+    s = gpu_ctx.solid()
+    o = O.bucketing(data, k, m, 5, 2, threads=4)
+    o.merge(k, 5, 2, HASH_SEQ, 2, 4)
+    assert canon_unitigs(*gu) == canon_unitigs(*o.unitigs(k))
+    osol = o.solid()
+    lo = s.keys[:, 0].tolist()
+    hi = s.keys[:, 1].tolist() if s.keys.shape[1] > 1 else [0] * len(lo)
+    assert sorted(zip(s.sub.tolist(), lo, hi, ext.tolist())) == \
+        sorted(zip(osol["sub"].tolist(), osol["key_lo"].tolist(), osol["key_hi"].tolist(), o.extension(k).tolist()))
