@@ -648,7 +648,10 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     M.slot_color = W ? P_<uint32_t>(ctx->slot_color) : nullptr;
     M.lds_try = ctx->tune.lds_max_occ ? ctx->tune.lds_max_occ : M.lds_cap * 16;
     const uint32_t item_occ = ctx->tune.item_occ ? ctx->tune.item_occ : M.lds_cap / 2;
-    const size_t lds_bytes = (size_t)M.lds_cap * entry_bytes + 64;
+    // identical records counted once (seq-hash keys up to k = 32, inline slots)
+    M.dedup = (p.hash_kind == GGCAT_GPU_HASH_SEQ && p.k <= 32 && ctx->slot_words <= 8 && !ctx->tune.no_dedup) ? 1 : 0;
+    M.rec_lds_off = (uint32_t)(((size_t)M.lds_cap * entry_bytes + 64 + 15) / 16 * 16);
+    const size_t lds_bytes = M.dedup ? M.rec_lds_off + rec_table_bytes(ctx->slot_words) : (size_t)M.lds_cap * entry_bytes + 64;
     M.pool_words = ctx->pool_words;
     // rabin-karp128 constants (crates/hashes/src/cn_rkhash.rs:65-78)
     M.rk_mult = mk128(0x3eb9402f3e733993ULL, 0xadd64d3ca00e1b6bULL);

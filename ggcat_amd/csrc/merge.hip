@@ -58,6 +58,8 @@ struct MergeArgs {
     uint32_t lds_cap;  // table entries in LDS (power of two)
     uint32_t lds_try;  // first pass: items with more occurrences go straight to partitioned passes
     uint32_t table_pct;  // table slots per 100 occurrences of a whole-item pass (power-of-two rounded)
+    uint32_t dedup;  // identical records of an entry are inserted once with their count (seq-hash, k <= 32)
+    uint32_t rec_lds_off;  // byte offset of the record table in dynamic LDS
     const WorkItem* items;
     const PartWork* parts;  // null: first pass, one work entry per item
     uint32_t n_work;  // entries of this pass
@@ -368,6 +370,181 @@ GG_DEV void insert_item(const MergeArgs& A, uint64_t r0, uint64_t r1, const Part
     }
 }
 
+// ---- identical-record path (seq-hash keys, k <= 32) -----------------------
+// The reference merges identical super-k-mers of a bucket before counting
+// (crates/minimizer_bucketing/src/compactor.rs:128-196, multiplicities
+// summed, :159-163); counting each distinct record once with its multiplicity
+// gives the same MapEntries.  Records are keyed by their whole slot (meta
+// word with length, flags, minimizer position and cross marks, then the
+// packed bases) plus their color; a record the table cannot take is counted
+// on its own.  After deduplication every thread takes k-mer positions of the
+// distinct records, so the k-mer work is spread evenly over the workgroup.
+constexpr uint32_t REC_CAP = 256;
+
+struct RecTable {
+    uint32_t* key;  // REC_CAP * SW
+    uint32_t* cnt;
+    uint32_t* state;  // 0 free, 1 being written, 2 ready
+    uint32_t* col;
+    uint32_t* list;  // distinct records (table slots)
+    uint32_t* off;  // REC_CAP + 1 k-mer offsets of the distinct records
+    uint32_t* n;  // [2]: distinct records, k-mers
+};
+
+GG_DEV RecTable rec_table(const MergeArgs& A, uint8_t* lds) {
+    RecTable R;
+    uint32_t* b = reinterpret_cast<uint32_t*>(lds + A.rec_lds_off);
+    R.key = b;
+    b += REC_CAP * A.slot_words;
+    R.cnt = b;
+    b += REC_CAP;
+    R.state = b;
+    b += REC_CAP;
+    R.col = b;
+    b += REC_CAP;
+    R.list = b;
+    b += REC_CAP;
+    R.off = b;
+    b += REC_CAP + 1;
+    R.n = b;
+    return R;
+}
+
+// Bytes of dynamic LDS the record table needs for `slot_words`.
+__host__ __device__ inline uint32_t rec_table_bytes(uint32_t slot_words) {
+    return (REC_CAP * slot_words + 5 * REC_CAP + 1 + 2) * 4;
+}
+
+GG_DEV uint32_t rec_hash(const uint32_t* w, uint32_t sw, uint32_t color) {
+    uint64_t h = 0x9E3779B97F4A7C15ULL ^ color;
+    for (uint32_t i = 0; i < sw; i++) h = (h ^ w[i]) * 0xbf58476d1ce4e5b9ULL;
+    h ^= h >> 31;
+    return (uint32_t)h;
+}
+
+// Canonical 2-bit k-mer (k <= 32) at base offset i of packed words (16 bases
+// per word, base j at bits 2(j % 16)); *fwd when the forward strand is the
+// smaller (the rolling path's fw < rc).
+GG_DEV uint64_t kmer_at(const uint32_t* bw, uint32_t i, uint32_t k, bool* fwd) {
+    const uint32_t wi = i >> 4, sh = (i & 15) * 2;
+    const uint64_t lo = (uint64_t)bw[wi] | ((uint64_t)bw[wi + 1] << 32);
+    uint64_t fw = sh ? (lo >> sh) | ((uint64_t)bw[wi + 2] << (64 - sh)) : lo;
+    const uint64_t mask = (k == 32) ? ~0ULL : ((1ULL << (2 * k)) - 1);
+    fw &= mask;
+    // reverse complement: complement every base, reverse the 2-bit groups
+    uint64_t x = fw ^ 0xAAAAAAAAAAAAAAAAULL;
+    x = ((uint64_t)__brev((uint32_t)x) << 32) | __brev((uint32_t)(x >> 32));
+    x = ((x >> 1) & 0x5555555555555555ULL) | ((x & 0x5555555555555555ULL) << 1);
+    const uint64_t rc = x >> (64 - 2 * k);
+    *fwd = fw < rc;
+    return fw < rc ? fw : rc;
+}
+
+template <class KP, class SP, class VP>
+GG_DEV void insert_item_dedup(const MergeArgs& A, uint64_t r0, uint64_t r1, const PartWork& w, KP klo, KP khi, SP st,
+                              VP vals, uint32_t* cbits, uint32_t mask, uint32_t probe_limit, uint32_t* overflow,
+                              const RecTable& R) {
+    const uint32_t W = A.color_words, SW = A.slot_words, k = A.k;
+    const uint32_t part_mask = (1u << w.log_parts) - 1;
+    auto insert_key = [&](uint64_t key, uint32_t bits, uint32_t color) {
+        if (*(volatile uint32_t*)overflow) return;
+        if (part_mask && ((tab_hash64(key) >> 16) & part_mask) != w.part) return;
+        uint32_t at = 0;
+        const uint32_t res = tab_insert64(klo, vals, mask, key, bits, probe_limit, at);
+        if (res == 2)
+            atomicOr(overflow, 1u);
+        else if (W)
+            atomicOr(&cbits[at * W + (color >> 5)], 1u << (color & 31));
+    };
+    for (uint32_t i = threadIdx.x; i < REC_CAP; i += blockDim.x) R.state[i] = 0;
+    if (threadIdx.x == 0) R.n[0] = R.n[1] = 0;
+    __syncthreads();
+    // phase A: distinct records (long records and table misses go direct)
+    for (uint64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+        const uint32_t* slot = A.slots + r * SW;
+        const uint32_t meta = slot[0];
+        const uint32_t color = W ? A.slot_color[r] : 0u;
+        bool direct = (meta & META_LONG) != 0;
+        if (!direct) {
+            uint32_t wv[8];
+            for (uint32_t q = 0; q < SW; q++) wv[q] = slot[q];
+            uint32_t i = rec_hash(wv, SW, color) & (REC_CAP - 1);
+            uint32_t probes = 0, spins = 0;
+            bool done = false;
+            while (!done) {
+                const uint32_t sv = __hip_atomic_load(&R.state[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (sv == 0) {
+                    if (atomicCAS(&R.state[i], 0u, 1u) == 0u) {
+                        for (uint32_t q = 0; q < SW; q++) R.key[i * SW + q] = wv[q];
+                        R.col[i] = color;
+                        R.cnt[i] = 1;
+                        __threadfence_block();
+                        atomicExch(&R.state[i], 2u);
+                        done = true;
+                    }
+                } else if (sv == 2) {
+                    bool eq = R.col[i] == color;
+                    for (uint32_t q = 0; q < SW && eq; q++) eq = R.key[i * SW + q] == wv[q];
+                    if (eq) {
+                        atomicAdd(&R.cnt[i], 1u);
+                        done = true;
+                    } else if (++probes >= 32) {
+                        direct = true;
+                        done = true;
+                    } else {
+                        i = (i + 1) & (REC_CAP - 1);
+                    }
+                } else if (++spins > (1u << 16)) {
+                    direct = true;
+                    done = true;
+                }
+            }
+        }
+        if (direct) {
+            for_each_kmer(A, slot, [&](uint64_t lo, uint64_t, bool is_fwd, uint32_t idx, uint32_t last) {
+                insert_key(lo, occurrence_bits(meta, is_fwd, idx, last), color);
+            });
+        }
+    }
+    __syncthreads();
+    // phase B1: list the distinct records and their k-mer counts
+    for (uint32_t i = threadIdx.x; i < REC_CAP; i += blockDim.x) {
+        if (R.state[i] != 2) continue;
+        const uint32_t u = atomicAdd(&R.n[0], 1u);
+        R.list[u] = i;
+    }
+    __syncthreads();
+    const uint32_t n_u = R.n[0];
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (uint32_t u = 0; u < n_u; u++) {
+            R.off[u] = acc;
+            const uint32_t len = meta_len(R.key[R.list[u] * SW]);
+            acc += len >= k ? len - k + 1 : 0;
+        }
+        R.off[n_u] = acc;
+        R.n[1] = acc;
+    }
+    __syncthreads();
+    // phase B2: every k-mer of the distinct records, one per thread
+    const uint32_t total = R.n[1];
+    for (uint32_t j = threadIdx.x; j < total; j += blockDim.x) {
+        uint32_t lo = 0, hi = n_u;  // last u with off[u] <= j
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (R.off[mid] <= j) lo = mid; else hi = mid;
+        }
+        const uint32_t si = R.list[lo], i = j - R.off[lo];
+        const uint32_t* rw = R.key + si * SW;
+        const uint32_t meta = rw[0];
+        const uint32_t len = meta_len(meta);
+        bool fwd;
+        const uint64_t key = kmer_at(rw + 1, i, k, &fwd);
+        const uint32_t bits = occurrence_bits(meta, fwd, i, len - k) - 1u + R.cnt[si];
+        insert_key(key, bits, R.col[si]);
+    }
+}
+
 // Work entry with the metadata a pass needs, fetched ahead by thread 0.
 struct Fetched {
     uint32_t e;
@@ -439,7 +616,10 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
         }
         for (uint32_t i = threadIdx.x; i < cap * W; i += blockDim.x) cbits[i] = 0;
         __syncthreads();
-        insert_item(A, cur.r0, cur.r1, w, klo, khi, st, vals, cbits, mask, 96u, &s_overflow);
+        if (A.dedup && cur.r1 - cur.r0 <= REC_CAP / 2)
+            insert_item_dedup(A, cur.r0, cur.r1, w, klo, khi, st, vals, cbits, mask, 96u, &s_overflow, rec_table(A, lds));
+        else
+            insert_item(A, cur.r0, cur.r1, w, klo, khi, st, vals, cbits, mask, 96u, &s_overflow);
         if (leader) fetch_meta(A, nxt);
         __syncthreads();
         if (s_overflow) {

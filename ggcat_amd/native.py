@@ -65,6 +65,7 @@ class Tuning(C.Structure):
         ("part_occ", C.c_uint32),
         ("lds_table_log", C.c_uint32),
         ("table_pct", C.c_uint32),
+        ("no_dedup", C.c_uint32),
     ]
 
 
@@ -278,8 +279,8 @@ class GpuContext:
         return used
 
     def set_tuning(self, item_occ: int = 0, lds_max_occ: int = 0, merge_wgs_per_cu: int = 0,
-                   part_occ: int = 0, lds_table_log: int = 0, table_pct: int = 0) -> None:
-        t = Tuning(item_occ, lds_max_occ, merge_wgs_per_cu, part_occ, lds_table_log, table_pct)
+                   part_occ: int = 0, lds_table_log: int = 0, table_pct: int = 0, no_dedup: int = 0) -> None:
+        t = Tuning(item_occ, lds_max_occ, merge_wgs_per_cu, part_occ, lds_table_log, table_pct, no_dedup)
         self._check(self.lib.ggcat_gpu_set_tuning(self.h, C.byref(t)), "ggcat_gpu_set_tuning")
 
     def records(self) -> Records:

@@ -141,7 +141,9 @@ int ggcat_gpu_solid_colors(ggcat_gpu_ctx* ctx, uint32_t* color_words, uint32_t* 
  *   table_pct         table slots per 100 occurrences of an item, rounded up
  *                     to a power of two (default 200: never overflows when
  *                     every k-mer is distinct; lower values trade overflow
- *                     redos for smaller tables) */
+ *                     redos for smaller tables)
+ *   no_dedup          1: insert every record's k-mers even when identical
+ *                     records could be counted once (seq-hash, k <= 32) */
 typedef struct {
     uint32_t item_occ;
     uint32_t lds_max_occ;
@@ -149,6 +151,7 @@ typedef struct {
     uint32_t part_occ;
     uint32_t lds_table_log;
     uint32_t table_pct;
+    uint32_t no_dedup;
 } ggcat_gpu_tuning;
 int ggcat_gpu_set_tuning(ggcat_gpu_ctx* ctx, const ggcat_gpu_tuning* tuning);
 
