@@ -417,9 +417,11 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     ctx->P = p;
     const uint32_t w = p.k - p.m;
     const uint32_t n_sub = ((1u << p.buckets_log) + 1) << p.second_buckets_log;
-    // physical units: about 2000 input bytes per unit, at most 2^23 units
+    // physical units: about `unit_bytes` input bytes per unit (default 2000),
+    // at most 2^23 units
     uint32_t xb = 0;
-    while (xb < 8 && (n_bytes >> xb) / n_sub > 2000 && ((uint64_t)n_sub << (xb + 1)) <= (1u << 23)) xb++;
+    const uint64_t unit_bytes = ctx->tune.unit_bytes ? ctx->tune.unit_bytes : 2000;
+    while (xb < 8 && (n_bytes >> xb) / n_sub > unit_bytes && ((uint64_t)n_sub << (xb + 1)) <= (1u << 23)) xb++;
     const uint32_t n_units = n_sub << xb;
     // slot: meta word + inline bases for the longest unique-minimizer record (2k - m)
     uint32_t sw = 1 + (2 * p.k - p.m + 15) / 16;
@@ -639,7 +641,7 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     M.kind = p.hash_kind;
     M.key_words = key_words;
     const uint32_t W = ctx->colored ? ctx->color_words : 0;
-    const size_t entry_bytes = 12 * key_words + 4 * W;
+    const size_t entry_bytes = 12 * key_words + 4 * W + 2;  // + claimed-slot list entry
     M.lds_cap = key_words == 1 ? 2048 : 1024;
     if (ctx->tune.lds_table_log) M.lds_cap = 1u << std::min<uint32_t>(std::max<uint32_t>(ctx->tune.lds_table_log, 8), 13);
     while (M.lds_cap > 256 && M.lds_cap * entry_bytes > 48 * 1024) M.lds_cap >>= 1;

@@ -261,16 +261,18 @@ GG_DEV void close_chunk(const MergeArgs& A, EmitState& es) {
 
 // Emit solid entries of a table of `cap` slots; cross-marked entries holding
 // only one flag bit go to the fixup list.
+// The table slots claimed by the entry are listed in `claimed` (n of them),
+// so emission and the reset afterwards touch only those.
 template <class KP, class SP, class VP>
 GG_DEV void emit_solid(const MergeArgs& A, uint32_t sub, KP klo, KP khi, SP st, VP vals, const uint32_t* cbits,
-                       uint32_t cap, EmitState& es) {
+                       const uint16_t* claimed, uint32_t n, EmitState& es) {
     const uint32_t W = A.color_words;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-    for (uint32_t i0 = 0; i0 < cap; i0 += blockDim.x) {
-        uint32_t i = i0 + threadIdx.x;
+    for (uint32_t i0 = 0; i0 < n; i0 += blockDim.x) {
+        const uint32_t i = (i0 + threadIdx.x < n) ? claimed[i0 + threadIdx.x] : 0u;
         bool solid = false, fix = false;
         uint32_t outv = 0;
-        if (i < cap) {
+        if (i0 + threadIdx.x < n) {
             bool occupied = (A.key_words == 1) ? (klo[i] != EMPTY64) : (st[i] == 2);
             if (occupied) {
                 uint32_t v = vals[i];
@@ -344,7 +346,8 @@ GG_DEV void emit_solid(const MergeArgs& A, uint32_t sub, KP klo, KP khi, SP st, 
 // *overflow and insertion stops (the entry is then redone as finer parts).
 template <class KP, class SP, class VP>
 GG_DEV void insert_item(const MergeArgs& A, uint64_t r0, uint64_t r1, const PartWork& w, KP klo, KP khi, SP st,
-                        VP vals, uint32_t* cbits, uint32_t mask, uint32_t probe_limit, uint32_t* overflow) {
+                        VP vals, uint32_t* cbits, uint32_t mask, uint32_t probe_limit, uint32_t* overflow,
+                        uint16_t* claimed, uint32_t* n_claimed) {
     const uint32_t W = A.color_words;
     const uint32_t SW = A.slot_words;
     const uint32_t part_mask = (1u << w.log_parts) - 1;
@@ -362,10 +365,12 @@ GG_DEV void insert_item(const MergeArgs& A, uint64_t r0, uint64_t r1, const Part
             uint32_t res = (A.key_words == 1)
                                ? tab_insert64(klo, vals, mask, lo, bits, probe_limit, at)
                                : tab_insert128(klo, khi, st, vals, mask, lo, hi, bits, probe_limit, at);
-            if (res == 2)
+            if (res == 2) {
                 atomicOr(overflow, 1u);
-            else if (W)
-                atomicOr(&cbits[at * W + (color >> 5)], 1u << (color & 31));
+                return;
+            }
+            if (res == 1) claimed[atomicAdd(n_claimed, 1u)] = (uint16_t)at;
+            if (W) atomicOr(&cbits[at * W + (color >> 5)], 1u << (color & 31));
         });
     }
 }
@@ -443,7 +448,7 @@ GG_DEV uint64_t kmer_at(const uint32_t* bw, uint32_t i, uint32_t k, bool* fwd) {
 template <class KP, class SP, class VP>
 GG_DEV void insert_item_dedup(const MergeArgs& A, uint64_t r0, uint64_t r1, const PartWork& w, KP klo, KP khi, SP st,
                               VP vals, uint32_t* cbits, uint32_t mask, uint32_t probe_limit, uint32_t* overflow,
-                              const RecTable& R) {
+                              uint16_t* claimed, uint32_t* n_claimed, const RecTable& R) {
     const uint32_t W = A.color_words, SW = A.slot_words, k = A.k;
     const uint32_t part_mask = (1u << w.log_parts) - 1;
     auto insert_key = [&](uint64_t key, uint32_t bits, uint32_t color) {
@@ -451,10 +456,12 @@ GG_DEV void insert_item_dedup(const MergeArgs& A, uint64_t r0, uint64_t r1, cons
         if (part_mask && ((tab_hash64(key) >> 16) & part_mask) != w.part) return;
         uint32_t at = 0;
         const uint32_t res = tab_insert64(klo, vals, mask, key, bits, probe_limit, at);
-        if (res == 2)
+        if (res == 2) {
             atomicOr(overflow, 1u);
-        else if (W)
-            atomicOr(&cbits[at * W + (color >> 5)], 1u << (color & 31));
+            return;
+        }
+        if (res == 1) claimed[atomicAdd(n_claimed, 1u)] = (uint16_t)at;
+        if (W) atomicOr(&cbits[at * W + (color >> 5)], 1u << (color & 31));
     };
     for (uint32_t i = threadIdx.x; i < REC_CAP; i += blockDim.x) R.state[i] = 0;
     if (threadIdx.x == 0) R.n[0] = R.n[1] = 0;
@@ -515,17 +522,33 @@ GG_DEV void insert_item_dedup(const MergeArgs& A, uint64_t r0, uint64_t r1, cons
     }
     __syncthreads();
     const uint32_t n_u = R.n[0];
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        for (uint32_t u = 0; u < n_u; u++) {
-            R.off[u] = acc;
+    // exclusive prefix of the distinct records' k-mer counts (REC_CAP <= 4
+    // waves of 64: lane prefix by shuffles, then the wave totals)
+    {
+        const uint32_t u = threadIdx.x, lane = u & 63, wave = u >> 6;
+        uint32_t c = 0;
+        if (u < n_u) {
             const uint32_t len = meta_len(R.key[R.list[u] * SW]);
-            acc += len >= k ? len - k + 1 : 0;
+            c = len >= k ? len - k + 1 : 0;
         }
-        R.off[n_u] = acc;
-        R.n[1] = acc;
+        uint32_t inc = c;
+        for (uint32_t d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(inc, d);
+            if (lane >= d) inc += y;
+        }
+        if (lane == 63) R.off[REC_CAP - 4 + wave] = inc;  // wave totals, parked past the used range
+        __syncthreads();
+        uint32_t base = 0;
+        for (uint32_t q = 0; q < wave; q++) base += R.off[REC_CAP - 4 + q];
+        const uint32_t all = R.off[REC_CAP - 4] + R.off[REC_CAP - 3] + R.off[REC_CAP - 2] + R.off[REC_CAP - 1];
+        __syncthreads();
+        if (u < n_u) R.off[u] = base + inc - c;
+        if (u == 0) {
+            R.off[n_u] = all;
+            R.n[1] = all;
+        }
+        __syncthreads();
     }
-    __syncthreads();
     // phase B2: every k-mer of the distinct records, one per thread
     const uint32_t total = R.n[1];
     for (uint32_t j = threadIdx.x; j < total; j += blockDim.x) {
@@ -577,11 +600,21 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
     uint32_t* st = vals + cap_max;  // only used for 128-bit keys
     uint32_t* cbits = vals + (size_t)cap_max * A.key_words;  // color_words per entry (colored builds)
     const uint32_t W = A.color_words;
+    uint16_t* claimed = reinterpret_cast<uint16_t*>(cbits + (size_t)cap_max * W);  // slots claimed by the entry
+    __shared__ uint32_t s_claimed;
     const bool leader = threadIdx.x == 0;
+    // the table starts empty; afterwards only claimed slots are reset
+    for (uint32_t i = threadIdx.x; i < cap_max; i += blockDim.x) {
+        klo[i] = EMPTY64;
+        vals[i] = 0;
+        if (A.key_words == 2) st[i] = 0;
+    }
+    for (uint32_t i = threadIdx.x; i < cap_max * W; i += blockDim.x) cbits[i] = 0;
     Fetched nxt{};
     if (leader) {
         es.chunk = ~0u;
         es.fill = 0;
+        s_claimed = 0;
         nxt.e = atomicAdd(A.work, 1u);
         fetch_meta(A, nxt);
     }
@@ -609,25 +642,29 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
         while (cap < want && cap < cap_max) cap <<= 1;
         if (w.log_parts) cap = cap_max;
         const uint32_t mask = cap - 1;
-        for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) {
-            klo[i] = EMPTY64;
-            vals[i] = 0;
-            if (A.key_words == 2) st[i] = 0;
-        }
-        for (uint32_t i = threadIdx.x; i < cap * W; i += blockDim.x) cbits[i] = 0;
-        __syncthreads();
         if (A.dedup && cur.r1 - cur.r0 <= REC_CAP / 2)
-            insert_item_dedup(A, cur.r0, cur.r1, w, klo, khi, st, vals, cbits, mask, 96u, &s_overflow, rec_table(A, lds));
+            insert_item_dedup(A, cur.r0, cur.r1, w, klo, khi, st, vals, cbits, mask, 96u, &s_overflow, claimed,
+                              &s_claimed, rec_table(A, lds));
         else
-            insert_item(A, cur.r0, cur.r1, w, klo, khi, st, vals, cbits, mask, 96u, &s_overflow);
+            insert_item(A, cur.r0, cur.r1, w, klo, khi, st, vals, cbits, mask, 96u, &s_overflow, claimed, &s_claimed);
         if (leader) fetch_meta(A, nxt);
         __syncthreads();
+        const uint32_t n_cl = s_claimed;
         if (s_overflow) {
             // more distinct k-mers than the table holds: redone as finer parts
             if (leader) A.redo[atomicAdd(A.redo_count, 1u)] = w;
         } else {
-            emit_solid(A, it.sub, klo, khi, st, vals, cbits, cap, es);
+            emit_solid(A, it.sub, klo, khi, st, vals, cbits, claimed, n_cl, es);
         }
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < n_cl; j += blockDim.x) {
+            const uint32_t i = claimed[j];
+            klo[i] = EMPTY64;
+            vals[i] = 0;
+            if (A.key_words == 2) st[i] = 0;
+            for (uint32_t q = 0; q < W; q++) cbits[i * W + q] = 0;
+        }
+        if (leader) s_claimed = 0;
         __syncthreads();
     }
     if (leader) close_chunk(A, es);

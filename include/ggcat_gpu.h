@@ -143,7 +143,9 @@ int ggcat_gpu_solid_colors(ggcat_gpu_ctx* ctx, uint32_t* color_words, uint32_t* 
  *                     every k-mer is distinct; lower values trade overflow
  *                     redos for smaller tables)
  *   no_dedup          1: insert every record's k-mers even when identical
- *                     records could be counted once (seq-hash, k <= 32) */
+ *                     records could be counted once (seq-hash, k <= 32)
+ *   unit_bytes        bucketing: input bytes per physical unit (sub-bucket
+ *                     refinement); default 2000 */
 typedef struct {
     uint32_t item_occ;
     uint32_t lds_max_occ;
@@ -152,6 +154,7 @@ typedef struct {
     uint32_t lds_table_log;
     uint32_t table_pct;
     uint32_t no_dedup;
+    uint32_t unit_bytes;
 } ggcat_gpu_tuning;
 int ggcat_gpu_set_tuning(ggcat_gpu_ctx* ctx, const ggcat_gpu_tuning* tuning);
 
