@@ -128,6 +128,7 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the colored query serving measurement")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default schedule")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson request rate of the serving run (1/s)")
     ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--calib-copy", action="store_true",
@@ -181,6 +182,20 @@ def main() -> None:
             raise SystemExit("[bench] parity gate FAILED: GPU solid table differs from the oracle")
         log(f"[bench] parity gate ok on {cut} bytes ({g.shape[0]} solid entries)")
 
+    tuning = None
+    if not a.no_autotune:
+        # schedule picked on a bounded sample of the same input (results are
+        # unchanged by construction and checked by the tuner)
+        from ggcat_amd.autotune import autotune
+
+        cut = min(wl.data.size, 256 << 20)
+        while cut < wl.data.size and wl.data[cut - 1] != ord("\n"):
+            cut += 1
+        t = time.perf_counter()
+        tuning = autotune(ctx, wl.data[:cut], params, colored=False)
+        log(f"[bench] autotune: {tuning['tried']} schedules on {cut} bytes in {time.perf_counter() - t:.1f} s -> "
+            f"{tuning['schedule']}")
+
     def step():
         ctx.bucketing(None, params, device_ptr=dptr, n_bytes=nbytes)
         ctx.kmers_merge()
@@ -227,6 +242,7 @@ def main() -> None:
         "counters": {"records": int(cnt[1]), "record_bases": int(cnt[2]), "kmer_occurrences": int(cnt[3]),
                      "hbm_table_subbuckets": int(cnt[4]), "solid_entries": int(n)},
         "h2d_ms": round(h2d_s * 1e3, 1),
+        "tuning": tuning,
         "pcie_inclusive_gbases_s": round(wl.bases / (dt / a.steps + h2d_s) / 1e9, 4),
     }
     ctx.free(dptr)

@@ -1,0 +1,111 @@
+"""Schedule autotuner for the build: times bucketing + merge of a sample
+under candidate schedules (ggcat_gpu_set_tuning knobs) and keeps the
+fastest.  The knobs change which kernel path and table size handle a work
+item, never the result; every candidate's solid table is checked against the
+first one's digest.  Choices are cached per (k, m, hash, colored).
+
+The bucket count (-b) changes the result (MapEntries are per sub-bucket), so
+it is only searched when the caller allows it, and then returned as a
+recommendation alongside the schedule.
+"""
+from __future__ import annotations
+
+import itertools
+import json
+import os
+import time
+from dataclasses import asdict, dataclass
+
+import numpy as np
+
+from .native import Params
+
+KNOBS = ("item_occ", "lds_max_occ", "merge_wgs_per_cu", "part_occ", "lds_table_log", "table_pct", "no_dedup",
+         "unit_bytes", "use_graphs")
+
+
+@dataclass
+class Schedule:
+    unit_bytes: int = 0
+    table_pct: int = 0
+    lds_table_log: int = 0
+    no_dedup: int = 0
+    merge_wgs_per_cu: int = 0
+    use_graphs: int = 0
+
+    def apply(self, ctx) -> None:
+        ctx.set_tuning(unit_bytes=self.unit_bytes, table_pct=self.table_pct, lds_table_log=self.lds_table_log,
+                       no_dedup=self.no_dedup, merge_wgs_per_cu=self.merge_wgs_per_cu, use_graphs=self.use_graphs)
+
+
+def default_grid(k: int) -> list[Schedule]:
+    logs = (10, 11) if k <= 32 else (9, 10)
+    out = []
+    for ub, pct, lg, nd, wg in itertools.product((1000, 2000, 4000), (100, 200), logs, (0, 1), (0, 4)):
+        out.append(Schedule(ub, pct, lg, nd, wg, 1))
+    return out
+
+
+def _digest(ctx) -> tuple:
+    s = ctx.solid()
+    keys = s.keys.astype(np.uint64)
+    with np.errstate(over="ignore"):
+        h = (keys[:, 0] * np.uint64(0x9E3779B97F4A7C15)) ^ (s.sub.astype(np.uint64) << np.uint64(32)) \
+            ^ (s.mult.astype(np.uint64) << np.uint64(8)) ^ s.flags.astype(np.uint64)
+        return int(s.mult.size), int(np.bitwise_xor.reduce(h)) if h.size else 0, int(h.sum(dtype=np.uint64))
+
+
+def autotune(ctx, sample: np.ndarray, params: Params, grid: list[Schedule] | None = None, repeats: int = 2,
+             cache_path: str | None = None, colored: bool = False, bucket_logs: tuple[int, ...] = ()) -> dict:
+    """Returns {"schedule": Schedule fields, "ms": best bucketing+merge ms on
+    the sample, "tried": n, "buckets_log": recommended b}; applies the best
+    schedule to ctx."""
+    key = f"k{params.k}-m{params.m}-h{params.hash_kind}-c{int(colored)}"
+    if cache_path and os.path.exists(cache_path):
+        cached = json.load(open(cache_path)).get(key)
+        if cached:
+            Schedule(**cached["schedule"]).apply(ctx)
+            return cached
+    grid = grid or default_grid(params.k)
+    sample = np.ascontiguousarray(sample, np.uint8)
+    ptr, n = ctx.upload(sample)
+    ref = None
+    best = None
+    try:
+        for sch in grid:
+            sch.apply(ctx)
+            ts = []
+            for _ in range(repeats):
+                t0 = time.perf_counter()
+                ctx.bucketing(None, params, device_ptr=ptr, n_bytes=n)
+                ctx.kmers_merge()
+                ts.append((time.perf_counter() - t0) * 1e3)
+            d = _digest(ctx)
+            if ref is None:
+                ref = d
+            elif d != ref:
+                raise RuntimeError(f"schedule {sch} changed the solid table")
+            ms = min(ts)
+            if best is None or ms < best[0]:
+                best = (ms, sch)
+        rec_b = params.buckets_log
+        for b in bucket_logs:
+            p2 = Params(params.k, params.m, b, params.second_buckets_log, params.forward_only, params.hash_kind,
+                        params.min_multiplicity)
+            best[1].apply(ctx)
+            t0 = time.perf_counter()
+            ctx.bucketing(None, p2, device_ptr=ptr, n_bytes=n)
+            ctx.kmers_merge()
+            ms = (time.perf_counter() - t0) * 1e3
+            if ms < best[0]:
+                rec_b = b
+    finally:
+        ctx.free(ptr)
+    best[1].apply(ctx)
+    out = {"schedule": asdict(best[1]), "ms": round(best[0], 3), "tried": len(grid), "buckets_log": rec_b,
+           "sample_bytes": int(sample.size)}
+    if cache_path:
+        allc = json.load(open(cache_path)) if os.path.exists(cache_path) else {}
+        allc[key] = out
+        json.dump(allc, open(cache_path, "w"), indent=1)
+    return out

@@ -20,6 +20,11 @@
 
 using namespace gg;
 
+__global__ void fill32_kernel(uint32_t* p, uint32_t v, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+
 namespace {
 
 struct DevBuf {
@@ -54,6 +59,13 @@ struct ggcat_gpu_ctx {
     uint32_t color_words = 0;  // bitmap words per solid entry of a colored build
     double ms[8] = {0};
     ggcat_gpu_tuning tune{};  // zero fields select the defaults
+    hipGraphExec_t graph_a = nullptr;  // captured bucketing pass A (graphs enabled)
+    struct GraphKeyA {
+        const void* in;
+        uint64_t n;
+        uint32_t k, m, b, sb, xb, T, canonical;
+        void *count, *seg, *off, *part;
+    } graph_a_key{};
     uint32_t n_color_ids = 0;  // largest color id + 1 of the colored build
     // query index over the solid table
     DevBuf idx_key, idx_pos, idx_boff, idx_bcount, idx_bcursor;
@@ -385,6 +397,7 @@ void ggcat_gpu_destroy(ggcat_gpu_ctx* ctx) {
                       &ctx->u_hidx, &ctx->u_seq, &ctx->u_sub, &ctx->u_flags, &ctx->u_visited, &ctx->u_rest,
                       &ctx->u_rest_n};
     if (ctx->server) (void)ggcat_gpu_serve_close(ctx);
+    if (ctx->graph_a) (void)hipGraphExecDestroy(ctx->graph_a);
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (int i = 0; i < 16; i++) (void)hipEventDestroy(ctx->ev[i]);
@@ -496,21 +509,51 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     nthash_tables(p.m, A.gtab);
     const TileGeom G = tile_geom(T, w, p.k, p.m, regs16);
 
-    HIPCHK(hipMemsetAsync(ctx->unit_count.p, 0, (size_t)n_units * 4, ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->seg_bits.p, 0, seg_words * 4, ctx->stream));
-    HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
-    if (n_tiles) {
-        if (regs16)
-            hipLaunchKernelGGL(bucket_count_kernel<16>, dim3((uint32_t)n_tiles), dim3(64), G.bytes, ctx->stream, A);
-        else
-            hipLaunchKernelGGL(bucket_count_kernel<0>, dim3((uint32_t)n_tiles), dim3(64), G.bytes, ctx->stream, A);
+    // Pass A: clear the unit counts and the group-start bitmap, count, scan.
+    // With graphs enabled the sequence is captured once per input shape and
+    // buffer set and replayed afterwards (fills are kernels, never memsets).
+    ENSURE(ctx->scan_part, (((size_t)n_units + SCAN_CHUNK - 1) / SCAN_CHUNK + 1) * 8);
+    auto pass_a = [&]() -> int {
+        hipLaunchKernelGGL(fill32_kernel, dim3(256), dim3(256), 0, ctx->stream, P_<uint32_t>(ctx->unit_count), 0u,
+                           (uint64_t)n_units);
+        hipLaunchKernelGGL(fill32_kernel, dim3(256), dim3(256), 0, ctx->stream, P_<uint32_t>(ctx->seg_bits), 0u,
+                           (uint64_t)seg_words);
+        if (n_tiles) {
+            if (regs16)
+                hipLaunchKernelGGL(bucket_count_kernel<16>, dim3((uint32_t)n_tiles), dim3(64), G.bytes, ctx->stream, A);
+            else
+                hipLaunchKernelGGL(bucket_count_kernel<0>, dim3((uint32_t)n_tiles), dim3(64), G.bytes, ctx->stream, A);
+        }
         HIPCHK(hipGetLastError());
-    }
-    HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
-    {
-        int r = device_scan(ctx, A.sub_count, P_<uint64_t>(ctx->unit_off), n_units);
+        return device_scan(ctx, A.sub_count, P_<uint64_t>(ctx->unit_off), n_units);
+    };
+    HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+    if (ctx->tune.use_graphs) {
+        ggcat_gpu_ctx::GraphKeyA key{d_in, n_bytes, p.k, p.m, p.buckets_log, p.second_buckets_log, xb, T, A.canonical,
+                      ctx->unit_count.p, ctx->seg_bits.p, ctx->unit_off.p, ctx->scan_part.p};
+        if (!ctx->graph_a || memcmp(&key, &ctx->graph_a_key, sizeof key) != 0) {
+            if (ctx->graph_a) (void)hipGraphExecDestroy(ctx->graph_a);
+            ctx->graph_a = nullptr;
+            hipGraph_t g;
+            HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            const int r = pass_a();
+            hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+            if (r != GGCAT_GPU_OK) return r;
+            if (e != hipSuccess) return fail(ctx, GGCAT_GPU_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+            e = hipGraphInstantiate(&ctx->graph_a, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                ctx->graph_a = nullptr;
+                return fail(ctx, GGCAT_GPU_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+            }
+            ctx->graph_a_key = key;
+        }
+        HIPCHK(hipGraphLaunch(ctx->graph_a, ctx->stream));
+    } else {
+        const int r = pass_a();
         if (r != GGCAT_GPU_OK) return r;
     }
+    HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
     uint64_t total = 0;
     HIPCHK(hipMemcpyAsync(&total, P_<uint64_t>(ctx->unit_off) + n_units, 8, hipMemcpyDeviceToHost, ctx->stream));

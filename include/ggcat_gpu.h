@@ -145,7 +145,9 @@ int ggcat_gpu_solid_colors(ggcat_gpu_ctx* ctx, uint32_t* color_words, uint32_t* 
  *   no_dedup          1: insert every record's k-mers even when identical
  *                     records could be counted once (seq-hash, k <= 32)
  *   unit_bytes        bucketing: input bytes per physical unit (sub-bucket
- *                     refinement); default 2000 */
+ *                     refinement); default 2000
+ *   use_graphs        1: capture the bucketing count/scan sequence in a HIP
+ *                     graph per input shape and replay it on later calls */
 typedef struct {
     uint32_t item_occ;
     uint32_t lds_max_occ;
@@ -155,6 +157,7 @@ typedef struct {
     uint32_t table_pct;
     uint32_t no_dedup;
     uint32_t unit_bytes;
+    uint32_t use_graphs;
 } ggcat_gpu_tuning;
 int ggcat_gpu_set_tuning(ggcat_gpu_ctx* ctx, const ggcat_gpu_tuning* tuning);
 

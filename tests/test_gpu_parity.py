@@ -278,3 +278,24 @@ def test_unitigs_and_extension_flags(gpu_ctx, k, m, rep, err):
     hi = s.keys[:, 1].tolist() if s.keys.shape[1] > 1 else [0] * len(lo)
     assert sorted(zip(s.sub.tolist(), lo, hi, ext.tolist())) == \
         sorted(zip(osol["sub"].tolist(), osol["key_lo"].tolist(), osol["key_hi"].tolist(), o.extension(k).tolist()))
+
+
+def test_autotune_and_graphs_keep_results(gpu_ctx):
+    from ggcat_amd.autotune import Schedule, autotune
+    wl = synth.config1(0.02)
+    p = _params(31, 15, wl.buckets_log, wl.second_buckets_log)
+    _say(f"case autotune/graphs bytes={wl.data.size}")
+    gpu_ctx.set_tuning()
+    gpu_ctx.bucketing(wl.data, p)
+    gpu_ctx.kmers_merge()
+    ref = solid_array(gpu_ctx.solid())
+    grid = [Schedule(1000, 100, 10, 0, 0, 1), Schedule(2000, 200, 11, 1, 4, 1), Schedule(4000, 200, 11, 0, 0, 0)]
+    res = autotune(gpu_ctx, wl.data, p, grid=grid, repeats=1)
+    assert res["tried"] == 3
+    try:
+        for _ in range(3):  # graph captured on the first call, replayed after
+            gpu_ctx.bucketing(wl.data, p)
+            gpu_ctx.kmers_merge()
+            assert np.array_equal(solid_array(gpu_ctx.solid()), ref)
+    finally:
+        gpu_ctx.set_tuning()
