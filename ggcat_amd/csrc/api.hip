@@ -471,7 +471,7 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     const uint64_t seg_words = n_tiles * T / 32 + 2;
     ENSURE(ctx->seg_bits, seg_words * 4);
     ENSURE(ctx->unit_count, (size_t)n_units * 4);
-    ENSURE(ctx->unit_cursor, (size_t)n_units * 4);
+    ENSURE(ctx->unit_cursor, (size_t)n_units * 8);
     ENSURE(ctx->unit_kmers, (size_t)n_units * 4);
     ENSURE(ctx->unit_off, ((size_t)n_units + 1) * 8);
     ENSURE(ctx->pool_cursor, 8);
@@ -497,7 +497,7 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     A.n_tiles = n_tiles;
     A.seg_bits = P_<uint32_t>(ctx->seg_bits);
     A.sub_count = P_<uint32_t>(ctx->unit_count);
-    A.sub_cursor = P_<uint32_t>(ctx->unit_cursor);
+    A.sub_cursor = P_<unsigned long long>(ctx->unit_cursor);
     A.sub_off = P_<uint64_t>(ctx->unit_off);
     A.sub_kmers = P_<uint32_t>(ctx->unit_kmers);
     A.slot_words = slot_words;
@@ -562,6 +562,7 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     ENSURE(ctx->slots, total * slot_words * 4);
     if (n_colors) ENSURE(ctx->slot_color, total * 4);
     A.slots = P_<uint32_t>(ctx->slots);
+    A.n_slots = total;
     A.slot_color = n_colors ? P_<uint32_t>(ctx->slot_color) : nullptr;
 
     uint64_t pool_cap = std::max<uint64_t>(ctx->pool.cap / 4, n_bytes / 64 + 65536);
@@ -569,7 +570,8 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
         ENSURE(ctx->pool, pool_cap * 4);
         A.pool = P_<uint32_t>(ctx->pool);
         A.pool_cap = pool_cap;
-        HIPCHK(hipMemsetAsync(ctx->unit_cursor.p, 0, (size_t)n_units * 4, ctx->stream));
+        hipLaunchKernelGGL(cursor_init_kernel, dim3(std::min<uint32_t>((n_units + 255) / 256, 4096)), dim3(256), 0,
+                           ctx->stream, A.sub_cursor, A.sub_off, n_units);
         HIPCHK(hipMemsetAsync(ctx->unit_kmers.p, 0, (size_t)n_units * 4, ctx->stream));
         HIPCHK(hipMemsetAsync(ctx->pool_cursor.p, 0, 8, ctx->stream));
         HIPCHK(hipMemsetAsync(ctx->error_flag.p, 0, 8, ctx->stream));
@@ -581,6 +583,8 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
                 hipLaunchKernelGGL(bucket_emit_kernel<0>, dim3((uint32_t)n_tiles), dim3(64), G.bytes, ctx->stream, A);
             HIPCHK(hipGetLastError());
         }
+        hipLaunchKernelGGL(cursor_check_kernel, dim3(std::min<uint32_t>((n_units + 255) / 256, 4096)), dim3(256), 0,
+                           ctx->stream, A.sub_cursor, A.sub_off, n_units, A.error_flag);
         HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
         uint64_t used = 0;
         uint32_t errf = 0;

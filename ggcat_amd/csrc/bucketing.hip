@@ -46,7 +46,8 @@ struct BucketArgs {
     uint32_t* seg_bits;  // [ceil(n/32)+2] group-start bitmap (pass A writes)
     uint32_t* sub_count;  // [n_units] records per unit (pass A)
     // pass B
-    uint32_t* sub_cursor;  // [n_units]
+    unsigned long long* sub_cursor;  // [n_units] next slot of each unit (starts at its offset)
+    uint64_t n_slots;  // slots in the arena
     const uint64_t* sub_off;  // [n_units+1] slot offsets
     uint32_t* slots;
     uint32_t slot_words;
@@ -658,9 +659,10 @@ __global__ __launch_bounds__(64) void bucket_emit_kernel(BucketArgs A) {
                 if (sub_of(vr, A.b, A.sb) == sub && unit_of(vr, A.b, A.sb, A.xb) != unit) cross |= rc ? META_XBEGIN : META_XEND;
             }
         }
-        const uint64_t slot = A.sub_off[unit] + atomicAdd(&A.sub_cursor[unit], 1u);
-        if (slot >= A.sub_off[unit + 1]) {
-            // pass A and pass B disagree on this unit: never write past it
+        const uint64_t slot = atomicAdd(&A.sub_cursor[unit], 1ULL);
+        if (slot >= A.n_slots) {
+            // pass A and pass B disagree: never write past the arena (the
+            // per-unit check after the pass reports it)
             atomicOr(A.error_flag, 1u);
             continue;
         }
@@ -668,21 +670,31 @@ __global__ __launch_bounds__(64) void bucket_emit_kernel(BucketArgs A) {
         uint32_t* out = A.slots + slot * SW;
         const int64_t rs = start - t.borg;
         if (len <= inline_cap && rs >= 0) {
-            out[0] = len | (flags << 8) | (minpos << 10) | (rc ? META_RC : 0u) | cross;
+            // the slot is written with 16-byte stores (slots are 16-byte
+            // aligned and a multiple of 4 words), one transaction per 16 bytes
             const uint32_t nw = (len + 15) / 16;
-            for (uint32_t q = 1; q < SW; q++) {
-                uint32_t wq = 0;
-                uint32_t qi = q - 1;
-                if (qi < nw) {
-                    if (!rc) {
-                        wq = extract16(t, rs + 16 * qi);
-                    } else {
-                        wq = rev_comp16(extract16(t, (end - t.borg) - 16 * (int64_t)qi - 16));
+            uint4* o4 = reinterpret_cast<uint4*>(out);
+            for (uint32_t g = 0; g < SW / 4; g++) {
+                uint32_t w4[4];
+#pragma unroll
+                for (uint32_t j = 0; j < 4; j++) {
+                    const uint32_t q = 4 * g + j;
+                    uint32_t wq = 0;
+                    if (q == 0) {
+                        wq = len | (flags << 8) | (minpos << 10) | (rc ? META_RC : 0u) | cross;
+                    } else if (q - 1 < nw) {
+                        const uint32_t qi = q - 1;
+                        if (!rc) {
+                            wq = extract16(t, rs + 16 * qi);
+                        } else {
+                            wq = rev_comp16(extract16(t, (end - t.borg) - 16 * (int64_t)qi - 16));
+                        }
+                        const uint32_t rem = len - 16 * qi;
+                        if (rem < 16) wq &= (1u << (2 * rem)) - 1u;
                     }
-                    uint32_t rem = len - 16 * qi;
-                    if (rem < 16) wq &= (1u << (2 * rem)) - 1u;
+                    w4[j] = wq;
                 }
-                out[q] = wq;
+                o4[g] = make_uint4(w4[0], w4[1], w4[2], w4[3]);
             }
         } else {
             // generic path: bases possibly outside the tile, or a long record
@@ -722,6 +734,16 @@ __global__ __launch_bounds__(64) void bucket_emit_kernel(BucketArgs A) {
             A.slot_color[slot] = A.colors[lo];
         }
     }
+}
+
+// Slot cursors start at each unit's offset; after pass B every cursor must
+// have reached the next unit's offset (passes A and B agree on every unit).
+__global__ void cursor_init_kernel(unsigned long long* cursor, const uint64_t* off, uint32_t n) {
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) cursor[u] = off[u];
+}
+__global__ void cursor_check_kernel(const unsigned long long* cursor, const uint64_t* off, uint32_t n, uint32_t* err) {
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x)
+        if (cursor[u] != off[u + 1]) atomicOr(err, 1u);
 }
 
 // Device-wide exclusive scan of uint32 counts into uint64 offsets (n+1
