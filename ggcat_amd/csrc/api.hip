@@ -42,7 +42,8 @@ struct ggcat_gpu_ctx {
     hipEvent_t ev[16];
     // bucketing buffers
     DevBuf input, seg_bits, unit_count, unit_cursor, unit_off, unit_kmers, scan_part, slots, pool, pool_cursor,
-        slot_color, color_starts, colors, error_flag;
+        slot_color, color_starts, colors, error_flag, stage_slots, stage_unit, stage_rank, stage_color, stage_cursor, defer,
+        defer_count;
     // merge buffers
     DevBuf items, n_items, work, parts, redo, redo_count, out_key, out_val, out_sub, chunk_cursor, chunk_fill_dev,
         fix_key, fix_val, fix_sub, fix_cursor, out_colors, fix_colors;
@@ -61,10 +62,9 @@ struct ggcat_gpu_ctx {
     ggcat_gpu_tuning tune{};  // zero fields select the defaults
     hipGraphExec_t graph_a = nullptr;  // captured bucketing pass A (graphs enabled)
     struct GraphKeyA {
-        const void* in;
-        uint64_t n;
-        uint32_t k, m, b, sb, xb, T, canonical;
-        void *count, *seg, *off, *part;
+        gg::BucketArgs args;  // every kernel argument of the captured launches
+        uint64_t seg_words;
+        uint32_t n_units;
     } graph_a_key{};
     uint32_t n_color_ids = 0;  // largest color id + 1 of the colored build
     // query index over the solid table
@@ -483,7 +483,8 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
         HIPCHK(hipMemcpy(ctx->colors.p, colors, (size_t)n_colors * 4, hipMemcpyHostToDevice));
     }
 
-    BucketArgs A{};
+    BucketArgs A;
+    memset(&A, 0, sizeof A);  // padding included: the struct is part of the graph key
     A.seq = d_in;
     A.n = n_bytes;
     A.k = p.k;
@@ -497,7 +498,7 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     A.n_tiles = n_tiles;
     A.seg_bits = P_<uint32_t>(ctx->seg_bits);
     A.sub_count = P_<uint32_t>(ctx->unit_count);
-    A.sub_cursor = P_<unsigned long long>(ctx->unit_cursor);
+    A.unit_ck = P_<unsigned long long>(ctx->unit_cursor);
     A.sub_off = P_<uint64_t>(ctx->unit_off);
     A.sub_kmers = P_<uint32_t>(ctx->unit_kmers);
     A.slot_words = slot_words;
@@ -508,95 +509,126 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     A.error_flag = P_<uint32_t>(ctx->error_flag);
     nthash_tables(p.m, A.gtab);
     const TileGeom G = tile_geom(T, w, p.k, p.m, regs16);
-
-    // Pass A: clear the unit counts and the group-start bitmap, count, scan.
-    // With graphs enabled the sequence is captured once per input shape and
-    // buffer set and replayed afterwards (fills are kernels, never memsets).
+    ENSURE(ctx->stage_cursor, 8);
+    ENSURE(ctx->defer_count, 8);
+    ENSURE(ctx->defer, std::max<uint64_t>(n_tiles, 1) * 24);
+    A.stage_cursor = P_<unsigned long long>(ctx->stage_cursor);
+    A.defer = P_<uint64_t>(ctx->defer);
+    A.defer_count = P_<uint32_t>(ctx->defer_count);
     ENSURE(ctx->scan_part, (((size_t)n_units + SCAN_CHUNK - 1) / SCAN_CHUNK + 1) * 8);
-    auto pass_a = [&]() -> int {
-        hipLaunchKernelGGL(fill32_kernel, dim3(256), dim3(256), 0, ctx->stream, P_<uint32_t>(ctx->unit_count), 0u,
-                           (uint64_t)n_units);
-        hipLaunchKernelGGL(fill32_kernel, dim3(256), dim3(256), 0, ctx->stream, P_<uint32_t>(ctx->seg_bits), 0u,
-                           (uint64_t)seg_words);
+
+    // Pass 1: clear the unit counts, k-mer counts, group-start bitmap and
+    // cursors; stage every record in tile order and count its unit; resolve
+    // the deferred far-start records.  With graphs enabled this sequence is
+    // captured once per argument set and replayed (fills are kernels, never
+    // memsets).  Staging is sized for the expected record count (about 2
+    // per w+1 windows) and grown to the counted size if that was short.
+    // tile regions hold 2.5 records per w+1 windows (about 2 on random
+    // sequence); the rest of a tile and deferred records go to the overflow
+    const uint32_t tile_cap = ctx->tune.stage_tile_cap ? std::min<uint32_t>(T, ctx->tune.stage_tile_cap)
+                                                       : std::min<uint32_t>(T, (T * 5 + 2 * (w + 1) - 1) / (2 * (w + 1)));
+    A.tile_cap = tile_cap;
+    const uint64_t tile_slots = n_tiles * tile_cap;
+    uint64_t ovf_cap = std::max<uint64_t>(tile_slots / 64 + 65536, ctx->stage_unit.cap / 4 > tile_slots ? ctx->stage_unit.cap / 4 - tile_slots : 0);
+    uint64_t pool_cap = std::max<uint64_t>(ctx->pool.cap / 4, n_bytes / 64 + 65536);
+    const uint32_t small_grid = (uint32_t)std::min<uint64_t>((std::max<uint64_t>(n_tiles, 1) + 255) / 256, 65535);
+    auto pass_1 = [&]() {
+        hipLaunchKernelGGL(fill32_kernel, dim3(256), dim3(256), 0, ctx->stream, reinterpret_cast<uint32_t*>(A.unit_ck),
+                           0u, 2 * (uint64_t)n_units);
+        hipLaunchKernelGGL(fill32_kernel, dim3(256), dim3(256), 0, ctx->stream, A.seg_bits, 0u, (uint64_t)seg_words);
+        hipLaunchKernelGGL(bucket_reset_kernel, dim3(1), dim3(64), 0, ctx->stream, A);
         if (n_tiles) {
             if (regs16)
-                hipLaunchKernelGGL(bucket_count_kernel<16>, dim3((uint32_t)n_tiles), dim3(64), G.bytes, ctx->stream, A);
+                hipLaunchKernelGGL(bucket_stage_kernel<16>, dim3((uint32_t)n_tiles), dim3(64), G.bytes, ctx->stream, A);
             else
-                hipLaunchKernelGGL(bucket_count_kernel<0>, dim3((uint32_t)n_tiles), dim3(64), G.bytes, ctx->stream, A);
+                hipLaunchKernelGGL(bucket_stage_kernel<0>, dim3((uint32_t)n_tiles), dim3(64), G.bytes, ctx->stream, A);
+            hipLaunchKernelGGL(bucket_defer_kernel, dim3(small_grid), dim3(256), 0, ctx->stream, A);
+        }
+        hipLaunchKernelGGL(unit_split_kernel, dim3(std::min<uint32_t>((n_units + 255) / 256, 4096)), dim3(256), 0,
+                           ctx->stream, A.unit_ck, A.sub_count, A.sub_kmers, n_units);
+    };
+    uint64_t staged = 0, pool_used = 0;
+    uint32_t errf = 0;
+    for (int attempt = 0;; attempt++) {
+        const uint64_t stage_cap = tile_slots + ovf_cap;
+        ENSURE(ctx->stage_slots, stage_cap * slot_words * 4);
+        ENSURE(ctx->stage_unit, stage_cap * 4);
+        ENSURE(ctx->stage_rank, stage_cap * 4);
+        if (n_colors) ENSURE(ctx->stage_color, stage_cap * 4);
+        ENSURE(ctx->pool, pool_cap * 4);
+        A.stage_slots = P_<uint32_t>(ctx->stage_slots);
+        A.stage_unit = P_<uint32_t>(ctx->stage_unit);
+        A.stage_rank = P_<uint32_t>(ctx->stage_rank);
+        A.stage_color = n_colors ? P_<uint32_t>(ctx->stage_color) : nullptr;
+        A.stage_cap = stage_cap;
+        A.pool = P_<uint32_t>(ctx->pool);
+        A.pool_cap = pool_cap;
+        HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+        if (ctx->tune.use_graphs) {
+            ggcat_gpu_ctx::GraphKeyA key;
+            memset(&key, 0, sizeof key);
+            key.args = A;
+            key.seg_words = seg_words;
+            key.n_units = n_units;
+            if (!ctx->graph_a || memcmp(&key, &ctx->graph_a_key, sizeof key) != 0) {
+                if (ctx->graph_a) (void)hipGraphExecDestroy(ctx->graph_a);
+                ctx->graph_a = nullptr;
+                hipGraph_t g;
+                HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                pass_1();
+                hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+                if (e != hipSuccess) return fail(ctx, GGCAT_GPU_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+                e = hipGraphInstantiate(&ctx->graph_a, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (e != hipSuccess) {
+                    ctx->graph_a = nullptr;
+                    return fail(ctx, GGCAT_GPU_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+                }
+                ctx->graph_a_key = key;
+            }
+            HIPCHK(hipGraphLaunch(ctx->graph_a, ctx->stream));
+        } else {
+            pass_1();
         }
         HIPCHK(hipGetLastError());
-        return device_scan(ctx, A.sub_count, P_<uint64_t>(ctx->unit_off), n_units);
-    };
-    HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
-    if (ctx->tune.use_graphs) {
-        ggcat_gpu_ctx::GraphKeyA key{d_in, n_bytes, p.k, p.m, p.buckets_log, p.second_buckets_log, xb, T, A.canonical,
-                      ctx->unit_count.p, ctx->seg_bits.p, ctx->unit_off.p, ctx->scan_part.p};
-        if (!ctx->graph_a || memcmp(&key, &ctx->graph_a_key, sizeof key) != 0) {
-            if (ctx->graph_a) (void)hipGraphExecDestroy(ctx->graph_a);
-            ctx->graph_a = nullptr;
-            hipGraph_t g;
-            HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-            const int r = pass_a();
-            hipError_t e = hipStreamEndCapture(ctx->stream, &g);
-            if (r != GGCAT_GPU_OK) return r;
-            if (e != hipSuccess) return fail(ctx, GGCAT_GPU_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
-            e = hipGraphInstantiate(&ctx->graph_a, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (e != hipSuccess) {
-                ctx->graph_a = nullptr;
-                return fail(ctx, GGCAT_GPU_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e));
-            }
-            ctx->graph_a_key = key;
-        }
-        HIPCHK(hipGraphLaunch(ctx->graph_a, ctx->stream));
-    } else {
-        const int r = pass_a();
-        if (r != GGCAT_GPU_OK) return r;
+        HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
+        HIPCHK(hipMemcpyAsync(&staged, ctx->stage_cursor.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(&pool_used, ctx->pool_cursor.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(&errf, ctx->error_flag.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (errf & 8) return fail(ctx, GGCAT_GPU_ENOMEM, "a physical unit exceeds 2^32 k-mer occurrences");
+        if (errf) return fail(ctx, GGCAT_GPU_EHIP, "bucketing: a group start was not found");
+        if (staged <= ovf_cap && pool_used <= pool_cap) break;
+        if (attempt == 1) return fail(ctx, GGCAT_GPU_ENOMEM, "bucketing staging overflow");
+        ovf_cap = std::max(ovf_cap, staged + 1024);
+        pool_cap = std::max(pool_cap, pool_used + 1024);
     }
-    HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
+    ctx->pool_words = pool_used;
+
+    // Unit offsets, then pass 2: every staged record to its unit's slots.
+    const int rs = device_scan(ctx, A.sub_count, P_<uint64_t>(ctx->unit_off), n_units);
+    if (rs != GGCAT_GPU_OK) return rs;
     HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
     uint64_t total = 0;
     HIPCHK(hipMemcpyAsync(&total, P_<uint64_t>(ctx->unit_off) + n_units, 8, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->n_records = total;
-    ENSURE(ctx->slots, total * slot_words * 4);
-    if (n_colors) ENSURE(ctx->slot_color, total * 4);
+    ENSURE(ctx->slots, std::max<uint64_t>(total, 1) * slot_words * 4);
+    if (n_colors) ENSURE(ctx->slot_color, std::max<uint64_t>(total, 1) * 4);
     A.slots = P_<uint32_t>(ctx->slots);
     A.n_slots = total;
     A.slot_color = n_colors ? P_<uint32_t>(ctx->slot_color) : nullptr;
-
-    uint64_t pool_cap = std::max<uint64_t>(ctx->pool.cap / 4, n_bytes / 64 + 65536);
-    for (int attempt = 0; attempt < 2; attempt++) {
-        ENSURE(ctx->pool, pool_cap * 4);
-        A.pool = P_<uint32_t>(ctx->pool);
-        A.pool_cap = pool_cap;
-        hipLaunchKernelGGL(cursor_init_kernel, dim3(std::min<uint32_t>((n_units + 255) / 256, 4096)), dim3(256), 0,
-                           ctx->stream, A.sub_cursor, A.sub_off, n_units);
-        HIPCHK(hipMemsetAsync(ctx->unit_kmers.p, 0, (size_t)n_units * 4, ctx->stream));
-        HIPCHK(hipMemsetAsync(ctx->pool_cursor.p, 0, 8, ctx->stream));
-        HIPCHK(hipMemsetAsync(ctx->error_flag.p, 0, 8, ctx->stream));
-        HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
-        if (n_tiles) {
-            if (regs16)
-                hipLaunchKernelGGL(bucket_emit_kernel<16>, dim3((uint32_t)n_tiles), dim3(64), G.bytes, ctx->stream, A);
-            else
-                hipLaunchKernelGGL(bucket_emit_kernel<0>, dim3((uint32_t)n_tiles), dim3(64), G.bytes, ctx->stream, A);
-            HIPCHK(hipGetLastError());
-        }
-        hipLaunchKernelGGL(cursor_check_kernel, dim3(std::min<uint32_t>((n_units + 255) / 256, 4096)), dim3(256), 0,
-                           ctx->stream, A.sub_cursor, A.sub_off, n_units, A.error_flag);
-        HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
-        uint64_t used = 0;
-        uint32_t errf = 0;
-        HIPCHK(hipMemcpyAsync(&used, ctx->pool_cursor.p, 8, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipMemcpyAsync(&errf, ctx->error_flag.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        if (errf) return fail(ctx, GGCAT_GPU_EHIP, "bucketing passes disagree on unit sizes");
-        ctx->pool_words = used;
-        if (used <= pool_cap) break;
-        if (attempt == 1) return fail(ctx, GGCAT_GPU_ENOMEM, "long-record pool overflow");
-        pool_cap = used + 1024;
+    HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
+    const uint64_t n_staged = tile_slots + staged;
+    if (n_staged) {
+        const uint32_t grid = (uint32_t)std::min<uint64_t>((n_staged + 255) / 256, (uint64_t)ctx->n_cu * 64);
+        hipLaunchKernelGGL(bucket_scatter_kernel, dim3(grid), dim3(256), 0, ctx->stream, A, n_staged);
     }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
+    HIPCHK(hipMemcpyAsync(&errf, ctx->error_flag.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (errf) return fail(ctx, GGCAT_GPU_EHIP, "bucketing: a record rank outside its unit");
     float t;
     HIPCHK(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[1]));
     ctx->ms[0] = t;
@@ -604,8 +636,7 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     ctx->ms[1] = t;
     HIPCHK(hipEventElapsedTime(&t, ctx->ev[3], ctx->ev[4]));
     ctx->ms[2] = t;
-    HIPCHK(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[4]));
-    ctx->ms[5] = t;
+    ctx->ms[5] = ctx->ms[0] + ctx->ms[1] + ctx->ms[2];
     // k-mer occurrence total (sizes the merge output)
     std::vector<uint32_t> occ(n_units);
     HIPCHK(hipMemcpy(occ.data(), ctx->unit_kmers.p, (size_t)n_units * 4, hipMemcpyDeviceToHost));

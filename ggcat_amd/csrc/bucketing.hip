@@ -17,11 +17,13 @@
 // before t0 (halo), enough to find the start of every group whose minimizer is
 // unique (such groups span at most w windows).  Groups with a non-unique
 // minimizer may be longer; their start is found in the global group-start
-// bitmap written by pass A.
+// bitmap once pass 1 has written all of it.
 //
-//   pass A (bucket_count_kernel): group-start bitmap + records per sub-bucket
-//   scan   (scan_kernel)        : sub-bucket offsets
-//   pass B (bucket_emit_kernel) : recompute, write one slot per super-k-mer
+//   pass 1 (bucket_stage_kernel)  : group-start bitmap; every record built and
+//                                   staged in tile order, counted per unit
+//          (bucket_defer_kernel)  : records whose group starts before the halo
+//   scan   (scan_*_kernel)        : unit offsets
+//   pass 2 (bucket_scatter_kernel): each staged record to its unit's next slot
 //
 // A group = maximal run of consecutive windows sharing the minimizer (value,
 // and position when unique).  Its super-k-mer is the set of k-mers touching
@@ -44,10 +46,23 @@ struct BucketArgs {
     uint32_t T;  // windows per tile (multiple of 64)
     uint64_t n_tiles;
     uint32_t* seg_bits;  // [ceil(n/32)+2] group-start bitmap (pass A writes)
-    uint32_t* sub_count;  // [n_units] records per unit (pass A)
+    uint32_t* sub_count;  // [n_units] records per unit
     // pass B
-    unsigned long long* sub_cursor;  // [n_units] next slot of each unit (starts at its offset)
+    unsigned long long* unit_ck;  // [n_units] records << 32 | k-mer occurrences, counted in pass 1
     uint64_t n_slots;  // slots in the arena
+    // staging: records in tile order before the scatter by unit
+    // Staging: tile i owns positions [i * tile_cap, (i + 1) * tile_cap);
+    // records beyond that and deferred records go to the overflow region
+    // that follows, [n_tiles * tile_cap, stage_cap).
+    uint32_t* stage_slots;  // [stage_cap * slot_words]
+    uint32_t* stage_unit;  // [stage_cap] unit, or NO_UNIT for a hole
+    uint32_t* stage_rank;  // [stage_cap] rank of the record inside its unit
+    uint32_t* stage_color;  // [stage_cap] (colored builds)
+    unsigned long long* stage_cursor;  // positions used in the overflow region
+    uint64_t stage_cap;
+    uint32_t tile_cap;
+    uint64_t* defer;  // [n_tiles * 3] records whose group starts before the tile: end window, minimizer, last flag
+    unsigned int* defer_count;
     const uint64_t* sub_off;  // [n_units+1] slot offsets
     uint32_t* slots;
     uint32_t slot_words;
@@ -506,43 +521,6 @@ GG_DEV uint32_t lane_end_mask16(const Tile& t, uint32_t* x0, int64_t* j0, uint32
     return okm & (lastm | (sg >> 1) | (next << 15)) & vm;
 }
 
-template <int WC>
-__global__ __launch_bounds__(64) void bucket_count_kernel(BucketArgs A) {
-    extern __shared__ __align__(16) uint8_t lds[];
-    Tile t;
-    tile_init(t, &A, blockIdx.x, lds, WC != 0);
-    tile_compute<WC>(t);
-    const uint32_t lane = threadIdx.x;
-    const uint32_t w = A.w;
-    if (WC == 16) {
-        uint32_t x0, segm;
-        int64_t j0;
-        uint32_t endm = lane_end_mask16(t, &x0, &j0, &segm);
-        if (lane >= 1 && lane <= A.T / 16) reinterpret_cast<uint16_t*>(A.seg_bits)[j0 >> 4] = (uint16_t)segm;
-        while (endm) {
-            const uint32_t i = __ffs(endm) - 1;
-            endm &= endm - 1;
-            atomicAdd(&A.sub_count[unit_of(t.wv[x0 + i], A.b, A.sb, A.xb)], 1u);
-        }
-        return;
-    }
-    for (uint32_t c = 0; c < A.T; c += 64) {
-        uint32_t x = w + c + lane;
-        int64_t j = t.t0 + c + lane;
-        bool s = t.bit(t.seg, x) && j < (int64_t)A.n;
-        uint64_t bs = __ballot(s);
-        if (lane == 0) {
-            uint64_t wi = (uint64_t)(t.t0 + c) >> 5;
-            A.seg_bits[wi] = (uint32_t)bs;
-            A.seg_bits[wi + 1] = (uint32_t)(bs >> 32);
-        }
-        if (j < (int64_t)A.n && tile_is_end(t, x)) {
-            uint32_t unit = unit_of(t.wv[x], A.b, A.sb, A.xb);
-            atomicAdd(&A.sub_count[unit], 1u);
-        }
-    }
-}
-
 // Code of the base at absolute position p (LDS when inside the tile, else HBM).
 GG_DEV uint32_t code_abs(const Tile& t, int64_t p) {
     int64_t r = p - t.borg;
@@ -556,26 +534,196 @@ GG_DEV bool invalid_abs(const Tile& t, int64_t p) {
     return !is_acgt(t.A->seq[p]);
 }
 
+constexpr uint32_t NO_UNIT = 0xFFFFFFFFu;
+
+// Color of the record starting at byte `start`: the last color range that
+// starts at or before it.
+GG_DEV uint32_t color_of(const BucketArgs& A, int64_t start) {
+    uint32_t lo = 0, hi = A.n_colors;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (A.color_starts[mid] <= (uint64_t)start) lo = mid; else hi = mid;
+    }
+    return A.colors[lo];
+}
+
+// Packs the bases [start, end) (reverse complement when rc) into dst words,
+// reading the tile's LDS codes where they cover the bases and the input
+// otherwise.
+GG_DEV void pack_bases(const Tile& t, int64_t start, int64_t end, bool rc, uint32_t* dst) {
+    const uint32_t len = (uint32_t)(end - start);
+    uint32_t acc = 0;
+    for (uint32_t i = 0; i < len; i++) {
+        const uint32_t cd = rc ? (code_abs(t, end - 1 - i) ^ 2u) : code_abs(t, start + i);
+        acc |= cd << (2 * (i & 15));
+        if ((i & 15) == 15 || i + 1 == len) {
+            dst[i >> 4] = acc;
+            acc = 0;
+        }
+    }
+}
+
+// Long record: bases to the pool, the slot holds length and pool offset.
+// Returns false when the pool is full (the host retries with a larger one).
+GG_DEV bool write_long(const BucketArgs& A, const Tile& t, uint32_t* out, int64_t start, int64_t end, bool rc,
+                       uint32_t flags, uint32_t minpos) {
+    const uint32_t len = (uint32_t)(end - start);
+    const uint64_t words = (len + 15) / 16;
+    const unsigned long long off = atomicAdd(A.pool_cursor, (unsigned long long)words);
+    out[0] = (flags << 8) | (minpos << 10) | META_LONG | (rc ? META_RC : 0u);
+    out[1] = len;
+    out[2] = (uint32_t)off;
+    out[3] = (uint32_t)(off >> 32);
+    for (uint32_t q = 4; q < A.slot_words; q++) out[q] = 0;
+    if (off + words > A.pool_cap) return false;
+    pack_bases(t, start, end, rc, A.pool + off);
+    return true;
+}
+
+// Counts a record in its unit and returns its rank there (records << 32 |
+// k-mer occurrences in one 64-bit atomic); flags a unit whose k-mer count
+// would carry into the record count.
+GG_DEV void count_record(const BucketArgs& A, uint32_t unit, uint32_t nk, unsigned long long* old) {
+    *old = atomicAdd(&A.unit_ck[unit], (1ULL << 32) | nk);
+}
+GG_DEV uint32_t rank_of(const BucketArgs& A, unsigned long long old, uint32_t nk) {
+    if ((uint32_t)old + nk < (uint32_t)old) atomicOr(A.error_flag, 8u);
+    return (uint32_t)(old >> 32);
+}
+
+// One record of a tile: everything the slot needs.
+struct RecInfo {
+    int64_t start, end;
+    uint32_t unit, meta, nk;  // meta: flags << 8 | minpos << 10 | rc | cross marks (length added on write)
+    bool rc, ok;
+};
+
+GG_DEV RecInfo rec_info(const Tile& t, uint32_t x) {
+    const BucketArgs& A = *t.A;
+    const uint32_t w = A.w, k = A.k, m = A.m;
+    RecInfo r;
+    r.ok = false;
+    const int64_t bwin = t.t0 + (int64_t)x - w;
+    // group start: latest group-start bit at or before x inside the tile
+    int64_t a = -1;
+    {
+        int32_t wi = (int32_t)(x >> 5);
+        uint32_t msk = t.seg[wi] & ((2u << (x & 31)) - 1u);
+        while (true) {
+            if (msk) {
+                a = t.wlo + (int64_t)wi * 32 + (31 - __clz(msk));
+                break;
+            }
+            if (--wi < 0) break;
+            msk = t.seg[wi];
+        }
+    }
+    if (a < 0) return r;  // group starts before the halo: deferred
+    const uint64_t v = t.wv[x];
+    const bool unique = v & 1;
+    const bool include_first = invalid_abs(t, a - 1);
+    const bool include_last = t.bit(t.last, x);
+    r.start = include_first ? a : a - 1;
+    r.end = include_last ? bwin + k - 1 : bwin + k;
+    const uint32_t prel = t.wp[x];
+    const int64_t mmabs = t.wlo + prel;
+    r.rc = A.canonical && unique && !t.fwd[prel];
+    uint32_t minpos = 0;
+    if (unique) minpos = r.rc ? (uint32_t)(r.end - mmabs - m) : (uint32_t)(mmabs - r.start);
+    const uint32_t flags = ((include_first ? 1u : 0u) << (r.rc ? 1 : 0)) | ((include_last ? 1u : 0u) << (r.rc ? 0 : 1));
+    const uint32_t sub = sub_of(v, A.b, A.sb);
+    r.unit = unit_of(v, A.b, A.sb, A.xb);
+    // Boundary k-mers shared with a neighbouring group of the same
+    // sub-bucket that was scattered to another unit: the merge must combine
+    // the two halves (cross marks, in stored orientation).  A unique group
+    // spans at most w windows, so window a-1 is inside the halo.
+    uint32_t cross = 0;
+    if (unique && A.xb) {
+        if (!include_first) {
+            const uint64_t vl = t.wv[a - t.wlo - 1];
+            if (sub_of(vl, A.b, A.sb) == sub && unit_of(vl, A.b, A.sb, A.xb) != r.unit) cross |= r.rc ? META_XEND : META_XBEGIN;
+        }
+        if (!include_last) {
+            const uint64_t vr = t.wv[x + 1];
+            if (sub_of(vr, A.b, A.sb) == sub && unit_of(vr, A.b, A.sb, A.xb) != r.unit) cross |= r.rc ? META_XBEGIN : META_XEND;
+        }
+    }
+    r.meta = (flags << 8) | (minpos << 10) | (r.rc ? META_RC : 0u) | cross;
+    r.nk = (uint32_t)(r.end - r.start) - k + 1;
+    r.ok = true;
+    return r;
+}
+
+// Writes the slot of record r to out (16-byte aligned, slot_words words).
+// Returns false when a long record did not fit the pool.
+GG_DEV bool write_record(const Tile& t, const RecInfo& r, uint32_t* out) {
+    const BucketArgs& A = *t.A;
+    const uint32_t SW = A.slot_words;
+    const uint32_t inline_cap = (SW - 1) * 16;
+    const uint32_t len = (uint32_t)(r.end - r.start);
+    const int64_t rs = r.start - t.borg;
+    if (len <= inline_cap && rs >= 0) {
+        // 16-byte stores (slots are a multiple of 4 words)
+        const uint32_t nw = (len + 15) / 16;
+        uint4* o4 = reinterpret_cast<uint4*>(out);
+        for (uint32_t g = 0; g < SW / 4; g++) {
+            uint32_t w4[4];
+#pragma unroll
+            for (uint32_t j = 0; j < 4; j++) {
+                const uint32_t q = 4 * g + j;
+                uint32_t wq = 0;
+                if (q == 0) {
+                    wq = len | r.meta;
+                } else if (q - 1 < nw) {
+                    const uint32_t qi = q - 1;
+                    if (!r.rc) {
+                        wq = extract16(t, rs + 16 * qi);
+                    } else {
+                        wq = rev_comp16(extract16(t, (r.end - t.borg) - 16 * (int64_t)qi - 16));
+                    }
+                    const uint32_t rem = len - 16 * qi;
+                    if (rem < 16) wq &= (1u << (2 * rem)) - 1u;
+                }
+                w4[j] = wq;
+            }
+            o4[g] = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+        }
+        return true;
+    }
+    if (len > inline_cap) return write_long(A, t, out, r.start, r.end, r.rc, (r.meta >> 8) & 3u, (r.meta >> 10) & 0xFFu);
+    // bases start before the tile's loaded range
+    out[0] = len | r.meta;
+    for (uint32_t q = 1; q < SW; q++) out[q] = 0;
+    pack_bases(t, r.start, r.end, r.rc, out + 1);
+    return true;
+}
+
+// Records staged per lane and pass of the record loop; their unit atomics
+// are all in flight before the records are written.
+constexpr uint32_t STAGE_R = 2;
+
+// Pass 1 (one wavefront per tile): window minima and group ends; every record
+// whose group starts inside the tile or its halo is counted in its unit
+// (which gives its rank there), built and staged at the tile's positions; the
+// first record of a tile whose group starts further back is deferred to
+// bucket_defer_kernel, which runs once the group-start bitmap is complete.
 template <int WC>
-__global__ __launch_bounds__(64) void bucket_emit_kernel(BucketArgs A) {
+__global__ __launch_bounds__(64) void bucket_stage_kernel(BucketArgs A) {
     extern __shared__ __align__(16) uint8_t lds[];
     Tile t;
     tile_init(t, &A, blockIdx.x, lds, WC != 0);
     tile_compute<WC>(t);
     const uint32_t lane = threadIdx.x;
-    const uint32_t w = A.w, k = A.k, m = A.m;
+    const uint32_t w = A.w;
     const uint32_t SW = A.slot_words;
-    const uint32_t inline_cap = (SW - 1) * 16;
-    // Compact the tile's group ends first, then emit them with every lane
-    // busy: the dependent slot atomics of a tile overlap instead of running
-    // one 64-window stripe at a time.
     __shared__ uint16_t ends[1024];
     __shared__ uint32_t n_ends;
+    __shared__ unsigned long long s_ovf;
     if (WC == 16) {
         uint32_t x0, segm;
         int64_t j0;
         uint32_t endm = lane_end_mask16(t, &x0, &j0, &segm);
-        // exclusive prefix of the lanes' end counts
+        if (lane >= 1 && lane <= A.T / 16) reinterpret_cast<uint16_t*>(A.seg_bits)[j0 >> 4] = (uint16_t)segm;
         const uint32_t cnt = __popc(endm);
         uint32_t inc = cnt;
         for (uint32_t d = 1; d < 64; d <<= 1) {
@@ -595,7 +743,15 @@ __global__ __launch_bounds__(64) void bucket_emit_kernel(BucketArgs A) {
         __syncthreads();
         for (uint32_t c = 0; c < A.T; c += 64) {
             const uint32_t x = w + c + lane;
-            const bool e = t.t0 + c + lane < (int64_t)A.n && tile_is_end(t, x);
+            const int64_t j = t.t0 + c + lane;
+            const bool sg = t.bit(t.seg, x) && j < (int64_t)A.n;
+            const uint64_t bs = __ballot(sg);
+            if (lane == 0) {
+                const uint64_t wi = (uint64_t)(t.t0 + c) >> 5;
+                A.seg_bits[wi] = (uint32_t)bs;
+                A.seg_bits[wi + 1] = (uint32_t)(bs >> 32);
+            }
+            const bool e = j < (int64_t)A.n && tile_is_end(t, x);
             const uint64_t be = __ballot(e);
             if (e) ends[n_ends + __popcll(be & ((1ULL << lane) - 1))] = (uint16_t)x;
             __syncthreads();
@@ -604,146 +760,150 @@ __global__ __launch_bounds__(64) void bucket_emit_kernel(BucketArgs A) {
         }
     }
     const uint32_t n_end = n_ends;
-    for (uint32_t ei = lane; ei < n_end; ei += 64) {
-        const uint32_t x = ends[ei];
-        const int64_t bwin = t.t0 + (int64_t)x - w;
-        // ---- group start: latest group-start bit at or before x ----
-        int64_t a = -1;
-        {
-            int32_t wi = (int32_t)(x >> 5);
-            uint32_t msk = t.seg[wi] & ((2u << (x & 31)) - 1u);
-            while (true) {
-                if (msk) {
-                    a = t.wlo + (int64_t)wi * 32 + (31 - __clz(msk));
-                    break;
-                }
-                if (--wi < 0) break;
-                msk = t.seg[wi];
-            }
-            if (a < 0 && t.wlo >= 0) {
-                // far start: scan the global bitmap below the tile's halo
-                int64_t g = t.wlo;
-                int64_t gw = g >> 5;
-                uint32_t gm = A.seg_bits[gw] & ((2u << (g & 31)) - 1u);
-                while (gm == 0 && gw > 0) gm = A.seg_bits[--gw];
-                if (gm) a = gw * 32 + (31 - __clz(gm));
-            }
-            if (a < 0) continue;  // unreachable for well-formed input (every run starts a group)
-        }
-        const uint64_t v = t.wv[x];
-        const bool unique = v & 1;
-        const bool include_first = invalid_abs(t, a - 1);
-        const bool include_last = t.bit(t.last, x);
-        const int64_t start = include_first ? a : a - 1;
-        const int64_t end = include_last ? bwin + k - 1 : bwin + k;
-        const uint32_t len = (uint32_t)(end - start);
-        const uint32_t prel = t.wp[x];
-        const int64_t mmabs = t.wlo + prel;
-        const bool rc = A.canonical && unique && !t.fwd[prel];
-        uint32_t minpos = 0;
-        if (unique) minpos = rc ? (uint32_t)(end - mmabs - m) : (uint32_t)(mmabs - start);
-        const uint32_t flags = ((include_first ? 1u : 0u) << (rc ? 1 : 0)) | ((include_last ? 1u : 0u) << (rc ? 0 : 1));
-        const uint32_t sub = sub_of(v, A.b, A.sb);
-        const uint32_t unit = unit_of(v, A.b, A.sb, A.xb);
-        // Boundary k-mers shared with a neighbouring group of the same
-        // sub-bucket that was scattered to another unit: the merge must combine
-        // the two halves (cross marks, in stored orientation).
-        uint32_t cross = 0;
-        if (unique && A.xb) {
-            if (!include_first) {
-                uint64_t vl = t.wv[a - t.wlo - 1];
-                if (sub_of(vl, A.b, A.sb) == sub && unit_of(vl, A.b, A.sb, A.xb) != unit) cross |= rc ? META_XEND : META_XBEGIN;
-            }
-            if (!include_last) {
-                uint64_t vr = t.wv[x + 1];
-                if (sub_of(vr, A.b, A.sb) == sub && unit_of(vr, A.b, A.sb, A.xb) != unit) cross |= rc ? META_XBEGIN : META_XEND;
-            }
-        }
-        const uint64_t slot = atomicAdd(&A.sub_cursor[unit], 1ULL);
-        if (slot >= A.n_slots) {
-            // pass A and pass B disagree: never write past the arena (the
-            // per-unit check after the pass reports it)
-            atomicOr(A.error_flag, 1u);
-            continue;
-        }
-        atomicAdd(&A.sub_kmers[unit], len - k + 1);
-        uint32_t* out = A.slots + slot * SW;
-        const int64_t rs = start - t.borg;
-        if (len <= inline_cap && rs >= 0) {
-            // the slot is written with 16-byte stores (slots are 16-byte
-            // aligned and a multiple of 4 words), one transaction per 16 bytes
-            const uint32_t nw = (len + 15) / 16;
-            uint4* o4 = reinterpret_cast<uint4*>(out);
-            for (uint32_t g = 0; g < SW / 4; g++) {
-                uint32_t w4[4];
+    const uint32_t cap = A.tile_cap;
+    const uint64_t tbase = (uint64_t)blockIdx.x * cap;
+    if (lane == 0) {
+        s_ovf = 0;
+        if (n_end > cap) s_ovf = (uint64_t)A.n_tiles * cap + atomicAdd(A.stage_cursor, (unsigned long long)(n_end - cap));
+    }
+    __syncthreads();
+    const uint64_t ovf = s_ovf;
+    for (uint32_t e = n_end + lane; e < cap; e += 64) A.stage_unit[tbase + e] = NO_UNIT;
+    for (uint32_t e0 = 0; e0 < n_end; e0 += 64 * STAGE_R) {
+        RecInfo ri[STAGE_R];
+        uint64_t pos[STAGE_R];
+        unsigned long long old[STAGE_R];
 #pragma unroll
-                for (uint32_t j = 0; j < 4; j++) {
-                    const uint32_t q = 4 * g + j;
-                    uint32_t wq = 0;
-                    if (q == 0) {
-                        wq = len | (flags << 8) | (minpos << 10) | (rc ? META_RC : 0u) | cross;
-                    } else if (q - 1 < nw) {
-                        const uint32_t qi = q - 1;
-                        if (!rc) {
-                            wq = extract16(t, rs + 16 * qi);
-                        } else {
-                            wq = rev_comp16(extract16(t, (end - t.borg) - 16 * (int64_t)qi - 16));
-                        }
-                        const uint32_t rem = len - 16 * qi;
-                        if (rem < 16) wq &= (1u << (2 * rem)) - 1u;
-                    }
-                    w4[j] = wq;
-                }
-                o4[g] = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+        for (uint32_t r = 0; r < STAGE_R; r++) {
+            const uint32_t ei = e0 + 64 * r + lane;
+            ri[r].ok = false;
+            pos[r] = ~0ULL;
+            old[r] = 0;
+            if (ei >= n_end) continue;
+            pos[r] = ei < cap ? tbase + ei : ovf + (ei - cap);
+            if (pos[r] >= A.stage_cap) {  // overflow region full: the host retries with the counted size
+                pos[r] = ~0ULL;
+                continue;
             }
-        } else {
-            // generic path: bases possibly outside the tile, or a long record
-            const bool is_long = len > inline_cap;
-            uint32_t* dst;
-            if (is_long) {
-                uint64_t words = (len + 15) / 16;
-                unsigned long long off = atomicAdd(A.pool_cursor, (unsigned long long)words);
-                out[0] = (flags << 8) | (minpos << 10) | META_LONG | (rc ? META_RC : 0u);
-                out[1] = len;
-                out[2] = (uint32_t)off;
-                out[3] = (uint32_t)(off >> 32);
-                for (uint32_t q = 4; q < SW; q++) out[q] = 0;
-                if (off + words > A.pool_cap) continue;  // overflow: host retries with a larger pool
-                dst = A.pool + off;
+            const uint32_t x = ends[ei];
+            ri[r] = rec_info(t, x);
+            if (ri[r].ok) {
+                count_record(A, ri[r].unit, ri[r].nk, &old[r]);
             } else {
-                out[0] = len | (flags << 8) | (minpos << 10) | (rc ? META_RC : 0u) | cross;
-                dst = out + 1;
-                for (uint32_t q = 1; q < SW; q++) out[q] = 0;
-            }
-            uint32_t acc = 0;
-            for (uint32_t i = 0; i < len; i++) {
-                uint32_t cd = rc ? (code_abs(t, end - 1 - i) ^ 2u) : code_abs(t, start + i);
-                acc |= cd << (2 * (i & 15));
-                if ((i & 15) == 15 || i + 1 == len) {
-                    dst[i >> 4] = acc;
-                    acc = 0;
+                // group starts before the halo (non-unique minimizer): defer
+                const uint32_t d = atomicAdd(A.defer_count, 1u);
+                if (d < A.n_tiles) {
+                    A.defer[3 * (uint64_t)d] = (uint64_t)(t.t0 + (int64_t)x - w);
+                    A.defer[3 * (uint64_t)d + 1] = t.wv[x];
+                    A.defer[3 * (uint64_t)d + 2] = t.bit(t.last, x) ? 1 : 0;
+                } else {
+                    atomicOr(A.error_flag, 4u);
                 }
             }
         }
-        if (A.n_colors) {
-            uint32_t lo = 0, hi = A.n_colors;  // last color range starting <= start
-            while (hi - lo > 1) {
-                uint32_t mid = (lo + hi) >> 1;
-                if (A.color_starts[mid] <= (uint64_t)start) lo = mid; else hi = mid;
+#pragma unroll
+        for (uint32_t r = 0; r < STAGE_R; r++) {
+            if (pos[r] == ~0ULL) continue;
+            if (!ri[r].ok) {
+                A.stage_unit[pos[r]] = NO_UNIT;
+                continue;
             }
-            A.slot_color[slot] = A.colors[lo];
+            if (!write_record(t, ri[r], A.stage_slots + pos[r] * SW)) continue;  // pool full: retried
+            A.stage_unit[pos[r]] = ri[r].unit;
+            if (A.n_colors) A.stage_color[pos[r]] = color_of(A, ri[r].start);
         }
+#pragma unroll
+        for (uint32_t r = 0; r < STAGE_R; r++)
+            if (pos[r] != ~0ULL && ri[r].ok) A.stage_rank[pos[r]] = rank_of(A, old[r], ri[r].nk);
     }
 }
 
-// Slot cursors start at each unit's offset; after pass B every cursor must
-// have reached the next unit's offset (passes A and B agree on every unit).
-__global__ void cursor_init_kernel(unsigned long long* cursor, const uint64_t* off, uint32_t n) {
-    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) cursor[u] = off[u];
+// Deferred records (group start before the tile's halo; only groups with a
+// non-unique minimizer are that long): the group-start bitmap is complete
+// now, so the start is the last group start at or before the end window.
+// One thread per record; bases are read from the input; staged in the
+// overflow region.
+__global__ void bucket_defer_kernel(BucketArgs A) {
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    const uint64_t n_defer = *A.defer_count;
+    if (i >= n_defer || i >= A.n_tiles) return;
+    const int64_t bwin = (int64_t)A.defer[3 * i];
+    const uint64_t v = A.defer[3 * i + 1];
+    const bool include_last = A.defer[3 * i + 2] != 0;
+    const uint32_t k = A.k, SW = A.slot_words, inline_cap = (SW - 1) * 16;
+    int64_t gw = bwin >> 5;
+    uint32_t gm = A.seg_bits[gw] & ((2u << (bwin & 31)) - 1u);
+    while (gm == 0 && gw > 0) gm = A.seg_bits[--gw];
+    if (!gm) {
+        atomicOr(A.error_flag, 4u);
+        return;
+    }
+    const int64_t a = gw * 32 + (31 - __clz(gm));
+    const bool include_first = (a == 0) || !is_acgt(A.seq[a - 1]);
+    const int64_t start = include_first ? a : a - 1;
+    const int64_t end = include_last ? bwin + k - 1 : bwin + k;
+    const uint32_t len = (uint32_t)(end - start);
+    // a non-unique minimizer: forward orientation, no minimizer position
+    const uint32_t flags = (include_first ? 1u : 0u) | ((include_last ? 1u : 0u) << 1);
+    const uint32_t unit = unit_of(v, A.b, A.sb, A.xb);
+    const uint64_t pos = (uint64_t)A.n_tiles * A.tile_cap + atomicAdd(A.stage_cursor, 1ULL);
+    if (pos >= A.stage_cap) return;  // the host retries with the counted size
+    unsigned long long old;
+    count_record(A, unit, len - k + 1, &old);
+    uint32_t* out = A.stage_slots + pos * SW;
+    Tile none;  // no LDS codes: every base comes from the input
+    none.A = &A;
+    none.borg = 0;
+    none.G.NB = 0;
+    bool ok = true;
+    if (len > inline_cap) {
+        ok = write_long(A, none, out, start, end, false, flags, 0);
+    } else {
+        out[0] = len | (flags << 8);
+        for (uint32_t q = 1; q < SW; q++) out[q] = 0;
+        pack_bases(none, start, end, false, out + 1);
+    }
+    A.stage_unit[pos] = ok ? unit : NO_UNIT;
+    if (A.n_colors) A.stage_color[pos] = color_of(A, start);
+    A.stage_rank[pos] = rank_of(A, old, len - k + 1);
 }
-__global__ void cursor_check_kernel(const unsigned long long* cursor, const uint64_t* off, uint32_t n, uint32_t* err) {
-    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x)
-        if (cursor[u] != off[u + 1]) atomicOr(err, 1u);
+
+// Splits the packed per-unit counts into records and k-mer occurrences.
+__global__ void unit_split_kernel(const unsigned long long* ck, uint32_t* count, uint32_t* kmers, uint32_t n) {
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+        const unsigned long long c = ck[u];
+        count[u] = (uint32_t)(c >> 32);
+        kmers[u] = (uint32_t)c;
+    }
+}
+
+// Clears the cursors of pass 1 (a kernel, so the pass can be captured in a graph).
+__global__ void bucket_reset_kernel(BucketArgs A) {
+    if (threadIdx.x == 0) {
+        *A.stage_cursor = 0;
+        *A.pool_cursor = 0;
+        *A.defer_count = 0;
+        *A.error_flag = 0;
+    }
+}
+
+// Pass 2: every staged record to slot unit offset + rank (no atomics).
+__global__ void bucket_scatter_kernel(BucketArgs A, uint64_t n_staged) {
+    const uint32_t SW = A.slot_words;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n_staged;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t unit = A.stage_unit[i];
+        if (unit == NO_UNIT) continue;
+        const uint64_t slot = A.sub_off[unit] + A.stage_rank[i];
+        if (slot >= A.sub_off[unit + 1]) {
+            atomicOr(A.error_flag, 1u);
+            continue;
+        }
+        const uint4* src = reinterpret_cast<const uint4*>(A.stage_slots + i * SW);
+        uint4* dst = reinterpret_cast<uint4*>(A.slots + slot * SW);
+        for (uint32_t g = 0; g < SW / 4; g++) dst[g] = src[g];
+        if (A.n_colors) A.slot_color[slot] = A.stage_color[i];
+    }
 }
 
 // Device-wide exclusive scan of uint32 counts into uint64 offsets (n+1

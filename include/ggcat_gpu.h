@@ -146,8 +146,12 @@ int ggcat_gpu_solid_colors(ggcat_gpu_ctx* ctx, uint32_t* color_words, uint32_t* 
  *                     records could be counted once (seq-hash, k <= 32)
  *   unit_bytes        bucketing: input bytes per physical unit (sub-bucket
  *                     refinement); default 2000
- *   use_graphs        1: capture the bucketing count/scan sequence in a HIP
- *                     graph per input shape and replay it on later calls */
+ *   use_graphs        1: capture the bucketing staging pass (clears, stage
+ *                     and deferred-record kernels) in a HIP graph per
+ *                     argument set and replay it on later calls
+ *   stage_tile_cap    bucketing: staging positions per tile before records
+ *                     spill to the shared overflow region (default 2.5
+ *                     records per w+1 windows); the result never depends on it */
 typedef struct {
     uint32_t item_occ;
     uint32_t lds_max_occ;
@@ -158,11 +162,12 @@ typedef struct {
     uint32_t no_dedup;
     uint32_t unit_bytes;
     uint32_t use_graphs;
+    uint32_t stage_tile_cap;
 } ggcat_gpu_tuning;
 int ggcat_gpu_set_tuning(ggcat_gpu_ctx* ctx, const ggcat_gpu_tuning* tuning);
 
 /* Device-side timings of the last calls, in milliseconds:
- * [0] bucketing pass A, [1] scan, [2] bucketing pass B,
+ * [0] bucketing staging pass, [1] scan, [2] bucketing scatter pass,
  * [3] merge (whole-item pass), [4] merge (partitioned passes),
  * [5] whole bucketing call, [6] whole merge call, [7] unitig kernels. */
 int ggcat_gpu_timings(ggcat_gpu_ctx* ctx, double* ms, uint32_t n);

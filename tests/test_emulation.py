@@ -77,3 +77,21 @@ def test_colored_merge_emulated(emu_ctx):
         assert np.array_equal(got, want)
     finally:
         emu_ctx.set_tuning()
+
+
+@pytest.mark.parametrize("tile_cap", [1, 7])
+def test_staging_overflow_emulated(emu_ctx, tile_cap):
+    # nearly every tile spills records to the shared overflow region
+    rng = np.random.default_rng(57)
+    data = np.concatenate([joined(adversarial_records(rng)[:12]), synth.reads(synth.genome(3, 4000), 8, 60, 120, 0.01)])
+    emu_ctx.set_tuning(merge_wgs_per_cu=2, stage_tile_cap=tile_cap)
+    try:
+        _check(emu_ctx, data, 31, 15, 4, 2, 1)
+        seps = np.flatnonzero(data == 10) + 1  # color ranges start at record boundaries
+        starts = np.array([0, seps[seps.size // 3], seps[seps.size // 2]], np.uint64)
+        colors = np.array([2, 0, 1], np.uint32)
+        emu_ctx.bucketing(data, Params(31, 15, 4, 2, 0, HASH_SEQ, 1), colors=(starts, colors))
+        o = O.bucketing_colored(data, 31, 15, 4, 2, starts, colors, threads=2)
+        assert record_tuples(emu_ctx.records()) == record_tuples(o.records())
+    finally:
+        emu_ctx.set_tuning()

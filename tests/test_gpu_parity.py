@@ -153,6 +153,17 @@ def test_schedule_knobs_do_not_change_results(gpu_ctx, tune):
         gpu_ctx.set_tuning()
 
 
+@pytest.mark.parametrize("tile_cap", [1, 24])
+def test_staging_overflow_region(gpu_ctx, tile_cap):
+    wl = synth.config1(0.01)
+    _say(f"case stage_tile_cap={tile_cap} bytes={wl.data.size}")
+    gpu_ctx.set_tuning(stage_tile_cap=tile_cap)
+    try:
+        _run_both(gpu_ctx, wl.data, 31, 15, wl.buckets_log, wl.second_buckets_log, exact=False)
+    finally:
+        gpu_ctx.set_tuning()
+
+
 def test_build_from_files_fastq(gpu_ctx, tmp_path):
     import gzip
 
