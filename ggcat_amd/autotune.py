@@ -21,7 +21,7 @@ import numpy as np
 from .native import Params
 
 KNOBS = ("item_occ", "lds_max_occ", "merge_wgs_per_cu", "part_occ", "lds_table_log", "table_pct", "no_dedup",
-         "unit_bytes", "use_graphs", "stage_tile_cap")
+         "unit_bytes", "use_graphs", "stage_tile_cap", "rec_cap")
 
 
 @dataclass

@@ -730,8 +730,9 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     const uint32_t item_occ = ctx->tune.item_occ ? ctx->tune.item_occ : M.lds_cap / 2;
     // identical records counted once (seq-hash keys up to k = 32, inline slots)
     M.dedup = (p.hash_kind == GGCAT_GPU_HASH_SEQ && p.k <= 32 && ctx->slot_words <= 8 && !ctx->tune.no_dedup) ? 1 : 0;
+    M.rec_cap = ctx->tune.rec_cap ? ctx->tune.rec_cap : REC_CAP_DEFAULT;
     M.rec_lds_off = (uint32_t)(((size_t)M.lds_cap * entry_bytes + 64 + 15) / 16 * 16);
-    const size_t lds_bytes = M.dedup ? M.rec_lds_off + rec_table_bytes(ctx->slot_words) : (size_t)M.lds_cap * entry_bytes + 64;
+    const size_t lds_bytes = M.dedup ? M.rec_lds_off + rec_table_bytes(ctx->slot_words, p.k, M.rec_cap) : (size_t)M.lds_cap * entry_bytes + 64;
     M.pool_words = ctx->pool_words;
     // rabin-karp128 constants (crates/hashes/src/cn_rkhash.rs:65-78)
     M.rk_mult = mk128(0x3eb9402f3e733993ULL, 0xadd64d3ca00e1b6bULL);
@@ -767,11 +768,15 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     HIPCHK(hipStreamSynchronize(ctx->stream));
     M.items = P_<WorkItem>(ctx->items);
 
-    // resident workgroups per CU: what 150 KiB of the 160 KiB LDS holds, at
-    // most 8 (32 waves of 64)
-    const uint32_t fit = (uint32_t)std::max<size_t>(1, std::min<size_t>(8, (150 * 1024) / lds_bytes));
+    // resident workgroups per CU: what the 160 KiB LDS holds with the
+    // kernel's static LDS, at most 8 (32 waves of 64)
+    hipFuncAttributes fa{};
+    HIPCHK(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(merge_lds_kernel)));
+    const size_t wg_lds = (lds_bytes + fa.sharedSizeBytes + 511) / 512 * 512;  // allocation granule
+    const uint32_t fit = (uint32_t)std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / wg_lds));
     const uint32_t lds_wgs = ctx->n_cu * (ctx->tune.merge_wgs_per_cu ? ctx->tune.merge_wgs_per_cu : fit);
-    uint64_t n_chunks = std::max<uint64_t>((ctx->kmer_occ / 6) / OUT_CHUNK + 2 * lds_wgs + 4096 + 64,
+    // every wavefront keeps one partly filled chunk open
+    uint64_t n_chunks = std::max<uint64_t>((ctx->kmer_occ / 6) / OUT_CHUNK + 8 * lds_wgs + 4096 + 64,
                                            ctx->out_val.cap / 4 / OUT_CHUNK);
     uint64_t out_cap = n_chunks * OUT_CHUNK;
     uint64_t fix_cap = std::max<uint64_t>(ctx->n_records / 64 + 4096, ctx->fix_val.cap / 4);
@@ -885,7 +890,7 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
         ctx->ms[4] = ms_parts;
         if (used_chunks > n_chunks || n_fix > fix_cap) {
             if (attempt == 1) return fail(ctx, GGCAT_GPU_ENOMEM, "solid output overflow");
-            n_chunks = std::max<uint64_t>(n_chunks, (uint64_t)used_chunks + 2 * lds_wgs);
+            n_chunks = std::max<uint64_t>(n_chunks, (uint64_t)used_chunks + 8 * lds_wgs);
             fix_cap = std::max(fix_cap, n_fix + 1024);
             continue;
         }
@@ -1022,6 +1027,8 @@ int ggcat_gpu_solid_colors(ggcat_gpu_ctx* ctx, uint32_t* color_words, uint32_t* 
 int ggcat_gpu_set_tuning(ggcat_gpu_ctx* ctx, const ggcat_gpu_tuning* t) {
     if (!ctx || !t) return GGCAT_GPU_EINVAL;
     if (t->merge_wgs_per_cu > 64) return fail(ctx, GGCAT_GPU_EINVAL, "merge_wgs_per_cu must be <= 64");
+    if (t->rec_cap && (t->rec_cap < 16 || t->rec_cap > 512 || (t->rec_cap & (t->rec_cap - 1))))
+        return fail(ctx, GGCAT_GPU_EINVAL, "rec_cap must be 0 or a power of two in [16, 512]");
     ctx->tune = *t;
     return GGCAT_GPU_OK;
 }

@@ -99,14 +99,16 @@ __host__ __device__ inline TileGeom tile_geom(uint32_t T, uint32_t w, uint32_t k
     uint32_t o = 0;
     g.off_g = o; o += 40 * 8;
     g.off_mm = o; o += regs ? 0 : g.NM * 8;
-    g.off_wv = o; o += g.NWB * w * 8;
+    // wv / wp / fwd are padded after every 16 entries (Tile::WV, WP, FWD) so
+    // that lanes writing 16-window blocks hit distinct LDS banks
+    g.off_wv = o; o += (g.NWB * w + g.NWB * w / 16 + 1) * 8;
     g.off_pk = o; o += g.NB / 16 * 4;
     g.off_inv = o; o += g.NB / 32 * 4;
     g.off_seg = o; o += (g.NW / 64 + 2) * 8;
     g.off_ok = o; o += (g.NW / 64 + 2) * 8;
     g.off_last = o; o += (g.NW / 64 + 2) * 8;
-    g.off_wp = o; o += g.NWB * w * 2;
-    g.off_fwd = o; o += g.NM;
+    g.off_wp = o; o += (g.NWB * w + 2 * (g.NWB * w / 16) + 2) * 2;
+    g.off_fwd = o; o += g.NM + 4 * (g.NM / 16) + 4;
     g.bytes = (o + 15) / 16 * 16;
     return g;
 }
@@ -152,6 +154,10 @@ struct Tile {
         return (inv[we] & hi_mask) != 0;
     }
     GG_DEV bool bit(const uint32_t* a, uint32_t x) const { return (a[x >> 5] >> (x & 31)) & 1; }
+    // window minimum, its m-mer position, and forward-strand flag per m-mer
+    GG_DEV uint64_t& WV(uint32_t x) const { return wv[x + (x >> 4)]; }
+    GG_DEV uint16_t& WP(uint32_t x) const { return wp[x + 2 * (x >> 4)]; }
+    GG_DEV uint8_t& FWD(uint32_t x) const { return fwd[x + 4 * (x >> 4)]; }
 };
 
 // 16 codes of relative positions [r, r+16) packed (position r at bits 0..1);
@@ -259,7 +265,7 @@ GG_DEV void tile_minima_lds(Tile& t) {
                 rc = rotr1_64(rc) ^ t.gt[2 * e + 1];
                 uint64_t mn = fw < rc ? fw : rc;
                 t.mm[p] = (mn << 1) | (fw != rc ? 1ULL : 0ULL);
-                t.fwd[p] = fw < rc ? 1 : 0;
+                t.FWD(p) = fw < rc ? 1 : 0;
             }
         }
     }
@@ -270,12 +276,12 @@ GG_DEV void tile_minima_lds(Tile& t) {
         const uint32_t base = q * w;
         uint64_t sv = t.mm[base + w - 1];
         uint32_t sp = base + w - 1;
-        t.wv[base + w - 1] = sv;
-        t.wp[base + w - 1] = (uint16_t)sp;
+        t.WV(base + w - 1) = sv;
+        t.WP(base + w - 1) = (uint16_t)sp;
         for (int i = (int)w - 2; i >= 0; i--) {
             mm_combine(t.mm[base + i], base + i, sv, sp, sv, sp);
-            t.wv[base + i] = sv;
-            t.wp[base + i] = (uint16_t)sp;
+            t.WV(base + i) = sv;
+            t.WP(base + i) = (uint16_t)sp;
         }
         uint64_t pv = 0;
         uint32_t pp = 0;
@@ -289,9 +295,9 @@ GG_DEV void tile_minima_lds(Tile& t) {
             }
             uint64_t rv;
             uint32_t rp;
-            mm_combine(t.wv[base + i], t.wp[base + i], pv, pp, rv, rp);
-            t.wv[base + i] = rv;
-            t.wp[base + i] = (uint16_t)rp;
+            mm_combine(t.WV(base + i), t.WP(base + i), pv, pp, rv, rp);
+            t.WV(base + i) = rv;
+            t.WP(base + i) = (uint16_t)rp;
         }
     }
 }
@@ -371,8 +377,8 @@ GG_DEV void tile_minima_w16(Tile& t) {
                 uint64_t rv = sv[i];
                 uint32_t rp = sp[i];
                 if (i > 0) mm_combine(sv[i], sp[i], pv, pp, rv, rp);
-                t.wv[p0 + i] = rv;
-                t.wp[p0 + i] = (uint16_t)rp;
+                t.WV(p0 + i) = rv;
+                t.WP(p0 + i) = (uint16_t)rp;
                 // flags of window x = p0 + i (bases rb + x .. rb + x + k - 2)
                 const uint32_t x = p0 + i;
                 if (x >= 1 && x < G.NW) {
@@ -403,17 +409,17 @@ GG_DEV void tile_minima_w16(Tile& t) {
             }
         }
 #pragma unroll
-        for (int i = 0; i < 16; i++) t.fwd[p0 + i] = (fbits >> i) & 1u;
+        for (int i = 0; i < 16; i++) t.FWD(p0 + i) = (fbits >> i) & 1u;
         if (q + 1 == G.NWB) {
 #pragma unroll
-            for (int i = 0; i < 16; i++) t.fwd[p0 + 16 + i] = (fbits1 >> i) & 1u;
+            for (int i = 0; i < 16; i++) t.FWD(p0 + 16 + i) = (fbits1 >> i) & 1u;
         }
     }
     __syncthreads();
     if (need_prev) {
         // window p0 starts a group when its minimum differs from window p0 - 1
-        const uint64_t v1 = t.wv[p0], v0 = t.wv[p0 - 1];
-        if (v0 != v1 || ((v1 & 1) && t.wp[p0] != t.wp[p0 - 1])) segm |= 1u;
+        const uint64_t v1 = t.WV(p0), v0 = t.WV(p0 - 1);
+        if (v0 != v1 || ((v1 & 1) && t.WP(p0) != t.WP(p0 - 1))) segm |= 1u;
     }
     // bitmap words past the last block stay zero
     const uint32_t npieces = (G.NW / 64 + 2) * 4;
@@ -444,8 +450,8 @@ GG_DEV void tile_flags(Tile& t) {
             okf = valid && !(first && lst);
             lastf = okf && lst;
             if (okf) {
-                uint64_t v0 = t.wv[x - 1], v1 = t.wv[x];
-                bool differs = (v0 != v1) || ((v1 & 1) && t.wp[x] != t.wp[x - 1]);
+                uint64_t v0 = t.WV(x - 1), v1 = t.WV(x);
+                bool differs = (v0 != v1) || ((v1 & 1) && t.WP(x) != t.WP(x - 1));
                 segf = first || differs;
             }
         }
@@ -619,15 +625,15 @@ GG_DEV RecInfo rec_info(const Tile& t, uint32_t x) {
         }
     }
     if (a < 0) return r;  // group starts before the halo: deferred
-    const uint64_t v = t.wv[x];
+    const uint64_t v = t.WV(x);
     const bool unique = v & 1;
     const bool include_first = invalid_abs(t, a - 1);
     const bool include_last = t.bit(t.last, x);
     r.start = include_first ? a : a - 1;
     r.end = include_last ? bwin + k - 1 : bwin + k;
-    const uint32_t prel = t.wp[x];
+    const uint32_t prel = t.WP(x);
     const int64_t mmabs = t.wlo + prel;
-    r.rc = A.canonical && unique && !t.fwd[prel];
+    r.rc = A.canonical && unique && !t.FWD(prel);
     uint32_t minpos = 0;
     if (unique) minpos = r.rc ? (uint32_t)(r.end - mmabs - m) : (uint32_t)(mmabs - r.start);
     const uint32_t flags = ((include_first ? 1u : 0u) << (r.rc ? 1 : 0)) | ((include_last ? 1u : 0u) << (r.rc ? 0 : 1));
@@ -640,11 +646,11 @@ GG_DEV RecInfo rec_info(const Tile& t, uint32_t x) {
     uint32_t cross = 0;
     if (unique && A.xb) {
         if (!include_first) {
-            const uint64_t vl = t.wv[a - t.wlo - 1];
+            const uint64_t vl = t.WV(a - t.wlo - 1);
             if (sub_of(vl, A.b, A.sb) == sub && unit_of(vl, A.b, A.sb, A.xb) != r.unit) cross |= r.rc ? META_XEND : META_XBEGIN;
         }
         if (!include_last) {
-            const uint64_t vr = t.wv[x + 1];
+            const uint64_t vr = t.WV(x + 1);
             if (sub_of(vr, A.b, A.sb) == sub && unit_of(vr, A.b, A.sb, A.xb) != r.unit) cross |= r.rc ? META_XBEGIN : META_XEND;
         }
     }
@@ -794,7 +800,7 @@ __global__ __launch_bounds__(64) void bucket_stage_kernel(BucketArgs A) {
                 const uint32_t d = atomicAdd(A.defer_count, 1u);
                 if (d < A.n_tiles) {
                     A.defer[3 * (uint64_t)d] = (uint64_t)(t.t0 + (int64_t)x - w);
-                    A.defer[3 * (uint64_t)d + 1] = t.wv[x];
+                    A.defer[3 * (uint64_t)d + 1] = t.WV(x);
                     A.defer[3 * (uint64_t)d + 2] = t.bit(t.last, x) ? 1 : 0;
                 } else {
                     atomicOr(A.error_flag, 4u);

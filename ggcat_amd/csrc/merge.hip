@@ -60,6 +60,7 @@ struct MergeArgs {
     uint32_t table_pct;  // table slots per 100 occurrences of a whole-item pass (power-of-two rounded)
     uint32_t dedup;  // identical records of an entry are inserted once with their count (seq-hash, k <= 32)
     uint32_t rec_lds_off;  // byte offset of the record table in dynamic LDS
+    uint32_t rec_cap;  // record table slots
     const WorkItem* items;
     const PartWork* parts;  // null: first pass, one work entry per item
     uint32_t n_work;  // entries of this pass
@@ -240,39 +241,40 @@ GG_DEV uint32_t occurrence_bits(uint32_t meta, bool is_fwd, uint32_t idx, uint32
     return 1u | (cross ? VAL_CROSS : 0u) | (fl << 30);
 }
 
-// Output is written in chunks of OUT_CHUNK entries.  A workgroup owns one
-// open chunk at a time (reserved with one atomic), fills it locally and
-// records its fill count when it moves on, so solid entries cost no global
-// atomics.  Entries of chunk c live at [c * OUT_CHUNK, c * OUT_CHUNK + fill[c]).
-constexpr uint32_t OUT_CHUNK = 8192;
+// Output is written in chunks of OUT_CHUNK entries.  Each wavefront owns one
+// open chunk at a time (reserved with one atomic), fills it without any
+// workgroup barrier and records its fill count when it moves on, so solid
+// entries cost no global atomics.  Entries of chunk c live at
+// [c * OUT_CHUNK, c * OUT_CHUNK + fill[c]).
+constexpr uint32_t OUT_CHUNK = 4096;
 
+// A wavefront's open chunk; every lane holds the same values.
 struct EmitState {
     uint32_t chunk;  // open chunk, ~0u when none
     uint32_t fill;
-    uint32_t wave_cnt[4];
-    uint32_t wave_base[4];
 };
 
 GG_DEV void close_chunk(const MergeArgs& A, EmitState& es) {
-    if (es.chunk != ~0u && es.chunk < A.n_chunks) A.chunk_fill[es.chunk] = es.fill;
+    if ((threadIdx.x & 63) == 0 && es.chunk != ~0u && es.chunk < A.n_chunks) A.chunk_fill[es.chunk] = es.fill;
     es.chunk = ~0u;
     es.fill = 0;
 }
 
-// Emit solid entries of a table of `cap` slots; cross-marked entries holding
-// only one flag bit go to the fixup list.
-// The table slots claimed by the entry are listed in `claimed` (n of them),
-// so emission and the reset afterwards touch only those.
+// Emit solid entries among the table slots claimed by the entry (`claimed`,
+// n of them); cross-marked entries holding only one flag bit go to the
+// fixup list.  Wavefronts take 64 claimed slots at a time.
 template <class KP, class SP, class VP>
 GG_DEV void emit_solid(const MergeArgs& A, uint32_t sub, KP klo, KP khi, SP st, VP vals, const uint32_t* cbits,
                        const uint16_t* claimed, uint32_t n, EmitState& es) {
     const uint32_t W = A.color_words;
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-    for (uint32_t i0 = 0; i0 < n; i0 += blockDim.x) {
-        const uint32_t i = (i0 + threadIdx.x < n) ? claimed[i0 + threadIdx.x] : 0u;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t below = (1ULL << lane) - 1;
+    for (uint32_t i0 = wave * 64; i0 < n; i0 += blockDim.x) {
+        const uint32_t j = i0 + lane;
+        const uint32_t i = j < n ? claimed[j] : 0u;
         bool solid = false, fix = false;
         uint32_t outv = 0;
-        if (i0 + threadIdx.x < n) {
+        if (j < n) {
             bool occupied = (A.key_words == 1) ? (klo[i] != EMPTY64) : (st[i] == 2);
             if (occupied) {
                 uint32_t v = vals[i];
@@ -285,37 +287,27 @@ GG_DEV void emit_solid(const MergeArgs& A, uint32_t sub, KP klo, KP khi, SP st, 
         }
         const bool main = solid && !fix;
         const uint64_t bal = __ballot(main), balf = __ballot(fix);
-        const uint64_t below = (1ULL << lane) - 1;
-        if (lane == 0) es.wave_cnt[wave] = (uint32_t)__popcll(bal);
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t tot = 0;
-            for (uint32_t q = 0; q < nwaves; q++) tot += es.wave_cnt[q];
-            if (tot) {
-                if (es.chunk == ~0u || es.fill + tot > OUT_CHUNK) {
-                    close_chunk(A, es);
-                    es.chunk = atomicAdd(A.chunk_cursor, 1u);
-                }
-                uint32_t run = es.fill;
-                for (uint32_t q = 0; q < nwaves; q++) {
-                    es.wave_base[q] = run;
-                    run += es.wave_cnt[q];
-                }
-                es.fill = run;
+        const uint32_t cnt = (uint32_t)__popcll(bal);
+        if (cnt) {
+            if (es.chunk == ~0u || es.fill + cnt > OUT_CHUNK) {
+                close_chunk(A, es);
+                uint32_t c = 0;
+                if (lane == 0) c = atomicAdd(A.chunk_cursor, 1u);
+                es.chunk = __shfl(c, 0);
             }
-        }
-        __syncthreads();
-        if (main && es.chunk < A.n_chunks) {
-            uint64_t pos = (uint64_t)es.chunk * OUT_CHUNK + es.wave_base[wave] + __popcll(bal & below);
-            if (A.key_words == 1) {
-                A.out_key[pos] = klo[i];
-            } else {
-                A.out_key[2 * pos] = klo[i];
-                A.out_key[2 * pos + 1] = khi[i];
+            if (main && es.chunk < A.n_chunks) {
+                const uint64_t pos = (uint64_t)es.chunk * OUT_CHUNK + es.fill + __popcll(bal & below);
+                if (A.key_words == 1) {
+                    A.out_key[pos] = klo[i];
+                } else {
+                    A.out_key[2 * pos] = klo[i];
+                    A.out_key[2 * pos + 1] = khi[i];
+                }
+                A.out_val[pos] = outv;
+                A.out_sub[pos] = sub;
+                for (uint32_t q = 0; q < W; q++) A.out_colors[pos * W + q] = cbits[i * W + q];
             }
-            A.out_val[pos] = outv;
-            A.out_sub[pos] = sub;
-            for (uint32_t j = 0; j < W; j++) A.out_colors[pos * W + j] = cbits[i * W + j];
+            es.fill += cnt;
         }
         if (balf) {
             unsigned long long basef = 0;
@@ -332,11 +324,10 @@ GG_DEV void emit_solid(const MergeArgs& A, uint32_t sub, KP klo, KP khi, SP st, 
                     }
                     A.fix_val[pos] = outv;
                     A.fix_sub[pos] = sub;
-                    for (uint32_t j = 0; j < W; j++) A.fix_colors[pos * W + j] = cbits[i * W + j];
+                    for (uint32_t q = 0; q < W; q++) A.fix_colors[pos * W + q] = cbits[i * W + q];
                 }
             }
         }
-        __syncthreads();
     }
 }
 
@@ -384,40 +375,63 @@ GG_DEV void insert_item(const MergeArgs& A, uint64_t r0, uint64_t r1, const Part
 // packed bases) plus their color; a record the table cannot take is counted
 // on its own.  After deduplication every thread takes k-mer positions of the
 // distinct records, so the k-mer work is spread evenly over the workgroup.
-constexpr uint32_t REC_CAP = 256;
+// Record table slots (a power of two, 16..512); chunks of rec_cap / 2
+// records are deduplicated at a time.
+constexpr uint32_t REC_CAP_DEFAULT = 256;
 
 struct RecTable {
-    uint32_t* key;  // REC_CAP * SW
+    uint32_t* key;  // rec_cap * SW
     uint32_t* cnt;
     uint32_t* state;  // 0 free, 1 being written, 2 ready
     uint32_t* col;
     uint32_t* list;  // distinct records (table slots)
-    uint32_t* off;  // REC_CAP + 1 k-mer offsets of the distinct records
+    uint32_t* off;  // rec_cap + 1 k-mer offsets of the distinct records
     uint32_t* n;  // [2]: distinct records, k-mers
+    uint8_t* rec_of;  // distinct record (index into list) of each k-mer position
 };
+
+// k-mers of one inline record at most.
+__host__ __device__ inline uint32_t rec_max_kmers(uint32_t slot_words, uint32_t k) {
+    return (slot_words - 1) * 16 >= k ? (slot_words - 1) * 16 - k + 1 : 0;
+}
 
 GG_DEV RecTable rec_table(const MergeArgs& A, uint8_t* lds) {
     RecTable R;
     uint32_t* b = reinterpret_cast<uint32_t*>(lds + A.rec_lds_off);
     R.key = b;
-    b += REC_CAP * A.slot_words;
+    b += A.rec_cap * A.slot_words;
     R.cnt = b;
-    b += REC_CAP;
+    b += A.rec_cap;
     R.state = b;
-    b += REC_CAP;
+    b += A.rec_cap;
     R.col = b;
-    b += REC_CAP;
+    b += A.rec_cap;
     R.list = b;
-    b += REC_CAP;
+    b += A.rec_cap;
     R.off = b;
-    b += REC_CAP + 1;
-    R.n = b;
+    b += A.rec_cap + 1;
+    R.n = b;  // [1]: k-mers of the chunk, [2..5]: wave totals of the scan
+    b += 6;
+    R.rec_of = reinterpret_cast<uint8_t*>(b);
     return R;
 }
 
-// Bytes of dynamic LDS the record table needs for `slot_words`.
-__host__ __device__ inline uint32_t rec_table_bytes(uint32_t slot_words) {
-    return (REC_CAP * slot_words + 5 * REC_CAP + 1 + 2) * 4;
+// Bytes of dynamic LDS the record table needs.
+__host__ __device__ inline uint32_t rec_table_bytes(uint32_t slot_words, uint32_t k, uint32_t rec_cap) {
+    return (rec_cap * slot_words + 5 * rec_cap + 1 + 6) * 4 + (rec_cap / 2) * rec_max_kmers(slot_words, k) + 4;
+}
+
+// Appends the slots claimed by this instruction's lanes to the claimed list
+// with one LDS atomic per wavefront.  Called by every lane of the wavefront.
+GG_DEV void append_claimed(bool cl, uint32_t at, uint16_t* claimed, uint32_t* n_claimed) {
+    const uint64_t bal = __ballot(cl);
+    if (!bal) return;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t first = (uint32_t)__ffsll((unsigned long long)bal) - 1;
+    uint32_t base = 0;
+    if (lane == first) base = atomicAdd(n_claimed, (uint32_t)__popcll(bal));
+    base = __shfl(base, (int)first);
+    if (cl) claimed[base + __popcll(bal & ((1ULL << lane) - 1))] = (uint16_t)at;
 }
 
 GG_DEV uint32_t rec_hash(const uint32_t* w, uint32_t sw, uint32_t color) {
@@ -449,7 +463,7 @@ template <class KP, class SP, class VP>
 GG_DEV void insert_item_dedup(const MergeArgs& A, uint64_t r0, uint64_t r1, const PartWork& w, KP klo, KP khi, SP st,
                               VP vals, uint32_t* cbits, uint32_t mask, uint32_t probe_limit, uint32_t* overflow,
                               uint16_t* claimed, uint32_t* n_claimed, const RecTable& R) {
-    const uint32_t W = A.color_words, SW = A.slot_words, k = A.k;
+    const uint32_t W = A.color_words, SW = A.slot_words, k = A.k, RC = A.rec_cap;
     const uint32_t part_mask = (1u << w.log_parts) - 1;
     auto insert_key = [&](uint64_t key, uint32_t bits, uint32_t color) {
         if (*(volatile uint32_t*)overflow) return;
@@ -463,9 +477,8 @@ GG_DEV void insert_item_dedup(const MergeArgs& A, uint64_t r0, uint64_t r1, cons
         if (res == 1) claimed[atomicAdd(n_claimed, 1u)] = (uint16_t)at;
         if (W) atomicOr(&cbits[at * W + (color >> 5)], 1u << (color & 31));
     };
-    for (uint32_t i = threadIdx.x; i < REC_CAP; i += blockDim.x) R.state[i] = 0;
-    if (threadIdx.x == 0) R.n[0] = R.n[1] = 0;
-    __syncthreads();
+    // the record table's states are all free here (cleared at kernel start
+    // and by phase B of the previous chunk)
     // phase A: distinct records (long records and table misses go direct)
     for (uint64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
         const uint32_t* slot = A.slots + r * SW;
@@ -475,7 +488,7 @@ GG_DEV void insert_item_dedup(const MergeArgs& A, uint64_t r0, uint64_t r1, cons
         if (!direct) {
             uint32_t wv[8];
             for (uint32_t q = 0; q < SW; q++) wv[q] = slot[q];
-            uint32_t i = rec_hash(wv, SW, color) & (REC_CAP - 1);
+            uint32_t i = rec_hash(wv, SW, color) & (RC - 1);
             uint32_t probes = 0, spins = 0;
             bool done = false;
             while (!done) {
@@ -499,7 +512,7 @@ GG_DEV void insert_item_dedup(const MergeArgs& A, uint64_t r0, uint64_t r1, cons
                         direct = true;
                         done = true;
                     } else {
-                        i = (i + 1) & (REC_CAP - 1);
+                        i = (i + 1) & (RC - 1);
                     }
                 } else if (++spins > (1u << 16)) {
                     direct = true;
@@ -514,61 +527,84 @@ GG_DEV void insert_item_dedup(const MergeArgs& A, uint64_t r0, uint64_t r1, cons
         }
     }
     __syncthreads();
-    // phase B1: list the distinct records and their k-mer counts
-    for (uint32_t i = threadIdx.x; i < REC_CAP; i += blockDim.x) {
-        if (R.state[i] != 2) continue;
-        const uint32_t u = atomicAdd(&R.n[0], 1u);
-        R.list[u] = i;
-    }
-    __syncthreads();
-    const uint32_t n_u = R.n[0];
-    // exclusive prefix of the distinct records' k-mer counts (REC_CAP <= 4
-    // waves of 64: lane prefix by shuffles, then the wave totals)
+    // phase B: one exclusive scan over the table slots of (distinct record
+    // << 16 | k-mers) numbers the distinct records and places their k-mers;
+    // each thread owns rec_cap / blockDim consecutive slots (at least one)
+    // and frees them for the next chunk once read.
     {
-        const uint32_t u = threadIdx.x, lane = u & 63, wave = u >> 6;
-        uint32_t c = 0;
-        if (u < n_u) {
-            const uint32_t len = meta_len(R.key[R.list[u] * SW]);
-            c = len >= k ? len - k + 1 : 0;
+        const uint32_t per = RC > blockDim.x ? RC / blockDim.x : 1u;
+        const uint32_t s0 = threadIdx.x * per;
+        const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        uint32_t pk[2] = {0u, 0u};  // per <= 2 (rec_cap <= 512, 256 threads)
+        uint32_t mine = 0;
+        for (uint32_t q = 0; q < per; q++) {
+            const uint32_t si = s0 + q;
+            if (si < RC && R.state[si] == 2) {
+                const uint32_t len = meta_len(R.key[si * SW]);
+                pk[q] = (1u << 16) | (len >= k ? len - k + 1 : 0u);
+                R.state[si] = 0;
+            }
+            mine += pk[q];
         }
-        uint32_t inc = c;
+        uint32_t inc = mine;
         for (uint32_t d = 1; d < 64; d <<= 1) {
             const uint32_t y = __shfl_up(inc, d);
             if (lane >= d) inc += y;
         }
-        if (lane == 63) R.off[REC_CAP - 4 + wave] = inc;  // wave totals, parked past the used range
+        if (lane == 63) R.n[2 + wave] = inc;  // wave totals
         __syncthreads();
-        uint32_t base = 0;
-        for (uint32_t q = 0; q < wave; q++) base += R.off[REC_CAP - 4 + q];
-        const uint32_t all = R.off[REC_CAP - 4] + R.off[REC_CAP - 3] + R.off[REC_CAP - 2] + R.off[REC_CAP - 1];
-        __syncthreads();
-        if (u < n_u) R.off[u] = base + inc - c;
-        if (u == 0) {
-            R.off[n_u] = all;
-            R.n[1] = all;
+        uint32_t base = 0, all = 0;
+        const uint32_t nw = blockDim.x >> 6;
+        for (uint32_t q = 0; q < nw; q++) {
+            const uint32_t t = R.n[2 + q];
+            if (q < wave) base += t;
+            all += t;
         }
+        uint32_t run = base + inc - mine;
+        for (uint32_t q = 0; q < per; q++) {
+            if (pk[q]) {
+                const uint32_t u = run >> 16, o = run & 0xFFFFu, c = pk[q] & 0xFFFFu;
+                R.list[u] = s0 + q;
+                R.off[u] = o;
+                for (uint32_t e = 0; e < c; e++) R.rec_of[o + e] = (uint8_t)u;
+            }
+            run += pk[q];
+        }
+        if (threadIdx.x == 0) R.n[1] = all & 0xFFFFu;
         __syncthreads();
     }
-    // phase B2: every k-mer of the distinct records, one per thread
+    // phase B2: every k-mer of the distinct records, one per thread; the loop
+    // is uniform over the workgroup so claims are appended per wavefront
     const uint32_t total = R.n[1];
-    for (uint32_t j = threadIdx.x; j < total; j += blockDim.x) {
-        uint32_t lo = 0, hi = n_u;  // last u with off[u] <= j
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (R.off[mid] <= j) lo = mid; else hi = mid;
+    for (uint32_t j0 = 0; j0 < total; j0 += blockDim.x) {
+        const uint32_t j = j0 + threadIdx.x;
+        bool cl = false;
+        uint32_t at = 0;
+        if (j < total) {
+            const uint32_t u = R.rec_of[j];
+            const uint32_t si = R.list[u], i = j - R.off[u];
+            const uint32_t* rw = R.key + si * SW;
+            const uint32_t meta = rw[0];
+            const uint32_t len = meta_len(meta);
+            bool fwd;
+            const uint64_t key = kmer_at(rw + 1, i, k, &fwd);
+            const uint32_t bits = occurrence_bits(meta, fwd, i, len - k) - 1u + R.cnt[si];
+            const uint32_t color = R.col[si];
+            if (!part_mask || ((tab_hash64(key) >> 16) & part_mask) == w.part) {
+                const uint32_t res = tab_insert64(klo, vals, mask, key, bits, probe_limit, at);
+                if (res == 2) {
+                    atomicOr(overflow, 1u);
+                } else {
+                    cl = res == 1;
+                    if (W) atomicOr(&cbits[at * W + (color >> 5)], 1u << (color & 31));
+                }
+            }
         }
-        const uint32_t si = R.list[lo], i = j - R.off[lo];
-        const uint32_t* rw = R.key + si * SW;
-        const uint32_t meta = rw[0];
-        const uint32_t len = meta_len(meta);
-        bool fwd;
-        const uint64_t key = kmer_at(rw + 1, i, k, &fwd);
-        const uint32_t bits = occurrence_bits(meta, fwd, i, len - k) - 1u + R.cnt[si];
-        insert_key(key, bits, R.col[si]);
+        append_claimed(cl, at, claimed, n_claimed);
     }
 }
 
-// Work entry with the metadata a pass needs, fetched ahead by thread 0.
+// Work entry with the metadata a pass needs.
 struct Fetched {
     uint32_t e;
     PartWork w;
@@ -576,23 +612,34 @@ struct Fetched {
     uint64_t r0, r1;
 };
 
-GG_DEV void fetch_meta(const MergeArgs& A, Fetched& f) {
-    if (f.e >= A.n_work) return;
-    f.w = A.parts ? A.parts[f.e] : PartWork{f.e, 0u, 0u, 0u};
-    f.it = A.items[f.w.item];
-    f.r0 = A.unit_off[f.it.u0];
-    f.r1 = A.unit_off[f.it.u1];
+// Thread 0's pipeline of work entries: the next entry's metadata is loaded
+// in three dependent steps (part entry, work item, unit offsets) spread over
+// the current entry's phases, and the entry after it is already claimed, so
+// no barrier waits on a chain of global loads.
+struct Pipe {
+    Fetched n1;  // next entry
+    uint32_t e2;  // the entry after it (claimed)
+};
+GG_DEV void pipe_part(const MergeArgs& A, Pipe& p) {
+    if (p.n1.e < A.n_work) p.n1.w = A.parts ? A.parts[p.n1.e] : PartWork{p.n1.e, 0u, 0u, 0u};
+}
+GG_DEV void pipe_item(const MergeArgs& A, Pipe& p) {
+    if (p.n1.e < A.n_work) p.n1.it = A.items[p.n1.w.item];
+}
+GG_DEV void pipe_units(const MergeArgs& A, Pipe& p) {
+    if (p.n1.e < A.n_work) {
+        p.n1.r0 = A.unit_off[p.n1.it.u0];
+        p.n1.r1 = A.unit_off[p.n1.it.u1];
+    }
 }
 
 // Persistent work-stealing merge: each workgroup takes work entries from a
-// global cursor.  Thread 0 claims the next entry when the current one starts
-// and loads its metadata between insertion and emission, so the claim and the
-// metadata loads overlap the current entry's work.
+// global cursor, thread 0 keeping the next entry's metadata in flight.
 __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
     extern __shared__ __align__(16) uint8_t lds[];
     __shared__ uint32_t s_overflow;
     __shared__ Fetched s_cur;
-    __shared__ EmitState es;
+    EmitState es{~0u, 0u};
     const uint32_t cap_max = A.lds_cap;  // the host allocates this many entries plus a margin
     uint64_t* klo = reinterpret_cast<uint64_t*>(lds);
     uint64_t* khi = klo + cap_max;  // only used for 128-bit keys
@@ -610,19 +657,27 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
         if (A.key_words == 2) st[i] = 0;
     }
     for (uint32_t i = threadIdx.x; i < cap_max * W; i += blockDim.x) cbits[i] = 0;
-    Fetched nxt{};
+    if (A.dedup) {
+        // record table slots start free; each chunk's phase B frees them again
+        const RecTable R = rec_table(A, lds);
+        for (uint32_t i = threadIdx.x; i < A.rec_cap; i += blockDim.x) R.state[i] = 0;
+    }
+    Pipe pp{};
     if (leader) {
-        es.chunk = ~0u;
-        es.fill = 0;
         s_claimed = 0;
-        nxt.e = atomicAdd(A.work, 1u);
-        fetch_meta(A, nxt);
+        pp.n1.e = atomicAdd(A.work, 1u);
+        pipe_part(A, pp);
+        pipe_item(A, pp);
+        pipe_units(A, pp);
+        pp.e2 = pp.n1.e < A.n_work ? atomicAdd(A.work, 1u) : A.n_work;
     }
     while (true) {
         if (leader) {
-            s_cur = nxt;
+            s_cur = pp.n1;
             s_overflow = 0;
-            nxt.e = (nxt.e < A.n_work) ? atomicAdd(A.work, 1u) : A.n_work;
+            pp.n1.e = pp.e2;
+            pp.e2 = (pp.e2 < A.n_work) ? atomicAdd(A.work, 1u) : A.n_work;
+            pipe_part(A, pp);
         }
         __syncthreads();
         const Fetched cur = s_cur;
@@ -633,7 +688,8 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
         if (!A.parts && it.occ > A.lds_try) {
             if (leader) {
                 A.redo[atomicAdd(A.redo_count, 1u)] = w;
-                fetch_meta(A, nxt);
+                pipe_item(A, pp);
+                pipe_units(A, pp);
             }
             continue;
         }
@@ -642,12 +698,21 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
         while (cap < want && cap < cap_max) cap <<= 1;
         if (w.log_parts) cap = cap_max;
         const uint32_t mask = cap - 1;
-        if (A.dedup && cur.r1 - cur.r0 <= REC_CAP / 2)
-            insert_item_dedup(A, cur.r0, cur.r1, w, klo, khi, st, vals, cbits, mask, 96u, &s_overflow, claimed,
-                              &s_claimed, rec_table(A, lds));
-        else
+        if (A.dedup) {
+            // records in chunks of rec_cap / 2 (the record table's load
+            // factor stays at most one half); identical records of one chunk
+            // are counted once, the k-mer table accumulates over the chunks
+            const uint32_t chunk = A.rec_cap / 2;
+            for (uint64_t c0 = cur.r0; c0 < cur.r1; c0 += chunk) {
+                const uint64_t c1 = c0 + chunk < cur.r1 ? c0 + chunk : cur.r1;
+                insert_item_dedup(A, c0, c1, w, klo, khi, st, vals, cbits, mask, 96u, &s_overflow, claimed,
+                                  &s_claimed, rec_table(A, lds));
+                __syncthreads();  // the next chunk reuses the record table
+            }
+        } else {
             insert_item(A, cur.r0, cur.r1, w, klo, khi, st, vals, cbits, mask, 96u, &s_overflow, claimed, &s_claimed);
-        if (leader) fetch_meta(A, nxt);
+        }
+        if (leader) pipe_item(A, pp);
         __syncthreads();
         const uint32_t n_cl = s_claimed;
         if (s_overflow) {
@@ -656,6 +721,7 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
         } else {
             emit_solid(A, it.sub, klo, khi, st, vals, cbits, claimed, n_cl, es);
         }
+        if (leader) pipe_units(A, pp);
         __syncthreads();
         for (uint32_t j = threadIdx.x; j < n_cl; j += blockDim.x) {
             const uint32_t i = claimed[j];
@@ -667,7 +733,7 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
         if (leader) s_claimed = 0;
         __syncthreads();
     }
-    if (leader) close_chunk(A, es);
+    close_chunk(A, es);
 }
 
 // One thread per sub-bucket: runs of its units become work items of at most

@@ -151,7 +151,10 @@ int ggcat_gpu_solid_colors(ggcat_gpu_ctx* ctx, uint32_t* color_words, uint32_t* 
  *                     argument set and replay it on later calls
  *   stage_tile_cap    bucketing: staging positions per tile before records
  *                     spill to the shared overflow region (default 2.5
- *                     records per w+1 windows); the result never depends on it */
+ *                     records per w+1 windows); the result never depends on it
+ *   rec_cap           merge: identical-record table slots (power of two in
+ *                     [16, 512], default 256); records are deduplicated in
+ *                     chunks of rec_cap / 2 */
 typedef struct {
     uint32_t item_occ;
     uint32_t lds_max_occ;
@@ -163,6 +166,7 @@ typedef struct {
     uint32_t unit_bytes;
     uint32_t use_graphs;
     uint32_t stage_tile_cap;
+    uint32_t rec_cap;
 } ggcat_gpu_tuning;
 int ggcat_gpu_set_tuning(ggcat_gpu_ctx* ctx, const ggcat_gpu_tuning* tuning);
 

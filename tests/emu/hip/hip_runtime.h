@@ -61,6 +61,7 @@ template <class T> inline T __hip_atomic_load(T* p, int, int) { return __atomic_
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 inline int __popc(unsigned x) { return __builtin_popcount(x); }
 inline int __ffs(unsigned x) { return __builtin_ffs((int)x); }
+inline int __ffsll(unsigned long long x) { return __builtin_ffsll((long long)x); }
 inline unsigned __brev(unsigned x) {
     x = ((x >> 1) & 0x55555555u) | ((x & 0x55555555u) << 1);
     x = ((x >> 2) & 0x33333333u) | ((x & 0x33333333u) << 2);
@@ -137,6 +138,13 @@ inline hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return 801; }
 constexpr unsigned hipEventDisableTiming = 2;
 inline hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
 inline hipError_t hipEventQuery(hipEvent_t) { return hipSuccess; }
+struct hipFuncAttributes {
+    size_t sharedSizeBytes;
+};
+inline hipError_t hipFuncGetAttributes(hipFuncAttributes* a, const void*) {
+    a->sharedSizeBytes = 0;
+    return hipSuccess;
+}
 inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b) {
     *ms = std::chrono::duration<float, std::milli>(b->t - a->t).count();
     return hipSuccess;
