@@ -41,14 +41,36 @@ def make_workload(config: int, scale: float):
 
 def kernel_bytes(cnt: np.ndarray, key_words: int) -> dict:
     """Algorithmic HBM bytes per launch of each kernel.
-    cnt: [input bytes, records, sum record bases, k-mer occurrences, big, solid, ...]."""
+    cnt: [input bytes, records, sum record bases, k-mer occurrences, big, solid, ...].
+    The super-k-mer arena is sum ceil(l/4) + 8 B of metadata per record; the
+    staging pass reads the input and writes the arena once, the scatter moves
+    it (read + write), the merge reads it and writes the solid entries."""
     in_b, n_rec, sum_len, _, _, n_solid = (float(x) for x in cnt[:6])
-    arena = (sum_len + 1.5 * n_rec) / 4.0 + 8.0 * n_rec  # sum ceil(l/4) + 8 B metadata per record
+    arena = (sum_len + 1.5 * n_rec) / 4.0 + 8.0 * n_rec
     return {
-        "bucket_count": in_b,
-        "bucket_emit": in_b + arena,
+        "bucket_stage": in_b + arena,
+        "bucket_scatter": 2.0 * arena,
         "merge": arena + n_solid * (8.0 * key_words + 8.0),
     }
+
+
+# rocprofv3 FETCH_SIZE / WRITE_SIZE of one launch per kernel at full size,
+# scaled by the counters' reading for a 1 GiB copy (profiles/ holds the
+# commands and the per-kernel table)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_config2.json")
+TRAFFIC_KERNELS = {"bucket_stage": "gg::bucket_stage_kernel<16>", "bucket_scatter": "gg::bucket_scatter_kernel",
+                   "merge": "gg::merge_lds_kernel"}
+
+
+def measured_traffic(workload: str, scale: float, kernel: str):
+    """HBM bytes (GB, read + write) of one launch of `kernel` as measured by
+    the committed counter runs of the same workload, else (None, None)."""
+    if not workload.startswith("config2") or scale != 1.0 or not os.path.exists(TRAFFIC_FILE):
+        return None, None
+    t = json.load(open(TRAFFIC_FILE)).get(TRAFFIC_KERNELS.get(kernel, ""))
+    if not t:
+        return None, None
+    return round(t["read_gb"] + t["write_gb"], 3), "profiles/hbm_traffic_config2.md (rocprofv3 --pmc, GB per launch)"
 
 
 def cpu_baseline(wl, threads: int) -> dict:
@@ -214,9 +236,10 @@ def main() -> None:
     n, kw = int(cnt[5]), (1 if (wl.k <= 32 and wl.hash_kind == 0) else 2)
     ms = np.mean(np.array(per_kernel), axis=0)
     kb = kernel_bytes(cnt, kw)
-    kern_ms = {"bucket_count": ms[0], "bucket_emit": ms[2], "merge": ms[3] + ms[4]}
+    kern_ms = {"bucket_stage": ms[0], "bucket_scatter": ms[2], "merge": ms[3] + ms[4]}
     dom = max(kern_ms, key=lambda x: kern_ms[x])
     achieved = kb[dom] / (kern_ms[dom] * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic(wl.name, a.scale, dom)
     ms_step = dt / a.steps * 1e3
     value = wl.bases / (dt / a.steps) / 1e9
     out = {
@@ -237,7 +260,8 @@ def main() -> None:
                    "hash": "seq-hash" if wl.hash_kind == 0 else "rabin-karp128", "input_bytes": int(wl.data.size),
                    "bases": int(wl.bases)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_gb": round(kb[dom] / 1e9, 3), "traffic_source": traffic_src},
         "kernels_ms": {k: round(float(v), 3) for k, v in kern_ms.items()},
         "counters": {"records": int(cnt[1]), "record_bases": int(cnt[2]), "kmer_occurrences": int(cnt[3]),
                      "hbm_table_subbuckets": int(cnt[4]), "solid_entries": int(n)},

@@ -15,6 +15,11 @@
 //   compaction (compact)                  pinned: multiset expansion equality
 //   color sets (merge_subbucket, colored) pinned: brute-force colored k-mer counts
 //   query counts (orc_query)              pinned: pure-Python brute force over the graph k-mer set
+//   unitig chains, order-free (unitig_chains) pinned: definition check (maximal paths of mutually unique
+//                                         unflagged neighbours) + every solid k-mer covered once
+//   reference extension loop (unitig_sequential) pinned against unitig_chains on inputs where the
+//                                         reference's visiting order cannot matter
+//   extension flags (orc_extension)       pinned: equal to neighbour counts of the solid set (same test)
 //   sorting extender (normal buckets)     parity unpinned: not restated; the reference intends it to
 //                                         give the hash-map rule's solid set, which is pinned above
 #include <algorithm>
