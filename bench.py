@@ -77,7 +77,7 @@ def cpu_baseline(wl, threads: int) -> dict:
     """The oracle restatement on a bounded prefix of the same reads."""
     from oracle import binding as O
 
-    target = 60_000_000 if wl.k <= 32 else 30_000_000  # bases in the sample (~10-30 s of CPU work)
+    target = 1_000_000_000 if wl.k <= 32 else 400_000_000  # bases in the sample (~10-20 s of CPU work)
     data = wl.data
     if data.size > target:
         cut = target

@@ -377,7 +377,7 @@ GG_DEV void insert_item(const MergeArgs& A, uint64_t r0, uint64_t r1, const Part
 // distinct records, so the k-mer work is spread evenly over the workgroup.
 // Record table slots (a power of two, 16..512); chunks of rec_cap / 2
 // records are deduplicated at a time.
-constexpr uint32_t REC_CAP_DEFAULT = 256;
+constexpr uint32_t REC_CAP_DEFAULT = 512;
 
 struct RecTable {
     uint32_t* key;  // rec_cap * SW
@@ -462,7 +462,8 @@ GG_DEV uint64_t kmer_at(const uint32_t* bw, uint32_t i, uint32_t k, bool* fwd) {
 template <class KP, class SP, class VP>
 GG_DEV void insert_item_dedup(const MergeArgs& A, uint64_t r0, uint64_t r1, const PartWork& w, KP klo, KP khi, SP st,
                               VP vals, uint32_t* cbits, uint32_t mask, uint32_t probe_limit, uint32_t* overflow,
-                              uint16_t* claimed, uint32_t* n_claimed, const RecTable& R) {
+                              uint16_t* claimed, uint32_t* n_claimed, const RecTable& R, const uint32_t* pre,
+                              uint32_t pre_color) {
     const uint32_t W = A.color_words, SW = A.slot_words, k = A.k, RC = A.rec_cap;
     const uint32_t part_mask = (1u << w.log_parts) - 1;
     auto insert_key = [&](uint64_t key, uint32_t bits, uint32_t color) {
@@ -479,15 +480,19 @@ GG_DEV void insert_item_dedup(const MergeArgs& A, uint64_t r0, uint64_t r1, cons
     };
     // the record table's states are all free here (cleared at kernel start
     // and by phase B of the previous chunk)
-    // phase A: distinct records (long records and table misses go direct)
-    for (uint64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+    // phase A: distinct records (long records and table misses go direct);
+    // thread t holds record r0 + t, loaded by the caller (pre, pre_color)
+    // while the previous chunk was processed
+    {
+        const uint64_t r = r0 + threadIdx.x;
+        if (r >= r1) goto phase_a_done;
         const uint32_t* slot = A.slots + r * SW;
-        const uint32_t meta = slot[0];
-        const uint32_t color = W ? A.slot_color[r] : 0u;
+        const uint32_t meta = pre[0];
+        const uint32_t color = pre_color;
         bool direct = (meta & META_LONG) != 0;
         if (!direct) {
             uint32_t wv[8];
-            for (uint32_t q = 0; q < SW; q++) wv[q] = slot[q];
+            for (uint32_t q = 0; q < SW; q++) wv[q] = pre[q];
             uint32_t i = rec_hash(wv, SW, color) & (RC - 1);
             uint32_t probes = 0, spins = 0;
             bool done = false;
@@ -526,6 +531,7 @@ GG_DEV void insert_item_dedup(const MergeArgs& A, uint64_t r0, uint64_t r1, cons
             });
         }
     }
+phase_a_done:
     __syncthreads();
     // phase B: one exclusive scan over the table slots of (distinct record
     // << 16 | k-mers) numbers the distinct records and places their k-mers;
@@ -699,14 +705,38 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
         if (w.log_parts) cap = cap_max;
         const uint32_t mask = cap - 1;
         if (A.dedup) {
-            // records in chunks of rec_cap / 2 (the record table's load
-            // factor stays at most one half); identical records of one chunk
-            // are counted once, the k-mer table accumulates over the chunks
-            const uint32_t chunk = A.rec_cap / 2;
+            // records in chunks of rec_cap / 2 <= 256 (the record table's load
+            // factor stays at most one half), one record per thread; identical
+            // records of one chunk are counted once, the k-mer table
+            // accumulates over the chunks.  Each thread's record of the next
+            // chunk is loaded while the current chunk is processed.
+            const uint32_t chunk = A.rec_cap / 2, SW = A.slot_words;
+            auto load_rec = [&](uint64_t r, uint32_t* wv, uint32_t& col) {
+                if (threadIdx.x < chunk && r < cur.r1) {
+                    const uint32_t* sp = A.slots + r * SW;
+                    if (SW == 4) {
+                        const uint4 v = *reinterpret_cast<const uint4*>(sp);
+                        wv[0] = v.x;
+                        wv[1] = v.y;
+                        wv[2] = v.z;
+                        wv[3] = v.w;
+                    } else {
+                        for (uint32_t q = 0; q < SW; q++) wv[q] = sp[q];
+                    }
+                    col = W ? A.slot_color[r] : 0u;
+                }
+            };
+            uint32_t cw[8] = {0, 0, 0, 0, 0, 0, 0, 0}, ccol = 0;
+            load_rec(cur.r0 + threadIdx.x, cw, ccol);
             for (uint64_t c0 = cur.r0; c0 < cur.r1; c0 += chunk) {
                 const uint64_t c1 = c0 + chunk < cur.r1 ? c0 + chunk : cur.r1;
+                uint32_t nw[8] = {0, 0, 0, 0, 0, 0, 0, 0}, ncol = 0;
+                load_rec(c1 + threadIdx.x, nw, ncol);
                 insert_item_dedup(A, c0, c1, w, klo, khi, st, vals, cbits, mask, 96u, &s_overflow, claimed,
-                                  &s_claimed, rec_table(A, lds));
+                                  &s_claimed, rec_table(A, lds), cw, ccol);
+#pragma unroll
+                for (uint32_t q = 0; q < 8; q++) cw[q] = nw[q];
+                ccol = ncol;
                 __syncthreads();  // the next chunk reuses the record table
             }
         } else {

@@ -153,7 +153,7 @@ int ggcat_gpu_solid_colors(ggcat_gpu_ctx* ctx, uint32_t* color_words, uint32_t* 
  *                     spill to the shared overflow region (default 2.5
  *                     records per w+1 windows); the result never depends on it
  *   rec_cap           merge: identical-record table slots (power of two in
- *                     [16, 512], default 256); records are deduplicated in
+ *                     [16, 512], default 512); records are deduplicated in
  *                     chunks of rec_cap / 2 */
 typedef struct {
     uint32_t item_occ;

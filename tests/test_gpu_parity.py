@@ -300,9 +300,10 @@ def test_autotune_and_graphs_keep_results(gpu_ctx):
     gpu_ctx.bucketing(wl.data, p)
     gpu_ctx.kmers_merge()
     ref = solid_array(gpu_ctx.solid())
-    grid = [Schedule(1000, 100, 10, 0, 0, 1), Schedule(2000, 200, 11, 1, 4, 1), Schedule(4000, 200, 11, 0, 0, 0)]
+    grid = [Schedule(1000, 100, 10, 0, 0, 1), Schedule(2000, 200, 11, 1, 4, 1), Schedule(4000, 200, 11, 0, 0, 0),
+            Schedule(2000, 200, 11, 0, 0, 1, rec_cap=512), Schedule(1400, 200, 11, 0, 0, 1, rec_cap=64)]
     res = autotune(gpu_ctx, wl.data, p, grid=grid, repeats=1)
-    assert res["tried"] == 3
+    assert res["tried"] == 5
     try:
         for _ in range(3):  # graph captured on the first call, replayed after
             gpu_ctx.bucketing(wl.data, p)
