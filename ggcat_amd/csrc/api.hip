@@ -760,7 +760,7 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->n_items.p, 0, 8, ctx->stream));
     hipLaunchKernelGGL(make_items_kernel, dim3((ctx->n_sub + 255) / 256), dim3(256), 0, ctx->stream,
-                       P_<uint32_t>(ctx->unit_kmers), ctx->n_sub, ctx->xb, item_occ,
+                       P_<uint32_t>(ctx->unit_kmers), P_<uint64_t>(ctx->unit_off), ctx->n_sub, ctx->xb, item_occ,
                        P_<WorkItem>(ctx->items), P_<unsigned int>(ctx->n_items));
     HIPCHK(hipGetLastError());
     uint32_t n_items = 0;
@@ -817,13 +817,15 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
         // items above lds_try) come back in `redo` and are processed again as
         // 2^L hash partitions, refined x4 while a partition still overflows.
         std::vector<PartWork> next;
-        std::vector<WorkItem> its;
         uint32_t n_work = n_items, n_split_items = 0, n_passes_run = 0;
         double ms_parts = 0;
         for (uint32_t pass = 0; n_work; pass++) {
             ENSURE(ctx->redo, (size_t)n_work * sizeof(PartWork));
             M.redo = P_<PartWork>(ctx->redo);
             M.n_work = n_work;
+            // batches of up to 8 entries per claim, while every workgroup
+            // still gets about 32 claims
+            M.claim = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(8, n_work / ((uint64_t)lds_wgs * 32)));
             M.parts = pass ? P_<PartWork>(ctx->parts) : nullptr;
             HIPCHK(hipMemsetAsync(ctx->work.p, 0, 8, ctx->stream));
             HIPCHK(hipMemsetAsync(ctx->redo_count.p, 0, 8, ctx->stream));
@@ -843,10 +845,6 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
             if (n_redo == 0) break;
             std::vector<PartWork> redo(n_redo);
             HIPCHK(hipMemcpy(redo.data(), ctx->redo.p, (size_t)n_redo * sizeof(PartWork), hipMemcpyDeviceToHost));
-            if (its.empty()) {
-                its.resize(n_items);
-                HIPCHK(hipMemcpy(its.data(), ctx->items.p, (size_t)n_items * sizeof(WorkItem), hipMemcpyDeviceToHost));
-            }
             next.clear();
             for (const PartWork& w : redo) {
                 if (w.item >= n_items) return fail(ctx, GGCAT_GPU_EHIP, "corrupt merge work list");
@@ -854,7 +852,7 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
                 if (w.log_parts == 0) {
                     // about half a table of occurrences per part
                     const uint64_t per = ctx->tune.part_occ ? ctx->tune.part_occ : M.lds_cap / 2;
-                    const uint64_t want = (its[w.item].occ + per - 1) / per;
+                    const uint64_t want = (w.occ + per - 1) / per;
                     L = 1;
                     while ((1ull << L) < want) L++;
                     base = 0;
@@ -868,7 +866,7 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
                 if (L > MAX_LOG_PARTS)
                     return fail(ctx, GGCAT_GPU_EHIP, "k-mer table overflow: a work item needs more than 2^16 parts");
                 const uint32_t n_new = w.log_parts == 0 ? (1u << L) : 4u;
-                for (uint32_t j = 0; j < n_new; j++) next.push_back(PartWork{w.item, base + j * stride, L, 0});
+                for (uint32_t j = 0; j < n_new; j++) next.push_back(PartWork{w.item, base + j * stride, L, w.occ});
             }
             n_work = (uint32_t)next.size();
             ENSURE(ctx->parts, (size_t)n_work * sizeof(PartWork));

@@ -33,6 +33,7 @@ struct WorkItem {
     uint32_t u0, u1;  // physical units [u0, u1)
     uint32_t occ;  // k-mer occurrences
     uint32_t sub;  // sub-bucket
+    uint64_t r0, r1;  // its slots [r0, r1)
 };
 
 // One pass over a work item restricted to the k-mers whose partition hash
@@ -42,7 +43,7 @@ struct PartWork {
     uint32_t item;
     uint32_t part;
     uint32_t log_parts;
-    uint32_t pad;
+    uint32_t occ;  // the item's k-mer occurrences (set on redo entries)
 };
 constexpr uint32_t MAX_LOG_PARTS = 16;
 
@@ -61,6 +62,7 @@ struct MergeArgs {
     uint32_t dedup;  // identical records of an entry are inserted once with their count (seq-hash, k <= 32)
     uint32_t rec_lds_off;  // byte offset of the record table in dynamic LDS
     uint32_t rec_cap;  // record table slots
+    uint32_t claim;  // work entries claimed per atomic on the work cursor
     const WorkItem* items;
     const PartWork* parts;  // null: first pass, one work entry per item
     uint32_t n_work;  // entries of this pass
@@ -619,13 +621,23 @@ struct Fetched {
 };
 
 // Thread 0's pipeline of work entries: the next entry's metadata is loaded
-// in three dependent steps (part entry, work item, unit offsets) spread over
+// in two dependent steps (part entry, work item with its slot range) spread over
 // the current entry's phases, and the entry after it is already claimed, so
-// no barrier waits on a chain of global loads.
+// no barrier waits on a chain of global loads.  Entries are claimed from the
+// global cursor `claim` at a time: one atomic on that single address per
+// batch instead of per entry (the address serialises its atomics).
 struct Pipe {
     Fetched n1;  // next entry
     uint32_t e2;  // the entry after it (claimed)
+    uint32_t bat, bat_end;  // claimed batch [bat, bat_end)
 };
+GG_DEV uint32_t pipe_claim(const MergeArgs& A, Pipe& p) {
+    if (p.bat >= p.bat_end) {
+        p.bat = atomicAdd(A.work, A.claim);
+        p.bat_end = p.bat + A.claim;
+    }
+    return p.bat < A.n_work ? p.bat++ : A.n_work;
+}
 GG_DEV void pipe_part(const MergeArgs& A, Pipe& p) {
     if (p.n1.e < A.n_work) p.n1.w = A.parts ? A.parts[p.n1.e] : PartWork{p.n1.e, 0u, 0u, 0u};
 }
@@ -634,8 +646,8 @@ GG_DEV void pipe_item(const MergeArgs& A, Pipe& p) {
 }
 GG_DEV void pipe_units(const MergeArgs& A, Pipe& p) {
     if (p.n1.e < A.n_work) {
-        p.n1.r0 = A.unit_off[p.n1.it.u0];
-        p.n1.r1 = A.unit_off[p.n1.it.u1];
+        p.n1.r0 = p.n1.it.r0;
+        p.n1.r1 = p.n1.it.r1;
     }
 }
 
@@ -668,21 +680,40 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
         const RecTable R = rec_table(A, lds);
         for (uint32_t i = threadIdx.x; i < A.rec_cap; i += blockDim.x) R.state[i] = 0;
     }
+    // each thread's record of the current chunk (dedup path)
+    const uint32_t chunk = A.rec_cap / 2, SW = A.slot_words;
+    auto load_rec = [&](uint64_t r, uint64_t r_end, uint32_t* wv, uint32_t& col) {
+        if (threadIdx.x < chunk && r < r_end) {
+            const uint32_t* sp = A.slots + r * SW;
+            if (SW == 4) {
+                const uint4 v = *reinterpret_cast<const uint4*>(sp);
+                wv[0] = v.x;
+                wv[1] = v.y;
+                wv[2] = v.z;
+                wv[3] = v.w;
+            } else {
+                for (uint32_t q = 0; q < SW; q++) wv[q] = sp[q];
+            }
+            col = W ? A.slot_color[r] : 0u;
+        }
+    };
+    uint32_t cw[8] = {0, 0, 0, 0, 0, 0, 0, 0}, ccol = 0;
     Pipe pp{};
     if (leader) {
         s_claimed = 0;
-        pp.n1.e = atomicAdd(A.work, 1u);
+        pp.bat = pp.bat_end = 0;
+        pp.n1.e = pipe_claim(A, pp);
         pipe_part(A, pp);
         pipe_item(A, pp);
         pipe_units(A, pp);
-        pp.e2 = pp.n1.e < A.n_work ? atomicAdd(A.work, 1u) : A.n_work;
+        pp.e2 = pp.n1.e < A.n_work ? pipe_claim(A, pp) : A.n_work;
     }
     while (true) {
         if (leader) {
             s_cur = pp.n1;
             s_overflow = 0;
             pp.n1.e = pp.e2;
-            pp.e2 = (pp.e2 < A.n_work) ? atomicAdd(A.work, 1u) : A.n_work;
+            pp.e2 = (pp.e2 < A.n_work) ? pipe_claim(A, pp) : A.n_work;
             pipe_part(A, pp);
         }
         __syncthreads();
@@ -693,7 +724,7 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
         const WorkItem it = cur.it;
         if (!A.parts && it.occ > A.lds_try) {
             if (leader) {
-                A.redo[atomicAdd(A.redo_count, 1u)] = w;
+                A.redo[atomicAdd(A.redo_count, 1u)] = PartWork{w.item, w.part, w.log_parts, it.occ};
                 pipe_item(A, pp);
                 pipe_units(A, pp);
             }
@@ -710,28 +741,11 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
             // records of one chunk are counted once, the k-mer table
             // accumulates over the chunks.  Each thread's record of the next
             // chunk is loaded while the current chunk is processed.
-            const uint32_t chunk = A.rec_cap / 2, SW = A.slot_words;
-            auto load_rec = [&](uint64_t r, uint32_t* wv, uint32_t& col) {
-                if (threadIdx.x < chunk && r < cur.r1) {
-                    const uint32_t* sp = A.slots + r * SW;
-                    if (SW == 4) {
-                        const uint4 v = *reinterpret_cast<const uint4*>(sp);
-                        wv[0] = v.x;
-                        wv[1] = v.y;
-                        wv[2] = v.z;
-                        wv[3] = v.w;
-                    } else {
-                        for (uint32_t q = 0; q < SW; q++) wv[q] = sp[q];
-                    }
-                    col = W ? A.slot_color[r] : 0u;
-                }
-            };
-            uint32_t cw[8] = {0, 0, 0, 0, 0, 0, 0, 0}, ccol = 0;
-            load_rec(cur.r0 + threadIdx.x, cw, ccol);
+            load_rec(cur.r0 + threadIdx.x, cur.r1, cw, ccol);
             for (uint64_t c0 = cur.r0; c0 < cur.r1; c0 += chunk) {
                 const uint64_t c1 = c0 + chunk < cur.r1 ? c0 + chunk : cur.r1;
                 uint32_t nw[8] = {0, 0, 0, 0, 0, 0, 0, 0}, ncol = 0;
-                load_rec(c1 + threadIdx.x, nw, ncol);
+                load_rec(c1 + threadIdx.x, cur.r1, nw, ncol);
                 insert_item_dedup(A, c0, c1, w, klo, khi, st, vals, cbits, mask, 96u, &s_overflow, claimed,
                                   &s_claimed, rec_table(A, lds), cw, ccol);
 #pragma unroll
@@ -747,7 +761,7 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
         const uint32_t n_cl = s_claimed;
         if (s_overflow) {
             // more distinct k-mers than the table holds: redone as finer parts
-            if (leader) A.redo[atomicAdd(A.redo_count, 1u)] = w;
+            if (leader) A.redo[atomicAdd(A.redo_count, 1u)] = PartWork{w.item, w.part, w.log_parts, it.occ};
         } else {
             emit_solid(A, it.sub, klo, khi, st, vals, cbits, claimed, n_cl, es);
         }
@@ -767,9 +781,10 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
 }
 
 // One thread per sub-bucket: runs of its units become work items of at most
-// `group_occ` occurrences (a single unit above that is an item of its own).
-__global__ void make_items_kernel(const uint32_t* unit_kmers, uint32_t n_sub, uint32_t xb, uint32_t group_occ,
-                                  WorkItem* items, unsigned int* n_items) {
+// `group_occ` occurrences (a single unit above that is an item of its own),
+// each with its slot range.
+__global__ void make_items_kernel(const uint32_t* unit_kmers, const uint64_t* unit_off, uint32_t n_sub, uint32_t xb,
+                                  uint32_t group_occ, WorkItem* items, unsigned int* n_items) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n_sub) return;
     const uint32_t u_begin = s << xb, u_end = (s + 1) << xb;
@@ -781,13 +796,13 @@ __global__ void make_items_kernel(const uint32_t* unit_kmers, uint32_t n_sub, ui
             continue;
         }
         if (acc > 0 && acc + o > group_occ) {
-            items[atomicAdd(n_items, 1u)] = WorkItem{start, u, acc, s};
+            items[atomicAdd(n_items, 1u)] = WorkItem{start, u, acc, s, unit_off[start], unit_off[u]};
             start = u;
             acc = 0;
         }
         acc += o;
     }
-    if (acc > 0) items[atomicAdd(n_items, 1u)] = WorkItem{start, u_end, acc, s};
+    if (acc > 0) items[atomicAdd(n_items, 1u)] = WorkItem{start, u_end, acc, s, unit_off[start], unit_off[u_end]};
 }
 
 }  // namespace gg
