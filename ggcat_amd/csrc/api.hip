@@ -43,7 +43,7 @@ struct ggcat_gpu_ctx {
     // bucketing buffers
     DevBuf input, seg_bits, unit_count, unit_cursor, unit_off, unit_kmers, scan_part, slots, pool, pool_cursor,
         slot_color, color_starts, colors, error_flag, stage_slots, stage_unit, stage_rank, stage_color, stage_cursor, defer,
-        defer_count;
+        defer_count, kmer_total;
     // merge buffers
     DevBuf items, n_items, work, parts, redo, redo_count, out_key, out_val, out_sub, chunk_cursor, chunk_fill_dev,
         fix_key, fix_val, fix_sub, fix_cursor, out_colors, fix_colors;
@@ -511,10 +511,12 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     const TileGeom G = tile_geom(T, w, p.k, p.m, regs16);
     ENSURE(ctx->stage_cursor, 8);
     ENSURE(ctx->defer_count, 8);
+    ENSURE(ctx->kmer_total, 8);
     ENSURE(ctx->defer, std::max<uint64_t>(n_tiles, 1) * 24);
     A.stage_cursor = P_<unsigned long long>(ctx->stage_cursor);
     A.defer = P_<uint64_t>(ctx->defer);
     A.defer_count = P_<uint32_t>(ctx->defer_count);
+    A.kmer_total = P_<unsigned long long>(ctx->kmer_total);
     ENSURE(ctx->scan_part, (((size_t)n_units + SCAN_CHUNK - 1) / SCAN_CHUNK + 1) * 8);
 
     // Pass 1: clear the unit counts, k-mer counts, group-start bitmap and
@@ -545,7 +547,7 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
             hipLaunchKernelGGL(bucket_defer_kernel, dim3(small_grid), dim3(256), 0, ctx->stream, A);
         }
         hipLaunchKernelGGL(unit_split_kernel, dim3(std::min<uint32_t>((n_units + 255) / 256, 4096)), dim3(256), 0,
-                           ctx->stream, A.unit_ck, A.sub_count, A.sub_kmers, n_units);
+                           ctx->stream, A.unit_ck, A.sub_count, A.sub_kmers, n_units, A.kmer_total);
     };
     uint64_t staged = 0, pool_used = 0;
     uint32_t errf = 0;
@@ -609,8 +611,9 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     const int rs = device_scan(ctx, A.sub_count, P_<uint64_t>(ctx->unit_off), n_units);
     if (rs != GGCAT_GPU_OK) return rs;
     HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
-    uint64_t total = 0;
+    uint64_t total = 0, kmer_total = 0;
     HIPCHK(hipMemcpyAsync(&total, P_<uint64_t>(ctx->unit_off) + n_units, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(&kmer_total, A.kmer_total, 8, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->n_records = total;
     ENSURE(ctx->slots, std::max<uint64_t>(total, 1) * slot_words * 4);
@@ -637,12 +640,7 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     HIPCHK(hipEventElapsedTime(&t, ctx->ev[3], ctx->ev[4]));
     ctx->ms[2] = t;
     ctx->ms[5] = ctx->ms[0] + ctx->ms[1] + ctx->ms[2];
-    // k-mer occurrence total (sizes the merge output)
-    std::vector<uint32_t> occ(n_units);
-    HIPCHK(hipMemcpy(occ.data(), ctx->unit_kmers.p, (size_t)n_units * 4, hipMemcpyDeviceToHost));
-    uint64_t s = 0;
-    for (uint32_t v : occ) s += v;
-    ctx->kmer_occ = s;
+    ctx->kmer_occ = kmer_total;  // sizes the merge output
     ctx->have_records = true;
     return GGCAT_GPU_OK;
 }

@@ -63,6 +63,7 @@ struct BucketArgs {
     uint32_t tile_cap;
     uint64_t* defer;  // [n_tiles * 3] records whose group starts before the tile: end window, minimizer, last flag
     unsigned int* defer_count;
+    unsigned long long* kmer_total;  // k-mer occurrences of all records
     const uint64_t* sub_off;  // [n_units+1] slot offsets
     uint32_t* slots;
     uint32_t slot_words;
@@ -875,11 +876,24 @@ __global__ void bucket_defer_kernel(BucketArgs A) {
 }
 
 // Splits the packed per-unit counts into records and k-mer occurrences.
-__global__ void unit_split_kernel(const unsigned long long* ck, uint32_t* count, uint32_t* kmers, uint32_t n) {
+// Also sums the k-mer occurrences into *total (cleared by the reset kernel;
+// blocks of 256 threads).
+__global__ __launch_bounds__(256) void unit_split_kernel(const unsigned long long* ck, uint32_t* count, uint32_t* kmers,
+                                                        uint32_t n, unsigned long long* total) {
+    __shared__ unsigned long long part[4];
+    unsigned long long sum = 0;
     for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
         const unsigned long long c = ck[u];
         count[u] = (uint32_t)(c >> 32);
         kmers[u] = (uint32_t)c;
+        sum += (uint32_t)c;
+    }
+    for (uint32_t d = 32; d; d >>= 1) sum += __shfl_down(sum, d);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long t = part[0] + part[1] + part[2] + part[3];
+        if (t) atomicAdd(total, t);
     }
 }
 
@@ -888,6 +902,7 @@ __global__ void bucket_reset_kernel(BucketArgs A) {
     if (threadIdx.x == 0) {
         *A.stage_cursor = 0;
         *A.pool_cursor = 0;
+        *A.kmer_total = 0;
         *A.defer_count = 0;
         *A.error_flag = 0;
     }

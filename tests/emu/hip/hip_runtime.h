@@ -91,6 +91,17 @@ template <class T> inline T __shfl(T v, int src) {
     emu::wbar[w]->arrive_and_wait();
     return r;
 }
+template <class T> inline T __shfl_down(T v, unsigned d) {
+    unsigned w = threadIdx.x / 64, l = threadIdx.x % 64;
+    static unsigned long long slots[16][64];
+    emu::wbar[w]->arrive_and_wait();
+    std::memcpy(&slots[w][l], &v, sizeof(T));
+    emu::wbar[w]->arrive_and_wait();
+    T r = v;
+    if (l + d < 64) std::memcpy(&r, &slots[w][l + d], sizeof(T));
+    emu::wbar[w]->arrive_and_wait();
+    return r;
+}
 template <class T> inline T __shfl_up(T v, unsigned d) {
     static thread_local std::vector<unsigned long long>* dummy = nullptr;
     (void)dummy;
