@@ -329,52 +329,55 @@ GG_DEV void tile_minima_w16(Tile& t) {
     const uint32_t p0 = q * 16;
     uint32_t segm = 0, okm = 0, lastm = 0;
     bool need_prev = false;
-    if (have) {
+    // every lane q <= NWB hashes m-mer block q once; lane q takes block q+1
+    // from lane q+1 by a lane shift instead of hashing it again
+    uint64_t sv[16], nv[16];
+    uint32_t sp[16];
+    uint32_t fbits = 0;
+    if (q <= G.NWB) {
         uint64_t fw = 0, rc = 0;
         for (uint32_t j = 0; j + 1 < m; j++) {
             const ulonglong2 g = gt2[16 + t.code_rel(rb + p0 + j)];
             fw = rotl1_64(fw) ^ g.x;
             rc = rotr1_64(rc) ^ g.y;
         }
-        uint64_t sv[16];
-        uint32_t sp[16];
-        uint32_t fbits = 0;
-        {
-            const uint32_t inw = extract16(t, rb + p0 + m - 1);
-            const uint32_t outw = extract16(t, rb + p0 - 1);
+        const uint32_t inw = extract16(t, rb + p0 + m - 1);
+        const uint32_t outw = extract16(t, rb + p0 - 1);
 #pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const uint32_t in = (inw >> (2 * i)) & 3u, out = (outw >> (2 * i)) & 3u;
-                const ulonglong2 g = gt2[(i == 0 ? 16u : out * 4u) + in];
-                fw = rotl1_64(fw) ^ g.x;
-                rc = rotr1_64(rc) ^ g.y;
-                const uint64_t mn = fw < rc ? fw : rc;
-                sv[i] = (mn << 1) | (fw != rc ? 1ULL : 0ULL);
-                sp[i] = p0 + i;
-                fbits |= (fw < rc ? 1u : 0u) << i;
-            }
+        for (int i = 0; i < 16; i++) {
+            const uint32_t in = (inw >> (2 * i)) & 3u, out = (outw >> (2 * i)) & 3u;
+            const ulonglong2 g = gt2[(i == 0 ? 16u : out * 4u) + in];
+            fw = rotl1_64(fw) ^ g.x;
+            rc = rotr1_64(rc) ^ g.y;
+            const uint64_t mn = fw < rc ? fw : rc;
+            sv[i] = (mn << 1) | (fw != rc ? 1ULL : 0ULL);
+            sp[i] = p0 + i;
+            fbits |= (fw < rc ? 1u : 0u) << i;
         }
+#pragma unroll
+        for (int i = 0; i < 16; i++) t.FWD(p0 + i) = (fbits >> i) & 1u;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            sv[i] = 0;
+            sp[i] = 0;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++) nv[i] = __shfl_down(sv[i], 1);  // every lane takes part
+    if (have) {
 #pragma unroll
         for (int i = 14; i >= 0; i--) mm_combine(sv[i], sp[i], sv[i + 1], sp[i + 1], sv[i], sp[i]);
         // invalid bits of relative bases rb + p0 - 1 .. rb + p0 + 62
         const uint64_t im = bits64(t.inv, rb + p0 - 1);
         const uint64_t kmask = (1ULL << (k - 1)) - 1;
-        uint32_t fbits1 = 0;
         {
             const uint32_t p1 = p0 + 16;
-            const uint32_t inw = extract16(t, rb + p1 + m - 1);
-            const uint32_t outw = extract16(t, rb + p1 - 1);
             uint64_t pv = 0, prv = 0;
             uint32_t pp = 0, prp = 0;
 #pragma unroll
             for (int i = 0; i < 16; i++) {
-                const uint32_t in = (inw >> (2 * i)) & 3u, out = (outw >> (2 * i)) & 3u;
-                const ulonglong2 g = gt2[out * 4u + in];
-                fw = rotl1_64(fw) ^ g.x;
-                rc = rotr1_64(rc) ^ g.y;
-                const uint64_t mn = fw < rc ? fw : rc;
-                const uint64_t v = (mn << 1) | (fw != rc ? 1ULL : 0ULL);
-                fbits1 |= (fw < rc ? 1u : 0u) << i;
+                const uint64_t v = nv[i];  // m-mer p1 + i
                 uint64_t rv = sv[i];
                 uint32_t rp = sp[i];
                 if (i > 0) mm_combine(sv[i], sp[i], pv, pp, rv, rp);
@@ -408,12 +411,6 @@ GG_DEV void tile_minima_w16(Tile& t) {
                     mm_combine(pv, pp, v, p1 + i, pv, pp);
                 }
             }
-        }
-#pragma unroll
-        for (int i = 0; i < 16; i++) t.FWD(p0 + i) = (fbits >> i) & 1u;
-        if (q + 1 == G.NWB) {
-#pragma unroll
-            for (int i = 0; i < 16; i++) t.FWD(p0 + 16 + i) = (fbits1 >> i) & 1u;
         }
     }
     __syncthreads();
