@@ -21,7 +21,8 @@ import numpy as np
 from .native import Params
 
 KNOBS = ("item_occ", "lds_max_occ", "merge_wgs_per_cu", "part_occ", "lds_table_log", "table_pct", "no_dedup",
-         "unit_bytes", "use_graphs", "stage_tile_cap", "rec_cap")
+         "unit_bytes", "use_graphs", "stage_tile_cap", "rec_cap",
+         "merge_threads")
 
 
 @dataclass
@@ -34,11 +35,12 @@ class Schedule:
     use_graphs: int = 0
     rec_cap: int = 0
     stage_tile_cap: int = 0
+    merge_threads: int = 0
 
     def apply(self, ctx) -> None:
         ctx.set_tuning(unit_bytes=self.unit_bytes, table_pct=self.table_pct, lds_table_log=self.lds_table_log,
                        no_dedup=self.no_dedup, merge_wgs_per_cu=self.merge_wgs_per_cu, use_graphs=self.use_graphs,
-                       rec_cap=self.rec_cap, stage_tile_cap=self.stage_tile_cap)
+                       rec_cap=self.rec_cap, stage_tile_cap=self.stage_tile_cap, merge_threads=self.merge_threads)
 
 
 def default_grid(k: int) -> list[Schedule]:
@@ -49,6 +51,9 @@ def default_grid(k: int) -> list[Schedule]:
     out = []
     for ub, rc, lg in itertools.product((1400, 2000, 3000), (256, 512), logs):
         out.append(Schedule(unit_bytes=ub, table_pct=200, lds_table_log=lg, rec_cap=rc, use_graphs=1))
+    # smaller workgroups with smaller tables over smaller units
+    out.append(Schedule(unit_bytes=1000, table_pct=200, lds_table_log=lg - 1, rec_cap=128, use_graphs=1,
+                        merge_threads=128))
     return out
 
 

@@ -728,7 +728,10 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     const uint32_t item_occ = ctx->tune.item_occ ? ctx->tune.item_occ : M.lds_cap / 2;
     // identical records counted once (seq-hash keys up to k = 32, inline slots)
     M.dedup = (p.hash_kind == GGCAT_GPU_HASH_SEQ && p.k <= 32 && ctx->slot_words <= 8 && !ctx->tune.no_dedup) ? 1 : 0;
-    M.rec_cap = ctx->tune.rec_cap ? ctx->tune.rec_cap : REC_CAP_DEFAULT;
+    // workgroup size of the merge (128 or 256 threads); a chunk of records
+    // (rec_cap / 2) has at most one record per thread
+    const uint32_t merge_threads = ctx->tune.merge_threads == 128 ? 128u : 256u;
+    M.rec_cap = std::min<uint32_t>(ctx->tune.rec_cap ? ctx->tune.rec_cap : REC_CAP_DEFAULT, 2 * merge_threads);
     M.rec_lds_off = (uint32_t)(((size_t)M.lds_cap * entry_bytes + 64 + 15) / 16 * 16);
     const size_t lds_bytes = M.dedup ? M.rec_lds_off + rec_table_bytes(ctx->slot_words, p.k, M.rec_cap) : (size_t)M.lds_cap * entry_bytes + 64;
     M.pool_words = ctx->pool_words;
@@ -771,7 +774,8 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     hipFuncAttributes fa{};
     HIPCHK(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(merge_lds_kernel)));
     const size_t wg_lds = (lds_bytes + fa.sharedSizeBytes + 511) / 512 * 512;  // allocation granule
-    const uint32_t fit = (uint32_t)std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / wg_lds));
+    const uint32_t fit =
+        (uint32_t)std::max<size_t>(1, std::min<size_t>(2048 / merge_threads, (160 * 1024) / wg_lds));
     const uint32_t lds_wgs = ctx->n_cu * (ctx->tune.merge_wgs_per_cu ? ctx->tune.merge_wgs_per_cu : fit);
     // every wavefront keeps one partly filled chunk open
     uint64_t n_chunks = std::max<uint64_t>((ctx->kmer_occ / 6) / OUT_CHUNK + 8 * lds_wgs + 4096 + 64,
@@ -828,7 +832,7 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
             HIPCHK(hipMemsetAsync(ctx->work.p, 0, 8, ctx->stream));
             HIPCHK(hipMemsetAsync(ctx->redo_count.p, 0, 8, ctx->stream));
             if (pass) HIPCHK(hipEventRecord(ctx->ev[8], ctx->stream));
-            hipLaunchKernelGGL(merge_lds_kernel, dim3(lds_wgs), dim3(256), lds_bytes, ctx->stream, M);
+            hipLaunchKernelGGL(merge_lds_kernel, dim3(lds_wgs), dim3(merge_threads), lds_bytes, ctx->stream, M);
             HIPCHK(hipGetLastError());
             n_passes_run++;
             HIPCHK(hipEventRecord(ctx->ev[pass ? 9 : 7], ctx->stream));
@@ -1023,6 +1027,8 @@ int ggcat_gpu_solid_colors(ggcat_gpu_ctx* ctx, uint32_t* color_words, uint32_t* 
 int ggcat_gpu_set_tuning(ggcat_gpu_ctx* ctx, const ggcat_gpu_tuning* t) {
     if (!ctx || !t) return GGCAT_GPU_EINVAL;
     if (t->merge_wgs_per_cu > 64) return fail(ctx, GGCAT_GPU_EINVAL, "merge_wgs_per_cu must be <= 64");
+    if (t->merge_threads && t->merge_threads != 128 && t->merge_threads != 256)
+        return fail(ctx, GGCAT_GPU_EINVAL, "merge_threads must be 0, 128 or 256");
     if (t->rec_cap && (t->rec_cap < 16 || t->rec_cap > 512 || (t->rec_cap & (t->rec_cap - 1))))
         return fail(ctx, GGCAT_GPU_EINVAL, "rec_cap must be 0 or a power of two in [16, 512]");
     ctx->tune = *t;

@@ -154,7 +154,8 @@ int ggcat_gpu_solid_colors(ggcat_gpu_ctx* ctx, uint32_t* color_words, uint32_t* 
  *                     records per w+1 windows); the result never depends on it
  *   rec_cap           merge: identical-record table slots (power of two in
  *                     [16, 512], default 512); records are deduplicated in
- *                     chunks of rec_cap / 2 */
+ *                     chunks of rec_cap / 2 (at most one record per thread)
+ *   merge_threads     threads per merge workgroup: 128 or 256 (default) */
 typedef struct {
     uint32_t item_occ;
     uint32_t lds_max_occ;
@@ -167,6 +168,7 @@ typedef struct {
     uint32_t use_graphs;
     uint32_t stage_tile_cap;
     uint32_t rec_cap;
+    uint32_t merge_threads;
 } ggcat_gpu_tuning;
 int ggcat_gpu_set_tuning(ggcat_gpu_ctx* ctx, const ggcat_gpu_tuning* tuning);
 
