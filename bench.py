@@ -73,6 +73,21 @@ def measured_traffic(workload: str, scale: float, kernel: str):
     return round(t["read_gb"] + t["write_gb"], 3), "profiles/hbm_traffic_config2.md (rocprofv3 --pmc, GB per launch)"
 
 
+def kernel_rooflines(kb: dict, kern_ms: dict, workload: str, scale: float) -> dict:
+    """Every timed kernel against the HBM peak: algorithmic bytes over its
+    HIP-event time, and the counter-measured bytes over the same time."""
+    out = {}
+    for name, ms in kern_ms.items():
+        alg = kb[name] / (ms * 1e-3) / 1e9
+        traffic, _ = measured_traffic(workload, scale, name)
+        out[name] = {"ms": round(float(ms), 3), "algorithmic_gb": round(kb[name] / 1e9, 3),
+                     "achieved_gbs": round(alg, 1), "frac": round(alg / HBM_PEAK_GBS, 4),
+                     "traffic_gb": traffic,
+                     "traffic_gbs": round(traffic / (ms * 1e-3), 1) if traffic else None,
+                     "traffic_frac": round(traffic / (ms * 1e-3) / HBM_PEAK_GBS, 4) if traffic else None}
+    return out
+
+
 def cpu_baseline(wl, threads: int) -> dict:
     """The oracle restatement on a bounded prefix of the same reads."""
     from oracle import binding as O
@@ -263,6 +278,7 @@ def main() -> None:
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_gb": round(kb[dom] / 1e9, 3), "traffic_source": traffic_src},
         "kernels_ms": {k: round(float(v), 3) for k, v in kern_ms.items()},
+        "kernels_roofline": kernel_rooflines(kb, kern_ms, wl.name, a.scale),
         "counters": {"records": int(cnt[1]), "record_bases": int(cnt[2]), "kmer_occurrences": int(cnt[3]),
                      "hbm_table_subbuckets": int(cnt[4]), "solid_entries": int(n)},
         "h2d_ms": round(h2d_s * 1e3, 1),
