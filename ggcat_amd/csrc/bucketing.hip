@@ -354,8 +354,13 @@ GG_DEV void tile_minima_w16(Tile& t) {
             sp[i] = p0 + i;
             fbits |= (fw < rc ? 1u : 0u) << i;
         }
+        // 16 flag bytes as four 32-bit stores (FWD(p0) is 4-byte aligned)
+        uint32_t* f4 = reinterpret_cast<uint32_t*>(&t.FWD(p0));
 #pragma unroll
-        for (int i = 0; i < 16; i++) t.FWD(p0 + i) = (fbits >> i) & 1u;
+        for (int g = 0; g < 4; g++) {
+            const uint32_t b = fbits >> (4 * g);
+            f4[g] = (b & 1u) | ((b >> 1) & 1u) << 8 | ((b >> 2) & 1u) << 16 | ((b >> 3) & 1u) << 24;
+        }
     } else {
 #pragma unroll
         for (int i = 0; i < 16; i++) {
@@ -374,7 +379,7 @@ GG_DEV void tile_minima_w16(Tile& t) {
         {
             const uint32_t p1 = p0 + 16;
             uint64_t pv = 0, prv = 0;
-            uint32_t pp = 0, prp = 0;
+            uint32_t pp = 0, prp = 0, prp_pair = 0;
 #pragma unroll
             for (int i = 0; i < 16; i++) {
                 const uint64_t v = nv[i];  // m-mer p1 + i
@@ -382,7 +387,9 @@ GG_DEV void tile_minima_w16(Tile& t) {
                 uint32_t rp = sp[i];
                 if (i > 0) mm_combine(sv[i], sp[i], pv, pp, rv, rp);
                 t.WV(p0 + i) = rv;
-                t.WP(p0 + i) = (uint16_t)rp;
+                // positions in pairs as 32-bit stores (WP(p0) is 4-byte aligned)
+                if (i & 1) reinterpret_cast<uint32_t*>(&t.WP(p0))[i >> 1] = (prp_pair & 0xFFFFu) | (rp << 16);
+                prp_pair = rp;
                 // flags of window x = p0 + i (bases rb + x .. rb + x + k - 2)
                 const uint32_t x = p0 + i;
                 if (x >= 1 && x < G.NW) {

@@ -107,6 +107,14 @@ def test_config1_medium_digest(gpu_ctx):
     _run_both(gpu_ctx, wl.data, 31, 15, wl.buckets_log, wl.second_buckets_log, exact=False)
 
 
+def test_config2_shape_digest(gpu_ctx):
+    # the benchmark's read set (250 bp at 40x, 1% errors, 20% repeats) at 1/100
+    # of its size, default schedule, compared with the oracle in full
+    wl = synth.config2(0.01)
+    _say(f"case {wl.name} bytes={wl.data.size}")
+    _run_both(gpu_ctx, wl.data, 31, 15, wl.buckets_log, wl.second_buckets_log, s=2, exact=False)
+
+
 @pytest.mark.parametrize("b,sb,s", [(3, 1, 2), (2, 1, 1)])
 def test_cross_unit_fixups(gpu_ctx, b, sb, s):
     # few sub-buckets for the input size: several physical units per
