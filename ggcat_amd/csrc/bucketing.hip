@@ -369,7 +369,7 @@ GG_DEV void tile_minima_w16(Tile& t) {
         }
     }
 #pragma unroll
-    for (int i = 0; i < 16; i++) nv[i] = __shfl_down(sv[i], 1);  // every lane takes part
+    for (int i = 0; i < 16; i++) nv[i] = lane_next64(sv[i]);  // every lane takes part
     if (have) {
 #pragma unroll
         for (int i = 14; i >= 0; i--) mm_combine(sv[i], sp[i], sv[i + 1], sp[i + 1], sv[i], sp[i]);

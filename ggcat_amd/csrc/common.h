@@ -53,6 +53,15 @@ GG_DEV void mm_combine(uint64_t av, uint32_t ap, uint64_t bv, uint32_t bp, uint6
 }
 
 // Multiplicative hash for LDS/global open-addressing tables.
+// Value of the next lane (lane + 1) of the wavefront: a DPP wavefront shift
+// (wave_shl:1); the last lane gets 0.
+GG_DEV uint32_t lane_next32(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x130, 0xF, 0xF, false);
+}
+GG_DEV uint64_t lane_next64(uint64_t x) {
+    return (uint64_t)lane_next32((uint32_t)x) | ((uint64_t)lane_next32((uint32_t)(x >> 32)) << 32);
+}
+
 GG_DEV uint32_t tab_hash64(uint64_t x) {
     x ^= x >> 29;
     x *= 0xbf58476d1ce4e5b9ULL;

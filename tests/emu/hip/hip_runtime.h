@@ -91,6 +91,19 @@ template <class T> inline T __shfl(T v, int src) {
     emu::wbar[w]->arrive_and_wait();
     return r;
 }
+// DPP move; only wave_shl:1 (0x130: each lane reads lane + 1, the last lane
+// gets `old`) is emulated
+inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int, int, bool) {
+    unsigned w = threadIdx.x / 64, l = threadIdx.x % 64;
+    static int slots[16][64];
+    if (ctrl != 0x130) std::abort();
+    emu::wbar[w]->arrive_and_wait();
+    slots[w][l] = src;
+    emu::wbar[w]->arrive_and_wait();
+    const int r = (l + 1 < 64) ? slots[w][l + 1] : old;
+    emu::wbar[w]->arrive_and_wait();
+    return r;
+}
 template <class T> inline T __shfl_down(T v, unsigned d) {
     unsigned w = threadIdx.x / 64, l = threadIdx.x % 64;
     static unsigned long long slots[16][64];
