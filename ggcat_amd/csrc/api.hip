@@ -188,6 +188,10 @@ void set_rk_constants(Args& M, uint32_t k) {
     for (uint32_t e = k - 1; e; e >>= 1, b *= b)
         if (e & 1) r *= b;
     M.rk_rmmult = r;
+    for (uint32_t c = 0; c < 4; c++) {
+        M.rk_fwd_rm[c] = M.rk_fwd[c] * r;
+        M.rk_bkw_rm[c] = M.rk_bkw[c] * r;
+    }
 }
 
 // Query records of a joined buffer: start offsets [n + 1] (entry q + 1 is one
@@ -735,19 +739,7 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     M.rec_lds_off = (uint32_t)(((size_t)M.lds_cap * entry_bytes + 64 + 15) / 16 * 16);
     const size_t lds_bytes = M.dedup ? M.rec_lds_off + rec_table_bytes(ctx->slot_words, p.k, M.rec_cap) : (size_t)M.lds_cap * entry_bytes + 64;
     M.pool_words = ctx->pool_words;
-    // rabin-karp128 constants (crates/hashes/src/cn_rkhash.rs:65-78)
-    M.rk_mult = mk128(0x3eb9402f3e733993ULL, 0xadd64d3ca00e1b6bULL);
-    M.rk_mult_inv = mk128(0x09cb6ff6f1b1a6d7ULL, 0x33e0952e899c3943ULL);
-    const u128_t A_ = mk128(0x4751137d01d863c5ULL, 0xb8c36de2b7d399dfULL);
-    const u128_t C_ = mk128(0x37ea3a13226503fbULL, 0x783f5cb69f4552bdULL);
-    const u128_t G_ = mk128(0x50796b285343f09aULL, 0x0c53113ae736572bULL);
-    const u128_t T_ = mk128(0x1e62d96a5e1f5adeULL, 0x2d4e68d8f88110b7ULL);
-    M.rk_fwd[0] = A_; M.rk_fwd[1] = C_; M.rk_fwd[2] = T_; M.rk_fwd[3] = G_;
-    M.rk_bkw[0] = T_; M.rk_bkw[1] = G_; M.rk_bkw[2] = A_; M.rk_bkw[3] = C_;
-    u128_t r = 1, b = M.rk_mult;
-    for (uint32_t e = p.k - 1; e; e >>= 1, b *= b)
-        if (e & 1) r *= b;
-    M.rk_rmmult = r;
+    set_rk_constants(M, p.k);
 
     // work items: runs of units of one sub-bucket
     ENSURE(ctx->items, (size_t)ctx->n_units * sizeof(WorkItem));

@@ -91,6 +91,7 @@ struct MergeArgs {
     // rabin-karp128 constants (crates/hashes/src/cn_rkhash.rs:65-78)
     u128_t rk_mult, rk_mult_inv, rk_rmmult;
     u128_t rk_fwd[4], rk_bkw[4];
+    u128_t rk_fwd_rm[4], rk_bkw_rm[4];  // rk_fwd[c] * rk_rmmult, rk_bkw[c] * rk_rmmult
 };
 
 GG_DEV uint32_t hash128(uint64_t lo, uint64_t hi) { return tab_hash64(lo ^ (hi * 0x9E3779B97F4A7C15ULL)); }
@@ -135,19 +136,19 @@ GG_DEV uint32_t tab_insert128(KP klo, KP khi, SP st, VP vals, uint32_t mask, uin
     uint32_t claimed = 0, probes = 0, spins = 0;
     bool done = false;
     while (!done) {
-        uint32_t s = __hip_atomic_load((uint32_t*)&st[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t s = __hip_atomic_load((uint32_t*)&st[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         if (s == 0) {
             if (atomicCAS((uint32_t*)&st[i], 0u, 1u) == 0u) {
                 klo[i] = lo;
                 khi[i] = hi;
-                __threadfence();
+                __threadfence_block();  // the table is in LDS: workgroup scope
                 atomicExch((uint32_t*)&st[i], 2u);
                 claimed = 1;
                 done = true;
             }
         } else if (s == 2) {
-            uint64_t a = __hip_atomic_load((uint64_t*)&klo[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            uint64_t b = __hip_atomic_load((uint64_t*)&khi[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint64_t a = __hip_atomic_load((uint64_t*)&klo[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            uint64_t b = __hip_atomic_load((uint64_t*)&khi[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             if (a == lo && b == hi) {
                 done = true;
             } else {
@@ -224,8 +225,8 @@ GG_DEV void for_each_kmer(const MergeArgs& A, const uint32_t* slot, CB&& cb) {
         for (uint32_t idx = k - 1; idx < len; idx++) {
             uint32_t in = code(idx), out = code(idx - (k - 1));
             u128_t cur_f = fh * A.rk_mult + A.rk_fwd[in];
-            fh = cur_f - A.rk_fwd[out] * A.rk_rmmult;
-            u128_t cur_b = bk * A.rk_mult_inv + A.rk_bkw[in] * A.rk_rmmult;
+            fh = cur_f - A.rk_fwd_rm[out];
+            u128_t cur_b = bk * A.rk_mult_inv + A.rk_bkw_rm[in];
             bk = cur_b - A.rk_bkw[out];
             u128_t key = cur_f < cur_b ? cur_f : cur_b;
             cb((uint64_t)key, (uint64_t)(key >> 64), cur_f < cur_b, idx + 1 - k, last);

@@ -88,6 +88,7 @@ struct QueryArgs {
     unsigned int* work;
     u128_t rk_mult, rk_mult_inv, rk_rmmult;
     u128_t rk_fwd[4], rk_bkw[4];
+    u128_t rk_fwd_rm[4], rk_bkw_rm[4];  // rk_fwd[c] * rk_rmmult, rk_bkw[c] * rk_rmmult
 };
 
 constexpr uint32_t NO_POS = 0xFFFFFFFFu;
@@ -212,8 +213,8 @@ __global__ __launch_bounds__(256) void query_kernel(QueryArgs A) {
                     if (!ascii_acgt(s[idx])) last_bad = (int64_t)idx;
                     const uint32_t in = ascii_code(s[idx]), out = ascii_code(s[idx - (k - 1)]);
                     const u128_t cur_f = fh * A.rk_mult + A.rk_fwd[in];
-                    fh = cur_f - A.rk_fwd[out] * A.rk_rmmult;
-                    const u128_t cur_b = bk * A.rk_mult_inv + A.rk_bkw[in] * A.rk_rmmult;
+                    fh = cur_f - A.rk_fwd_rm[out];
+                    const u128_t cur_b = bk * A.rk_mult_inv + A.rk_bkw_rm[in];
                     bk = cur_b - A.rk_bkw[out];
                     if (last_bad < (int64_t)(idx + 1 - k)) {
                         const u128_t key = cur_f < cur_b ? cur_f : cur_b;
