@@ -285,6 +285,15 @@ def main() -> None:
         "tuning": tuning,
         "pcie_inclusive_gbases_s": round(wl.bases / (dt / a.steps + h2d_s) / 1e9, 4),
     }
+    # unitig chains inside the sub-buckets of the last step's graph (device
+    # kernels only; the sequences stay in HBM)
+    t = time.perf_counter()
+    ctx.index_build()
+    t_index = time.perf_counter() - t
+    t = time.perf_counter()
+    n_unitigs, unitig_bytes = ctx.unitig_counts()
+    out["unitigs"] = {"unitigs": n_unitigs, "bytes": unitig_bytes, "kernels_ms": round(float(ctx.timings()[7]), 2),
+                      "call_ms": round((time.perf_counter() - t) * 1e3, 1), "index_ms": round(t_index * 1e3, 1)}
     ctx.free(dptr)
     if not a.no_serve:
         out["query"] = serve_measurement(ctx, a.serve_rate, a.serve_requests)

@@ -324,6 +324,13 @@ class GpuContext:
         return out
 
     # ---- unitigs -------------------------------------------------------
+    def unitig_counts(self) -> tuple[int, int]:
+        """Builds the intra-sub-bucket unitigs on the device and returns
+        (unitigs, bytes) without copying them out."""
+        n, nb = C.c_uint64(), C.c_uint64()
+        self._check(self.lib.ggcat_gpu_unitigs(self.h, C.byref(n), C.byref(nb)), "ggcat_gpu_unitigs")
+        return int(n.value), int(nb.value)
+
     def unitigs(self):
         """Intra-sub-bucket unitigs: (list of ASCII sequences, sub-bucket per
         unitig, flags: bit0 open start, bit1 open end, bit2 circular)."""
