@@ -63,10 +63,17 @@ GG_DEV u128_t key_at(const UnitigArgs& A, uint64_t p) {
     return ((u128_t)hi << 64) | lo;
 }
 
+// Reverse complement of a k-mer (base j at bits 2j, A0 C1 T2 G3, so the
+// complement of a base is code ^ 2): complement all 64 positions, reverse
+// the 128 bits, swap the bits inside each 2-bit group back, and drop the
+// 64 - k positions above the k-mer, which the reversal put at the bottom.
 GG_DEV u128_t kmer_rc128(u128_t x, uint32_t k) {
-    u128_t r = 0;
-    for (uint32_t i = 0; i < k; i++) r |= (u128_t)((((uint32_t)(x >> (2 * i))) & 3u) ^ 2u) << (2 * (k - 1 - i));
-    return r;
+    constexpr uint64_t AA = 0xAAAAAAAAAAAAAAAAULL, M5 = 0x5555555555555555ULL;
+    uint64_t lo = __brevll((uint64_t)(x >> 64) ^ AA), hi = __brevll((uint64_t)x ^ AA);
+    lo = ((lo >> 1) & M5) | ((lo & M5) << 1);
+    hi = ((hi >> 1) & M5) | ((hi & M5) << 1);
+    const u128_t r = ((u128_t)hi << 64) | lo;
+    return r >> (128 - 2 * k);
 }
 
 GG_DEV bool side_flagged(const UnitigArgs& A, uint64_t p, uint32_t s) {
@@ -95,13 +102,15 @@ __global__ void ext_kernel(UnitigArgs A, uint64_t n_pos) {
             continue;
         }
         const u128_t x = key_at(A, p);
+        const u128_t xr = kmer_rc128(x, k);
         const uint32_t sub = A.out_sub[p];
         uint32_t degs = 0;
         for (uint32_t s = 0; s < 2; s++) {
             uint32_t cnt = 0, only = NO_LINK;
             for (uint32_t b = 0; b < 4; b++) {
+                // neighbour y and its reverse complement, both from x and rc(x)
                 const u128_t y = s ? ((x >> 2) | ((u128_t)b << (2 * (k - 1)))) : (((x << 2) | b) & mask);
-                const u128_t yr = kmer_rc128(y, k);
+                const u128_t yr = s ? (((xr << 2) | (b ^ 2u)) & mask) : ((xr >> 2) | ((u128_t)(b ^ 2u) << (2 * (k - 1))));
                 const u128_t c = y < yr ? y : yr;
                 const uint32_t q = lookup_in_sub(A, c, sub);
                 if (q == NO_LINK) continue;

@@ -70,6 +70,9 @@ inline unsigned __brev(unsigned x) {
     return (x >> 16) | (x << 16);
 }
 inline int __clz(unsigned x) { return x ? __builtin_clz(x) : 32; }
+inline unsigned long long __brevll(unsigned long long x) {
+    return ((unsigned long long)__brev((unsigned)x) << 32) | __brev((unsigned)(x >> 32));
+}
 inline unsigned long long __ballot(int pred) {
     unsigned w = threadIdx.x / 64, l = threadIdx.x % 64;
     emu::wbar[w]->arrive_and_wait();
