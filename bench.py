@@ -280,7 +280,7 @@ def main() -> None:
         "kernels_ms": {k: round(float(v), 3) for k, v in kern_ms.items()},
         "kernels_roofline": kernel_rooflines(kb, kern_ms, wl.name, a.scale),
         "counters": {"records": int(cnt[1]), "record_bases": int(cnt[2]), "kmer_occurrences": int(cnt[3]),
-                     "hbm_table_subbuckets": int(cnt[4]), "solid_entries": int(n)},
+                     "split_items": int(cnt[4]), "solid_entries": int(n)},
         "h2d_ms": round(h2d_s * 1e3, 1),
         "tuning": tuning,
         "pcie_inclusive_gbases_s": round(wl.bases / (dt / a.steps + h2d_s) / 1e9, 4),
