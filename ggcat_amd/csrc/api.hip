@@ -715,7 +715,6 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     M.slots = P_<uint32_t>(ctx->slots);
     M.slot_words = ctx->slot_words;
     M.pool = P_<uint32_t>(ctx->pool);
-    M.unit_off = P_<uint64_t>(ctx->unit_off);
     M.k = p.k;
     M.min_mult = p.min_multiplicity;
     M.kind = p.hash_kind;

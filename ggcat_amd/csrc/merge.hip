@@ -51,7 +51,6 @@ struct MergeArgs {
     const uint32_t* slots;
     uint32_t slot_words;
     const uint32_t* pool;
-    const uint64_t* unit_off;  // [n_units+1] slot offsets
     uint32_t k;
     uint32_t min_mult;
     uint32_t kind;  // 0 = canonical seq-hash, 1 = canonical rabin-karp128
@@ -617,8 +616,7 @@ phase_a_done:
 struct Fetched {
     uint32_t e;
     PartWork w;
-    WorkItem it;
-    uint64_t r0, r1;
+    WorkItem it;  // with its slot range [it.r0, it.r1)
 };
 
 // Thread 0's pipeline of work entries: the next entry's metadata is loaded
@@ -644,12 +642,6 @@ GG_DEV void pipe_part(const MergeArgs& A, Pipe& p) {
 }
 GG_DEV void pipe_item(const MergeArgs& A, Pipe& p) {
     if (p.n1.e < A.n_work) p.n1.it = A.items[p.n1.w.item];
-}
-GG_DEV void pipe_units(const MergeArgs& A, Pipe& p) {
-    if (p.n1.e < A.n_work) {
-        p.n1.r0 = p.n1.it.r0;
-        p.n1.r1 = p.n1.it.r1;
-    }
 }
 
 // Persistent work-stealing merge: each workgroup takes work entries from a
@@ -706,7 +698,6 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
         pp.n1.e = pipe_claim(A, pp);
         pipe_part(A, pp);
         pipe_item(A, pp);
-        pipe_units(A, pp);
         pp.e2 = pp.n1.e < A.n_work ? pipe_claim(A, pp) : A.n_work;
     }
     while (true) {
@@ -727,7 +718,6 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
             if (leader) {
                 A.redo[atomicAdd(A.redo_count, 1u)] = PartWork{w.item, w.part, w.log_parts, it.occ};
                 pipe_item(A, pp);
-                pipe_units(A, pp);
             }
             continue;
         }
@@ -742,11 +732,11 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
             // records of one chunk are counted once, the k-mer table
             // accumulates over the chunks.  Each thread's record of the next
             // chunk is loaded while the current chunk is processed.
-            load_rec(cur.r0 + threadIdx.x, cur.r1, cw, ccol);
-            for (uint64_t c0 = cur.r0; c0 < cur.r1; c0 += chunk) {
-                const uint64_t c1 = c0 + chunk < cur.r1 ? c0 + chunk : cur.r1;
+            load_rec(it.r0 + threadIdx.x, it.r1, cw, ccol);
+            for (uint64_t c0 = it.r0; c0 < it.r1; c0 += chunk) {
+                const uint64_t c1 = c0 + chunk < it.r1 ? c0 + chunk : it.r1;
                 uint32_t nw[8] = {0, 0, 0, 0, 0, 0, 0, 0}, ncol = 0;
-                load_rec(c1 + threadIdx.x, cur.r1, nw, ncol);
+                load_rec(c1 + threadIdx.x, it.r1, nw, ncol);
                 insert_item_dedup(A, c0, c1, w, klo, khi, st, vals, cbits, mask, 96u, &s_overflow, claimed,
                                   &s_claimed, rec_table(A, lds), cw, ccol);
 #pragma unroll
@@ -755,7 +745,7 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
                 __syncthreads();  // the next chunk reuses the record table
             }
         } else {
-            insert_item(A, cur.r0, cur.r1, w, klo, khi, st, vals, cbits, mask, 96u, &s_overflow, claimed, &s_claimed);
+            insert_item(A, it.r0, it.r1, w, klo, khi, st, vals, cbits, mask, 96u, &s_overflow, claimed, &s_claimed);
         }
         if (leader) pipe_item(A, pp);
         __syncthreads();
@@ -766,7 +756,6 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
         } else {
             emit_solid(A, it.sub, klo, khi, st, vals, cbits, claimed, n_cl, es);
         }
-        if (leader) pipe_units(A, pp);
         __syncthreads();
         for (uint32_t j = threadIdx.x; j < n_cl; j += blockDim.x) {
             const uint32_t i = claimed[j];
