@@ -386,10 +386,8 @@ struct RecTable {
     uint32_t* cnt;
     uint32_t* state;  // 0 free, 1 being written, 2 ready
     uint32_t* col;
-    uint32_t* list;  // distinct records (table slots)
-    uint32_t* off;  // rec_cap + 1 k-mer offsets of the distinct records
-    uint32_t* n;  // [2]: distinct records, k-mers
-    uint8_t* rec_of;  // distinct record (index into list) of each k-mer position
+    uint32_t* n;  // [1]: k-mers of the chunk, [2..5]: wave totals of the scan
+    uint16_t* rec_of;  // each k-mer position of the chunk: record slot << 7 | k-mer index in the record
 };
 
 // k-mers of one inline record at most.
@@ -408,32 +406,15 @@ GG_DEV RecTable rec_table(const MergeArgs& A, uint8_t* lds) {
     b += A.rec_cap;
     R.col = b;
     b += A.rec_cap;
-    R.list = b;
-    b += A.rec_cap;
-    R.off = b;
-    b += A.rec_cap + 1;
-    R.n = b;  // [1]: k-mers of the chunk, [2..5]: wave totals of the scan
+    R.n = b;
     b += 6;
-    R.rec_of = reinterpret_cast<uint8_t*>(b);
+    R.rec_of = reinterpret_cast<uint16_t*>(b);
     return R;
 }
 
 // Bytes of dynamic LDS the record table needs.
 __host__ __device__ inline uint32_t rec_table_bytes(uint32_t slot_words, uint32_t k, uint32_t rec_cap) {
-    return (rec_cap * slot_words + 5 * rec_cap + 1 + 6) * 4 + (rec_cap / 2) * rec_max_kmers(slot_words, k) + 4;
-}
-
-// Appends the slots claimed by this instruction's lanes to the claimed list
-// with one LDS atomic per wavefront.  Called by every lane of the wavefront.
-GG_DEV void append_claimed(bool cl, uint32_t at, uint16_t* claimed, uint32_t* n_claimed) {
-    const uint64_t bal = __ballot(cl);
-    if (!bal) return;
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t first = (uint32_t)__ffsll((unsigned long long)bal) - 1;
-    uint32_t base = 0;
-    if (lane == first) base = atomicAdd(n_claimed, (uint32_t)__popcll(bal));
-    base = __shfl(base, (int)first);
-    if (cl) claimed[base + __popcll(bal & ((1ULL << lane) - 1))] = (uint16_t)at;
+    return (rec_cap * slot_words + 3 * rec_cap + 6) * 4 + (rec_cap / 2) * rec_max_kmers(slot_words, k) * 2 + 4;
 }
 
 GG_DEV uint32_t rec_hash(const uint32_t* w, uint32_t sw, uint32_t color) {
@@ -535,10 +516,11 @@ GG_DEV void insert_item_dedup(const MergeArgs& A, uint64_t r0, uint64_t r1, cons
     }
 phase_a_done:
     __syncthreads();
-    // phase B: one exclusive scan over the table slots of (distinct record
-    // << 16 | k-mers) numbers the distinct records and places their k-mers;
-    // each thread owns rec_cap / blockDim consecutive slots (at least one)
-    // and frees them for the next chunk once read.
+    // phase B: one exclusive scan over the table slots of their k-mer counts
+    // places the distinct records' k-mers (each position tagged with its
+    // record's slot and k-mer index); each thread owns rec_cap / blockDim
+    // consecutive slots (at least one) and frees them for the next chunk
+    // once read.
     {
         const uint32_t per = RC > blockDim.x ? RC / blockDim.x : 1u;
         const uint32_t s0 = threadIdx.x * per;
@@ -549,7 +531,7 @@ phase_a_done:
             const uint32_t si = s0 + q;
             if (si < RC && R.state[si] == 2) {
                 const uint32_t len = meta_len(R.key[si * SW]);
-                pk[q] = (1u << 16) | (len >= k ? len - k + 1 : 0u);
+                pk[q] = len >= k ? len - k + 1 : 0u;
                 R.state[si] = 0;
             }
             mine += pk[q];
@@ -571,33 +553,30 @@ phase_a_done:
         uint32_t run = base + inc - mine;
         for (uint32_t q = 0; q < per; q++) {
             if (pk[q]) {
-                const uint32_t u = run >> 16, o = run & 0xFFFFu, c = pk[q] & 0xFFFFu;
-                R.list[u] = s0 + q;
-                R.off[u] = o;
-                for (uint32_t e = 0; e < c; e++) R.rec_of[o + e] = (uint8_t)u;
+                const uint32_t o = run, c = pk[q], tag = (s0 + q) << 7;
+                for (uint32_t e = 0; e < c; e++) R.rec_of[o + e] = (uint16_t)(tag | e);
             }
             run += pk[q];
         }
-        if (threadIdx.x == 0) R.n[1] = all & 0xFFFFu;
+        if (threadIdx.x == 0) R.n[1] = all;
         __syncthreads();
     }
-    // phase B2: every k-mer of the distinct records, one per thread; the loop
-    // is uniform over the workgroup so claims are appended per wavefront
+    // phase B2: every k-mer of the distinct records, one per thread
     const uint32_t total = R.n[1];
     for (uint32_t j0 = 0; j0 < total; j0 += blockDim.x) {
         const uint32_t j = j0 + threadIdx.x;
         bool cl = false;
         uint32_t at = 0;
         if (j < total) {
-            const uint32_t u = R.rec_of[j];
-            const uint32_t si = R.list[u], i = j - R.off[u];
+            const uint32_t v = R.rec_of[j];
+            const uint32_t si = v >> 7, i = v & 127u;
             const uint32_t* rw = R.key + si * SW;
+            const uint32_t cnt = R.cnt[si], color = W ? R.col[si] : 0u;
             const uint32_t meta = rw[0];
             const uint32_t len = meta_len(meta);
             bool fwd;
             const uint64_t key = kmer_at(rw + 1, i, k, &fwd);
-            const uint32_t bits = occurrence_bits(meta, fwd, i, len - k) - 1u + R.cnt[si];
-            const uint32_t color = R.col[si];
+            const uint32_t bits = occurrence_bits(meta, fwd, i, len - k) - 1u + cnt;
             if (!part_mask || ((tab_hash64(key) >> 16) & part_mask) == w.part) {
                 const uint32_t res = tab_insert64(klo, vals, mask, key, bits, probe_limit, at);
                 if (res == 2) {
@@ -608,7 +587,7 @@ phase_a_done:
                 }
             }
         }
-        append_claimed(cl, at, claimed, n_claimed);
+        if (cl) claimed[atomicAdd(n_claimed, 1u)] = (uint16_t)at;
     }
 }
 
