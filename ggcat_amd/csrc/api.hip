@@ -60,7 +60,7 @@ struct ggcat_gpu_ctx {
     uint32_t color_words = 0;  // bitmap words per solid entry of a colored build
     double ms[8] = {0};
     ggcat_gpu_tuning tune{};  // zero fields select the defaults
-    hipGraphExec_t graph_a = nullptr;  // captured bucketing pass A (graphs enabled)
+    hipGraphExec_t graph_a = nullptr;  // captured bucketing staging pass (graphs enabled)
     struct GraphKeyA {
         gg::BucketArgs args;  // every kernel argument of the captured launches
         uint64_t seg_words;

@@ -45,13 +45,11 @@ struct BucketArgs {
     uint32_t canonical;
     uint32_t T;  // windows per tile (multiple of 64)
     uint64_t n_tiles;
-    uint32_t* seg_bits;  // [ceil(n/32)+2] group-start bitmap (pass A writes)
+    uint32_t* seg_bits;  // [ceil(n/32)+2] group-start bitmap (written by pass 1)
     uint32_t* sub_count;  // [n_units] records per unit
-    // pass B
     unsigned long long* unit_ck;  // [n_units] records << 32 | k-mer occurrences, counted in pass 1
     uint64_t n_slots;  // slots in the arena
-    // staging: records in tile order before the scatter by unit
-    // Staging: tile i owns positions [i * tile_cap, (i + 1) * tile_cap);
+    // Staging (records in tile order before the scatter by unit): tile i owns positions [i * tile_cap, (i + 1) * tile_cap);
     // records beyond that and deferred records go to the overflow region
     // that follows, [n_tiles * tile_cap, stage_cap).
     uint32_t* stage_slots;  // [stage_cap * slot_words]
