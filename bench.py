@@ -140,6 +140,8 @@ def serve_measurement(ctx, rate_hz: float, n_requests: int) -> dict:
     results = {}
     try:
         rep = serve_poisson(ctx, reqs, rate_hz, results=results)
+        # capacity: the same requests all at once
+        burst = serve_poisson(ctx, reqs, 1e9)
     finally:
         ctx.serve_close()
     # each result equals the same request run alone (spot check of three)
@@ -152,6 +154,8 @@ def serve_measurement(ctx, rate_hz: float, n_requests: int) -> dict:
             "p50_ms": round(rep.percentile(50), 2), "p99_ms": round(rep.percentile(99), 2),
             "max_ms": round(max(rep.latencies_ms), 2), "wall_s": round(rep.wall_s, 2),
             "kmers_per_s": round(rep.kmers / rep.wall_s, 1), "max_admitted": rep.max_admitted,
+            "burst_wall_s": round(burst.wall_s, 3), "burst_kmers_per_s": round(burst.kmers / burst.wall_s, 1),
+            "burst_p99_ms": round(burst.percentile(99), 2),
             "graph_build_s": round(t_build, 3), "index_build_ms": round(t_index * 1e3, 1)}
 
 
