@@ -201,8 +201,14 @@ bool query_offsets(const uint8_t* q, uint64_t n_bytes, uint32_t n, std::vector<u
     off.clear();
     off.reserve((size_t)n + 1);
     off.push_back(0);
-    for (uint64_t i = 0; i < n_bytes; i++)
-        if (q[i] == '\n') off.push_back(i + 1);
+    const uint8_t* p = q;
+    const uint8_t* end = q + n_bytes;
+    while (p < end) {
+        const void* f = memchr(p, '\n', (size_t)(end - p));
+        if (!f) break;
+        p = static_cast<const uint8_t*>(f) + 1;
+        off.push_back((uint64_t)(p - q));
+    }
     return off.size() == (size_t)n + 1 && (n_bytes == 0 || q[n_bytes - 1] == '\n');
 }
 
@@ -267,7 +273,9 @@ struct RangeAlloc {
 };
 
 struct ServeReq {
-    std::vector<uint8_t> data;
+    std::vector<uint8_t> data;  // own copy of the query bytes while the request waits
+    const uint8_t* src = nullptr;  // the query bytes (the caller's buffer during submit)
+    uint64_t n_bytes = 0;
     std::vector<uint64_t> off;
     uint32_t n_queries = 0;
     bool admitted = false, done = false;
@@ -308,7 +316,7 @@ int serve_admit(ggcat_gpu_ctx* ctx) {
             o = (o + bytes + 255) & ~255ull;
             return at;
         };
-        r.o_seq = place(r.data.size() + 16);
+        r.o_seq = place(r.n_bytes + 16);
         r.o_off = place(((uint64_t)nq + 1) * 8);
         r.o_work = place(8);
         r.o_matched = place((uint64_t)nq * 8);
@@ -324,7 +332,7 @@ int serve_admit(ggcat_gpu_ctx* ctx) {
         r.dev_off = doff;
         r.pin_off = poff;
         uint8_t* hp = S.pin + poff;
-        memcpy(hp + r.o_seq, r.data.data(), r.data.size());
+        memcpy(hp + r.o_seq, r.src, r.n_bytes);
         memcpy(hp + r.o_off, r.off.data(), r.off.size() * 8);
         memset(hp + r.o_work, 0, 8);
         int si = 0;
@@ -1304,12 +1312,20 @@ int ggcat_gpu_serve_submit(ggcat_gpu_ctx* ctx, const uint8_t* queries, uint64_t 
         S.reqs.erase(t);
         return fail(ctx, GGCAT_GPU_EINVAL, "queries must be n_queries records, each ending with '\\n'");
     }
-    r.data.assign(queries, queries + n_bytes);
+    r.src = queries;
+    r.n_bytes = n_bytes;
     r.n_queries = n_queries;
     S.waiting.push_back(t);
     *ticket = t;
     HIPCHK(hipSetDevice(ctx->device));
-    return serve_admit(ctx);
+    // admitted at once: the bytes go straight from the caller's buffer to the
+    // pinned staging; a request left waiting keeps its own copy
+    const int rc = serve_admit(ctx);
+    if (!r.admitted) {
+        r.data.assign(queries, queries + n_bytes);
+        r.src = r.data.data();
+    }
+    return rc;
 }
 
 int ggcat_gpu_serve_poll(ggcat_gpu_ctx* ctx, int block, uint64_t* ticket) {
