@@ -442,10 +442,12 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     ctx->P = p;
     const uint32_t w = p.k - p.m;
     const uint32_t n_sub = ((1u << p.buckets_log) + 1) << p.second_buckets_log;
-    // physical units: about `unit_bytes` input bytes per unit (default 2000),
+    // physical units: about `unit_bytes` input bytes per unit (default 2000,
+    // 1000 for colored builds),
     // at most 2^23 units
     uint32_t xb = 0;
-    const uint64_t unit_bytes = ctx->tune.unit_bytes ? ctx->tune.unit_bytes : 2000;
+    // (colored builds have smaller LDS tables per workgroup: smaller units)
+    const uint64_t unit_bytes = ctx->tune.unit_bytes ? ctx->tune.unit_bytes : (n_colors ? 1000 : 2000);
     while (xb < 8 && (n_bytes >> xb) / n_sub > unit_bytes && ((uint64_t)n_sub << (xb + 1)) <= (1u << 23)) xb++;
     const uint32_t n_units = n_sub << xb;
     // slot: meta word + inline bases for the longest unique-minimizer record (2k - m)

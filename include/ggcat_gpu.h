@@ -145,7 +145,7 @@ int ggcat_gpu_solid_colors(ggcat_gpu_ctx* ctx, uint32_t* color_words, uint32_t* 
  *   no_dedup          1: insert every record's k-mers even when identical
  *                     records could be counted once (seq-hash, k <= 32)
  *   unit_bytes        bucketing: input bytes per physical unit (sub-bucket
- *                     refinement); default 2000
+ *                     refinement); default 2000, 1000 for colored builds
  *   use_graphs        1: capture the bucketing staging pass (clears, stage
  *                     and deferred-record kernels) in a HIP graph per
  *                     argument set and replay it on later calls
