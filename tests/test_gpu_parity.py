@@ -107,6 +107,15 @@ def test_config1_medium_digest(gpu_ctx):
     _run_both(gpu_ctx, wl.data, 31, 15, wl.buckets_log, wl.second_buckets_log, exact=False)
 
 
+@pytest.mark.parametrize("k,m", [(21, 9), (27, 11), (25, 9), (32, 12)])
+def test_other_k_m(gpu_ctx, k, m):
+    # the generic tile path (w != 16) and the register path with k != 31
+    g = synth.genome(60 + k, 150_000, repeat_frac=0.2, rep_min=300, rep_max=1500)
+    data = synth.reads(g, 61 + k, 15_000, 150, 0.01)
+    _say(f"case k={k} m={m} bytes={data.size}")
+    _run_both(gpu_ctx, data, k, m, 6, 3, s=2, exact=False)
+
+
 def test_config2_shape_digest(gpu_ctx):
     # the benchmark's read set (250 bp at 40x, 1% errors, 20% repeats) at 1/100
     # of its size, default schedule, compared with the oracle in full
