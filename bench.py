@@ -298,6 +298,11 @@ def main() -> None:
     n_unitigs, unitig_bytes = ctx.unitig_counts()
     out["unitigs"] = {"unitigs": n_unitigs, "bytes": unitig_bytes, "kernels_ms": round(float(ctx.timings()[7]), 2),
                       "call_ms": round((time.perf_counter() - t) * 1e3, 1), "index_ms": round(t_index * 1e3, 1)}
+    # maximal unitigs of the whole solid set (joined across sub-buckets)
+    t = time.perf_counter()
+    n_max, max_bytes = ctx.unitig_counts(joined=True)
+    out["maximal_unitigs"] = {"unitigs": n_max, "bytes": max_bytes, "kernels_ms": round(float(ctx.timings()[7]), 2),
+                              "call_ms": round((time.perf_counter() - t) * 1e3, 1)}
     ctx.free(dptr)
     if not a.no_serve:
         out["query"] = serve_measurement(ctx, a.serve_rate, a.serve_requests)

@@ -75,8 +75,9 @@ struct ggcat_gpu_ctx {
     struct Server* server = nullptr;
     // unitig chains
     DevBuf u_nb, u_ext, u_link, u_hbytes, u_hcount, u_hoff, u_hidx, u_seq, u_sub, u_flags, u_visited, u_rest,
-        u_rest_n;
+        u_rest_n, u_rep;
     bool have_unitigs = false;
+    bool u_joined = false;
     uint64_t u_n = 0, u_bytes = 0, u_npos = 0;
     std::vector<uint8_t> cyc_seq, cyc_flags;
     std::vector<uint32_t> cyc_sub;
@@ -407,7 +408,7 @@ void ggcat_gpu_destroy(ggcat_gpu_ctx* ctx) {
                       &ctx->q_seq, &ctx->q_off, &ctx->q_matched, &ctx->q_counts, &ctx->q_work,
                       &ctx->u_nb, &ctx->u_ext, &ctx->u_link, &ctx->u_hbytes, &ctx->u_hcount, &ctx->u_hoff,
                       &ctx->u_hidx, &ctx->u_seq, &ctx->u_sub, &ctx->u_flags, &ctx->u_visited, &ctx->u_rest,
-                      &ctx->u_rest_n};
+                      &ctx->u_rest_n, &ctx->u_rep};
     if (ctx->server) (void)ggcat_gpu_serve_close(ctx);
     if (ctx->graph_a) (void)hipGraphExecDestroy(ctx->graph_a);
     for (DevBuf* b : bufs)
@@ -1408,8 +1409,11 @@ int ggcat_gpu_serve_close(ggcat_gpu_ctx* ctx) {
     return GGCAT_GPU_OK;
 }
 
-int ggcat_gpu_unitigs(ggcat_gpu_ctx* ctx, uint64_t* n_unitigs, uint64_t* n_bytes) {
-    if (!ctx) return GGCAT_GPU_EINVAL;
+namespace {
+
+// Unitigs of the last merge's solid table: per sub-bucket (joined == false)
+// or over the whole table, joined across sub-buckets (joined == true).
+int build_unitigs(ggcat_gpu_ctx* ctx, bool joined, uint64_t* n_unitigs, uint64_t* n_bytes) {
     if (!ctx->have_solid) return fail(ctx, GGCAT_GPU_ESTATE, "no merge result");
     if (ctx->P.hash_kind != GGCAT_GPU_HASH_SEQ)
         return fail(ctx, GGCAT_GPU_EINVAL, "unitig extension needs seq-hash keys (the k-mer is the key)");
@@ -1429,6 +1433,9 @@ int ggcat_gpu_unitigs(ggcat_gpu_ctx* ctx, uint64_t* n_unitigs, uint64_t* n_bytes
     ENSURE(ctx->u_hidx, (ne + 1) * 8);
     ENSURE(ctx->u_visited, n_pos + 8);
     ENSURE(ctx->u_rest_n, 8);
+    ENSURE(ctx->u_rep, n_pos + 8);
+    if (ne >= (uint64_t)NO_LINK) return fail(ctx, GGCAT_GPU_EINVAL, "solid table too large for unitig links");
+    ctx->have_unitigs = false;
     UnitigArgs U{};
     U.out_key = P_<uint64_t>(ctx->out_key);
     U.out_val = P_<uint32_t>(ctx->out_val);
@@ -1451,9 +1458,12 @@ int ggcat_gpu_unitigs(ggcat_gpu_ctx* ctx, uint64_t* n_unitigs, uint64_t* n_bytes
     U.head_off = P_<uint64_t>(ctx->u_hoff);
     U.head_idx = P_<uint64_t>(ctx->u_hidx);
     U.visited = P_<uint8_t>(ctx->u_visited);
+    U.global = joined ? 1u : 0u;
+    U.rep = P_<uint8_t>(ctx->u_rep);
     const uint32_t grid = (uint32_t)std::min<uint64_t>((ne + 255) / 256 + 1, (uint64_t)ctx->n_cu * 16);
     HIPCHK(hipEventRecord(ctx->ev[11], ctx->stream));
     if (n_pos) {
+        if (joined) hipLaunchKernelGGL(rep_kernel, dim3(grid), dim3(256), 0, ctx->stream, U, n_pos);
         hipLaunchKernelGGL(ext_kernel, dim3(grid), dim3(256), 0, ctx->stream, U, n_pos);
         hipLaunchKernelGGL(link_kernel, dim3(grid), dim3(256), 0, ctx->stream, U, n_pos);
         hipLaunchKernelGGL(walk_kernel<false>, dim3(grid), dim3(256), 0, ctx->stream, U, n_pos);
@@ -1537,7 +1547,7 @@ int ggcat_gpu_unitigs(ggcat_gpu_ctx* ctx, uint64_t* n_unitigs, uint64_t* n_bytes
                 cur = nx;
             }
             ctx->cyc_seq.push_back('\n');
-            ctx->cyc_sub.push_back(sub);
+            ctx->cyc_sub.push_back(joined ? 0u : sub);
             ctx->cyc_flags.push_back(4);
         }
     }
@@ -1548,8 +1558,49 @@ int ggcat_gpu_unitigs(ggcat_gpu_ctx* ctx, uint64_t* n_unitigs, uint64_t* n_bytes
     ctx->u_bytes = tot_bytes;
     ctx->u_npos = n_pos;
     ctx->have_unitigs = true;
+    ctx->u_joined = joined;
     if (n_unitigs) *n_unitigs = tot_n + ctx->cyc_sub.size();
     if (n_bytes) *n_bytes = tot_bytes + ctx->cyc_seq.size();
+    return GGCAT_GPU_OK;
+}
+
+}  // namespace
+
+int ggcat_gpu_unitigs(ggcat_gpu_ctx* ctx, uint64_t* n_unitigs, uint64_t* n_bytes) {
+    if (!ctx) return GGCAT_GPU_EINVAL;
+    return build_unitigs(ctx, false, n_unitigs, n_bytes);
+}
+
+int ggcat_gpu_maximal_unitigs(ggcat_gpu_ctx* ctx, uint64_t* n_unitigs, uint64_t* n_bytes) {
+    if (!ctx) return GGCAT_GPU_EINVAL;
+    return build_unitigs(ctx, true, n_unitigs, n_bytes);
+}
+
+int ggcat_gpu_unitigs_write_fasta(ggcat_gpu_ctx* ctx, const char* path, uint64_t* n_written) {
+    if (!ctx || !path) return GGCAT_GPU_EINVAL;
+    if (!ctx->have_unitigs) return fail(ctx, GGCAT_GPU_ESTATE, "no unitig result");
+    HIPCHK(hipSetDevice(ctx->device));
+    const uint64_t total = ctx->u_bytes + ctx->cyc_seq.size();
+    std::vector<uint8_t> seq(total);
+    if (ctx->u_bytes) HIPCHK(hipMemcpy(seq.data(), ctx->u_seq.p, ctx->u_bytes, hipMemcpyDeviceToHost));
+    if (!ctx->cyc_seq.empty()) memcpy(seq.data() + ctx->u_bytes, ctx->cyc_seq.data(), ctx->cyc_seq.size());
+    FILE* f = fopen(path, "wb");
+    if (!f) return fail(ctx, GGCAT_GPU_EINVAL, std::string("cannot create ") + path);
+    std::vector<char> buf(1 << 20);
+    setvbuf(f, buf.data(), _IOFBF, buf.size());
+    uint64_t id = 0;
+    bool ok = true;
+    for (uint64_t o = 0; o < total && ok;) {
+        const uint8_t* nl = static_cast<const uint8_t*>(memchr(seq.data() + o, '\n', total - o));
+        const uint64_t len = nl ? (uint64_t)(nl - (seq.data() + o)) : total - o;
+        ok = fprintf(f, ">%llu LN:i:%llu\n", (unsigned long long)id, (unsigned long long)len) > 0 &&
+             fwrite(seq.data() + o, 1, len, f) == len && fputc('\n', f) != EOF;
+        o += len + 1;
+        id++;
+    }
+    ok = (fclose(f) == 0) && ok;
+    if (!ok) return fail(ctx, GGCAT_GPU_EINVAL, std::string("write failed: ") + path);
+    if (n_written) *n_written = id;
     return GGCAT_GPU_OK;
 }
 

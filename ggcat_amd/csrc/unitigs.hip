@@ -18,6 +18,14 @@
 //   * a unitig is a maximal linked path, written from its end with the
 //     smaller (key, side); an end is open when its outer side is flagged.
 // Cycles (paths without ends) are rare and are walked on the host.
+//
+// Joined mode (global != 0) builds the maximal unitigs of the whole solid
+// set, i.e. the partial unitigs of every sub-bucket joined across buckets
+// (what the links / reorganize / build_final_unitigs stages of
+// crates/assembler_pipeline/src/extend_unitigs.rs:348-993 produce): a k-mer
+// held by several sub-buckets (a boundary k-mer, counted on both sides) is
+// one node, represented by its smallest table position; neighbours are
+// looked up over the whole table and no side is flagged.
 #include "common.h"
 
 namespace gg {
@@ -49,12 +57,20 @@ struct UnitigArgs {
     uint32_t* uni_sub;
     uint8_t* uni_flags;
     uint8_t* visited;  // [n_pos]
+    uint32_t global;  // joined mode: one graph over all sub-buckets
+    uint8_t* rep;  // [n_pos] joined mode: 1 where p is its key's smallest position
 };
 
-GG_DEV bool unitig_valid(const UnitigArgs& A, uint64_t p) {
+// Position p holds a solid entry (chunks are filled from their start).
+GG_DEV bool slot_filled(const UnitigArgs& A, uint64_t p) {
     const uint64_t c = p / A.chunk;
     if (c < A.n_chunks) return (p - c * A.chunk) < A.chunk_fill[c];
     return p - (uint64_t)A.n_chunks * A.chunk < A.n_tail;
+}
+
+// Position p is a node of the graph being compacted.
+GG_DEV bool unitig_valid(const UnitigArgs& A, uint64_t p) {
+    return slot_filled(A, p) && (!A.global || A.rep[p]);
 }
 
 GG_DEV u128_t key_at(const UnitigArgs& A, uint64_t p) {
@@ -77,19 +93,33 @@ GG_DEV u128_t kmer_rc128(u128_t x, uint32_t k) {
 }
 
 GG_DEV bool side_flagged(const UnitigArgs& A, uint64_t p, uint32_t s) {
+    if (A.global) return false;
     const uint32_t fl = A.out_val[p] >> 30;
     return (fl == 1 && s == 0) || (fl == 2 && s == 1);
 }
 
-// Position of key `c` inside sub-bucket `sub`, or NO_LINK.
+// Position of key `c` inside sub-bucket `sub`, or NO_LINK.  In joined mode
+// the sub-bucket is ignored and the key's smallest position is returned.
 GG_DEV uint32_t lookup_in_sub(const UnitigArgs& A, u128_t c, uint32_t sub) {
     const uint64_t lo = (uint64_t)c, hi = (uint64_t)(c >> 64);
     const uint32_t b = A.log_buckets ? (uint32_t)(index_hash(lo, hi) >> (64 - A.log_buckets)) : 0u;
+    uint32_t best = NO_LINK;
     for (uint64_t e = A.boff[b]; e < A.boff[b + 1]; e++) {
         const bool eq = A.key_words == 1 ? A.idx_key[e] == lo : (A.idx_key[2 * e] == lo && A.idx_key[2 * e + 1] == hi);
-        if (eq && A.out_sub[A.idx_pos[e]] == sub) return A.idx_pos[e];
+        if (!eq) continue;
+        if (!A.global) {
+            if (A.out_sub[A.idx_pos[e]] == sub) return A.idx_pos[e];
+        } else if (A.idx_pos[e] < best) {
+            best = A.idx_pos[e];
+        }
     }
-    return NO_LINK;
+    return best;
+}
+
+// Joined mode: marks each filled position that is its key's smallest one.
+__global__ void rep_kernel(UnitigArgs A, uint64_t n_pos) {
+    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < n_pos; p += (uint64_t)gridDim.x * blockDim.x)
+        A.rep[p] = slot_filled(A, p) && lookup_in_sub(A, key_at(A, p), 0) == (uint32_t)p;
 }
 
 __global__ void ext_kernel(UnitigArgs A, uint64_t n_pos) {
@@ -191,7 +221,7 @@ __global__ void walk_kernel(UnitigArgs A, uint64_t n_pos) {
         out[o] = '\n';
         const uint64_t qend = cur >> 1;
         const uint32_t exit_side = (uint32_t)((cur ^ 1) & 1);
-        A.uni_sub[ui] = A.out_sub[p];
+        A.uni_sub[ui] = A.global ? 0u : A.out_sub[p];
         A.uni_flags[ui] = (uint8_t)((side_flagged(A, p, s) ? 1u : 0u) | (side_flagged(A, qend, exit_side) ? 2u : 0u));
     }
 }

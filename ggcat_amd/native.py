@@ -120,6 +120,8 @@ def gpu_lib(path: str = GPU_LIB) -> C.CDLL:
     lib.ggcat_gpu_serve_close.argtypes = [V]
     lib.ggcat_gpu_unitigs.argtypes = [V, _u64p, _u64p]
     lib.ggcat_gpu_unitigs_copy.argtypes = [V, _u8p, _u32p, _u8p]
+    lib.ggcat_gpu_maximal_unitigs.argtypes = [V, _u64p, _u64p]
+    lib.ggcat_gpu_unitigs_write_fasta.argtypes = [V, C.c_char_p, _u64p]
     lib.ggcat_gpu_extension_copy.argtypes = [V, _u8p]
     lib.ggcat_gpu_read_files.argtypes = [V, C.POINTER(C.c_char_p), C.c_uint32, C.c_uint32, C.c_int,
                                          C.POINTER(_u8p), _u64p, C.POINTER(_u64p), C.POINTER(_u32p), _u32p, _u64p]
@@ -138,6 +140,7 @@ def gpu_lib(path: str = GPU_LIB) -> C.CDLL:
                  "ggcat_gpu_copy_device", "ggcat_gpu_index_build", "ggcat_gpu_index_colors", "ggcat_gpu_query",
                  "ggcat_gpu_serve_open", "ggcat_gpu_serve_submit", "ggcat_gpu_serve_poll", "ggcat_gpu_serve_fetch",
                  "ggcat_gpu_serve_stats", "ggcat_gpu_serve_close", "ggcat_gpu_unitigs", "ggcat_gpu_unitigs_copy",
+                 "ggcat_gpu_maximal_unitigs", "ggcat_gpu_unitigs_write_fasta",
                  "ggcat_gpu_extension_copy"):
         getattr(lib, name).restype = C.c_int
     _gpu[path] = lib
@@ -324,18 +327,32 @@ class GpuContext:
         return out
 
     # ---- unitigs -------------------------------------------------------
-    def unitig_counts(self) -> tuple[int, int]:
-        """Builds the intra-sub-bucket unitigs on the device and returns
-        (unitigs, bytes) without copying them out."""
+    def _build_unitigs(self, joined: bool) -> tuple[int, int]:
         n, nb = C.c_uint64(), C.c_uint64()
-        self._check(self.lib.ggcat_gpu_unitigs(self.h, C.byref(n), C.byref(nb)), "ggcat_gpu_unitigs")
+        fn = "ggcat_gpu_maximal_unitigs" if joined else "ggcat_gpu_unitigs"
+        self._check(getattr(self.lib, fn)(self.h, C.byref(n), C.byref(nb)), fn)
         return int(n.value), int(nb.value)
 
-    def unitigs(self):
+    def unitig_counts(self, joined: bool = False) -> tuple[int, int]:
+        """Builds the intra-sub-bucket unitigs (joined=True: the maximal
+        unitigs of the whole solid set) on the device and returns (unitigs,
+        bytes) without copying them out."""
+        return self._build_unitigs(joined)
+
+    def write_unitigs_fasta(self, path: str) -> int:
+        """Writes the last unitig result as FASTA ('>id LN:i:len'); returns
+        the number of records."""
+        n = C.c_uint64()
+        self._check(self.lib.ggcat_gpu_unitigs_write_fasta(self.h, os.fsencode(path), C.byref(n)),
+                    "ggcat_gpu_unitigs_write_fasta")
+        return int(n.value)
+
+    def unitigs(self, joined: bool = False):
         """Intra-sub-bucket unitigs: (list of ASCII sequences, sub-bucket per
-        unitig, flags: bit0 open start, bit1 open end, bit2 circular)."""
+        unitig, flags: bit0 open start, bit1 open end, bit2 circular).
+        joined=True: the maximal unitigs of the whole solid set (sub-bucket 0)."""
         n, nb = C.c_uint64(), C.c_uint64()
-        self._check(self.lib.ggcat_gpu_unitigs(self.h, C.byref(n), C.byref(nb)), "ggcat_gpu_unitigs")
+        n.value, nb.value = self._build_unitigs(joined)
         seq = np.zeros(nb.value, np.uint8)
         sub, fl = np.zeros(n.value, np.uint32), np.zeros(n.value, np.uint8)
         self._check(self.lib.ggcat_gpu_unitigs_copy(self.h, _ptr(seq, _u8p), _ptr(sub, _u32p), _ptr(fl, _u8p)),

@@ -245,6 +245,25 @@ int ggcat_gpu_unitigs_copy(ggcat_gpu_ctx* ctx, uint8_t* seq, uint32_t* sub, uint
  * follow it.  Valid after ggcat_gpu_unitigs. */
 int ggcat_gpu_extension_copy(ggcat_gpu_ctx* ctx, uint8_t* ext);
 
+/* ---- maximal unitigs (partial unitigs joined across buckets) -------------
+ * Replaces build_final_unitigs (crates/assembler_pipeline/src/lib.rs:111,
+ * whose extend_unitigs at crates/assembler_pipeline/src/extend_unitigs.rs:
+ * 348-993 joins the partial unitigs of all buckets; called from
+ * crates/assembler/src/lib.rs:339-349) for uncolored seq-hash builds: the
+ * maximal unitigs of the whole solid k-mer set of the last merge.  A k-mer
+ * held by two sub-buckets (a boundary k-mer) is one node; every other rule is
+ * the one of ggcat_gpu_unitigs with no side flagged.  Results replace the
+ * previous unitig result and are read with ggcat_gpu_unitigs_copy (sub-bucket
+ * 0 and flags 0 for every unitig, 4 for a circular one) and
+ * ggcat_gpu_extension_copy (degrees over the whole set, for the first solid
+ * entry of each k-mer; 0 for its other entries). */
+int ggcat_gpu_maximal_unitigs(ggcat_gpu_ctx* ctx, uint64_t* n_unitigs, uint64_t* n_bytes);
+/* Writes the last unitig result as FASTA, one record per unitig,
+ * ">{id} LN:i:{length}" then the sequence on one line (the uncolored header
+ * of crates/sequence_output/src/structured_sequences/fasta.rs:153-200; ids
+ * count from 0 in result order).  *n_written (optional) = records written. */
+int ggcat_gpu_unitigs_write_fasta(ggcat_gpu_ctx* ctx, const char* path, uint64_t* n_written);
+
 /* ---- queries ------------------------------------------------------------
  * Replaces the counting part of run_query (crates/querier/src/lib.rs:54-200):
  * the graph is the solid k-mer table of the last merge (the k-mer set of the

@@ -173,11 +173,12 @@ class Result:
         L.orc_extension(self.h, k, _p(out, _u8p))
         return out
 
-    def unitigs(self, k: int, sequential: bool = False):
+    def unitigs(self, k: int, sequential: bool = False, joined: bool = False):
         """Intra-sub-bucket unitigs of the merged table: (list of ASCII sequences,
-        sub-bucket per unitig, flags: bit0 open start, bit1 open end, bit2 circular)."""
+        sub-bucket per unitig, flags: bit0 open start, bit1 open end, bit2 circular).
+        joined=True: the maximal unitigs of the whole solid k-mer set."""
         L = lib()
-        n = L.orc_unitigs(self.h, k, 1 if sequential else 0)
+        n = L.orc_unitigs(self.h, k, 2 if joined else (1 if sequential else 0))
         seq = np.zeros(L.orc_unitigs_bytes(self.h), np.uint8)
         sub, fl = np.zeros(n, np.uint32), np.zeros(n, np.uint8)
         L.orc_unitigs_get(self.h, _p(seq, _u8p), _p(sub, _u32p), _p(fl, _u8p))

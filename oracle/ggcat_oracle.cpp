@@ -818,14 +818,30 @@ void orc_merge(void* hv, uint32_t k, uint32_t b, uint32_t sb, int kind, uint32_t
 
 // Unitig chains over the merged solid table (seq-hash keys).  mode 0: the
 // order-free rule; mode 1: the reference's greedy loop, starting k-mers in
-// ascending key order per sub-bucket.  Returns the number of unitigs; the
-// data stays in the handle (orc_unitigs_get).
+// ascending key order per sub-bucket; mode 2: the maximal unitigs of the
+// whole solid k-mer set (the output of build_final_unitigs,
+// crates/assembler_pipeline/src/lib.rs:111, extend_unitigs.rs:348-993): one
+// node per distinct k-mer over all sub-buckets, no side flagged, sub-bucket
+// 0.  Returns the number of unitigs; the data stays in the handle
+// (orc_unitigs_get).
 size_t orc_unitigs(void* hv, uint32_t k, int mode) {
     Handle* h = (Handle*)hv;
     const SolidSet& S = h->solid;
+    h->unitigs = UnitigOut();
+    if (mode == 2) {
+        SubGraph G;
+        G.k = k;
+        for (size_t i = 0; i < S.sub.size(); i++) {
+            const u128 key = ((u128)S.key_hi[i] << 64) | S.key_lo[i];
+            if (!G.at.emplace(key, (uint32_t)G.key.size()).second) continue;
+            G.key.push_back(key);
+            G.flags.push_back(0);
+        }
+        unitig_chains(G, 0, h->unitigs);
+        return h->unitigs.sub.size();
+    }
     std::map<uint32_t, std::vector<size_t>> by_sub;
     for (size_t i = 0; i < S.sub.size(); i++) by_sub[S.sub[i]].push_back(i);
-    h->unitigs = UnitigOut();
     for (auto& kv : by_sub) {
         SubGraph G;
         G.k = k;

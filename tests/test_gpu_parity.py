@@ -308,6 +308,25 @@ def test_unitigs_and_extension_flags(gpu_ctx, k, m, rep, err):
         sorted(zip(osol["sub"].tolist(), osol["key_lo"].tolist(), osol["key_hi"].tolist(), o.extension(k).tolist()))
 
 
+@pytest.mark.parametrize("k,m,rep", [(31, 15, 0.2), (45, 21, 0.2)])
+def test_maximal_unitigs_and_fasta(gpu_ctx, tmp_path, k, m, rep):
+    from tests.test_unitigs import canon_unitigs
+    g = synth.genome(79, 200_000, repeat_frac=rep, rep_min=300, rep_max=2000)
+    data = synth.reads(g, 80, 30_000, 150, 0.005)
+    _say(f"case maximal unitigs k={k} m={m} bytes={data.size}")
+    gpu_ctx.bucketing(data, _params(k, m, 5, 2, 2))
+    gpu_ctx.kmers_merge()
+    gu = gpu_ctx.unitigs(joined=True)
+    o = O.bucketing(data, k, m, 5, 2, threads=4)
+    o.merge(k, 5, 2, HASH_SEQ, 2, 4)
+    assert canon_unitigs(*gu) == canon_unitigs(*o.unitigs(k, joined=True))
+    path = str(tmp_path / "unitigs.fa")
+    assert gpu_ctx.write_unitigs_fasta(path) == len(gu[0])
+    lines = open(path, "rb").read().split(b"\n")
+    assert lines[1::2][:len(gu[0])] == gu[0]
+    assert lines[0] == b">0 LN:i:%d" % len(gu[0][0])
+
+
 def test_autotune_and_graphs_keep_results(gpu_ctx):
     from ggcat_amd.autotune import Schedule, autotune
     wl = synth.config1(0.02)
