@@ -75,7 +75,7 @@ struct ggcat_gpu_ctx {
     struct Server* server = nullptr;
     // unitig chains
     DevBuf u_nb, u_ext, u_link, u_hbytes, u_hcount, u_hoff, u_hidx, u_seq, u_sub, u_flags, u_visited, u_rest,
-        u_rest_n, u_rep;
+        u_rest_n, u_rep, u_tab;
     bool have_unitigs = false;
     bool u_joined = false;
     uint64_t u_n = 0, u_bytes = 0, u_npos = 0;
@@ -408,7 +408,7 @@ void ggcat_gpu_destroy(ggcat_gpu_ctx* ctx) {
                       &ctx->q_seq, &ctx->q_off, &ctx->q_matched, &ctx->q_counts, &ctx->q_work,
                       &ctx->u_nb, &ctx->u_ext, &ctx->u_link, &ctx->u_hbytes, &ctx->u_hcount, &ctx->u_hoff,
                       &ctx->u_hidx, &ctx->u_seq, &ctx->u_sub, &ctx->u_flags, &ctx->u_visited, &ctx->u_rest,
-                      &ctx->u_rest_n, &ctx->u_rep};
+                      &ctx->u_rest_n, &ctx->u_rep, &ctx->u_tab};
     if (ctx->server) (void)ggcat_gpu_serve_close(ctx);
     if (ctx->graph_a) (void)hipGraphExecDestroy(ctx->graph_a);
     for (DevBuf* b : bufs)
@@ -1417,10 +1417,6 @@ int build_unitigs(ggcat_gpu_ctx* ctx, bool joined, uint64_t* n_unitigs, uint64_t
     if (!ctx->have_solid) return fail(ctx, GGCAT_GPU_ESTATE, "no merge result");
     if (ctx->P.hash_kind != GGCAT_GPU_HASH_SEQ)
         return fail(ctx, GGCAT_GPU_EINVAL, "unitig extension needs seq-hash keys (the k-mer is the key)");
-    if (!ctx->have_index) {
-        int r = ggcat_gpu_index_build(ctx);
-        if (r != GGCAT_GPU_OK) return r;
-    }
     HIPCHK(hipSetDevice(ctx->device));
     const uint64_t n_pos = (uint64_t)ctx->n_chunks_used * OUT_CHUNK + ctx->n_tail;
     const uint64_t ne = 2 * n_pos;
@@ -1434,6 +1430,9 @@ int build_unitigs(ggcat_gpu_ctx* ctx, bool joined, uint64_t* n_unitigs, uint64_t
     ENSURE(ctx->u_visited, n_pos + 8);
     ENSURE(ctx->u_rest_n, 8);
     ENSURE(ctx->u_rep, n_pos + 8);
+    uint64_t tab_slots = 1024;
+    while (tab_slots < 2 * n_pos) tab_slots <<= 1;
+    ENSURE(ctx->u_tab, tab_slots * 8);
     if (ne >= (uint64_t)NO_LINK) return fail(ctx, GGCAT_GPU_EINVAL, "solid table too large for unitig links");
     ctx->have_unitigs = false;
     UnitigArgs U{};
@@ -1446,10 +1445,8 @@ int build_unitigs(ggcat_gpu_ctx* ctx, bool joined, uint64_t* n_unitigs, uint64_t
     U.chunk = OUT_CHUNK;
     U.key_words = ctx->key_words;
     U.k = ctx->P.k;
-    U.idx_key = P_<uint64_t>(ctx->idx_key);
-    U.idx_pos = P_<uint32_t>(ctx->idx_pos);
-    U.boff = P_<uint64_t>(ctx->idx_boff);
-    U.log_buckets = ctx->idx_log_buckets;
+    U.tab = P_<uint64_t>(ctx->u_tab);
+    U.tab_mask = tab_slots - 1;
     U.nb = P_<uint32_t>(ctx->u_nb);
     U.ext = P_<uint8_t>(ctx->u_ext);
     U.link = P_<uint32_t>(ctx->u_link);
@@ -1460,9 +1457,14 @@ int build_unitigs(ggcat_gpu_ctx* ctx, bool joined, uint64_t* n_unitigs, uint64_t
     U.visited = P_<uint8_t>(ctx->u_visited);
     U.global = joined ? 1u : 0u;
     U.rep = P_<uint8_t>(ctx->u_rep);
+    U.n_nodes = P_<unsigned int>(ctx->u_rest_n) + 1;
+    U.mark_visited = 0;
     const uint32_t grid = (uint32_t)std::min<uint64_t>((ne + 255) / 256 + 1, (uint64_t)ctx->n_cu * 16);
     HIPCHK(hipEventRecord(ctx->ev[11], ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->u_rest_n.p, 0, 8, ctx->stream));  // [0] cycle nodes, [1] graph nodes
     if (n_pos) {
+        HIPCHK(hipMemsetAsync(ctx->u_tab.p, 0, tab_slots * 8, ctx->stream));
+        hipLaunchKernelGGL(utab_insert_kernel, dim3(grid), dim3(256), 0, ctx->stream, U, n_pos);
         if (joined) hipLaunchKernelGGL(rep_kernel, dim3(grid), dim3(256), 0, ctx->stream, U, n_pos);
         hipLaunchKernelGGL(ext_kernel, dim3(grid), dim3(256), 0, ctx->stream, U, n_pos);
         hipLaunchKernelGGL(link_kernel, dim3(grid), dim3(256), 0, ctx->stream, U, n_pos);
@@ -1474,23 +1476,34 @@ int build_unitigs(ggcat_gpu_ctx* ctx, bool joined, uint64_t* n_unitigs, uint64_t
     r = device_scan(ctx, U.head_count, P_<uint64_t>(ctx->u_hidx), (uint32_t)ne);
     if (r != GGCAT_GPU_OK) return r;
     uint64_t tot_bytes = 0, tot_n = 0;
+    uint32_t counts[2] = {0, 0};
     HIPCHK(hipMemcpyAsync(&tot_bytes, P_<uint64_t>(ctx->u_hoff) + ne, 8, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(&tot_n, P_<uint64_t>(ctx->u_hidx) + ne, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(counts, ctx->u_rest_n.p, 8, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    // Paths cover every node unless there are cycles: only then are the path
+    // nodes marked, so that the remaining ones can be collected.
+    uint64_t n_nodes = counts[1];
+    if (!joined) {
+        n_nodes = ctx->n_tail;
+        for (uint32_t c = 0; c < ctx->n_chunks_used; c++) n_nodes += ctx->chunk_fill[c];
+    }
+    const bool cycles = tot_bytes - (uint64_t)ctx->P.k * tot_n != n_nodes;
+    U.mark_visited = cycles ? 1u : 0u;
     ENSURE(ctx->u_seq, tot_bytes + 8);
     ENSURE(ctx->u_sub, tot_n * 4 + 8);
     ENSURE(ctx->u_flags, tot_n + 8);
     U.seq = P_<uint8_t>(ctx->u_seq);
     U.uni_sub = P_<uint32_t>(ctx->u_sub);
     U.uni_flags = P_<uint8_t>(ctx->u_flags);
-    HIPCHK(hipMemsetAsync(ctx->u_visited.p, 0, n_pos + 8, ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->u_rest_n.p, 0, 8, ctx->stream));
+    if (cycles) HIPCHK(hipMemsetAsync(ctx->u_visited.p, 0, n_pos + 8, ctx->stream));
     const uint32_t cap = 1u << 20;
     ENSURE(ctx->u_rest, (size_t)cap * 4);
     if (n_pos) {
         hipLaunchKernelGGL(walk_kernel<true>, dim3(grid), dim3(256), 0, ctx->stream, U, n_pos);
-        hipLaunchKernelGGL(unvisited_kernel, dim3(grid), dim3(256), 0, ctx->stream, U, n_pos, P_<uint32_t>(ctx->u_rest),
-                           P_<unsigned int>(ctx->u_rest_n), cap);
+        if (cycles)
+            hipLaunchKernelGGL(unvisited_kernel, dim3(grid), dim3(256), 0, ctx->stream, U, n_pos,
+                               P_<uint32_t>(ctx->u_rest), P_<unsigned int>(ctx->u_rest_n), cap);
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipEventRecord(ctx->ev[12], ctx->stream));

@@ -42,10 +42,8 @@ struct UnitigArgs {
     uint32_t chunk;
     uint32_t key_words;
     uint32_t k;
-    const uint64_t* idx_key;
-    const uint32_t* idx_pos;
-    const uint64_t* boff;
-    uint32_t log_buckets;
+    uint64_t* tab;  // [tab_mask + 1] open addressing: hash bits 32-63 << 32 | (position + 1); 0 = empty
+    uint64_t tab_mask;
     uint32_t* nb;  // [2 n_pos] only neighbour (q << 1 | t) or NO_LINK
     uint8_t* ext;  // [n_pos] degree side 0 (0, 1, 2 = more) | degree side 1 << 2
     uint32_t* link;  // [2 n_pos]
@@ -59,6 +57,8 @@ struct UnitigArgs {
     uint8_t* visited;  // [n_pos]
     uint32_t global;  // joined mode: one graph over all sub-buckets
     uint8_t* rep;  // [n_pos] joined mode: 1 where p is its key's smallest position
+    unsigned int* n_nodes;  // joined mode: number of representatives
+    uint32_t mark_visited;  // write pass: mark path nodes (needed only when cycles exist)
 };
 
 // Position p holds a solid entry (chunks are filled from their start).
@@ -98,28 +98,58 @@ GG_DEV bool side_flagged(const UnitigArgs& A, uint64_t p, uint32_t s) {
     return (fl == 1 && s == 0) || (fl == 2 && s == 1);
 }
 
+// Every filled position goes into the lookup table (linear probing; the
+// table has at least twice as many slots as positions, so probing ends).
+__global__ void utab_insert_kernel(UnitigArgs A, uint64_t n_pos) {
+    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < n_pos;
+         p += (uint64_t)gridDim.x * blockDim.x) {
+        if (!slot_filled(A, p)) continue;
+        const u128_t c = key_at(A, p);
+        const uint64_t h = index_hash((uint64_t)c, (uint64_t)(c >> 64));
+        const uint64_t v = (h & 0xFFFFFFFF00000000ULL) | (p + 1);
+        for (uint64_t i = h & A.tab_mask;; i = (i + 1) & A.tab_mask)
+            if (atomicCAS(reinterpret_cast<unsigned long long*>(A.tab + i), 0ULL, (unsigned long long)v) == 0ULL) break;
+    }
+}
+
 // Position of key `c` inside sub-bucket `sub`, or NO_LINK.  In joined mode
 // the sub-bucket is ignored and the key's smallest position is returned.
 GG_DEV uint32_t lookup_in_sub(const UnitigArgs& A, u128_t c, uint32_t sub) {
-    const uint64_t lo = (uint64_t)c, hi = (uint64_t)(c >> 64);
-    const uint32_t b = A.log_buckets ? (uint32_t)(index_hash(lo, hi) >> (64 - A.log_buckets)) : 0u;
+    const uint64_t h = index_hash((uint64_t)c, (uint64_t)(c >> 64));
+    const uint32_t fp = (uint32_t)(h >> 32);
     uint32_t best = NO_LINK;
-    for (uint64_t e = A.boff[b]; e < A.boff[b + 1]; e++) {
-        const bool eq = A.key_words == 1 ? A.idx_key[e] == lo : (A.idx_key[2 * e] == lo && A.idx_key[2 * e + 1] == hi);
-        if (!eq) continue;
+    for (uint64_t i = h & A.tab_mask;; i = (i + 1) & A.tab_mask) {
+        const uint64_t v = A.tab[i];
+        if (v == 0) break;
+        if ((uint32_t)(v >> 32) != fp) continue;
+        const uint32_t q = (uint32_t)v - 1u;
+        if (key_at(A, q) != c) continue;
         if (!A.global) {
-            if (A.out_sub[A.idx_pos[e]] == sub) return A.idx_pos[e];
-        } else if (A.idx_pos[e] < best) {
-            best = A.idx_pos[e];
+            if (A.out_sub[q] == sub) return q;
+        } else if (q < best) {
+            best = q;
         }
     }
     return best;
 }
 
 // Joined mode: marks each filled position that is its key's smallest one.
-__global__ void rep_kernel(UnitigArgs A, uint64_t n_pos) {
-    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < n_pos; p += (uint64_t)gridDim.x * blockDim.x)
-        A.rep[p] = slot_filled(A, p) && lookup_in_sub(A, key_at(A, p), 0) == (uint32_t)p;
+// Counts them into *n_nodes (blocks of 256 threads: one atomic per block).
+__global__ __launch_bounds__(256) void rep_kernel(UnitigArgs A, uint64_t n_pos) {
+    __shared__ unsigned int part[4];
+    unsigned int cnt = 0;
+    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < n_pos; p += (uint64_t)gridDim.x * blockDim.x) {
+        const bool r = slot_filled(A, p) && lookup_in_sub(A, key_at(A, p), 0) == (uint32_t)p;
+        A.rep[p] = r;
+        cnt += r ? 1u : 0u;
+    }
+    for (uint32_t d = 32; d; d >>= 1) cnt += __shfl_down(cnt, d);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned int t = part[0] + part[1] + part[2] + part[3];
+        if (t) atomicAdd(A.n_nodes, t);
+    }
 }
 
 __global__ void ext_kernel(UnitigArgs A, uint64_t n_pos) {
@@ -169,6 +199,41 @@ __global__ void link_kernel(UnitigArgs A, uint64_t n_pos) {
     }
 }
 
+// Last base of k-mer q read in orientation t (0: canonical, 1: reverse
+// complement): base k-1 of the key, or the complement of its base 0.
+GG_DEV uint32_t entry_base(const UnitigArgs& A, uint64_t q, uint32_t t) {
+    if (t) return ((uint32_t)A.out_key[q * A.key_words] & 3u) ^ 2u;
+    const uint32_t j = A.k - 1;
+    const uint64_t w = A.out_key[q * A.key_words + (j >> 5)];
+    return (uint32_t)(w >> (2 * (j & 31))) & 3u;
+}
+
+// Writes the bytes of one unitig at seq[off...] through 4-byte stores; the
+// first and last words, which it may share with its neighbours, byte by byte.
+struct SeqWriter {
+    uint8_t* seq;
+    uint64_t off, a;
+    uint32_t acc = 0;
+    GG_DEV SeqWriter(uint8_t* s, uint64_t o) : seq(s), off(o), a(o) {}
+    GG_DEV void put(uint32_t ch) {
+        acc |= ch << (8 * (uint32_t)(a & 3));
+        if ((a & 3) == 3) {
+            if (a - 3 >= off) {
+                *reinterpret_cast<uint32_t*>(seq + a - 3) = acc;
+            } else {
+                for (uint64_t b = off; b <= a; b++) seq[b] = (uint8_t)(acc >> (8 * (uint32_t)(b & 3)));
+            }
+            acc = 0;
+        }
+        a++;
+    }
+    GG_DEV void finish() {
+        if ((a & 3) == 0) return;  // the last word was stored
+        const uint64_t w0 = a & ~(uint64_t)3;
+        for (uint64_t b = w0 > off ? w0 : off; b < a; b++) seq[b] = (uint8_t)(acc >> (8 * (uint32_t)(b & 3)));
+    }
+};
+
 // Pass 1 (write == false): every end of a path walks to its other end; the
 // end with the smaller (key, side) records the unitig's bytes.  Pass 2
 // (write == true): those ends write the sequence, its sub-bucket and flags.
@@ -203,22 +268,21 @@ __global__ void walk_kernel(UnitigArgs A, uint64_t n_pos) {
             continue;
         }
         if (!head || A.head_off[e + 1] == A.head_off[e]) continue;
-        uint8_t* out = A.seq + A.head_off[e];
+        SeqWriter out(A.seq, A.head_off[e]);
         const uint64_t ui = A.head_idx[e];
-        const char B[4] = {'A', 'C', 'T', 'G'};
+        constexpr uint32_t ASCII = 'A' | ('C' << 8) | ('T' << 16) | ('G' << 24);
         const u128_t x = s == 0 ? key_at(A, p) : kmer_rc128(key_at(A, p), k);
-        for (uint32_t i = 0; i < k; i++) out[i] = (uint8_t)B[(uint32_t)(x >> (2 * i)) & 3u];
-        uint64_t o = k;
-        A.visited[p] = 1;
+        for (uint32_t i = 0; i < k; i++) out.put((ASCII >> (8 * ((uint32_t)(x >> (2 * i)) & 3u))) & 0xFFu);
+        if (A.mark_visited) A.visited[p] = 1;
         uint64_t cur = e;
         while (A.link[cur ^ 1] != NO_LINK) {
             cur = A.link[cur ^ 1];
             const uint64_t q = cur >> 1;
-            A.visited[q] = 1;
-            const u128_t y = (cur & 1) == 0 ? key_at(A, q) : kmer_rc128(key_at(A, q), k);
-            out[o++] = (uint8_t)B[(uint32_t)(y >> (2 * (k - 1))) & 3u];
+            if (A.mark_visited) A.visited[q] = 1;
+            out.put((ASCII >> (8 * entry_base(A, q, (uint32_t)(cur & 1)))) & 0xFFu);
         }
-        out[o] = '\n';
+        out.put('\n');
+        out.finish();
         const uint64_t qend = cur >> 1;
         const uint32_t exit_side = (uint32_t)((cur ^ 1) & 1);
         A.uni_sub[ui] = A.global ? 0u : A.out_sub[p];
