@@ -713,6 +713,101 @@ int ggcat_gpu_records_copy(ggcat_gpu_ctx* ctx, uint32_t* sub, uint32_t* len, uin
     return GGCAT_GPU_OK;
 }
 
+namespace {
+
+// crates/io/src/varint.rs:8-22 (encode_varint): 7 bits per byte, low first,
+// bit 7 set while more bytes follow.
+void put_varint(std::vector<uint8_t>& o, uint64_t v) {
+    do {
+        o.push_back((uint8_t)((v & 0x7F) | (v > 0x7F ? 0x80 : 0)));
+        v >>= 7;
+    } while (v);
+}
+
+// crates/io/src/varint.rs:26-57 (encode_varint_flags) with FlagsCount = 2:
+// the flags in bits 6-7 of the first byte, 5 value bits and a continuation
+// bit below them, then plain varint bytes.
+void put_varint_flags2(std::vector<uint8_t>& o, uint64_t v, uint8_t flags) {
+    o.push_back((uint8_t)((flags << 6) | (v & 0x1F) | (v > 0x1F ? 0x20 : 0)));
+    v >>= 5;
+    while (v) {
+        o.push_back((uint8_t)((v & 0x7F) | (v > 0x7F ? 0x80 : 0)));
+        v >>= 7;
+    }
+}
+
+}  // namespace
+
+int ggcat_gpu_records_serialize(ggcat_gpu_ctx* ctx, uint8_t* out, uint64_t cap, uint64_t* bucket_off,
+                                uint64_t* n_bytes) {
+    if (!ctx) return GGCAT_GPU_EINVAL;
+    if (!ctx->have_records) return fail(ctx, GGCAT_GPU_ESTATE, "no bucketing result");
+    HIPCHK(hipSetDevice(ctx->device));
+    const uint32_t SW = ctx->slot_words, k = ctx->P.k, sb = ctx->P.second_buckets_log;
+    const uint32_t n_buckets = ctx->n_sub >> sb;
+    uint32_t min_size_log = 0;
+    while ((1u << min_size_log) < k) min_size_log++;  // k.next_power_of_two().ilog2()
+    std::vector<uint32_t> slots(ctx->n_records * SW), pool(ctx->pool_words), cols;
+    std::vector<uint64_t> uoff(ctx->n_units + 1);
+    if (!slots.empty()) HIPCHK(hipMemcpy(slots.data(), ctx->slots.p, slots.size() * 4, hipMemcpyDeviceToHost));
+    if (!pool.empty()) HIPCHK(hipMemcpy(pool.data(), ctx->pool.p, pool.size() * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(uoff.data(), ctx->unit_off.p, uoff.size() * 8, hipMemcpyDeviceToHost));
+    if (ctx->colored && ctx->n_records) {
+        cols.resize(ctx->n_records);
+        HIPCHK(hipMemcpy(cols.data(), ctx->slot_color.p, cols.size() * 4, hipMemcpyDeviceToHost));
+    }
+    // Records are stored by unit, so the records of a bucket (its 2^sb
+    // sub-buckets) are consecutive and each bucket is one stream.
+    std::vector<uint8_t> o;
+    std::vector<uint64_t> boff(n_buckets + 1, 0);
+    uint32_t u = 0, cur_bucket = 0;
+    uint32_t last_color = 0xFFFFFFFFu;  // MinBkSingleColor::default()
+    for (uint64_t r = 0; r < ctx->n_records; r++) {
+        while (u + 1 < uoff.size() && uoff[u + 1] <= r) u++;
+        const uint32_t sub = u >> ctx->xb, bucket = sub >> sb;
+        if (bucket >= n_buckets) return fail(ctx, GGCAT_GPU_EHIP, "record outside the bucket range");
+        while (cur_bucket < bucket) {
+            boff[++cur_bucket] = o.size();
+            last_color = 0xFFFFFFFFu;  // each bucket stream starts afresh
+        }
+        const uint32_t* sl = slots.data() + r * SW;
+        const uint32_t meta = sl[0];
+        uint32_t L;
+        const uint32_t* bw;
+        if (meta & META_LONG) {
+            L = sl[1];
+            bw = pool.data() + ((uint64_t)sl[2] | ((uint64_t)sl[3] << 32));
+        } else {
+            L = meta & 0xFF;
+            bw = sl + 1;
+        }
+        if (L < k) return fail(ctx, GGCAT_GPU_EHIP, "record shorter than k");
+        // creads_utils.rs:374-432 write_to: second bucket byte, no
+        // multiplicity, length with minimizer position and flags
+        // (compressed_read.rs:435-467 encode_length), packed bases, color.
+        o.push_back((uint8_t)(sub & ((1u << sb) - 1)));
+        put_varint_flags2(o, ((uint64_t)(L - k) << min_size_log) | ((meta >> 10) & 0xFF), (uint8_t)((meta >> 8) & 3));
+        const uint8_t* bb = reinterpret_cast<const uint8_t*>(bw);
+        o.insert(o.end(), bb, bb + (L + 3) / 4);
+        const uint32_t tail = L & 3;  // bases past the end of the last byte are zero
+        if (tail) o.back() &= (uint8_t)((1u << (2 * tail)) - 1);
+        if (ctx->colored) {
+            // crates/colors/src/parsers/separate.rs:98-115: 0 repeats the
+            // previous record's color, otherwise color + 1
+            put_varint(o, cols[r] == last_color ? 0 : (uint64_t)cols[r] + 1);
+            last_color = cols[r];
+        }
+    }
+    while (cur_bucket < n_buckets) boff[++cur_bucket] = o.size();
+    if (n_bytes) *n_bytes = o.size();
+    if (bucket_off) memcpy(bucket_off, boff.data(), boff.size() * 8);
+    if (out) {
+        if (cap < o.size()) return fail(ctx, GGCAT_GPU_EINVAL, "serialization buffer too small");
+        memcpy(out, o.data(), o.size());
+    }
+    return GGCAT_GPU_OK;
+}
+
 int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     if (!ctx) return GGCAT_GPU_EINVAL;
     if (!ctx->have_records) return fail(ctx, GGCAT_GPU_ESTATE, "no bucketing result");

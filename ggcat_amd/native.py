@@ -121,6 +121,7 @@ def gpu_lib(path: str = GPU_LIB) -> C.CDLL:
     lib.ggcat_gpu_unitigs.argtypes = [V, _u64p, _u64p]
     lib.ggcat_gpu_unitigs_copy.argtypes = [V, _u8p, _u32p, _u8p]
     lib.ggcat_gpu_maximal_unitigs.argtypes = [V, _u64p, _u64p]
+    lib.ggcat_gpu_records_serialize.argtypes = [V, _u8p, C.c_uint64, _u64p, _u64p]
     lib.ggcat_gpu_unitigs_write_fasta.argtypes = [V, C.c_char_p, _u64p]
     lib.ggcat_gpu_extension_copy.argtypes = [V, _u8p]
     lib.ggcat_gpu_read_files.argtypes = [V, C.POINTER(C.c_char_p), C.c_uint32, C.c_uint32, C.c_int,
@@ -140,7 +141,7 @@ def gpu_lib(path: str = GPU_LIB) -> C.CDLL:
                  "ggcat_gpu_copy_device", "ggcat_gpu_index_build", "ggcat_gpu_index_colors", "ggcat_gpu_query",
                  "ggcat_gpu_serve_open", "ggcat_gpu_serve_submit", "ggcat_gpu_serve_poll", "ggcat_gpu_serve_fetch",
                  "ggcat_gpu_serve_stats", "ggcat_gpu_serve_close", "ggcat_gpu_unitigs", "ggcat_gpu_unitigs_copy",
-                 "ggcat_gpu_maximal_unitigs", "ggcat_gpu_unitigs_write_fasta",
+                 "ggcat_gpu_maximal_unitigs", "ggcat_gpu_unitigs_write_fasta", "ggcat_gpu_records_serialize",
                  "ggcat_gpu_extension_copy"):
         getattr(lib, name).restype = C.c_int
     _gpu[path] = lib
@@ -239,6 +240,7 @@ class GpuContext:
 
     def __init__(self, device: int = 0, lib_path: str = GPU_LIB):
         self.lib = gpu_lib(lib_path)
+        self._params: Params | None = None  # of the last bucketing call
         h = C.c_void_p()
         rc = self.lib.ggcat_gpu_create(device, C.byref(h))
         if rc != OK:
@@ -264,6 +266,7 @@ class GpuContext:
     def bucketing(self, data, params: Params, colors: tuple[np.ndarray, np.ndarray] | None = None,
                   device_ptr: int | None = None, n_bytes: int | None = None) -> None:
         """data: uint8 numpy array (host) or None with device_ptr/n_bytes (device-resident)."""
+        self._params = params
         if device_ptr is not None:
             ptr, n, on_dev = C.c_void_p(device_ptr), int(n_bytes), 1
         else:
@@ -304,6 +307,21 @@ class GpuContext:
                                                     _ptr(r.flags, _u8p), _ptr(r.minpos, _u16p), _ptr(r.color, _u32p),
                                                     _ptr(r.off, _u64p), _ptr(r.bases, _u8p)), "records_copy")
         return r
+
+    def records_serialize(self) -> tuple[bytes, np.ndarray]:
+        """The last bucketing result in the reference's bucket-file record
+        layout: (all bucket streams concatenated, bucket offsets)."""
+        nr, ns, nb = C.c_uint64(), C.c_uint32(), C.c_uint64()
+        self._check(self.lib.ggcat_gpu_records_info(self.h, C.byref(nr), C.byref(ns), C.byref(nb)), "records_info")
+        sb = self._params.second_buckets_log if self._params else 0
+        off = np.zeros((ns.value >> sb) + 1, np.uint64)
+        n = C.c_uint64()
+        self._check(self.lib.ggcat_gpu_records_serialize(self.h, None, 0, _ptr(off, _u64p), C.byref(n)),
+                    "ggcat_gpu_records_serialize")
+        out = np.zeros(max(n.value, 1), np.uint8)
+        self._check(self.lib.ggcat_gpu_records_serialize(self.h, _ptr(out, _u8p), out.size, _ptr(off, _u64p),
+                                                         C.byref(n)), "ggcat_gpu_records_serialize")
+        return out[:n.value].tobytes(), off
 
     def kmers_merge(self) -> None:
         self._check(self.lib.ggcat_gpu_kmers_merge(self.h), "ggcat_gpu_kmers_merge")

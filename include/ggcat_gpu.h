@@ -98,6 +98,29 @@ int ggcat_gpu_records_info(ggcat_gpu_ctx* ctx, uint64_t* n_records, uint32_t* n_
 int ggcat_gpu_records_copy(ggcat_gpu_ctx* ctx, uint32_t* sub, uint32_t* len, uint8_t* flags, uint16_t* minpos,
                            uint32_t* color, uint64_t* off, uint8_t* bases);
 
+/* Serializes the last bucketing result in the byte layout of the
+ * reference's bucket files, so that its unchanged downstream can read it:
+ * CompressedReadsBucketDataSerializer::write_to
+ * (crates/io/src/concurrent/temp_reads/creads_utils.rs:374-432) as
+ * instantiated by the minimizer bucketing
+ * (crates/minimizer_bucketing/src/lib.rs:271-283: WithSecondBucket,
+ * NoMultiplicity, AssemblerMinimizerPosition, two flag bits, min_size = k).
+ * Per record: the second-bucket byte; varint_flags(((len - k) <<
+ * log2(next_pow2(k))) | minimizer position, flags)
+ * (crates/io/src/compressed_read.rs:435-467, crates/io/src/varint.rs:26-57);
+ * the bases packed 4 per byte, base i in bits 2(i mod 4) of byte i / 4, as
+ * stored (crates/io/src/compressed_read.rs:610-633); for colored builds the
+ * color as a varint, 0 when it repeats the previous record's, else color + 1
+ * (crates/colors/src/parsers/separate.rs:98-115).  Bucket b's stream (its
+ * 2^sb sub-buckets; the duplicates bucket is the last) is
+ * out[bucket_off[b] .. bucket_off[b + 1]); bucket_off holds
+ * n_sub / 2^sb + 1 entries.  The record order inside a stream is the
+ * device's (the reference's depends on thread timing).  out == NULL: only
+ * *n_bytes and bucket_off are filled.  Host buffers; out needs cap >=
+ * *n_bytes. */
+int ggcat_gpu_records_serialize(ggcat_gpu_ctx* ctx, uint8_t* out, uint64_t cap, uint64_t* bucket_off,
+                                uint64_t* n_bytes);
+
 /* Per-sub-bucket canonical k-mer counting of the last bucketing result.
  * Replaces kmers_merge (crates/assembler_kmers_merge/src/lib.rs:159-284),
  * called at crates/assembler/src/lib.rs:269-319, for its k-mer table: every

@@ -308,6 +308,17 @@ def test_unitigs_and_extension_flags(gpu_ctx, k, m, rep, err):
         sorted(zip(osol["sub"].tolist(), osol["key_lo"].tolist(), osol["key_hi"].tolist(), o.extension(k).tolist()))
 
 
+@pytest.mark.parametrize("k,m", [(31, 15), (63, 21)])
+def test_records_serialize(gpu_ctx, k, m):
+    from tests.test_serialize import _device_streams, _oracle_streams
+    g = synth.genome(93, 100_000, repeat_frac=0.2, rep_min=300, rep_max=2000)
+    data = synth.reads(g, 94, 10_000, 150, 0.01)
+    _say(f"case serialize k={k} m={m} bytes={data.size}")
+    gpu_ctx.bucketing(data, _params(k, m, 5, 2, 2))
+    o = O.bucketing(data, k, m, 5, 2, threads=4)
+    assert _device_streams(gpu_ctx, k, False, 5) == _oracle_streams(o, k, 2, False)
+
+
 @pytest.mark.parametrize("k,m,rep", [(31, 15, 0.2), (45, 21, 0.2)])
 def test_maximal_unitigs_and_fasta(gpu_ctx, tmp_path, k, m, rep):
     from tests.test_unitigs import canon_unitigs
