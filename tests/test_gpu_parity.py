@@ -322,15 +322,19 @@ def test_records_serialize(gpu_ctx, k, m):
 @pytest.mark.parametrize("k,m,rep", [(31, 15, 0.2), (45, 21, 0.2)])
 def test_maximal_unitigs_and_fasta(gpu_ctx, tmp_path, k, m, rep):
     from tests.test_unitigs import canon_unitigs
+    from tests.test_maximal_unitigs import _random_genome, _tiled
     g = synth.genome(79, 200_000, repeat_frac=rep, rep_min=300, rep_max=2000)
-    data = synth.reads(g, 80, 30_000, 150, 0.005)
+    circ = _random_genome(80, 3000)  # a circular genome: one cycle among the paths
+    data = np.concatenate([synth.reads(g, 80, 30_000, 150, 0.005), _tiled([circ + circ[:200]])])
     _say(f"case maximal unitigs k={k} m={m} bytes={data.size}")
     gpu_ctx.bucketing(data, _params(k, m, 5, 2, 2))
     gpu_ctx.kmers_merge()
     gu = gpu_ctx.unitigs(joined=True)
     o = O.bucketing(data, k, m, 5, 2, threads=4)
     o.merge(k, 5, 2, HASH_SEQ, 2, 4)
-    assert canon_unitigs(*gu) == canon_unitigs(*o.unitigs(k, joined=True))
+    want = canon_unitigs(*o.unitigs(k, joined=True))
+    assert canon_unitigs(*gu) == want
+    assert sum(1 for _, _, f in want if f == 4) >= 1
     path = str(tmp_path / "unitigs.fa")
     assert gpu_ctx.write_unitigs_fasta(path) == len(gu[0])
     lines = open(path, "rb").read().split(b"\n")
