@@ -292,12 +292,9 @@ def main() -> None:
     # unitig chains inside the sub-buckets of the last step's graph (device
     # kernels only; the sequences stay in HBM)
     t = time.perf_counter()
-    ctx.index_build()
-    t_index = time.perf_counter() - t
-    t = time.perf_counter()
     n_unitigs, unitig_bytes = ctx.unitig_counts()
     out["unitigs"] = {"unitigs": n_unitigs, "bytes": unitig_bytes, "kernels_ms": round(float(ctx.timings()[7]), 2),
-                      "call_ms": round((time.perf_counter() - t) * 1e3, 1), "index_ms": round(t_index * 1e3, 1)}
+                      "call_ms": round((time.perf_counter() - t) * 1e3, 1)}
     # maximal unitigs of the whole solid set (joined across sub-buckets)
     t = time.perf_counter()
     n_max, max_bytes = ctx.unitig_counts(joined=True)
