@@ -70,8 +70,11 @@ def test_empty_and_separators(gpu_ctx):
         gpu_ctx.bucketing(data, _params(31, 15, 4, 2))
         r = gpu_ctx.records()
         assert r.sub.size == 0
+        buf, off = gpu_ctx.records_serialize()
+        assert buf == b"" and not off.any()
         gpu_ctx.kmers_merge()
         assert gpu_ctx.solid().mult.size == 0
+        assert gpu_ctx.unitigs()[0] == [] and gpu_ctx.unitigs(joined=True)[0] == []
 
 
 def test_rk128_k63(gpu_ctx):

@@ -194,3 +194,16 @@ def test_serialized_colored_streams(emu_ctx):
     emu_ctx.bucketing(data, Params(31, 15, 3, 2, 0, HASH_SEQ, 1), colors=(starts, colors))
     o = O.bucketing_colored(data, 31, 15, 3, 2, starts, colors, threads=2)
     assert _device_streams(emu_ctx, 31, True, 3) == _oracle_streams(o, 31, 2, True)
+
+
+def test_empty_inputs_emulated(emu_ctx, tmp_path):
+    """Empty and separator-only inputs through every output of a build."""
+    from ggcat_amd.native import Params
+    for data in (np.zeros(0, np.uint8), np.frombuffer(b"\n\nNNNN\nACGT\n", np.uint8).copy()):
+        emu_ctx.bucketing(data, Params(31, 15, 4, 2, 0, HASH_SEQ, 2))
+        buf, off = emu_ctx.records_serialize()
+        assert buf == b"" and off.size == (1 << 4) + 2 and not off.any()
+        emu_ctx.kmers_merge()
+        assert emu_ctx.unitigs()[0] == [] and emu_ctx.unitigs(joined=True)[0] == []
+        path = str(tmp_path / "empty.fa")
+        assert emu_ctx.write_unitigs_fasta(path) == 0 and open(path, "rb").read() == b""
