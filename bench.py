@@ -109,6 +109,27 @@ def cpu_baseline(wl, threads: int) -> dict:
                       f"{threads} threads, {dt:.2f} s"}
 
 
+def cpu_unitig_baseline(wl, threads: int) -> tuple[dict, dict]:
+    """The oracle's unitig constructions (single-threaded) on the same read-set
+    shape at 1/40 of the size (built with `threads` threads first)."""
+    from ggcat_amd import synth
+    from oracle import binding as O
+
+    sw = synth.config2(0.025)
+    r = O.bucketing(sw.data, sw.k, sw.m, sw.buckets_log, sw.second_buckets_log, threads=threads)
+    r.merge(sw.k, sw.buckets_log, sw.second_buckets_log, sw.hash_kind, sw.min_mult, threads)
+    n = int(r.solid()["sub"].size)
+    out = []
+    for joined in (False, True):
+        t = time.perf_counter()
+        r.unitigs(sw.k, joined=joined)
+        dt = time.perf_counter() - t
+        out.append({"value": round(n / dt, 1), "unit": "solid k-mers/s", "cores": 1, "kind": "port",
+                    "sample": f"{sw.name} ({sw.bases} bases, {n} solid k-mers); oracle "
+                              f"{'joined' if joined else 'per-sub-bucket'} unitigs, {dt:.2f} s"})
+    return out[0], out[1]
+
+
 def serve_measurement(ctx, rate_hz: float, n_requests: int) -> dict:
     """Colored query serving: the 256-genome colored graph (one color per
     genome, k=31 m=15 s=2) built and indexed in HBM, then a Poisson stream of
@@ -293,13 +314,17 @@ def main() -> None:
     # kernels only; the sequences stay in HBM)
     t = time.perf_counter()
     n_unitigs, unitig_bytes = ctx.unitig_counts()
-    out["unitigs"] = {"unitigs": n_unitigs, "bytes": unitig_bytes, "kernels_ms": round(float(ctx.timings()[7]), 2),
-                      "call_ms": round((time.perf_counter() - t) * 1e3, 1)}
+    k_ms = float(ctx.timings()[7])
+    out["unitigs"] = {"unitigs": n_unitigs, "bytes": unitig_bytes, "kernels_ms": round(k_ms, 2),
+                      "call_ms": round((time.perf_counter() - t) * 1e3, 1),
+                      "solid_kmers_per_s": round(n / (k_ms / 1e3), 1) if k_ms > 0 else None}
     # maximal unitigs of the whole solid set (joined across sub-buckets)
     t = time.perf_counter()
     n_max, max_bytes = ctx.unitig_counts(joined=True)
-    out["maximal_unitigs"] = {"unitigs": n_max, "bytes": max_bytes, "kernels_ms": round(float(ctx.timings()[7]), 2),
-                              "call_ms": round((time.perf_counter() - t) * 1e3, 1)}
+    k_ms = float(ctx.timings()[7])
+    out["maximal_unitigs"] = {"unitigs": n_max, "bytes": max_bytes, "kernels_ms": round(k_ms, 2),
+                              "call_ms": round((time.perf_counter() - t) * 1e3, 1),
+                              "solid_kmers_per_s": round(n / (k_ms / 1e3), 1) if k_ms > 0 else None}
     ctx.free(dptr)
     if not a.no_serve:
         out["query"] = serve_measurement(ctx, a.serve_rate, a.serve_requests)
@@ -308,6 +333,9 @@ def main() -> None:
         threads = min(os.cpu_count() or 1, 64)
         log(f"[bench] CPU baseline with {threads} threads ...")
         out["cpu_baseline"] = cpu_baseline(wl, threads)
+        if wl.hash_kind == 0:
+            log("[bench] CPU unitig baselines ...")
+            out["unitigs"]["cpu_baseline"], out["maximal_unitigs"]["cpu_baseline"] = cpu_unitig_baseline(wl, threads)
     print(json.dumps(out), flush=True)
 
 
