@@ -207,3 +207,6 @@ def test_empty_inputs_emulated(emu_ctx, tmp_path):
         assert emu_ctx.unitigs()[0] == [] and emu_ctx.unitigs(joined=True)[0] == []
         path = str(tmp_path / "empty.fa")
         assert emu_ctx.write_unitigs_fasta(path) == 0 and open(path, "rb").read() == b""
+    from ggcat_amd.native import GGCATGpuError
+    with pytest.raises(GGCATGpuError, match="cannot create"):
+        emu_ctx.write_unitigs_fasta(str(tmp_path / "missing-dir" / "u.fa"))
