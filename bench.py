@@ -130,6 +130,52 @@ def cpu_unitig_baseline(wl, threads: int) -> tuple[dict, dict]:
     return out[0], out[1]
 
 
+def end_to_end(ctx, wl, target_bases: int = 1_000_000_000) -> dict:
+    """File -> solid table: the first ~1 Gbase of the read set written as a
+    FASTA file (one '>' header line per read), then ggcat_gpu_build_from_files
+    (read + parse on the host, upload, bucketing, merge) timed as a whole."""
+    import tempfile
+
+    from ggcat_amd.native import BuildOptions
+
+    data = wl.data
+    cut = min(target_bases, data.size)
+    while cut < data.size and data[cut - 1] != ord("\n"):
+        cut += 1
+    data = data[:cut]
+    nl = data == ord("\n")
+    n_reads = int(np.count_nonzero(nl))
+    # record index of every byte; each record gains the two bytes ">\n" in front
+    rec = np.zeros(data.size, np.uint32)
+    np.cumsum(nl[:-1], dtype=np.uint32, out=rec[1:])
+    fasta = np.empty(data.size + 2 * n_reads, np.uint8)
+    pos = np.arange(data.size, dtype=np.uint64) + 2 * (rec.astype(np.uint64) + 1)
+    fasta[pos] = data
+    starts = np.concatenate([[0], np.flatnonzero(nl)[:-1] + 1]).astype(np.uint64)
+    hdr = starts + 2 * np.arange(n_reads, dtype=np.uint64)
+    fasta[hdr] = ord(">")
+    fasta[hdr + 1] = ord("\n")
+    del rec, pos
+    fd, path = tempfile.mkstemp(suffix=".fa")
+    os.close(fd)
+    try:
+        fasta.tofile(path)
+        bases = int(data.size - n_reads)
+        del fasta
+        opts = BuildOptions(wl.k, wl.m, wl.min_mult, 0, 0, wl.hash_kind, 0)
+        t = time.perf_counter()
+        used = ctx.build_from_files([path], opts)
+        dt = time.perf_counter() - t
+        tm = ctx.timings()
+    finally:
+        os.unlink(path)
+    return {"gbases_s": round(bases / dt / 1e9, 4), "wall_s": round(dt, 3), "bases": bases,
+            "file_bytes": int(data.size + 2 * n_reads), "device_bucketing_ms": round(float(tm[5]), 1),
+            "device_merge_ms": round(float(tm[6]), 1), "buckets_log": int(used.buckets_log),
+            "sample": "first ~1 Gbase of the same read set as a FASTA file; file read + parse + upload + "
+                      "bucketing + merge"}
+
+
 def serve_measurement(ctx, rate_hz: float, n_requests: int) -> dict:
     """Colored query serving: the 256-genome colored graph (one color per
     genome, k=31 m=15 s=2) built and indexed in HBM, then a Poisson stream of
@@ -191,6 +237,7 @@ def main() -> None:
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the colored query serving measurement")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default schedule")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the file-to-solid-table measurement")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson request rate of the serving run (1/s)")
     ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--calib-copy", action="store_true",
@@ -326,6 +373,9 @@ def main() -> None:
                               "call_ms": round((time.perf_counter() - t) * 1e3, 1),
                               "solid_kmers_per_s": round(n / (k_ms / 1e3), 1) if k_ms > 0 else None}
     ctx.free(dptr)
+    if not a.no_e2e:
+        log("[bench] end-to-end from a FASTA file ...")
+        out["end_to_end"] = end_to_end(ctx, wl)
     if not a.no_serve:
         out["query"] = serve_measurement(ctx, a.serve_rate, a.serve_requests)
     ctx.close()
