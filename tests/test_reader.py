@@ -197,3 +197,23 @@ def test_plan_bucket_logs(bases, b, sb):
     # an explicit bucket count wins; the second-bucket count stays automatic
     q = native.plan_build(native.BuildOptions(k=31, m=15, buckets_log=9), bases)
     assert (q.buckets_log, q.second_buckets_log) == (9, sb)
+
+
+def test_many_files_keep_their_order(tmp_path):
+    # files are read concurrently and concatenated in the order given
+    rng = np.random.default_rng(3)
+    paths, want = [], []
+    for i in range(40):
+        recs = [bytes(rng.choice(np.frombuffer(b"ACGT", np.uint8), int(rng.integers(1, 300))))
+                for _ in range(int(rng.integers(1, 6)))]
+        body = b"".join(b">r\n" + r + b"\n" for r in recs)
+        paths.append(write(tmp_path, f"f{i}.fa", body))
+        want.append(recs)
+    recs, r = records(paths, colors=True)
+    assert recs == [x for f in want for x in f]
+    starts = np.cumsum([0] + [sum(len(x) + 1 for x in f) for f in want[:-1]])
+    assert r.color_starts.tolist() == starts.tolist()
+    assert r.colors.tolist() == list(range(40))
+    bad = paths[:5] + [str(tmp_path / "missing.fa")] + paths[5:]
+    with pytest.raises(native.GGCATGpuError, match="cannot open"):
+        native.read_files(bad, 31)
