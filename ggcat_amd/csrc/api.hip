@@ -1515,6 +1515,7 @@ int build_unitigs(ggcat_gpu_ctx* ctx, bool joined, uint64_t* n_unitigs, uint64_t
     HIPCHK(hipSetDevice(ctx->device));
     const uint64_t n_pos = (uint64_t)ctx->n_chunks_used * OUT_CHUNK + ctx->n_tail;
     const uint64_t ne = 2 * n_pos;
+    if (ne >= (uint64_t)NO_LINK) return fail(ctx, GGCAT_GPU_EINVAL, "solid table too large for unitig links");
     ENSURE(ctx->u_nb, ne * 4 + 8);
     ENSURE(ctx->u_link, ne * 4 + 8);
     ENSURE(ctx->u_ext, n_pos + 8);
@@ -1528,7 +1529,6 @@ int build_unitigs(ggcat_gpu_ctx* ctx, bool joined, uint64_t* n_unitigs, uint64_t
     uint64_t tab_slots = 1024;
     while (tab_slots < 2 * n_pos) tab_slots <<= 1;
     ENSURE(ctx->u_tab, tab_slots * 8);
-    if (ne >= (uint64_t)NO_LINK) return fail(ctx, GGCAT_GPU_EINVAL, "solid table too large for unitig links");
     ctx->have_unitigs = false;
     UnitigArgs U{};
     U.out_key = P_<uint64_t>(ctx->out_key);
