@@ -375,7 +375,10 @@ def main() -> None:
     ctx.free(dptr)
     if not a.no_e2e:
         log("[bench] end-to-end from a FASTA file ...")
-        out["end_to_end"] = end_to_end(ctx, wl)
+        try:
+            out["end_to_end"] = end_to_end(ctx, wl)
+        except (OSError, RuntimeError, MemoryError) as e:  # e.g. no room for the temporary file
+            out["end_to_end"] = {"error": f"{type(e).__name__}: {e}"}
     if not a.no_serve:
         out["query"] = serve_measurement(ctx, a.serve_rate, a.serve_requests)
     ctx.close()
