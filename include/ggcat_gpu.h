@@ -284,7 +284,10 @@ int ggcat_gpu_maximal_unitigs(ggcat_gpu_ctx* ctx, uint64_t* n_unitigs, uint64_t*
 /* Writes the last unitig result as FASTA, one record per unitig,
  * ">{id} LN:i:{length}" then the sequence on one line (the uncolored header
  * of crates/sequence_output/src/structured_sequences/fasta.rs:153-200; ids
- * count from 0 in result order).  *n_written (optional) = records written. */
+ * count from 0 in result order).  For colored builds the color-run tags
+ * (C:...) of the reference's colored header are not written: their subset
+ * ids come from the colormap, which this library does not produce.
+ * *n_written (optional) = records written. */
 int ggcat_gpu_unitigs_write_fasta(ggcat_gpu_ctx* ctx, const char* path, uint64_t* n_written);
 
 /* ---- queries ------------------------------------------------------------
