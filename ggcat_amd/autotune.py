@@ -61,7 +61,12 @@ def _digest(ctx) -> tuple:
     s = ctx.solid()
     keys = s.keys.astype(np.uint64)
     with np.errstate(over="ignore"):
-        h = (keys[:, 0] * np.uint64(0x9E3779B97F4A7C15)) ^ (s.sub.astype(np.uint64) << np.uint64(32)) \
+        # every key word is mixed in (2-word keys for k > 32 and rabin-karp128)
+        kh = np.zeros(keys.shape[0], np.uint64)
+        for j in range(keys.shape[1]):
+            kh = (kh ^ keys[:, j]) * np.uint64(0x9E3779B97F4A7C15)
+            kh ^= kh >> np.uint64(29)
+        h = kh ^ (s.sub.astype(np.uint64) << np.uint64(32)) \
             ^ (s.mult.astype(np.uint64) << np.uint64(8)) ^ s.flags.astype(np.uint64)
         return int(s.mult.size), int(np.bitwise_xor.reduce(h)) if h.size else 0, int(h.sum(dtype=np.uint64))
 

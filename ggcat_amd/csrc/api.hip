@@ -305,25 +305,32 @@ struct Server {
 
 namespace {
 
-// Admits waiting requests in arrival order while their memory fits.
+// Layout of one request inside its device / pinned range (same offsets in
+// both): query bytes, offsets, work counter, then the results.
+void serve_layout(const ggcat_gpu_ctx* ctx, ServeReq& r) {
+    const uint32_t nq = r.n_queries, nc = ctx->colored ? ctx->n_color_ids : 0;
+    uint64_t o = 0;
+    auto place = [&](uint64_t bytes) {
+        const uint64_t at = o;
+        o = (o + bytes + 255) & ~255ull;
+        return at;
+    };
+    r.o_seq = place(r.n_bytes + 16);
+    r.o_off = place(((uint64_t)nq + 1) * 8);
+    r.o_work = place(8);
+    r.o_matched = place((uint64_t)nq * 8);
+    r.o_counts = place((uint64_t)nq * nc * 4 + 4);
+    r.bytes = o;
+}
+
+// Admits waiting requests in arrival order while their memory fits.  Every
+// waiting request fits the budget on its own (serve_submit rejects the
+// others), so the queue always drains once earlier requests are fetched.
 int serve_admit(ggcat_gpu_ctx* ctx) {
     Server& S = *ctx->server;
     while (!S.waiting.empty()) {
         ServeReq& r = S.reqs[S.waiting.front()];
         const uint32_t nq = r.n_queries, nc = ctx->colored ? ctx->n_color_ids : 0;
-        uint64_t o = 0;
-        auto place = [&](uint64_t bytes) {
-            const uint64_t at = o;
-            o = (o + bytes + 255) & ~255ull;
-            return at;
-        };
-        r.o_seq = place(r.n_bytes + 16);
-        r.o_off = place(((uint64_t)nq + 1) * 8);
-        r.o_work = place(8);
-        r.o_matched = place((uint64_t)nq * 8);
-        r.o_counts = place((uint64_t)nq * nc * 4 + 4);
-        r.bytes = o;
-        if (r.bytes > S.budget) return fail(ctx, GGCAT_GPU_ENOMEM, "request larger than the serving memory budget");
         uint64_t doff, poff;
         if (!S.dalloc.alloc(r.bytes, &doff)) break;
         if (!S.palloc.alloc(r.bytes, &poff)) {
@@ -1413,6 +1420,14 @@ int ggcat_gpu_serve_submit(ggcat_gpu_ctx* ctx, const uint8_t* queries, uint64_t 
     r.src = queries;
     r.n_bytes = n_bytes;
     r.n_queries = n_queries;
+    serve_layout(ctx, r);
+    if (r.bytes > S.budget) {
+        // could never be admitted: rejected here, so it cannot block the queue
+        const uint64_t need = r.bytes;
+        S.reqs.erase(t);
+        return fail(ctx, GGCAT_GPU_ENOMEM, "request larger than the serving memory budget (" +
+                                               std::to_string(need) + " > " + std::to_string(S.budget) + " bytes)");
+    }
     S.waiting.push_back(t);
     *ticket = t;
     HIPCHK(hipSetDevice(ctx->device));

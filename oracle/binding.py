@@ -78,6 +78,13 @@ def lib() -> C.CDLL:
     L.orc_solid_colors_total.restype = C.c_size_t
     L.orc_solid_colors_get.argtypes = [V, _u32p, _u32p]
     L.orc_solid_colors_get.restype = None
+    L.orc_merge_sorting.argtypes = [V, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_uint32, C.c_int,
+                                    _u64p]
+    L.orc_merge_sorting.restype = None
+    L.orc_sorting_count.argtypes = [V]
+    L.orc_sorting_count.restype = C.c_size_t
+    L.orc_sorting_get.argtypes = [V, _u32p, _u64p, _u64p, _u32p, _u8p]
+    L.orc_sorting_get.restype = None
     L.orc_bruteforce_counts.restype = V
     _lib = L
     return L
@@ -135,6 +142,21 @@ class Result:
 
     def merge(self, k: int, b: int, sb: int, kind: int, min_mult: int, threads: int = 1) -> None:
         lib().orc_merge(self.h, k, b, sb, kind, min_mult, threads)
+
+    def merge_sorting(self, k: int, m: int, b: int, sb: int, kind: int, min_mult: int, threads: int = 1) -> dict:
+        """The sorting extender's rule on the normal sub-buckets: entries
+        (sub, key_lo, key_hi, mult, flags = extra bases, bit 0 backward, bit 1
+        forward) and stats (both_extra, groups, centred)."""
+        L = lib()
+        st = np.zeros(3, np.uint64)
+        L.orc_merge_sorting(self.h, k, m, b, sb, kind, min_mult, threads, _p(st, _u64p))
+        n = L.orc_sorting_count(self.h)
+        s = dict(sub=np.empty(n, np.uint32), key_lo=np.empty(n, np.uint64), key_hi=np.empty(n, np.uint64),
+                 mult=np.empty(n, np.uint32), flags=np.empty(n, np.uint8))
+        L.orc_sorting_get(self.h, _p(s["sub"], _u32p), _p(s["key_lo"], _u64p), _p(s["key_hi"], _u64p),
+                          _p(s["mult"], _u32p), _p(s["flags"], _u8p))
+        s["stats"] = dict(both_extra=int(st[0]), groups=int(st[1]), centred=int(st[2]))
+        return s
 
     def solid(self) -> dict:
         L = lib()

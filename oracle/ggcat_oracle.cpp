@@ -20,8 +20,10 @@
 //   reference extension loop (unitig_sequential) pinned against unitig_chains on inputs where the
 //                                         reference's visiting order cannot matter
 //   extension flags (orc_extension)       pinned: equal to neighbour counts of the solid set (same test)
-//   sorting extender (normal buckets)     parity unpinned: not restated; the reference intends it to
-//                                         give the hash-map rule's solid set, which is pinned above
+//   sorting extender (normal buckets,     pinned: independent pure-Python restatement (explicit
+//     merge_subbucket_sorting)            supertigs) on small inputs; equal (k-mer, multiplicity) set
+//                                         to the hash-map rule on every test shape; exact while no
+//                                         centred k-mer holds both extra bases (counted, tests assert 0)
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -471,6 +473,121 @@ void merge_subbucket(const RecordSet& R, const std::vector<size_t>& idx, uint32_
     }
 }
 
+// ---------------------------------------------------- sorting extender ---
+// Normal sub-buckets (neither the duplicates bucket nor an outlier) are not
+// counted with the hash map in the reference.  add_read
+// (crates/assembler_kmers_merge/src/map_processor.rs:127-166) groups the
+// sub-bucket's records by the hash of their minimizer m-mer in a
+// FuzzyHashmap (crates/utils/src/fuzzy_hashmap.rs:5-127); process_map
+// (crates/assembler_kmers_merge/src/final_executor.rs:316-437) outputs a
+// group of one record as it is when its multiplicity reaches the cutoff and
+// runs SortingExtender::process_reads
+// (crates/assembler_kmers_merge/src/unitigs_extender/sorting.rs:641-1050)
+// on larger groups.  process_reads sorts the records by the bases after and
+// before their minimizer (centred suffix and prefix) and walks runs of
+// records that share at least k bases around the minimizer
+// (sorting.rs:420-444): a k-mer is identified by its bases together with
+// its offset from the minimizer position, and the multiplicity of a
+// supertig is the sum of the multiplicities of the records holding it there
+// (:431).  A supertig of exactly one k-mer that is the last k-mer of a record
+// without READ_FLAG_INCL_END and the first k-mer of a record without
+// READ_FLAG_INCL_BEGIN is a confirmed double-counted k-mer: its
+// multiplicity is halved and both extra-base flags are cleared (:454-486).
+// A supertig is kept iff its multiplicity >= min_multiplicity (:501).
+//
+// Restated here per centred k-mer: groups are keyed by the exact minimizer
+// m-mer (records whose minimizer m-mers differ share no centred k-mer, since
+// the k shared bases would cover the minimizer, so the fuzzy grouping only
+// changes which records are sorted together, not what they share), and the
+// halving is applied to every centred k-mer that has both extra bases.  The
+// second step equals the reference whenever such a k-mer is alone in its
+// supertig; `both_extra` counts those k-mers (a centred k-mer cannot be the
+// last k-mer of one group and the first of another: both groups would have
+// the same minimizer at the same position, which is no split,
+// crates/hashes/src/rolling/batch_minqueue.rs:155-160), and the tests check
+// that it stays zero.  Output flags are the extra bases in stored
+// orientation: bit 0 backward (first k-mer, no INCL_BEGIN), bit 1 forward.
+struct CentredKey {
+    u128 key;  // canonical k-mer key
+    uint64_t minimizer;  // 2-bit packed minimizer m-mer (stored orientation), m <= 32
+    int32_t offset;  // k-mer start - minimizer position
+    uint8_t fwd;  // orientation of the stored k-mer relative to its canonical key
+    bool operator==(const CentredKey& o) const {
+        return key == o.key && minimizer == o.minimizer && offset == o.offset && fwd == o.fwd;
+    }
+};
+struct CentredHash {
+    size_t operator()(const CentredKey& c) const {
+        return U128Hash()(c.key ^ ((u128)c.minimizer << 64)) ^ ((size_t)(uint32_t)c.offset * 0x9E3779B97F4A7C15ULL) ^
+               c.fwd;
+    }
+};
+struct CentredEntry {
+    uint64_t count = 0;
+    bool fwd_extra = false, bwd_extra = false;
+    std::vector<uint32_t> cols;
+};
+
+struct SortingStats {
+    uint64_t both_extra = 0;  // centred k-mers holding both extra bases (halved)
+    uint64_t groups = 0;  // minimizer groups with more than one record
+    uint64_t centred = 0;  // centred k-mers seen
+};
+
+void merge_subbucket_sorting(const RecordSet& R, const std::vector<size_t>& idx, uint32_t sub, unsigned k,
+                             unsigned m, int kind, uint32_t min_mult, bool colored, SolidSet& out,
+                             SortingStats& st, std::vector<uint8_t>& codes) {
+    if (colored && out.colors_off.empty()) out.colors_off.push_back(0);
+    std::unordered_map<uint64_t, std::vector<size_t>> groups;
+    for (size_t r : idx) {
+        const uint8_t* b = R.bases.data() + R.off[r] + R.minpos[r];
+        uint64_t mm = 0;
+        for (unsigned j = 0; j < m; j++) mm |= (uint64_t)cbase(b[j]) << (2 * j);
+        groups[mm].push_back(r);
+    }
+    std::unordered_map<CentredKey, CentredEntry, CentredHash> map;
+    for (auto& g : groups) {
+        map.clear();
+        if (g.second.size() > 1) st.groups++;
+        for (size_t r : g.second) {
+            const uint32_t L = R.len[r];
+            const uint8_t* b = R.bases.data() + R.off[r];
+            codes.resize(L);
+            for (uint32_t i = 0; i < L; i++) codes[i] = cbase(b[i]);
+            const uint8_t fl = R.flags[r];
+            const uint32_t mult = R.mult[r];
+            const size_t last = L - k;
+            const int32_t p = R.minpos[r];
+            kmer_hashes(codes.data(), L, k, kind, [&](size_t i, u128 key, bool is_fwd) {
+                CentredEntry& e = map[CentredKey{key, g.first, (int32_t)i - p, (uint8_t)is_fwd}];
+                e.count += mult;
+                if (i == last && (fl & 2) == 0) e.fwd_extra = true;
+                if (i == 0 && (fl & 1) == 0) e.bwd_extra = true;
+                if (colored && (e.cols.empty() || e.cols.back() != R.color[r])) e.cols.push_back(R.color[r]);
+            });
+        }
+        for (auto& kv : map) {
+            st.centred++;
+            CentredEntry& e = kv.second;
+            uint64_t mult = e.count;
+            uint8_t flags = (uint8_t)((e.bwd_extra ? 1 : 0) | (e.fwd_extra ? 2 : 0));
+            // a group of one record is output as it is (final_executor.rs:316-320)
+            if (e.fwd_extra && e.bwd_extra && g.second.size() > 1) {
+                st.both_extra++;
+                mult /= 2;
+                flags = 0;
+            }
+            if (mult < min_mult) continue;
+            out.sub.push_back(sub);
+            out.key_lo.push_back((uint64_t)kv.first.key);
+            out.key_hi.push_back((uint64_t)(kv.first.key >> 64));
+            out.count.push_back((uint32_t)mult);
+            out.flags.push_back(flags);
+            if (colored) append_color_set(e.cols, out.colors_off, out.colors);
+        }
+    }
+}
+
 // ----------------------------------------------------- unitig chains ----
 // Intra-sub-bucket unitig extension over the solid seq-hash table
 // (crates/assembler_kmers_merge/src/unitigs_extender/hashmap.rs:162-297,
@@ -661,6 +778,8 @@ void unitig_sequential(const SubGraph& G, uint32_t sub, const std::vector<uint32
 struct Handle {
     RecordSet rec;
     SolidSet solid;
+    SolidSet solid_sorting;  // normal sub-buckets under the sorting extender's rule
+    SortingStats sorting_stats;
     UnitigOut unitigs;
     bool colored = false;
     double t_bucket = 0, t_merge = 0;
@@ -814,6 +933,59 @@ void orc_merge(void* hv, uint32_t k, uint32_t b, uint32_t sb, int kind, uint32_t
     h->solid = SolidSet();
     for (auto& p : parts) h->solid.append(p);
     h->t_merge = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// The sorting extender's solid k-mers (see merge_subbucket_sorting) for the
+// sub-buckets of the normal buckets (the duplicates bucket 2^b is counted
+// with the hash map in the reference too); stats[0..3] = centred k-mers
+// with both extra bases, minimizer groups of more than one record, centred
+// k-mers.  Entries stay in the handle (orc_sorting_count / orc_sorting_get).
+void orc_merge_sorting(void* hv, uint32_t k, uint32_t m, uint32_t b, uint32_t sb, int kind, uint32_t min_mult,
+                       int threads, uint64_t* stats) {
+    Handle* h = (Handle*)hv;
+    const uint32_t n_normal = (1u << b) << sb;
+    auto groups = group_by_sub(h->rec, ((1u << b) + 1) << sb);
+    if (threads < 1) threads = 1;
+    std::vector<SolidSet> parts(threads);
+    std::vector<SortingStats> pst(threads);
+    std::atomic<uint32_t> next{0};
+    std::vector<std::thread> th;
+    for (int t = 0; t < threads; t++) {
+        th.emplace_back([&, t] {
+            std::vector<uint8_t> codes;
+            for (;;) {
+                uint32_t s = next.fetch_add(1);
+                if (s >= n_normal) break;
+                if (groups[s].empty()) continue;
+                merge_subbucket_sorting(h->rec, groups[s], s, k, m, kind, min_mult, h->colored, parts[t], pst[t],
+                                        codes);
+            }
+        });
+    }
+    for (auto& x : th) x.join();
+    h->solid_sorting = SolidSet();
+    h->sorting_stats = SortingStats();
+    for (int t = 0; t < threads; t++) {
+        h->solid_sorting.append(parts[t]);
+        h->sorting_stats.both_extra += pst[t].both_extra;
+        h->sorting_stats.groups += pst[t].groups;
+        h->sorting_stats.centred += pst[t].centred;
+    }
+    if (stats) {
+        stats[0] = h->sorting_stats.both_extra;
+        stats[1] = h->sorting_stats.groups;
+        stats[2] = h->sorting_stats.centred;
+    }
+}
+size_t orc_sorting_count(void* hv) { return ((Handle*)hv)->solid_sorting.sub.size(); }
+void orc_sorting_get(void* hv, uint32_t* sub, uint64_t* key_lo, uint64_t* key_hi, uint32_t* count, uint8_t* flags) {
+    SolidSet& S = ((Handle*)hv)->solid_sorting;
+    size_t n = S.sub.size();
+    memcpy(sub, S.sub.data(), n * 4);
+    memcpy(key_lo, S.key_lo.data(), n * 8);
+    memcpy(key_hi, S.key_hi.data(), n * 8);
+    memcpy(count, S.count.data(), n * 4);
+    memcpy(flags, S.flags.data(), n);
 }
 
 // Unitig chains over the merged solid table (seq-hash keys).  mode 0: the

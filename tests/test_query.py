@@ -109,5 +109,18 @@ def test_query_and_serving_emulated():
         for i, rq in enumerate(reqs):
             alone = ctx.query(rq)
             assert all(np.array_equal(a, b) for a, b in zip(results[i], alone))
+        # a request larger than the whole budget is refused at submission and
+        # does not block the requests after it
+        ctx.serve_open(1, 8192)
+        from ggcat_amd.native import GGCATGpuError
+        with pytest.raises(GGCATGpuError, match="budget"):
+            ctx.serve_submit(synth.queries(gs, 70, 50, 400, 600, 0.02))
+        small = synth.queries(gs, 71, 2, 40, 60, 0.02)
+        t = ctx.serve_submit(small)
+        assert ctx.serve_poll(block=True) == t
+        got = ctx.serve_fetch(t, 2)
+        assert all(np.array_equal(a, b) for a, b in zip(got, ctx.query(small)))
+        assert ctx.serve_stats() == (0, 0)
+        ctx.serve_close()
     finally:
         ctx.close()
