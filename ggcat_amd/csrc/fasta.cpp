@@ -4,14 +4,15 @@
 // process_fastq): '>' starts a FASTA record, ';' lines are comments, sequence
 // lines are concatenated; FASTQ records are 4 lines.  Records longer than
 // max(4 MiB, 2(k-1)) are flushed in chunks that repeat the last k-1 bases
-// (:119-173), each chunk a record of its own.  Compressed inputs are read
-// through zlib (gzip); plain files pass through unchanged.  Normalisation
+// (:119-173), each chunk a record of its own.  Compressed inputs (gzip, lz4,
+// bz2, xz, zstd by the last extension) are decoded by decompress.cpp; plain
+// files pass through unchanged.  Normalisation
 // (:26-54) is not applied here: the device reads a/c/g/t as A/C/G/T and every
 // other byte ends an ACGT run, exactly like the 'N' it would become.
 //
 // Output: records, each followed by '\n'; with colors, the byte offset where
 // each file's records start and the file's color (its index in the list).
-#include <zlib.h>
+#include "decompress.cpp"
 
 #include <algorithm>
 #include <atomic>
@@ -67,11 +68,11 @@ inline std::string last_ext(const std::string& path) {
 // partial = true for a non-empty line piece cut by the block end (a trailing
 // '\r' removed), and a final call (empty, false, true).
 template <class F>
-bool for_each_line_piece(gzFile f, F&& cb) {
+bool for_each_line_piece(ByteSource& f, F&& cb) {
     const size_t BLOCK = 4u << 20;
     std::vector<uint8_t> buf(BLOCK);
     while (true) {
-        int r = gzread(f, buf.data(), (unsigned)BLOCK);
+        const long r = f.read(buf.data(), BLOCK);
         if (r < 0) return false;
         if (r == 0) {
             cb((const uint8_t*)nullptr, (size_t)0, false, true);
@@ -103,16 +104,9 @@ inline bool read_file(const std::string& path, uint32_t k, ReadResult& R) {
         return false;
     }
     const std::string ext = last_ext(path);
-    if (ext == "lz4" || ext == "bz2" || ext == "xz" || ext == "zst" || ext == "zstd") {
-        R.error = "unsupported compression '." + ext + "' of '" + path + "' (plain and gzip only)";
-        return false;
-    }
-    gzFile f = gzopen(path.c_str(), "rb");
-    if (!f) {
-        R.error = "cannot open '" + path + "'";
-        return false;
-    }
-    gzbuffer(f, 1 << 20);
+    std::unique_ptr<ByteSource> src = open_source(path, ext, &R.error);
+    if (!src) return false;
+    ByteSource& f = *src;
     {
         // FastaFileSequencesStream::get_estimated_bases_count: the file size,
         // halved when the last extension names a compression
@@ -184,7 +178,7 @@ inline bool read_file(const std::string& path, uint32_t k, ReadResult& R) {
             if (!partial) state = (state + 1) % 3;
         });
     }
-    gzclose(f);
+    src.reset();
     if (!ok) {
         R.error = "error while reading '" + path + "'";
         return false;
