@@ -13,6 +13,7 @@
 #include <deque>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <cstring>
 #include <string>
 #include <tuple>
@@ -383,6 +384,10 @@ int serve_admit(ggcat_gpu_ctx* ctx) {
 
 extern "C" {
 
+namespace {
+int noalloc_scratch_setup(int device);
+}  // namespace
+
 int ggcat_gpu_create(int device, ggcat_gpu_ctx** out) {
     if (!out) return GGCAT_GPU_EINVAL;
     *out = nullptr;
@@ -393,6 +398,8 @@ int ggcat_gpu_create(int device, ggcat_gpu_ctx** out) {
     for (int i = 0; i < 16 && e == hipSuccess; i++) e = hipEventCreate(&ctx->ev[i]);
     hipDeviceProp_t prop;
     if (e == hipSuccess && hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->n_cu = prop.multiProcessorCount;
+    // the no-allocation entry point's scratch array: its address on this device
+    if (e == hipSuccess && noalloc_scratch_setup(device) != GGCAT_GPU_OK) e = hipErrorInvalidSymbol;
     if (e != hipSuccess) {
         delete ctx;
         return GGCAT_GPU_EHIP;
@@ -1209,6 +1216,13 @@ int ggcat_gpu_copy_to_device(ggcat_gpu_ctx* ctx, void* dst, const void* src, uin
     return GGCAT_GPU_OK;
 }
 
+int ggcat_gpu_copy_to_host(ggcat_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return GGCAT_GPU_EINVAL;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return GGCAT_GPU_OK;
+}
+
 int ggcat_gpu_read_files(ggcat_gpu_ctx* ctx, const char* const* paths, uint32_t n_paths, uint32_t k, int colors,
                          uint8_t** joined, uint64_t* n_bytes, uint64_t** color_starts, uint32_t** color_ids,
                          uint32_t* n_colors, uint64_t* estimated_bases) {
@@ -1764,3 +1778,5 @@ int ggcat_gpu_extension_copy(ggcat_gpu_ctx* ctx, uint8_t* ext) {
 }
 
 }  // extern "C"
+
+#include "noalloc.hip"

@@ -109,6 +109,8 @@ def gpu_lib(path: str = GPU_LIB) -> C.CDLL:
     lib.ggcat_gpu_device_free.argtypes = [V, V]
     lib.ggcat_gpu_copy_to_device.argtypes = [V, V, V, C.c_uint64]
     lib.ggcat_gpu_copy_device.argtypes = [V, V, V, C.c_uint64]
+    lib.ggcat_gpu_copy_to_host.argtypes = [V, V, V, C.c_uint64]
+    lib.ggcat_gpu_query_device_noalloc.argtypes = [V, V, C.c_uint64, C.c_uint32, V, V, V]
     lib.ggcat_gpu_index_build.argtypes = [V]
     lib.ggcat_gpu_index_colors.argtypes = [V, _u32p]
     lib.ggcat_gpu_query.argtypes = [V, V, C.c_uint64, C.c_uint32, _u64p, _u64p, _u32p]
@@ -142,7 +144,7 @@ def gpu_lib(path: str = GPU_LIB) -> C.CDLL:
                  "ggcat_gpu_serve_open", "ggcat_gpu_serve_submit", "ggcat_gpu_serve_poll", "ggcat_gpu_serve_fetch",
                  "ggcat_gpu_serve_stats", "ggcat_gpu_serve_close", "ggcat_gpu_unitigs", "ggcat_gpu_unitigs_copy",
                  "ggcat_gpu_maximal_unitigs", "ggcat_gpu_unitigs_write_fasta", "ggcat_gpu_records_serialize",
-                 "ggcat_gpu_extension_copy"):
+                 "ggcat_gpu_extension_copy", "ggcat_gpu_copy_to_host", "ggcat_gpu_query_device_noalloc"):
         getattr(lib, name).restype = C.c_int
     _gpu[path] = lib
     return lib
@@ -457,6 +459,20 @@ class GpuContext:
 
     def copy_device(self, dst: int, src: int, nbytes: int) -> None:
         self._check(self.lib.ggcat_gpu_copy_device(self.h, C.c_void_p(dst), C.c_void_p(src), nbytes), "copy_device")
+
+    def download(self, ptr: int, out: np.ndarray) -> np.ndarray:
+        """Copies device bytes at ptr into the host array `out` (its whole size)."""
+        self._check(self.lib.ggcat_gpu_copy_to_host(self.h, out.ctypes.data_as(C.c_void_p), C.c_void_p(ptr),
+                                                    out.nbytes), "copy_to_host")
+        return out
+
+    def query_device_noalloc(self, d_queries: int, n_bytes: int, n_queries: int, d_query_kmers: int | None,
+                             d_matched: int, d_color_counts: int | None) -> None:
+        """ggcat_gpu_query_device_noalloc on caller-owned device buffers (pointers)."""
+        self._check(self.lib.ggcat_gpu_query_device_noalloc(self.h, C.c_void_p(d_queries), n_bytes, n_queries,
+                                                            C.c_void_p(d_query_kmers), C.c_void_p(d_matched),
+                                                            C.c_void_p(d_color_counts)),
+                    "ggcat_gpu_query_device_noalloc")
 
     def free(self, ptr: int) -> None:
         self._check(self.lib.ggcat_gpu_device_free(self.h, C.c_void_p(ptr)), "device_free")

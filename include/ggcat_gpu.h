@@ -349,6 +349,25 @@ int ggcat_gpu_copy_to_device(ggcat_gpu_ctx* ctx, void* dst, const void* src, uin
 /* Device-to-device copy on the context's stream with a plain copy kernel
  * (16-byte aligned pointers and size); returns when the copy is done. */
 int ggcat_gpu_copy_device(ggcat_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+/* Synchronous device-to-host copy. */
+int ggcat_gpu_copy_to_host(ggcat_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+
+/* ---- query batch that allocates nothing ----------------------------------
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (tight serving loops, code that must not call hipMalloc).  Same
+ * results as ggcat_gpu_query, but the queries (joined layout, n_queries
+ * records each ending with '\n') and the outputs are device pointers owned
+ * by the caller: d_query_kmers (n_queries entries, may be NULL), d_matched
+ * (n_queries) and, for a colored index, d_color_counts (n_queries *
+ * n_colors).  Everything else it needs (work cursor, newline counts, query
+ * offsets) lives in a fixed 4 MiB device array of the library, one per GPU,
+ * whose address is looked up by ggcat_gpu_create; the call makes no
+ * allocation.  Limits: at most 500000 queries and 256 MiB of query bytes per
+ * call (GGCAT_GPU_EINVAL beyond).  Calls that overlap on one GPU share that
+ * array and race: serialise them.  Returns when the results are written. */
+int ggcat_gpu_query_device_noalloc(ggcat_gpu_ctx* ctx, const uint8_t* d_queries, uint64_t n_bytes,
+                                   uint32_t n_queries, uint64_t* d_query_kmers, uint64_t* d_matched,
+                                   uint32_t* d_color_counts);
 
 #ifdef __cplusplus
 }

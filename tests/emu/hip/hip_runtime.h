@@ -152,6 +152,10 @@ inline hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }
 inline hipError_t hipEventRecord(hipEvent_t e, hipStream_t) { e->t = std::chrono::steady_clock::now(); return hipSuccess; }
 inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
 constexpr hipError_t hipErrorNotReady = 600;
+constexpr hipError_t hipErrorInvalidSymbol = 13;
+// device symbols are host variables here: their address is the symbol's own
+#define HIP_SYMBOL(x) (&(x))
+template <class T> inline hipError_t hipGetSymbolAddress(void** p, T* sym) { *p = (void*)sym; return hipSuccess; }
 // HIP graphs are not emulated: capture fails, so callers keep direct launches.
 typedef void* hipGraph_t;
 typedef void* hipGraphExec_t;
