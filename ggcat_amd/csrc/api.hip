@@ -6,6 +6,7 @@
 #include "query.hip"
 #include "unitigs.hip"
 #include "fasta.cpp"
+#include "colormap.cpp"
 
 #include <algorithm>
 #include <cstdio>
@@ -82,6 +83,13 @@ struct ggcat_gpu_ctx {
     uint64_t u_n = 0, u_bytes = 0, u_npos = 0;
     std::vector<uint8_t> cyc_seq, cyc_flags;
     std::vector<uint32_t> cyc_sub;
+    DevBuf u_path;  // colored builds: table position of every unitig k-mer, in written order
+    std::vector<uint32_t> cyc_pos;  // the same for the unitigs of cycles (walked on the host)
+    // color subsets of the solid table: ids for the color map and the colored
+    // FASTA headers (assigned on first use after a merge)
+    bool have_subsets = false;
+    std::vector<uint32_t> pos_subset;  // per table position, ~0u for an empty position
+    std::vector<std::vector<uint32_t>> subsets;
 };
 
 namespace {
@@ -422,7 +430,7 @@ void ggcat_gpu_destroy(ggcat_gpu_ctx* ctx) {
                       &ctx->q_seq, &ctx->q_off, &ctx->q_matched, &ctx->q_counts, &ctx->q_work,
                       &ctx->u_nb, &ctx->u_ext, &ctx->u_link, &ctx->u_hbytes, &ctx->u_hcount, &ctx->u_hoff,
                       &ctx->u_hidx, &ctx->u_seq, &ctx->u_sub, &ctx->u_flags, &ctx->u_visited, &ctx->u_rest,
-                      &ctx->u_rest_n, &ctx->u_rep, &ctx->u_tab};
+                      &ctx->u_rest_n, &ctx->u_rep, &ctx->u_tab, &ctx->u_path};
     if (ctx->server) (void)ggcat_gpu_serve_close(ctx);
     if (ctx->graph_a) (void)hipGraphExecDestroy(ctx->graph_a);
     for (DevBuf* b : bufs)
@@ -826,7 +834,7 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     if (!ctx) return GGCAT_GPU_EINVAL;
     if (!ctx->have_records) return fail(ctx, GGCAT_GPU_ESTATE, "no bucketing result");
     if (ctx->server) return fail(ctx, GGCAT_GPU_ESTATE, "close the query server before a new merge");
-    ctx->have_index = ctx->have_unitigs = false;
+    ctx->have_index = ctx->have_unitigs = ctx->have_subsets = false;
     HIPCHK(hipSetDevice(ctx->device));
     const ggcat_gpu_params& p = ctx->P;
     const uint32_t key_words = (p.hash_kind == GGCAT_GPU_HASH_SEQ && p.k <= 32) ? 1 : 2;
@@ -1620,6 +1628,11 @@ int build_unitigs(ggcat_gpu_ctx* ctx, bool joined, uint64_t* n_unitigs, uint64_t
     U.seq = P_<uint8_t>(ctx->u_seq);
     U.uni_sub = P_<uint32_t>(ctx->u_sub);
     U.uni_flags = P_<uint8_t>(ctx->u_flags);
+    if (ctx->colored) {
+        // the k-mers' table positions give each unitig's color runs
+        ENSURE(ctx->u_path, (tot_bytes - (uint64_t)ctx->P.k * tot_n) * 4 + 8);
+        U.path_pos = P_<uint32_t>(ctx->u_path);
+    }
     if (cycles) HIPCHK(hipMemsetAsync(ctx->u_visited.p, 0, n_pos + 8, ctx->stream));
     const uint32_t cap = 1u << 20;
     ENSURE(ctx->u_rest, (size_t)cap * 4);
@@ -1640,6 +1653,7 @@ int build_unitigs(ggcat_gpu_ctx* ctx, bool joined, uint64_t* n_unitigs, uint64_t
     ctx->cyc_seq.clear();
     ctx->cyc_sub.clear();
     ctx->cyc_flags.clear();
+    ctx->cyc_pos.clear();
     if (n_rest) {
         std::vector<uint32_t> rest(n_rest);
         HIPCHK(hipMemcpy(rest.data(), ctx->u_rest.p, (size_t)n_rest * 4, hipMemcpyDeviceToHost));
@@ -1666,6 +1680,7 @@ int build_unitigs(ggcat_gpu_ctx* ctx, bool joined, uint64_t* n_unitigs, uint64_t
             seen[p0] = true;
             const u128_t x = node[p0].first;
             for (uint32_t i = 0; i < k; i++) ctx->cyc_seq.push_back((uint8_t)B[(uint32_t)(x >> (2 * i)) & 3u]);
+            ctx->cyc_pos.push_back(p0);
             uint64_t cur = 2 * (uint64_t)p0;  // entered at side 0
             uint32_t sub = 0;
             HIPCHK(hipMemcpy(&sub, P_<uint32_t>(ctx->out_sub) + p0, 4, hipMemcpyDeviceToHost));
@@ -1681,6 +1696,7 @@ int build_unitigs(ggcat_gpu_ctx* ctx, bool joined, uint64_t* n_unitigs, uint64_t
                 if (jt == node.end()) return fail(ctx, GGCAT_GPU_EHIP, "cycle walk left the cycle set");
                 const u128_t y = (nx & 1) == 0 ? jt->second.first : rc(jt->second.first);
                 ctx->cyc_seq.push_back((uint8_t)B[(uint32_t)(y >> (2 * (k - 1))) & 3u]);
+                ctx->cyc_pos.push_back(q);
                 cur = nx;
             }
             ctx->cyc_seq.push_back('\n');
@@ -1713,6 +1729,166 @@ int ggcat_gpu_maximal_unitigs(ggcat_gpu_ctx* ctx, uint64_t* n_unitigs, uint64_t*
     return build_unitigs(ctx, true, n_unitigs, n_bytes);
 }
 
+namespace {
+
+// Color subset ids of the solid table (colored builds): every distinct color
+// set gets one id, ids ordered by the sets' bitmaps so that they do not depend
+// on the table's layout (the reference assigns them first-come,
+// crates/colors/src/colors_memmap_writer.rs:50-59, so its ids vary between
+// runs; the sets do not).
+int ensure_subsets(ggcat_gpu_ctx* ctx) {
+    if (ctx->have_subsets) return GGCAT_GPU_OK;
+    if (!ctx->have_solid) return fail(ctx, GGCAT_GPU_ESTATE, "no merge result");
+    if (!ctx->colored) return fail(ctx, GGCAT_GPU_ESTATE, "the last build was not colored");
+    HIPCHK(hipSetDevice(ctx->device));
+    const uint32_t W = ctx->color_words;
+    const uint64_t n_pos = (uint64_t)ctx->n_chunks_used * OUT_CHUNK + ctx->n_tail;
+    std::vector<uint32_t> bm(n_pos * W);
+    if (n_pos) HIPCHK(hipMemcpy(bm.data(), ctx->out_colors.p, n_pos * W * 4, hipMemcpyDeviceToHost));
+    auto filled = [&](uint64_t p) {
+        const uint64_t c = p / OUT_CHUNK;
+        return c < ctx->n_chunks_used ? (p - c * OUT_CHUNK) < ctx->chunk_fill[c]
+                                      : p - (uint64_t)ctx->n_chunks_used * OUT_CHUNK < ctx->n_tail;
+    };
+    auto less = [&](uint64_t a, uint64_t b) {
+        return std::lexicographical_compare(&bm[a * W], &bm[a * W] + W, &bm[b * W], &bm[b * W] + W);
+    };
+    std::vector<std::pair<uint64_t, uint32_t>> hp;  // (hash of the bitmap, position)
+    hp.reserve(n_pos);
+    for (uint64_t p = 0; p < n_pos; p++) {
+        if (!filled(p)) continue;
+        uint64_t h = 0x9E3779B97F4A7C15ULL;
+        for (uint32_t w = 0; w < W; w++) h = (h ^ bm[p * W + w]) * 0xbf58476d1ce4e5b9ULL;
+        hp.emplace_back(h ^ (h >> 31), (uint32_t)p);
+    }
+    std::sort(hp.begin(), hp.end(), [&](const std::pair<uint64_t, uint32_t>& a, const std::pair<uint64_t, uint32_t>& b) {
+        return a.first != b.first ? a.first < b.first : less(a.second, b.second);
+    });
+    // distinct bitmaps (equal bitmaps are adjacent), then ordered by bitmap
+    std::vector<uint32_t> rep;
+    std::vector<uint32_t> group(hp.size());
+    for (size_t i = 0; i < hp.size(); i++) {
+        const bool same = i > 0 && hp[i].first == hp[i - 1].first &&
+                          std::equal(&bm[(uint64_t)hp[i].second * W], &bm[(uint64_t)hp[i].second * W] + W,
+                                     &bm[(uint64_t)hp[i - 1].second * W]);
+        if (!same) rep.push_back(hp[i].second);
+        group[i] = (uint32_t)rep.size() - 1;
+    }
+    std::vector<uint32_t> order(rep.size());
+    for (uint32_t i = 0; i < order.size(); i++) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return less(rep[a], rep[b]); });
+    std::vector<uint32_t> id_of(rep.size());
+    ctx->subsets.assign(rep.size(), {});
+    for (uint32_t i = 0; i < order.size(); i++) {
+        id_of[order[i]] = i;
+        const uint64_t p = rep[order[i]];
+        for (uint32_t w = 0; w < W; w++)
+            for (uint32_t bits = bm[p * W + w]; bits; bits &= bits - 1)
+                ctx->subsets[i].push_back(w * 32 + (uint32_t)__builtin_ctz(bits));
+        if (ctx->subsets[i].empty()) return fail(ctx, GGCAT_GPU_EHIP, "a solid k-mer without colors");
+    }
+    ctx->pos_subset.assign(n_pos, ~0u);
+    for (size_t i = 0; i < hp.size(); i++) ctx->pos_subset[hp[i].second] = id_of[group[i]];
+    ctx->have_subsets = true;
+    return GGCAT_GPU_OK;
+}
+
+}  // namespace
+
+int ggcat_gpu_color_subsets(ggcat_gpu_ctx* ctx, uint64_t* n_subsets, uint32_t* subset_of_entry) {
+    if (!ctx) return GGCAT_GPU_EINVAL;
+    const int r = ensure_subsets(ctx);
+    if (r != GGCAT_GPU_OK) return r;
+    if (n_subsets) *n_subsets = ctx->subsets.size();
+    if (subset_of_entry) {
+        uint64_t o = 0;
+        for (uint64_t p = 0; p < ctx->pos_subset.size(); p++)
+            if (ctx->pos_subset[p] != ~0u) subset_of_entry[o++] = ctx->pos_subset[p];
+    }
+    return GGCAT_GPU_OK;
+}
+
+int ggcat_gpu_colormap_write(ggcat_gpu_ctx* ctx, const char* path, const char* const* color_names, uint32_t n_names,
+                             uint64_t* n_subsets) {
+    if (!ctx || !path || (!color_names && n_names)) return GGCAT_GPU_EINVAL;
+    int r = ensure_subsets(ctx);
+    if (r != GGCAT_GPU_OK) return r;
+    if (n_names < ctx->n_color_ids)
+        return fail(ctx, GGCAT_GPU_EINVAL, "every color id of the build needs a name");
+    std::vector<std::string> names;
+    for (uint32_t i = 0; i < n_names; i++) {
+        if (!color_names[i]) return fail(ctx, GGCAT_GPU_EINVAL, "null color name");
+        names.emplace_back(color_names[i]);
+    }
+    std::string err;
+    if (!gg_host::colormap_write(path, names, ctx->subsets, &err)) return fail(ctx, GGCAT_GPU_EINVAL, err);
+    if (n_subsets) *n_subsets = ctx->subsets.size();
+    return GGCAT_GPU_OK;
+}
+
+int ggcat_gpu_colormap_write_sets(const char* path, const char* const* color_names, uint32_t n_names,
+                                  const uint64_t* subset_off, const uint32_t* colors, uint64_t n_subsets) {
+    if (!path || (!color_names && n_names) || !subset_off || (!colors && subset_off[n_subsets]))
+        return fail(nullptr, GGCAT_GPU_EINVAL, "null argument");
+    std::vector<std::string> names;
+    for (uint32_t i = 0; i < n_names; i++) {
+        if (!color_names[i]) return fail(nullptr, GGCAT_GPU_EINVAL, "null color name");
+        names.emplace_back(color_names[i]);
+    }
+    std::vector<std::vector<uint32_t>> sets(n_subsets);
+    for (uint64_t s = 0; s < n_subsets; s++) {
+        if (subset_off[s + 1] <= subset_off[s]) return fail(nullptr, GGCAT_GPU_EINVAL, "empty color subset");
+        sets[s].assign(colors + subset_off[s], colors + subset_off[s + 1]);
+        for (size_t i = 1; i < sets[s].size(); i++)
+            if (sets[s][i] <= sets[s][i - 1])
+                return fail(nullptr, GGCAT_GPU_EINVAL, "color subsets must be strictly increasing");
+        if (sets[s].back() >= n_names) return fail(nullptr, GGCAT_GPU_EINVAL, "a color without a name");
+    }
+    std::string err;
+    if (!gg_host::colormap_write(path, names, sets, &err)) return fail(nullptr, GGCAT_GPU_EINVAL, err);
+    return GGCAT_GPU_OK;
+}
+
+struct ggcat_gpu_colormap {
+    gg_host::ColorMap m;
+};
+
+int ggcat_gpu_colormap_open(const char* path, ggcat_gpu_colormap** out) {
+    if (!path || !out) return fail(nullptr, GGCAT_GPU_EINVAL, "null argument");
+    *out = nullptr;
+    std::unique_ptr<ggcat_gpu_colormap> c(new ggcat_gpu_colormap());
+    std::string err;
+    if (!gg_host::colormap_read(path, c->m, &err)) return fail(nullptr, GGCAT_GPU_EINVAL, err);
+    *out = c.release();
+    return GGCAT_GPU_OK;
+}
+
+int ggcat_gpu_colormap_info(const ggcat_gpu_colormap* c, uint32_t* n_colors, uint64_t* n_subsets) {
+    if (!c) return GGCAT_GPU_EINVAL;
+    if (n_colors) *n_colors = (uint32_t)c->m.names.size();
+    if (n_subsets) *n_subsets = c->m.subset_off.size() - 1;
+    return GGCAT_GPU_OK;
+}
+
+const char* ggcat_gpu_colormap_color_name(const ggcat_gpu_colormap* c, uint32_t color) {
+    if (!c || color >= c->m.names.size()) return nullptr;
+    return c->m.names[color].c_str();
+}
+
+int ggcat_gpu_colormap_subset(const ggcat_gpu_colormap* c, uint64_t subset, uint32_t* colors, uint32_t cap,
+                              uint32_t* n) {
+    if (!c || !n || subset + 1 >= c->m.subset_off.size()) return GGCAT_GPU_EINVAL;
+    const uint64_t a = c->m.subset_off[subset], b = c->m.subset_off[subset + 1];
+    *n = (uint32_t)(b - a);
+    if (colors) {
+        if (cap < b - a) return GGCAT_GPU_EINVAL;
+        memcpy(colors, c->m.colors.data() + a, (b - a) * 4);
+    }
+    return GGCAT_GPU_OK;
+}
+
+void ggcat_gpu_colormap_close(ggcat_gpu_colormap* c) { delete c; }
+
 int ggcat_gpu_unitigs_write_fasta(ggcat_gpu_ctx* ctx, const char* path, uint64_t* n_written) {
     if (!ctx || !path) return GGCAT_GPU_EINVAL;
     if (!ctx->have_unitigs) return fail(ctx, GGCAT_GPU_ESTATE, "no unitig result");
@@ -1721,17 +1897,50 @@ int ggcat_gpu_unitigs_write_fasta(ggcat_gpu_ctx* ctx, const char* path, uint64_t
     std::vector<uint8_t> seq(total);
     if (ctx->u_bytes) HIPCHK(hipMemcpy(seq.data(), ctx->u_seq.p, ctx->u_bytes, hipMemcpyDeviceToHost));
     if (!ctx->cyc_seq.empty()) memcpy(seq.data() + ctx->u_bytes, ctx->cyc_seq.data(), ctx->cyc_seq.size());
+    // colored builds: each unitig's k-mers' color subset ids as runs
+    // " C:{id hex}:{count}" (crates/colors/src/managers/multiple.rs:707-740)
+    std::vector<uint32_t> path_pos;
+    const uint32_t k = ctx->P.k;
+    if (ctx->colored) {
+        const int r = ensure_subsets(ctx);
+        if (r != GGCAT_GPU_OK) return r;
+        const uint64_t n_path = ctx->u_bytes - (uint64_t)k * ctx->u_n;
+        path_pos.resize(n_path + ctx->cyc_pos.size());
+        if (n_path) HIPCHK(hipMemcpy(path_pos.data(), ctx->u_path.p, n_path * 4, hipMemcpyDeviceToHost));
+        std::copy(ctx->cyc_pos.begin(), ctx->cyc_pos.end(), path_pos.begin() + n_path);
+    }
     FILE* f = fopen(path, "wb");
     if (!f) return fail(ctx, GGCAT_GPU_EINVAL, std::string("cannot create ") + path);
     std::vector<char> buf(1 << 20);
     setvbuf(f, buf.data(), _IOFBF, buf.size());
-    uint64_t id = 0;
+    uint64_t id = 0, kp = 0;
     bool ok = true;
     for (uint64_t o = 0; o < total && ok;) {
         const uint8_t* nl = static_cast<const uint8_t*>(memchr(seq.data() + o, '\n', total - o));
         const uint64_t len = nl ? (uint64_t)(nl - (seq.data() + o)) : total - o;
-        ok = fprintf(f, ">%llu LN:i:%llu\n", (unsigned long long)id, (unsigned long long)len) > 0 &&
-             fwrite(seq.data() + o, 1, len, f) == len && fputc('\n', f) != EOF;
+        ok = fprintf(f, ">%llu LN:i:%llu", (unsigned long long)id, (unsigned long long)len) > 0;
+        if (ctx->colored && ok) {
+            const uint64_t nk = len - k + 1;
+            if (kp + nk > path_pos.size()) {
+                fclose(f);
+                return fail(ctx, GGCAT_GPU_EHIP, "unitig k-mer positions do not cover the unitigs");
+            }
+            uint32_t run_id = ~0u;
+            uint64_t run_n = 0;
+            for (uint64_t i = 0; i < nk && ok; i++) {
+                const uint32_t s = ctx->pos_subset[path_pos[kp + i]];
+                if (s == run_id) {
+                    run_n++;
+                    continue;
+                }
+                if (run_n) ok = fprintf(f, " C:%x:%llu", run_id, (unsigned long long)run_n) > 0;
+                run_id = s;
+                run_n = 1;
+            }
+            if (run_n && ok) ok = fprintf(f, " C:%x:%llu", run_id, (unsigned long long)run_n) > 0;
+            kp += nk;
+        }
+        ok = ok && fputc('\n', f) != EOF && fwrite(seq.data() + o, 1, len, f) == len && fputc('\n', f) != EOF;
         o += len + 1;
         id++;
     }

@@ -59,6 +59,8 @@ struct UnitigArgs {
     uint8_t* rep;  // [n_pos] joined mode: 1 where p is its key's smallest position
     unsigned int* n_nodes;  // joined mode: number of representatives
     uint32_t mark_visited;  // write pass: mark path nodes (needed only when cycles exist)
+    uint32_t* path_pos;  // write pass, optional: the table position of every k-mer of every unitig, in
+                         // written order (unitig i's k-mers start at head_off - i * k)
 };
 
 // Position p holds a solid entry (chunks are filled from their start).
@@ -274,11 +276,15 @@ __global__ void walk_kernel(UnitigArgs A, uint64_t n_pos) {
         const u128_t x = s == 0 ? key_at(A, p) : kmer_rc128(key_at(A, p), k);
         for (uint32_t i = 0; i < k; i++) out.put((ASCII >> (8 * ((uint32_t)(x >> (2 * i)) & 3u))) & 0xFFu);
         if (A.mark_visited) A.visited[p] = 1;
+        uint32_t* pp = A.path_pos ? A.path_pos + (A.head_off[e] - ui * k) : nullptr;
+        if (pp) *pp++ = (uint32_t)p;
         uint64_t cur = e;
-        while (A.link[cur ^ 1] != NO_LINK) {
+        // the count pass found this path's end within n_pos steps
+        for (uint64_t steps = 1; A.link[cur ^ 1] != NO_LINK && steps <= n_pos; steps++) {
             cur = A.link[cur ^ 1];
             const uint64_t q = cur >> 1;
             if (A.mark_visited) A.visited[q] = 1;
+            if (pp) *pp++ = (uint32_t)q;
             out.put((ASCII >> (8 * entry_base(A, q, (uint32_t)(cur & 1)))) & 0xFFu);
         }
         out.put('\n');

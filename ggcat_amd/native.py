@@ -111,6 +111,17 @@ def gpu_lib(path: str = GPU_LIB) -> C.CDLL:
     lib.ggcat_gpu_copy_device.argtypes = [V, V, V, C.c_uint64]
     lib.ggcat_gpu_copy_to_host.argtypes = [V, V, V, C.c_uint64]
     lib.ggcat_gpu_query_device_noalloc.argtypes = [V, V, C.c_uint64, C.c_uint32, V, V, V]
+    lib.ggcat_gpu_color_subsets.argtypes = [V, _u64p, _u32p]
+    lib.ggcat_gpu_colormap_write.argtypes = [V, C.c_char_p, C.POINTER(C.c_char_p), C.c_uint32, _u64p]
+    lib.ggcat_gpu_colormap_write_sets.argtypes = [C.c_char_p, C.POINTER(C.c_char_p), C.c_uint32, _u64p, _u32p,
+                                                  C.c_uint64]
+    lib.ggcat_gpu_colormap_open.argtypes = [C.c_char_p, C.POINTER(V)]
+    lib.ggcat_gpu_colormap_info.argtypes = [V, _u32p, _u64p]
+    lib.ggcat_gpu_colormap_color_name.argtypes = [V, C.c_uint32]
+    lib.ggcat_gpu_colormap_color_name.restype = C.c_char_p
+    lib.ggcat_gpu_colormap_subset.argtypes = [V, C.c_uint64, _u32p, C.c_uint32, _u32p]
+    lib.ggcat_gpu_colormap_close.argtypes = [V]
+    lib.ggcat_gpu_colormap_close.restype = None
     lib.ggcat_gpu_index_build.argtypes = [V]
     lib.ggcat_gpu_index_colors.argtypes = [V, _u32p]
     lib.ggcat_gpu_query.argtypes = [V, V, C.c_uint64, C.c_uint32, _u64p, _u64p, _u32p]
@@ -144,7 +155,9 @@ def gpu_lib(path: str = GPU_LIB) -> C.CDLL:
                  "ggcat_gpu_serve_open", "ggcat_gpu_serve_submit", "ggcat_gpu_serve_poll", "ggcat_gpu_serve_fetch",
                  "ggcat_gpu_serve_stats", "ggcat_gpu_serve_close", "ggcat_gpu_unitigs", "ggcat_gpu_unitigs_copy",
                  "ggcat_gpu_maximal_unitigs", "ggcat_gpu_unitigs_write_fasta", "ggcat_gpu_records_serialize",
-                 "ggcat_gpu_extension_copy", "ggcat_gpu_copy_to_host", "ggcat_gpu_query_device_noalloc"):
+                 "ggcat_gpu_extension_copy", "ggcat_gpu_copy_to_host", "ggcat_gpu_query_device_noalloc",
+                 "ggcat_gpu_color_subsets", "ggcat_gpu_colormap_write", "ggcat_gpu_colormap_write_sets",
+                 "ggcat_gpu_colormap_open", "ggcat_gpu_colormap_info", "ggcat_gpu_colormap_subset"):
         getattr(lib, name).restype = C.c_int
     _gpu[path] = lib
     return lib
@@ -207,6 +220,49 @@ def read_files(paths, k: int, colors: bool = False, lib_path: str = GPU_LIB) -> 
         for p in (jp, csp, cvp):
             lib.ggcat_gpu_free_host(C.cast(p, C.c_void_p))
     return ReadInput(joined, cs, cv, est.value)
+
+
+def _names(names) -> tuple:
+    arr = (C.c_char_p * max(len(names), 1))(*[n.encode() for n in names])
+    return arr, len(names)
+
+
+def colormap_write_sets(path: str, names: list[str], subsets: list[list[int]], lib_path: str = GPU_LIB) -> None:
+    """Writes a color map of explicit subsets (host code; needs no GPU)."""
+    lib = gpu_lib(lib_path)
+    off = np.zeros(len(subsets) + 1, np.uint64)
+    off[1:] = np.cumsum([len(s) for s in subsets]) if subsets else []
+    cols = np.array([c for s in subsets for c in s], np.uint32)
+    arr, n = _names(names)
+    rc = lib.ggcat_gpu_colormap_write_sets(str(path).encode(), arr, n, _ptr(off, _u64p),
+                                           _ptr(cols, _u32p) if cols.size else C.cast(None, _u32p), len(subsets))
+    if rc != OK:
+        raise GGCATGpuError(f"ggcat_gpu_colormap_write_sets failed (status {rc}): "
+                            f"{lib.ggcat_gpu_last_error(None).decode(errors='replace')}")
+
+
+def colormap_read(path: str, lib_path: str = GPU_LIB) -> tuple[list[str], list[list[int]]]:
+    """Reads a color map: (color names, subsets in id order) (host code)."""
+    lib = gpu_lib(lib_path)
+    h = C.c_void_p()
+    rc = lib.ggcat_gpu_colormap_open(str(path).encode(), C.byref(h))
+    if rc != OK:
+        raise GGCATGpuError(f"ggcat_gpu_colormap_open failed (status {rc}): "
+                            f"{lib.ggcat_gpu_last_error(None).decode(errors='replace')}")
+    try:
+        nc, ns = C.c_uint32(), C.c_uint64()
+        lib.ggcat_gpu_colormap_info(h, C.byref(nc), C.byref(ns))
+        names = [lib.ggcat_gpu_colormap_color_name(h, i).decode() for i in range(nc.value)]
+        subsets = []
+        buf = np.zeros(max(nc.value, 1), np.uint32)
+        cnt = C.c_uint32()
+        for s in range(ns.value):
+            if lib.ggcat_gpu_colormap_subset(h, s, _ptr(buf, _u32p), buf.size, C.byref(cnt)) != OK:
+                raise GGCATGpuError("ggcat_gpu_colormap_subset failed")
+            subsets.append(buf[:cnt.value].tolist())
+        return names, subsets
+    finally:
+        lib.ggcat_gpu_colormap_close(h)
 
 
 def plan_build(options: BuildOptions, estimated_bases: int, lib_path: str = GPU_LIB) -> Params:
@@ -459,6 +515,21 @@ class GpuContext:
 
     def copy_device(self, dst: int, src: int, nbytes: int) -> None:
         self._check(self.lib.ggcat_gpu_copy_device(self.h, C.c_void_p(dst), C.c_void_p(src), nbytes), "copy_device")
+
+    def color_subsets(self) -> tuple[int, np.ndarray]:
+        """(number of distinct color sets, subset id of every solid entry in solid() order)."""
+        n = C.c_uint64()
+        self._check(self.lib.ggcat_gpu_color_subsets(self.h, C.byref(n), C.cast(None, _u32p)), "color_subsets")
+        out = np.zeros(int(self.counters()[5]), np.uint32)
+        self._check(self.lib.ggcat_gpu_color_subsets(self.h, C.byref(n), _ptr(out, _u32p)), "color_subsets")
+        return n.value, out
+
+    def colormap_write(self, path: str, names: list[str]) -> int:
+        arr, n = _names(names)
+        ns = C.c_uint64()
+        self._check(self.lib.ggcat_gpu_colormap_write(self.h, str(path).encode(), arr, n, C.byref(ns)),
+                    "colormap_write")
+        return ns.value
 
     def download(self, ptr: int, out: np.ndarray) -> np.ndarray:
         """Copies device bytes at ptr into the host array `out` (its whole size)."""

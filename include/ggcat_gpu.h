@@ -282,13 +282,50 @@ int ggcat_gpu_extension_copy(ggcat_gpu_ctx* ctx, uint8_t* ext);
  * entry of each k-mer; 0 for its other entries). */
 int ggcat_gpu_maximal_unitigs(ggcat_gpu_ctx* ctx, uint64_t* n_unitigs, uint64_t* n_bytes);
 /* Writes the last unitig result as FASTA, one record per unitig,
- * ">{id} LN:i:{length}" then the sequence on one line (the uncolored header
- * of crates/sequence_output/src/structured_sequences/fasta.rs:153-200; ids
- * count from 0 in result order).  For colored builds the color-run tags
- * (C:...) of the reference's colored header are not written: their subset
- * ids come from the colormap, which this library does not produce.
- * *n_written (optional) = records written. */
+ * ">{id} LN:i:{length}" then the sequence on one line
+ * (crates/sequence_output/src/structured_sequences/fasta.rs:153-200; ids
+ * count from 0 in result order).  For colored builds the header continues
+ * with the unitig's k-mers' color subset ids as runs " C:{id hex}:{count}"
+ * in sequence order (crates/colors/src/managers/multiple.rs:707-740), ids as
+ * ggcat_gpu_colormap_write writes them.  *n_written (optional) = records
+ * written. */
 int ggcat_gpu_unitigs_write_fasta(ggcat_gpu_ctx* ctx, const char* path, uint64_t* n_written);
+
+/* ---- color map file (<graph>.colors.dat) ---------------------------------
+ * The reference's run-length color map (magic "GGCAT_CMAP_RNLEN", version 1):
+ * header, LZ4 frame of the color names, the color subsets in id order in LZ4
+ * frames of 20000, and an index of the frames
+ * (crates/colors/src/storage/serializer.rs:17-241, run_length.rs:26-334,
+ * deserializer.rs:1-181).  Subset ids of a colored build are assigned to
+ * the distinct color sets of its solid table in bitmap order (the reference
+ * assigns them first-come, crates/colors/src/colors_memmap_writer.rs:50-59,
+ * so its ids differ between runs while the sets do not). */
+
+/* Subset id of every solid entry, in ggcat_gpu_solid_copy order (NULL to get
+ * only *n_subsets).  Colored builds only. */
+int ggcat_gpu_color_subsets(ggcat_gpu_ctx* ctx, uint64_t* n_subsets, uint32_t* subset_of_entry);
+/* Writes the color map of the last colored build: color_names[i] names color
+ * id i (n_names >= the build's largest color id + 1); *n_subsets (optional)
+ * = subsets written.  Replaces the ColorsMemMapWriter of run_assembler
+ * (crates/assembler/src/lib.rs:130-136). */
+int ggcat_gpu_colormap_write(ggcat_gpu_ctx* ctx, const char* path, const char* const* color_names, uint32_t n_names,
+                             uint64_t* n_subsets);
+/* Writes a color map of explicit subsets (host only, no context): subset s
+ * holds colors[subset_off[s] .. subset_off[s + 1]), strictly increasing and
+ * below n_names.  Errors: ggcat_gpu_last_error(NULL). */
+int ggcat_gpu_colormap_write_sets(const char* path, const char* const* color_names, uint32_t n_names,
+                                  const uint64_t* subset_off, const uint32_t* colors, uint64_t n_subsets);
+/* Reads a whole color map (host only; errors: ggcat_gpu_last_error(NULL)).
+ * Mirrors ColorsDeserializer::new + get_color_mappings. */
+typedef struct ggcat_gpu_colormap ggcat_gpu_colormap;
+int ggcat_gpu_colormap_open(const char* path, ggcat_gpu_colormap** out);
+int ggcat_gpu_colormap_info(const ggcat_gpu_colormap* map, uint32_t* n_colors, uint64_t* n_subsets);
+/* NULL when color is out of range; valid until ggcat_gpu_colormap_close. */
+const char* ggcat_gpu_colormap_color_name(const ggcat_gpu_colormap* map, uint32_t color);
+/* Colors of one subset, ascending: *n always set, colors (cap entries) may be NULL. */
+int ggcat_gpu_colormap_subset(const ggcat_gpu_colormap* map, uint64_t subset, uint32_t* colors, uint32_t cap,
+                              uint32_t* n);
+void ggcat_gpu_colormap_close(ggcat_gpu_colormap* map);
 
 /* ---- queries ------------------------------------------------------------
  * Replaces the counting part of run_query (crates/querier/src/lib.rs:54-200):

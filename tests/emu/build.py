@@ -28,7 +28,7 @@ def build(sanitize: bool = False) -> str:
     os.makedirs(os.path.join(src, "ggcat_amd", "csrc"))
     shutil.copytree(os.path.join(ROOT, "include"), os.path.join(src, "include"))
     for name in os.listdir(os.path.join(ROOT, "ggcat_amd", "csrc")):
-        if not (name.endswith(".hip") or name.endswith(".h") or name == "fasta.cpp"):
+        if not (name.endswith(".hip") or name.endswith(".h") or name.endswith(".cpp")):
             continue
         text = open(os.path.join(ROOT, "ggcat_amd", "csrc", name)).read()
         text = re.sub(r"extern __shared__ __align__\(16\) uint8_t lds\[\];", "uint8_t* lds = emu::lds;", text)
