@@ -1989,3 +1989,4 @@ int ggcat_gpu_extension_copy(ggcat_gpu_ctx* ctx, uint8_t* ext) {
 }  // extern "C"
 
 #include "noalloc.hip"
+#include "query_graph.cpp"

@@ -121,6 +121,8 @@ def gpu_lib(path: str = GPU_LIB) -> C.CDLL:
     lib.ggcat_gpu_colormap_color_name.restype = C.c_char_p
     lib.ggcat_gpu_colormap_subset.argtypes = [V, C.c_uint64, _u32p, C.c_uint32, _u32p]
     lib.ggcat_gpu_colormap_close.argtypes = [V]
+    lib.ggcat_gpu_query_graph.argtypes = [V, C.c_char_p, C.c_char_p, C.c_char_p, C.c_uint32, C.c_uint32, C.c_int,
+                                          C.c_uint32, C.c_int, C.c_uint32, C.c_uint32, C.c_char_p, C.c_uint64]
     lib.ggcat_gpu_colormap_close.restype = None
     lib.ggcat_gpu_index_build.argtypes = [V]
     lib.ggcat_gpu_index_colors.argtypes = [V, _u32p]
@@ -157,7 +159,8 @@ def gpu_lib(path: str = GPU_LIB) -> C.CDLL:
                  "ggcat_gpu_maximal_unitigs", "ggcat_gpu_unitigs_write_fasta", "ggcat_gpu_records_serialize",
                  "ggcat_gpu_extension_copy", "ggcat_gpu_copy_to_host", "ggcat_gpu_query_device_noalloc",
                  "ggcat_gpu_color_subsets", "ggcat_gpu_colormap_write", "ggcat_gpu_colormap_write_sets",
-                 "ggcat_gpu_colormap_open", "ggcat_gpu_colormap_info", "ggcat_gpu_colormap_subset"):
+                 "ggcat_gpu_colormap_open", "ggcat_gpu_colormap_info", "ggcat_gpu_colormap_subset",
+                 "ggcat_gpu_query_graph"):
         getattr(lib, name).restype = C.c_int
     _gpu[path] = lib
     return lib
@@ -523,6 +526,16 @@ class GpuContext:
         out = np.zeros(int(self.counters()[5]), np.uint32)
         self._check(self.lib.ggcat_gpu_color_subsets(self.h, C.byref(n), _ptr(out, _u32p)), "color_subsets")
         return n.value, out
+
+    def query_graph(self, graph: str, queries: str, out_prefix: str, k: int, colors: bool = False,
+                    names: bool = False, hash_kind: int = HASH_SEQ) -> str:
+        """ggcat_gpu_query_graph: query a unitig FASTA (+ color map) with a query
+        file; returns the output file path."""
+        buf = C.create_string_buffer(4096)
+        self._check(self.lib.ggcat_gpu_query_graph(self.h, os.fsencode(graph), os.fsencode(queries),
+                                                   os.fsencode(out_prefix), k, 1, 0, 0, int(colors), int(names),
+                                                   hash_kind, buf, len(buf)), "ggcat_gpu_query_graph")
+        return buf.value.decode()
 
     def colormap_write(self, path: str, names: list[str]) -> int:
         arr, n = _names(names)

@@ -353,6 +353,26 @@ int ggcat_gpu_index_colors(ggcat_gpu_ctx* ctx, uint32_t* n_colors);
 int ggcat_gpu_query(ggcat_gpu_ctx* ctx, const uint8_t* queries, uint64_t n_bytes, uint32_t n_queries,
                     uint64_t* query_kmers, uint64_t* matched, uint32_t* color_counts);
 
+/* Queries a graph given as files.  Mirrors ggcat_query_graph
+ * (crates/capi/src/lib.rs:635-651, crates/api/src/lib.rs:325-382,
+ * crates/querier/src/lib.rs:54-199): input_graph is a unitig FASTA (plain or
+ * compressed); with colors != 0 its headers carry " C:{subset hex}:{count}"
+ * runs and the color map is input_graph with its extension replaced by
+ * "colors.dat".  Every k-mer of the unitigs is a graph k-mer (hash_kind as in
+ * ggcat_gpu_params, canonical).  input_query is a FASTA/FASTQ file.  Writes
+ * output_file_prefix (".csv" / ".jsonl" added when it has no extension;
+ * compressed by its extension: lz4, gz, zst, bz2, xz) as the reference does:
+ * CSV query_index,matched_kmers,query_kmers,match_percentage, or one JSON line
+ * per query with matches, colors as numbers (color_output_format 0) or names
+ * (1).  The output path is copied to out_path (cap bytes, may be NULL).
+ * threads_count and minimizer_length are accepted and not needed;
+ * forward_only must be 0.  The graph replaces the context's table (a later
+ * ggcat_gpu_query queries it). */
+int ggcat_gpu_query_graph(ggcat_gpu_ctx* ctx, const char* input_graph, const char* input_query,
+                          const char* output_file_prefix, uint32_t k, uint32_t threads_count, int forward_only,
+                          uint32_t minimizer_length, int colors, uint32_t color_output_format, uint32_t hash_kind,
+                          char* out_path, uint64_t out_path_cap);
+
 /* ---- serving: continuous batching of query requests ---------------------
  * No counterpart in the reference (each ggcat query call re-buckets the whole
  * graph).  Requests are admitted while their device and pinned staging
