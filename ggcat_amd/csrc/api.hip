@@ -1950,6 +1950,99 @@ int ggcat_gpu_unitigs_write_fasta(ggcat_gpu_ctx* ctx, const char* path, uint64_t
     return GGCAT_GPU_OK;
 }
 
+namespace {
+
+// CanonicalSeqHash of a k-mer of ASCII bases in sequence orientation:
+// (fw, rc) with base i at bits 2i of fw (crates/hashes/src/base/cn_seqhash_base.rs:22-72).
+void seqhash_pair(const uint8_t* s, uint32_t k, u128_t* fw, u128_t* rc) {
+    u128_t f = 0, r = 0;
+    for (uint32_t i = 0; i < k; i++) {
+        const u128_t c = (s[i] >> 1) & 3;
+        f |= c << (2 * i);
+        r |= (c ^ 2) << (2 * (k - 1 - i));
+    }
+    *fw = f;
+    *rc = r;
+}
+
+// CanonicalSeqHashFactory::get_bucket(0, log, hash)
+// (crates/hashes/src/base/cn_seqhash_base.rs:140-150, constants
+// crates/hashes/src/cn_seqhash.rs:15-27): hash * MULTIPLIER + BASIS, rotated
+// right by 3, low `log` bits; 64-bit arithmetic for k <= 32, 128-bit above.
+uint32_t seqhash_bucket(u128_t h, uint32_t k, uint32_t log) {
+    if (k <= 32) {
+        uint64_t x = (uint64_t)h * 0x00000100000001b3ULL + 0xcbf29ce484222325ULL;
+        x = (x >> 3) | (x << 61);
+        return (uint32_t)(x & ((1ull << log) - 1));
+    }
+    const u128_t mul = ((u128_t)0x0000000001000000ULL << 64) | 0x000000000000013bULL;
+    const u128_t basis = ((u128_t)0x6c62272e07bb0142ULL << 64) | 0x62b821756295c58dULL;
+    u128_t x = h * mul + basis;
+    x = (x >> 3) | (x << 125);
+    return (uint32_t)((uint64_t)x & ((1ull << log) - 1));
+}
+
+}  // namespace
+
+int ggcat_gpu_partial_unitigs(ggcat_gpu_ctx* ctx, uint32_t buckets_log, uint8_t* kind, uint16_t* bucket,
+                              uint8_t* rc, uint8_t* last_align, uint8_t* flags) {
+    if (!ctx) return GGCAT_GPU_EINVAL;
+    if (!ctx->have_unitigs || ctx->u_joined) return fail(ctx, GGCAT_GPU_ESTATE, "no per-sub-bucket unitig result");
+    if (buckets_log > 16) return fail(ctx, GGCAT_GPU_EINVAL, "buckets_log must be <= 16");
+    HIPCHK(hipSetDevice(ctx->device));
+    const uint32_t k = ctx->P.k;
+    const uint64_t n = ctx->u_n + ctx->cyc_sub.size();
+    std::vector<uint8_t> seq(ctx->u_bytes + ctx->cyc_seq.size()), fl(n);
+    if (ctx->u_bytes) HIPCHK(hipMemcpy(seq.data(), ctx->u_seq.p, ctx->u_bytes, hipMemcpyDeviceToHost));
+    memcpy(seq.data() + ctx->u_bytes, ctx->cyc_seq.data(), ctx->cyc_seq.size());
+    if (ctx->u_n) HIPCHK(hipMemcpy(fl.data(), ctx->u_flags.p, ctx->u_n, hipMemcpyDeviceToHost));
+    memcpy(fl.data() + ctx->u_n, ctx->cyc_flags.data(), ctx->cyc_flags.size());
+    // ParallelKmersMergeFinalExecutor::output_sequence
+    // (crates/assembler_kmers_merge/src/final_executor.rs:103-246)
+    uint64_t o = 0;
+    for (uint64_t i = 0; i < n; i++) {
+        const uint8_t* s = seq.data() + o;
+        const uint8_t* nl = static_cast<const uint8_t*>(memchr(s, '\n', seq.size() - o));
+        const uint64_t len = nl ? (uint64_t)(nl - s) : seq.size() - o;
+        o += len + 1;
+        if (len < k) return fail(ctx, GGCAT_GPU_EHIP, "unitig shorter than k");
+        const bool back = fl[i] & 1, fwd = (fl[i] & 2) != 0, circ = (fl[i] & 4) != 0;
+        uint8_t kd = 0, r = 0, la = 0, f = 0;
+        uint16_t bk = 0xFFFF;
+        if (circ) {
+            kd = 1;  // lonely circular unitig
+        } else if (back || fwd) {
+            kd = 2;
+            uint32_t lb = 0xFFFF, rb = 0xFFFF;
+            bool lrc = false, rrc = true;
+            u128_t hf, hr;
+            if (back) {  // get_extremal_hash(out_seq, k, true): the first k-mer
+                seqhash_pair(s, k, &hf, &hr);
+                lb = seqhash_bucket(hf < hr ? hf : hr, k, buckets_log);
+                lrc = !(hf < hr);
+            }
+            if (fwd) {  // the last k-mer
+                seqhash_pair(s + len - k, k, &hf, &hr);
+                rb = seqhash_bucket(hf < hr ? hf : hr, k, buckets_log);
+                rrc = !(hf < hr);
+            }
+            // always the smaller bucket
+            const bool hash_beginning = lb <= rb;
+            const bool should_rc = hash_beginning ? lrc : rrc;
+            bk = (uint16_t)(hash_beginning ? lb : rb);
+            r = should_rc;
+            la = (hash_beginning ^ should_rc) ? 0 : (uint8_t)((len - k) % 4);
+            f = (uint8_t)((((!hash_beginning) ^ should_rc) ? 2 : 0) | (back && fwd ? 4 : 0));
+        }
+        if (kind) kind[i] = kd;
+        if (bucket) bucket[i] = bk;
+        if (rc) rc[i] = r;
+        if (last_align) last_align[i] = la;
+        if (flags) flags[i] = f;
+    }
+    return GGCAT_GPU_OK;
+}
+
 int ggcat_gpu_unitigs_copy(ggcat_gpu_ctx* ctx, uint8_t* seq, uint32_t* sub, uint8_t* flags) {
     if (!ctx) return GGCAT_GPU_EINVAL;
     if (!ctx->have_unitigs) return fail(ctx, GGCAT_GPU_ESTATE, "no unitig result");

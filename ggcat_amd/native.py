@@ -121,6 +121,7 @@ def gpu_lib(path: str = GPU_LIB) -> C.CDLL:
     lib.ggcat_gpu_colormap_color_name.restype = C.c_char_p
     lib.ggcat_gpu_colormap_subset.argtypes = [V, C.c_uint64, _u32p, C.c_uint32, _u32p]
     lib.ggcat_gpu_colormap_close.argtypes = [V]
+    lib.ggcat_gpu_partial_unitigs.argtypes = [V, C.c_uint32, _u8p, C.POINTER(C.c_uint16), _u8p, _u8p, _u8p]
     lib.ggcat_gpu_query_graph.argtypes = [V, C.c_char_p, C.c_char_p, C.c_char_p, C.c_uint32, C.c_uint32, C.c_int,
                                           C.c_uint32, C.c_int, C.c_uint32, C.c_uint32, C.c_char_p, C.c_uint64]
     lib.ggcat_gpu_colormap_close.restype = None
@@ -160,7 +161,7 @@ def gpu_lib(path: str = GPU_LIB) -> C.CDLL:
                  "ggcat_gpu_extension_copy", "ggcat_gpu_copy_to_host", "ggcat_gpu_query_device_noalloc",
                  "ggcat_gpu_color_subsets", "ggcat_gpu_colormap_write", "ggcat_gpu_colormap_write_sets",
                  "ggcat_gpu_colormap_open", "ggcat_gpu_colormap_info", "ggcat_gpu_colormap_subset",
-                 "ggcat_gpu_query_graph"):
+                 "ggcat_gpu_query_graph", "ggcat_gpu_partial_unitigs"):
         getattr(lib, name).restype = C.c_int
     _gpu[path] = lib
     return lib
@@ -437,6 +438,17 @@ class GpuContext:
         self._check(self.lib.ggcat_gpu_unitigs_copy(self.h, _ptr(seq, _u8p), _ptr(sub, _u32p), _ptr(fl, _u8p)),
                     "ggcat_gpu_unitigs_copy")
         return seq.tobytes().split(b"\n")[:-1], sub, fl
+
+    def partial_unitigs(self, buckets_log: int, n: int) -> dict:
+        """Routing of the last per-sub-bucket unitigs (n of them) as the
+        reference's merge writes them: kind, bucket, rc, last_align, flags."""
+        out = {"kind": np.zeros(n, np.uint8), "bucket": np.zeros(n, np.uint16), "rc": np.zeros(n, np.uint8),
+               "last_align": np.zeros(n, np.uint8), "flags": np.zeros(n, np.uint8)}
+        self._check(self.lib.ggcat_gpu_partial_unitigs(
+            self.h, buckets_log, _ptr(out["kind"], _u8p), out["bucket"].ctypes.data_as(C.POINTER(C.c_uint16)),
+            _ptr(out["rc"], _u8p), _ptr(out["last_align"], _u8p), _ptr(out["flags"], _u8p)),
+            "ggcat_gpu_partial_unitigs")
+        return out
 
     def extension_flags(self) -> np.ndarray:
         n, kw = C.c_uint64(), C.c_uint32()

@@ -267,6 +267,24 @@ int ggcat_gpu_unitigs_copy(ggcat_gpu_ctx* ctx, uint8_t* seq, uint32_t* sub, uint
  * it in its canonical orientation (2 = two or more), bits 2-3 = that can
  * follow it.  Valid after ggcat_gpu_unitigs. */
 int ggcat_gpu_extension_copy(ggcat_gpu_ctx* ctx, uint8_t* ext);
+/* Where the reference's merge sends each per-sub-bucket unitig of the last
+ * ggcat_gpu_unitigs call (ParallelKmersMergeFinalExecutor::output_sequence,
+ * crates/assembler_kmers_merge/src/final_executor.rs:103-246), in
+ * ggcat_gpu_unitigs_copy order; any output may be NULL:
+ *   kind        0 lonely unitig (no open end: straight to the output),
+ *               1 lonely circular unitig, 2 partial unitig (an open end)
+ *   bucket      partial unitigs: the result bucket, the smaller of the
+ *               seq-hash buckets (get_bucket(0, buckets_log, ...)) of its
+ *               open ends' k-mers; 0xFFFF otherwise
+ *   rc          1 when the unitig is stored reverse-complemented
+ *   last_align  the stored minimizer position: (length - k) % 4 when the
+ *               hashed k-mer ends the stored sequence, else 0
+ *   flags       HASH_ENDING (2) | OTHER_END (4)
+ *               (crates/io/src/partial_unitigs_extra_data.rs:16-18)
+ * buckets_log is the build's bucket count log (the result buckets have the
+ * bucketing's count). */
+int ggcat_gpu_partial_unitigs(ggcat_gpu_ctx* ctx, uint32_t buckets_log, uint8_t* kind, uint16_t* bucket, uint8_t* rc,
+                              uint8_t* last_align, uint8_t* flags);
 
 /* ---- maximal unitigs (partial unitigs joined across buckets) -------------
  * Replaces build_final_unitigs (crates/assembler_pipeline/src/lib.rs:111,
