@@ -88,6 +88,41 @@ def kernel_rooflines(kb: dict, kern_ms: dict, workload: str, scale: float) -> di
     return out
 
 
+def solid_matrix(keys, sub, mult, flags) -> np.ndarray:
+    """Rows (sub, key_lo, key_hi, multiplicity, flags), sorted."""
+    hi = keys[:, -1] if keys.shape[1] > 1 else np.zeros(keys.shape[0], np.uint64)
+    m = np.stack([sub.astype(np.uint64), keys[:, 0].astype(np.uint64), hi.astype(np.uint64),
+                  mult.astype(np.uint64), flags.astype(np.uint64)], 1)
+    return m[np.lexsort(m.T[::-1])]
+
+
+def parity_gate(ctx, wl, params, sample: np.ndarray, threads: int) -> dict:
+    """GPU solid table of `sample` under the schedule that is about to be
+    timed, against the oracle, bit for bit; exits on a mismatch."""
+    from oracle import binding as O
+
+    ctx.bucketing(sample, params)
+    ctx.kmers_merge()
+    split = int(ctx.counters()[4])
+    s = ctx.solid()
+    g = solid_matrix(s.keys, s.sub, s.mult, s.flags)
+    t = time.perf_counter()
+    o = O.bucketing(sample, wl.k, wl.m, wl.buckets_log, wl.second_buckets_log, threads=threads)
+    o.merge(wl.k, wl.buckets_log, wl.second_buckets_log, wl.hash_kind, wl.min_mult, threads)
+    os_ = o.solid()
+    keys = np.stack([os_["key_lo"], os_["key_hi"]], 1) if g.shape[0] and s.keys.shape[1] > 1 else \
+        os_["key_lo"].reshape(-1, 1)
+    e = solid_matrix(keys, os_["sub"], os_["mult"], os_["flags"])
+    dt = time.perf_counter() - t
+    if not np.array_equal(g, e):
+        raise SystemExit(f"[bench] parity gate FAILED: GPU solid table ({g.shape[0]} entries) differs from the "
+                         f"oracle ({e.shape[0]} entries)")
+    bases = int(np.count_nonzero(sample != ord("\n")))
+    return {"bytes": int(sample.size), "solid_entries": int(g.shape[0]), "split_items": split,
+            "oracle_s": round(dt, 2), "oracle_threads": threads, "oracle_gbases_s": round(bases / dt / 1e9, 4),
+            "result": "bit-exact"}
+
+
 def cpu_baseline(wl, threads: int) -> dict:
     """The oracle restatement on a bounded prefix of the same reads."""
     from oracle import binding as O
@@ -226,6 +261,94 @@ def serve_measurement(ctx, rate_hz: float, n_requests: int) -> dict:
             "graph_build_s": round(t_build, 3), "index_build_ms": round(t_index * 1e3, 1)}
 
 
+def measure(ctx, wl, steps: int, warmup: int, tune: bool, check: bool, threads: int, scale: float,
+            sample_cap: int = 256 << 20) -> dict:
+    """Upload the workload, pick the schedule on a sample, gate it against the
+    oracle on that sample, then time `steps` bucketing + merge passes over the
+    device-resident input.  Leaves the last step's solid table in ctx."""
+    from ggcat_amd.native import Params
+
+    params = Params(wl.k, wl.m, wl.buckets_log, wl.second_buckets_log, 0, wl.hash_kind, wl.min_mult)
+    t = time.perf_counter()
+    dptr, nbytes = ctx.upload(wl.data)
+    h2d_s = time.perf_counter() - t
+    log(f"[bench] {wl.name}: input uploaded to HBM in {h2d_s * 1e3:.1f} ms")
+    cut = min(wl.data.size, sample_cap)
+    while cut < wl.data.size and wl.data[cut - 1] != ord("\n"):
+        cut += 1
+    sample = wl.data[:cut]
+    tuning = None
+    ctx.set_tuning()
+    if tune:
+        # schedule picked on a bounded sample of the same input; every
+        # candidate's solid table must have the same digest
+        from ggcat_amd.autotune import autotune
+
+        t = time.perf_counter()
+        tuning = autotune(ctx, sample, params, colored=False)
+        log(f"[bench] autotune: {tuning['tried']} schedules on {cut} bytes in {time.perf_counter() - t:.1f} s -> "
+            f"{tuning['schedule']}")
+    gate = None
+    if check:
+        gate = parity_gate(ctx, wl, params, sample, threads)
+        log(f"[bench] parity gate ok (schedule that is timed): {gate}")
+
+    def step():
+        ctx.bucketing(None, params, device_ptr=dptr, n_bytes=nbytes)
+        ctx.kmers_merge()
+
+    for i in range(warmup):
+        step()
+        log(f"[bench] warmup {i + 1}/{warmup} done")
+    per_kernel = []
+    t0 = time.perf_counter()
+    for i in range(steps):
+        step()
+        per_kernel.append(ctx.timings().copy())
+        log(f"[bench] step {i + 1}/{steps} done")
+    dt = time.perf_counter() - t0
+    ctx.free(dptr)
+    cnt = ctx.counters()
+    n, kw = int(cnt[5]), (1 if (wl.k <= 32 and wl.hash_kind == 0) else 2)
+    ms = np.mean(np.array(per_kernel), axis=0)
+    kb = kernel_bytes(cnt, kw)
+    kern_ms = {"bucket_stage": ms[0], "bucket_scatter": ms[2], "merge": ms[3] + ms[4]}
+    dom = max(kern_ms, key=lambda x: kern_ms[x])
+    achieved = kb[dom] / (kern_ms[dom] * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic(wl.name, scale, dom)
+    ms_step = dt / steps * 1e3
+    value = wl.bases / (dt / steps) / 1e9
+    return {
+        "metric": "input Gbases/s bucketing+kmer-merge (bit-exact); query p50/p99 ms",
+        "value": round(value, 4),
+        "unit": "Gbases/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8/u64 integer" if kw == 1 else "u8/u128 integer",
+        "data": "synthetic (seeded generator, same shape as the named configuration)",
+        "config": {"workload": wl.name, "k": wl.k, "m": wl.m, "min_multiplicity": wl.min_mult,
+                   "buckets_log": wl.buckets_log, "second_buckets_log": wl.second_buckets_log,
+                   "hash": "seq-hash" if wl.hash_kind == 0 else "rabin-karp128", "input_bytes": int(wl.data.size),
+                   "bases": int(wl.bases)},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_gb": round(kb[dom] / 1e9, 3), "traffic_source": traffic_src},
+        "kernels_ms": {k: round(float(v), 3) for k, v in kern_ms.items()},
+        "kernels_roofline": kernel_rooflines(kb, kern_ms, wl.name, scale),
+        "counters": {"records": int(cnt[1]), "record_bases": int(cnt[2]), "kmer_occurrences": int(cnt[3]),
+                     "split_items": int(cnt[4]), "solid_entries": int(n), "merge_passes": int(cnt[7])},
+        "h2d_ms": round(h2d_s * 1e3, 1),
+        "tuning": tuning,
+        "parity_gate": gate,
+        "pcie_inclusive_gbases_s": round(wl.bases / (dt / steps + h2d_s) / 1e9, 4),
+    }
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -238,6 +361,7 @@ def main() -> None:
     ap.add_argument("--no-serve", action="store_true", help="skip the colored query serving measurement")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default schedule")
     ap.add_argument("--no-e2e", action="store_true", help="skip the file-to-solid-table measurement")
+    ap.add_argument("--no-k63", action="store_true", help="skip the k=63 rabin-karp128 measurement (config 4)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson request rate of the serving run (1/s)")
     ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--calib-copy", action="store_true",
@@ -254,7 +378,6 @@ def main() -> None:
     wl = make_workload(a.config, a.scale)
     log(f"[bench] workload {wl.name}: {wl.data.size} bytes, {wl.bases} bases, k={wl.k} m={wl.m} "
         f"b={wl.buckets_log} sb={wl.second_buckets_log} ({time.perf_counter() - t:.1f} s to generate)")
-    params = Params(wl.k, wl.m, wl.buckets_log, wl.second_buckets_log, 0, wl.hash_kind, wl.min_mult)
     ctx = GpuContext(0)
     if a.calib_copy:
         src, dst = ctx.alloc(1 << 30), ctx.alloc(1 << 30)
@@ -262,101 +385,9 @@ def main() -> None:
         ctx.free(src)
         ctx.free(dst)
         log("[bench] calibration copy (copy16_kernel): 1073741824 bytes read, 1073741824 bytes written")
-    t = time.perf_counter()
-    dptr, nbytes = ctx.upload(wl.data)
-    h2d_s = time.perf_counter() - t
-    log(f"[bench] input uploaded to HBM in {h2d_s * 1e3:.1f} ms")
-
-    if not a.no_check:
-        # parity gate on a sample: GPU vs oracle solid tables, bit-exact
-        from oracle import binding as O
-
-        cut = min(wl.data.size, 3_000_000)
-        while cut < wl.data.size and wl.data[cut - 1] != ord("\n"):
-            cut += 1
-        sample = wl.data[:cut]
-        ctx.bucketing(sample, params)
-        ctx.kmers_merge()
-        s = ctx.solid()
-        o = O.bucketing(sample, wl.k, wl.m, wl.buckets_log, wl.second_buckets_log, threads=8)
-        o.merge(wl.k, wl.buckets_log, wl.second_buckets_log, wl.hash_kind, wl.min_mult, 8)
-        os_ = o.solid()
-        g = np.stack([s.sub.astype(np.uint64), s.keys[:, 0], s.keys[:, -1] if s.keys.shape[1] > 1 else 0 * s.keys[:, 0],
-                      s.mult.astype(np.uint64), s.flags.astype(np.uint64)], 1)
-        e = np.stack([os_["sub"].astype(np.uint64), os_["key_lo"], os_["key_hi"], os_["mult"].astype(np.uint64),
-                      os_["flags"].astype(np.uint64)], 1)
-        g = g[np.lexsort(g.T[::-1])]
-        e = e[np.lexsort(e.T[::-1])]
-        if not np.array_equal(g, e):
-            raise SystemExit("[bench] parity gate FAILED: GPU solid table differs from the oracle")
-        log(f"[bench] parity gate ok on {cut} bytes ({g.shape[0]} solid entries)")
-
-    tuning = None
-    if not a.no_autotune:
-        # schedule picked on a bounded sample of the same input (results are
-        # unchanged by construction and checked by the tuner)
-        from ggcat_amd.autotune import autotune
-
-        cut = min(wl.data.size, 256 << 20)
-        while cut < wl.data.size and wl.data[cut - 1] != ord("\n"):
-            cut += 1
-        t = time.perf_counter()
-        tuning = autotune(ctx, wl.data[:cut], params, colored=False)
-        log(f"[bench] autotune: {tuning['tried']} schedules on {cut} bytes in {time.perf_counter() - t:.1f} s -> "
-            f"{tuning['schedule']}")
-
-    def step():
-        ctx.bucketing(None, params, device_ptr=dptr, n_bytes=nbytes)
-        ctx.kmers_merge()
-
-    for i in range(a.warmup):
-        step()
-        log(f"[bench] warmup {i + 1}/{a.warmup} done")
-    per_kernel = []
-    t0 = time.perf_counter()
-    for i in range(a.steps):
-        step()
-        per_kernel.append(ctx.timings().copy())
-        log(f"[bench] step {i + 1}/{a.steps} done")
-    dt = time.perf_counter() - t0
-    cnt = ctx.counters()
-    n, kw = int(cnt[5]), (1 if (wl.k <= 32 and wl.hash_kind == 0) else 2)
-    ms = np.mean(np.array(per_kernel), axis=0)
-    kb = kernel_bytes(cnt, kw)
-    kern_ms = {"bucket_stage": ms[0], "bucket_scatter": ms[2], "merge": ms[3] + ms[4]}
-    dom = max(kern_ms, key=lambda x: kern_ms[x])
-    achieved = kb[dom] / (kern_ms[dom] * 1e-3) / 1e9
-    traffic, traffic_src = measured_traffic(wl.name, a.scale, dom)
-    ms_step = dt / a.steps * 1e3
-    value = wl.bases / (dt / a.steps) / 1e9
-    out = {
-        "metric": "input Gbases/s bucketing+kmer-merge (bit-exact); query p50/p99 ms",
-        "value": round(value, 4),
-        "unit": "Gbases/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": round(ms_step, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u8/u64 integer",
-        "data": "synthetic (seeded generator, same shape as the named configuration)",
-        "config": {"workload": wl.name, "k": wl.k, "m": wl.m, "min_multiplicity": wl.min_mult,
-                   "buckets_log": wl.buckets_log, "second_buckets_log": wl.second_buckets_log,
-                   "hash": "seq-hash" if wl.hash_kind == 0 else "rabin-karp128", "input_bytes": int(wl.data.size),
-                   "bases": int(wl.bases)},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_gb": round(kb[dom] / 1e9, 3), "traffic_source": traffic_src},
-        "kernels_ms": {k: round(float(v), 3) for k, v in kern_ms.items()},
-        "kernels_roofline": kernel_rooflines(kb, kern_ms, wl.name, a.scale),
-        "counters": {"records": int(cnt[1]), "record_bases": int(cnt[2]), "kmer_occurrences": int(cnt[3]),
-                     "split_items": int(cnt[4]), "solid_entries": int(n)},
-        "h2d_ms": round(h2d_s * 1e3, 1),
-        "tuning": tuning,
-        "pcie_inclusive_gbases_s": round(wl.bases / (dt / a.steps + h2d_s) / 1e9, 4),
-    }
+    threads = min(os.cpu_count() or 1, 64)
+    out = measure(ctx, wl, a.steps, a.warmup, not a.no_autotune, not a.no_check, threads, a.scale)
+    n = out["counters"]["solid_entries"]
     # unitig chains inside the sub-buckets of the last step's graph (device
     # kernels only; the sequences stay in HBM)
     t = time.perf_counter()
@@ -372,7 +403,18 @@ def main() -> None:
     out["maximal_unitigs"] = {"unitigs": n_max, "bytes": max_bytes, "kernels_ms": round(k_ms, 2),
                               "call_ms": round((time.perf_counter() - t) * 1e3, 1),
                               "solid_kmers_per_s": round(n / (k_ms / 1e3), 1) if k_ms > 0 else None}
-    ctx.free(dptr)
+    schedule = (out.get("tuning") or {}).get("schedule")
+    if a.config == 2 and not a.no_k63:
+        # the 128-bit key path at full size: k = 63, rabin-karp128, 15 Gbases
+        t = time.perf_counter()
+        wl4 = make_workload(4, a.scale)
+        log(f"[bench] workload {wl4.name}: {wl4.data.size} bytes, {wl4.bases} bases "
+            f"({time.perf_counter() - t:.1f} s to generate)")
+        out["config4"] = measure(ctx, wl4, max(1, min(a.steps, 2)), 1, not a.no_autotune, not a.no_check, threads,
+                                 a.scale, sample_cap=64 << 20)
+        del wl4
+    # the file-to-table run uses the same read set, so the same schedule
+    ctx.set_tuning(**schedule) if schedule else ctx.set_tuning()
     if not a.no_e2e:
         log("[bench] end-to-end from a FASTA file ...")
         try:

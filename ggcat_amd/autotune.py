@@ -27,6 +27,7 @@ KNOBS = ("item_occ", "lds_max_occ", "merge_wgs_per_cu", "part_occ", "lds_table_l
 
 @dataclass
 class Schedule:
+    item_occ: int = 0
     unit_bytes: int = 0
     table_pct: int = 0
     lds_table_log: int = 0
@@ -38,7 +39,7 @@ class Schedule:
     merge_threads: int = 0
 
     def apply(self, ctx) -> None:
-        ctx.set_tuning(unit_bytes=self.unit_bytes, table_pct=self.table_pct, lds_table_log=self.lds_table_log,
+        ctx.set_tuning(item_occ=self.item_occ, unit_bytes=self.unit_bytes, table_pct=self.table_pct, lds_table_log=self.lds_table_log,
                        no_dedup=self.no_dedup, merge_wgs_per_cu=self.merge_wgs_per_cu, use_graphs=self.use_graphs,
                        rec_cap=self.rec_cap, stage_tile_cap=self.stage_tile_cap, merge_threads=self.merge_threads)
 
@@ -54,6 +55,21 @@ def default_grid(k: int) -> list[Schedule]:
     # smaller workgroups with smaller tables over smaller units
     out.append(Schedule(unit_bytes=1000, table_pct=200, lds_table_log=lg - 1, rec_cap=128, use_graphs=1,
                         merge_threads=128))
+    return out
+
+
+def load_grid(best: Schedule, k: int) -> list[Schedule]:
+    """Second stage around the best first-stage schedule: the hash table's
+    load factor (table slots per 100 k-mer occurrences of a work item) and
+    the work-item size, which together set how full the LDS tables run."""
+    lds = 1 << (best.lds_table_log or (11 if k <= 32 else 10))
+    out = []
+    for pct, occ in itertools.product((100, 200, 400), (lds // 2, lds, 2 * lds)):
+        if (pct, occ) == (best.table_pct or 200, best.item_occ or lds // 2):
+            continue
+        s = Schedule(**asdict(best))
+        s.table_pct, s.item_occ = pct, occ
+        out.append(s)
     return out
 
 
@@ -82,28 +98,39 @@ def autotune(ctx, sample: np.ndarray, params: Params, grid: list[Schedule] | Non
         if cached:
             Schedule(**cached["schedule"]).apply(ctx)
             return cached
+    two_stage = grid is None
     grid = grid or default_grid(params.k)
     sample = np.ascontiguousarray(sample, np.uint8)
     ptr, n = ctx.upload(sample)
     ref = None
     best = None
+    tried = 0
+
+    def run(sch):
+        nonlocal ref, best, tried
+        sch.apply(ctx)
+        ts = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            ctx.bucketing(None, params, device_ptr=ptr, n_bytes=n)
+            ctx.kmers_merge()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        d = _digest(ctx)
+        if ref is None:
+            ref = d
+        elif d != ref:
+            raise RuntimeError(f"schedule {sch} changed the solid table")
+        tried += 1
+        ms = min(ts)
+        if best is None or ms < best[0]:
+            best = (ms, sch)
+
     try:
         for sch in grid:
-            sch.apply(ctx)
-            ts = []
-            for _ in range(repeats):
-                t0 = time.perf_counter()
-                ctx.bucketing(None, params, device_ptr=ptr, n_bytes=n)
-                ctx.kmers_merge()
-                ts.append((time.perf_counter() - t0) * 1e3)
-            d = _digest(ctx)
-            if ref is None:
-                ref = d
-            elif d != ref:
-                raise RuntimeError(f"schedule {sch} changed the solid table")
-            ms = min(ts)
-            if best is None or ms < best[0]:
-                best = (ms, sch)
+            run(sch)
+        if two_stage:
+            for sch in load_grid(best[1], params.k):
+                run(sch)
         rec_b = params.buckets_log
         for b in bucket_logs:
             p2 = Params(params.k, params.m, b, params.second_buckets_log, params.forward_only, params.hash_kind,
@@ -118,7 +145,7 @@ def autotune(ctx, sample: np.ndarray, params: Params, grid: list[Schedule] | Non
     finally:
         ctx.free(ptr)
     best[1].apply(ctx)
-    out = {"schedule": asdict(best[1]), "ms": round(best[0], 3), "tried": len(grid), "buckets_log": rec_b,
+    out = {"schedule": asdict(best[1]), "ms": round(best[0], 3), "tried": tried, "buckets_log": rec_b,
            "sample_bytes": int(sample.size)}
     if cache_path:
         allc = json.load(open(cache_path)) if os.path.exists(cache_path) else {}

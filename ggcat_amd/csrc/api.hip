@@ -467,11 +467,12 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     const uint32_t n_sub = ((1u << p.buckets_log) + 1) << p.second_buckets_log;
     // physical units: about `unit_bytes` input bytes per unit (default 2000,
     // 1000 for colored builds),
-    // at most 2^23 units
+    // at most 2^25 units (unit sizes then stay near unit_bytes up to ~30 GB
+    // of input, so a schedule tuned on a sample holds at full size)
     uint32_t xb = 0;
     // (colored builds have smaller LDS tables per workgroup: smaller units)
     const uint64_t unit_bytes = ctx->tune.unit_bytes ? ctx->tune.unit_bytes : (n_colors ? 1000 : 2000);
-    while (xb < 8 && (n_bytes >> xb) / n_sub > unit_bytes && ((uint64_t)n_sub << (xb + 1)) <= (1u << 23)) xb++;
+    while (xb < 10 && (n_bytes >> xb) / n_sub > unit_bytes && ((uint64_t)n_sub << (xb + 1)) <= (1u << 25)) xb++;
     const uint32_t n_units = n_sub << xb;
     // slot: meta word + inline bases for the longest unique-minimizer record (2k - m)
     uint32_t sw = 1 + (2 * p.k - p.m + 15) / 16;
@@ -679,6 +680,94 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     ctx->ms[5] = ctx->ms[0] + ctx->ms[1] + ctx->ms[2];
     ctx->kmer_occ = kmer_total;  // sizes the merge output
     ctx->have_records = true;
+    return GGCAT_GPU_OK;
+}
+
+// Outlier sub-buckets of the last bucketing as crates/kmers_transform/src/lib.rs:171-371
+// would find them: buckets ordered by size (largest first, after the
+// min(normal / 8, 2 * threads) smallest ones, then alternating a large one
+// with small ones that match its size, :195-243), a running average of
+// records per non-empty sub-bucket over the buckets taken so far (at least
+// MIN_AVERAGE_CAP = 5000), and a sub-bucket above 8x that average
+// (MAX_SUBBUCKET_AVERAGE_MULTIPLIER, crates/config/src/lib.rs:62-66) is
+// resplit into next_pow2(records / average) clamped to [4, 1024] buckets.
+// The reference orders by compressed bucket-file size and counts compacted
+// records; this uses the serialized record bytes and the records as staged.
+int ggcat_gpu_outliers(ggcat_gpu_ctx* ctx, uint32_t threads, uint32_t* n_outliers, uint32_t* sub, uint32_t* resplit,
+                       uint64_t* records, uint32_t cap) {
+    if (!ctx || !n_outliers) return GGCAT_GPU_EINVAL;
+    if (!ctx->have_records) return fail(ctx, GGCAT_GPU_ESTATE, "no bucketing result");
+    HIPCHK(hipSetDevice(ctx->device));
+    const uint32_t n_sub = ctx->n_sub, sb = ctx->P.second_buckets_log;
+    std::vector<uint64_t> rec(n_sub), bas(n_sub);
+    DevBuf tmp;
+    if (ensure(ctx, tmp, (size_t)n_sub * 16) != GGCAT_GPU_OK) return GGCAT_GPU_ENOMEM;
+    hipLaunchKernelGGL(sub_totals_kernel, dim3((n_sub + 255) / 256), dim3(256), 0, ctx->stream,
+                       P_<uint32_t>(ctx->unit_count), P_<uint32_t>(ctx->unit_kmers), n_sub, ctx->xb, ctx->P.k,
+                       P_<uint64_t>(tmp), P_<uint64_t>(tmp) + n_sub);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(rec.data(), tmp.p, (size_t)n_sub * 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(bas.data(), P_<uint64_t>(tmp) + n_sub, (size_t)n_sub * 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(tmp.p);
+    if (e != hipSuccess) return fail(ctx, GGCAT_GPU_EHIP, std::string("outlier counts: ") + hipGetErrorString(e));
+    const uint32_t n_buckets = n_sub >> sb, normal = n_buckets - 1;
+    // bucket sizes: serialized record bytes (second-bucket byte, two length
+    // bytes, packed bases)
+    std::vector<std::pair<uint64_t, uint32_t>> sizes(n_buckets);
+    for (uint32_t b = 0; b < n_buckets; b++) {
+        uint64_t sz = 0;
+        for (uint32_t s = b << sb; s < (b + 1) << sb; s++) sz += rec[s] * 3 + (bas[s] + 3 * rec[s]) / 4;
+        sizes[b] = {sz, b};
+    }
+    // sort_by_key (stable, ascending) then reverse
+    std::stable_sort(sizes.begin(), sizes.end(),
+                     [](const std::pair<uint64_t, uint32_t>& a, const std::pair<uint64_t, uint32_t>& b) {
+                         return a.first < b.first;
+                     });
+    std::reverse(sizes.begin(), sizes.end());
+    std::vector<uint32_t> order;
+    size_t start = 0, end = sizes.size();
+    int64_t matched = 0;
+    uint64_t estimators = std::min<uint64_t>(normal / 8, 2ull * std::max<uint32_t>(threads, 1));
+    while (start != end && estimators > 0) {
+        end--;
+        estimators--;
+        order.push_back(sizes[end].second);
+    }
+    while (start != end) {
+        if (matched <= 0) {
+            matched = (int64_t)sizes[start].first;
+            order.push_back(sizes[start++].second);
+        } else {
+            matched -= (int64_t)sizes[end - 1].first;
+            order.push_back(sizes[--end].second);
+        }
+    }
+    uint64_t tot_seq = 0, tot_cnt = 0;
+    uint32_t n = 0;
+    for (uint32_t b : order) {
+        for (uint32_t s = b << sb; s < (b + 1) << sb; s++)
+            if (rec[s]) {
+                tot_seq += rec[s];
+                tot_cnt++;
+            }
+        const uint64_t avg = std::max<uint64_t>(tot_seq / std::max<uint64_t>(tot_cnt, 1), 5000);
+        for (uint32_t s = b << sb; s < (b + 1) << sb; s++) {
+            if (!rec[s] || rec[s] <= avg * 8) continue;
+            uint64_t c = 1;
+            while (c < rec[s] / avg) c <<= 1;
+            c = std::min<uint64_t>(std::max<uint64_t>(c, 4), 1024);
+            if (n < cap) {
+                if (sub) sub[n] = s;
+                if (resplit) resplit[n] = (uint32_t)c;
+                if (records) records[n] = rec[s];
+            }
+            n++;
+        }
+    }
+    *n_outliers = n;
     return GGCAT_GPU_OK;
 }
 

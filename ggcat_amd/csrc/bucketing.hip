@@ -899,6 +899,20 @@ __global__ __launch_bounds__(256) void unit_split_kernel(const unsigned long lon
     }
 }
 
+// Records and bases per sub-bucket (sums over its 2^xb units).
+__global__ void sub_totals_kernel(const uint32_t* unit_count, const uint32_t* unit_kmers, uint32_t n_sub, uint32_t xb,
+                                  uint32_t k, uint64_t* records, uint64_t* bases) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_sub) return;
+    uint64_t r = 0, b = 0;
+    for (uint32_t u = s << xb; u < (s + 1) << xb; u++) {
+        r += unit_count[u];
+        b += unit_kmers[u] + (uint64_t)unit_count[u] * (k - 1);
+    }
+    records[s] = r;
+    bases[s] = b;
+}
+
 // Clears the cursors of pass 1 (a kernel, so the pass can be captured in a graph).
 __global__ void bucket_reset_kernel(BucketArgs A) {
     if (threadIdx.x == 0) {

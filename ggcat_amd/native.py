@@ -122,6 +122,7 @@ def gpu_lib(path: str = GPU_LIB) -> C.CDLL:
     lib.ggcat_gpu_colormap_subset.argtypes = [V, C.c_uint64, _u32p, C.c_uint32, _u32p]
     lib.ggcat_gpu_colormap_close.argtypes = [V]
     lib.ggcat_gpu_partial_unitigs.argtypes = [V, C.c_uint32, _u8p, C.POINTER(C.c_uint16), _u8p, _u8p, _u8p]
+    lib.ggcat_gpu_outliers.argtypes = [V, C.c_uint32, _u32p, _u32p, _u32p, _u64p, C.c_uint32]
     lib.ggcat_gpu_query_graph.argtypes = [V, C.c_char_p, C.c_char_p, C.c_char_p, C.c_uint32, C.c_uint32, C.c_int,
                                           C.c_uint32, C.c_int, C.c_uint32, C.c_uint32, C.c_char_p, C.c_uint64]
     lib.ggcat_gpu_colormap_close.restype = None
@@ -161,7 +162,7 @@ def gpu_lib(path: str = GPU_LIB) -> C.CDLL:
                  "ggcat_gpu_extension_copy", "ggcat_gpu_copy_to_host", "ggcat_gpu_query_device_noalloc",
                  "ggcat_gpu_color_subsets", "ggcat_gpu_colormap_write", "ggcat_gpu_colormap_write_sets",
                  "ggcat_gpu_colormap_open", "ggcat_gpu_colormap_info", "ggcat_gpu_colormap_subset",
-                 "ggcat_gpu_query_graph", "ggcat_gpu_partial_unitigs"):
+                 "ggcat_gpu_query_graph", "ggcat_gpu_partial_unitigs", "ggcat_gpu_outliers"):
         getattr(lib, name).restype = C.c_int
     _gpu[path] = lib
     return lib
@@ -438,6 +439,16 @@ class GpuContext:
         self._check(self.lib.ggcat_gpu_unitigs_copy(self.h, _ptr(seq, _u8p), _ptr(sub, _u32p), _ptr(fl, _u8p)),
                     "ggcat_gpu_unitigs_copy")
         return seq.tobytes().split(b"\n")[:-1], sub, fl
+
+    def outliers(self, threads: int) -> list[tuple[int, int, int]]:
+        """(sub-bucket, resplit bucket count, records) of every outlier sub-bucket."""
+        n = C.c_uint32()
+        self._check(self.lib.ggcat_gpu_outliers(self.h, threads, C.byref(n), C.cast(None, _u32p),
+                                                C.cast(None, _u32p), C.cast(None, _u64p), 0), "outliers")
+        s, r, c = np.zeros(n.value + 1, np.uint32), np.zeros(n.value + 1, np.uint32), np.zeros(n.value + 1, np.uint64)
+        self._check(self.lib.ggcat_gpu_outliers(self.h, threads, C.byref(n), _ptr(s, _u32p), _ptr(r, _u32p),
+                                                _ptr(c, _u64p), n.value), "outliers")
+        return [(int(s[i]), int(r[i]), int(c[i])) for i in range(n.value)]
 
     def partial_unitigs(self, buckets_log: int, n: int) -> dict:
         """Routing of the last per-sub-bucket unitigs (n of them) as the

@@ -84,6 +84,21 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
                         const uint64_t* color_starts, const uint32_t* colors, uint32_t n_colors,
                         const ggcat_gpu_params* params);
 
+/* Sub-buckets of the last bucketing that the reference would resplit as
+ * outliers (crates/kmers_transform/src/lib.rs:171-371): buckets in the
+ * reference's processing order (largest first after the min(normal / 8,
+ * 2 * threads) smallest, then a large bucket alternated with small ones
+ * matching its size), a running average of records per non-empty sub-bucket
+ * (at least 5000), and a sub-bucket above 8x it resplit into next_pow2(records
+ * / average) buckets clamped to [4, 1024] (crates/config/src/lib.rs:62-66).
+ * The order uses the serialized record bytes of each bucket and the records
+ * as bucketed (the reference uses compressed file sizes and compacted
+ * counts).  Writes up to cap entries (sub-bucket, resplit bucket count,
+ * records); *n_outliers = all of them.  Detection only: the merge treats
+ * outliers like every other sub-bucket. */
+int ggcat_gpu_outliers(ggcat_gpu_ctx* ctx, uint32_t threads, uint32_t* n_outliers, uint32_t* sub, uint32_t* resplit,
+                       uint64_t* records, uint32_t cap);
+
 /* Sizes of the last bucketing result: records, sub-buckets, and the bytes
  * needed by ggcat_gpu_records_copy for the bases buffer. */
 int ggcat_gpu_records_info(ggcat_gpu_ctx* ctx, uint64_t* n_records, uint32_t* n_sub, uint64_t* n_bases);

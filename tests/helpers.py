@@ -133,3 +133,38 @@ def adversarial_records(rng: np.random.Generator) -> list[bytes]:
     for _ in range(40):
         recs.append(rnd(int(rng.integers(20, 400))))
     return recs
+
+
+def inline_capacity(k: int, m: int) -> int:
+    """Bases a device record slot holds inline: the slot has 1 + ceil((2k - m)
+    / 16) words rounded up to a power of two (at least 4), one of them the meta
+    word, 16 bases per word.  Longer records go to the long-record pool."""
+    sw = 1 + (2 * k - m + 15) // 16
+    w = 4
+    while w < sw:
+        w <<= 1
+    return (w - 1) * 16
+
+
+def edge_records_inline(rng: np.random.Generator, k: int) -> list[bytes]:
+    """The reference's edge cases that stay within an inline slot: empty and
+    short records (k-1, k, k+1), N runs, lowercase and IUPAC / other non-ACGT
+    bytes (all of them split records, crates/minimizer_bucketing/src/sequences_splitter.rs:15-40),
+    short homopolymers and tandem repeats (windows whose minimum occurs twice:
+    duplicates bucket, crates/assembler_minimizer_bucketing/src/lib.rs:221-227)
+    embedded in random sequence, and long random contigs spanning many tiles."""
+    def rnd(n):
+        return bytes(rng.choice(np.frombuffer(b"ACGT", np.uint8), n))
+
+    recs = [b"", b"ACGT", rnd(k - 1), rnd(k), rnd(k + 1), rnd(k - 1) + b"N" + rnd(k), b"N" * 40, rnd(k).lower(),
+            rnd(60) + b"N" + rnd(40), rnd(10) + b"NNNN" + rnd(k) + b"N" + rnd(k - 2) + b"NN" + rnd(80),
+            rnd(70).lower(), rnd(20) + rnd(40).lower() + rnd(20)]
+    for c in b"RYKMSWBDHVUX-.*n":
+        recs.append(rnd(int(rng.integers(k, 3 * k))) + bytes([c]) + rnd(int(rng.integers(k, 3 * k))))
+    for unit in (b"A", b"C", b"G", b"T", b"AC", b"GT", b"ACG", b"GATTACA"):
+        for reps in (k // len(unit) + 1, k // len(unit) + 6):
+            recs.append(rnd(int(rng.integers(0, 40))) + unit * reps + rnd(int(rng.integers(0, 40))))
+    recs += [rnd(20_000), rnd(7_777), rnd(3000).lower() + rnd(3000)]
+    for _ in range(120):
+        recs.append(rnd(int(rng.integers(k - 2, 400))))
+    return recs

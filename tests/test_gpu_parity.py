@@ -7,9 +7,11 @@ import numpy as np
 import pytest
 
 from ggcat_amd import synth
+from ggcat_amd import native
 from ggcat_amd.native import HASH_RK128, HASH_SEQ, Params
 from oracle import binding as O
-from tests.helpers import adversarial_records, joined, record_digests, record_tuples, solid_array
+from tests.helpers import (adversarial_records, edge_records_inline, inline_capacity, joined,
+                           record_digests, record_tuples, solid_array)
 
 pytestmark = pytest.mark.gpu
 
@@ -62,6 +64,38 @@ def test_adversarial(gpu_ctx, k, m):
     _say(f"case adversarial k={k} m={m} bytes={data.size}")
     _run_both(gpu_ctx, data, k, m, 4, 2, s=1)
     _run_both(gpu_ctx, data, k, m, 6, 3, s=2)
+
+
+def _inline_only(records: list[bytes], k: int, m: int, b: int, sb: int) -> list[bytes]:
+    """The records whose super-k-mers all fit an inline slot (checked with the
+    oracle, on the CPU, before anything reaches the GPU)."""
+    cap = inline_capacity(k, m)
+    keep = []
+    for r in records:
+        o = O.bucketing(joined([r]), k, m, b, sb, threads=1).records()
+        if o["length"].size == 0 or int(o["length"].max()) <= cap:
+            keep.append(r)
+    return keep
+
+
+@pytest.mark.parametrize("k,m", [(31, 15), (21, 9), (63, 21), (32, 12), (11, 5), (45, 21)])
+def test_edge_inputs_inline_records(gpu_ctx, k, m):
+    # N runs, lowercase, IUPAC and other non-ACGT bytes, records of length
+    # k-1 / k / k+1, short homopolymers and tandem repeats (duplicate window
+    # minima: duplicates-bucket records), contigs spanning many tiles.  Only
+    # inputs whose records fit inline slots: records longer than a slot (the
+    # long-record pool) stay off the GPU, see LONG_RECORD_HANG.
+    rng = np.random.default_rng(2000 + k * 7 + m)
+    recs = _inline_only(edge_records_inline(rng, k), k, m, 6, 3)
+    data = joined(recs)
+    o = O.bucketing(data, k, m, 6, 3, threads=4).records()
+    assert int(o["length"].max()) <= inline_capacity(k, m)
+    n_dup = int(np.count_nonzero((o["sub"] >> 3) == (1 << 6)))
+    assert n_dup > 0  # duplicate-minimizer windows are present
+    _say(f"case edge inputs k={k} m={m} records={len(recs)} bytes={data.size} duplicates-bucket={n_dup}")
+    _run_both(gpu_ctx, data, k, m, 6, 3, s=1)
+    _run_both(gpu_ctx, data, k, m, 4, 2, s=2)
+    _run_both(gpu_ctx, data, k, m, 6, 3, s=2, kind=HASH_RK128)
 
 
 def test_empty_and_separators(gpu_ctx):
@@ -343,6 +377,215 @@ def test_maximal_unitigs_and_fasta(gpu_ctx, tmp_path, k, m, rep):
     lines = open(path, "rb").read().split(b"\n")
     assert lines[1::2][:len(gu[0])] == gu[0]
     assert lines[0] == b">0 LN:i:%d" % len(gu[0][0])
+
+
+def _outliers_restated(sub: np.ndarray, length: np.ndarray, n_sub: int, sb: int, threads: int) -> list:
+    # crates/kmers_transform/src/lib.rs:171-371 on the oracle's records:
+    # bucket order (sizes descending, the min(normal / 8, 2 threads) smallest
+    # first, then a large bucket and the small ones matching its size), the
+    # running average over non-empty sub-buckets (at least 5000), 8x rule
+    rec = np.bincount(sub, minlength=n_sub).astype(np.int64)
+    bas = np.bincount(sub, weights=length, minlength=n_sub).astype(np.int64)
+    nb = n_sub >> sb
+    sizes = []
+    for b in range(nb):
+        r, ba = rec[b << sb:(b + 1) << sb], bas[b << sb:(b + 1) << sb]
+        sizes.append((int((r * 3 + (ba + 3 * r) // 4).sum()), b))
+    sizes = sorted(sizes, key=lambda x: x[0])[::-1]
+    order, start, end, matched = [], 0, len(sizes), 0
+    est = min((nb - 1) // 8, 2 * threads)
+    while start != end and est > 0:
+        end -= 1
+        est -= 1
+        order.append(sizes[end][1])
+    while start != end:
+        if matched <= 0:
+            matched = sizes[start][0]
+            order.append(sizes[start][1])
+            start += 1
+        else:
+            matched -= sizes[end - 1][0]
+            end -= 1
+            order.append(sizes[end][1])
+    tot, cnt, out = 0, 0, []
+    for b in order:
+        subs = range(b << sb, (b + 1) << sb)
+        for s in subs:
+            if rec[s]:
+                tot, cnt = tot + int(rec[s]), cnt + 1
+        avg = max(tot // max(cnt, 1), 5000)
+        for s in subs:
+            if rec[s] and rec[s] > avg * 8:
+                c = 1
+                while c < rec[s] // avg:
+                    c <<= 1
+                out.append((s, min(max(c, 4), 1024), int(rec[s])))
+    return out
+
+
+def test_outlier_detection(gpu_ctx):
+    # a repeat-heavy input: short tandem-repeat reads (inline records) pile
+    # up in one duplicates-bucket sub-bucket far above 8x the average
+    rng = np.random.default_rng(3)
+    g = synth.genome(93, 200_000)
+    ac = [(b"AC" * 30)[p:p + 40] for p in rng.integers(0, 2, 220_000)]
+    data = np.concatenate([synth.reads(g, 94, 13_000, 150, 0.005), joined(ac)])
+    _say(f"case outlier detection bytes={data.size}")
+    rec = _run_both(gpu_ctx, data, 31, 15, 4, 2, s=2, exact=False)
+    want = _outliers_restated(rec.sub.astype(np.int64), rec.length.astype(np.int64), ((1 << 4) + 1) << 2, 2, 4)
+    assert want and want[0][0] == (1 << 4) << 2  # the duplicates bucket's first sub-bucket
+    assert gpu_ctx.outliers(4) == want
+    assert gpu_ctx.outliers(64) == _outliers_restated(rec.sub.astype(np.int64), rec.length.astype(np.int64),
+                                                      ((1 << 4) + 1) << 2, 2, 64)
+
+
+def _seqhash_bucket(kmer: bytes, k: int, log: int) -> tuple[int, bool]:
+    # CanonicalSeqHashFactory::get_bucket(0, log, min(fw, rc)) with wrapping
+    # arithmetic (crates/hashes/src/base/cn_seqhash_base.rs:140-150,
+    # constants crates/hashes/src/cn_seqhash.rs:15-27), and is_forward (fw < rc)
+    code = [(c >> 1) & 3 for c in kmer]
+    fw = sum(c << (2 * i) for i, c in enumerate(code))
+    rc = sum((c ^ 2) << (2 * (k - 1 - i)) for i, c in enumerate(code))
+    h = min(fw, rc)
+    if k <= 32:
+        bits, mul, basis = 64, 0x00000100000001B3, 0xCBF29CE484222325
+    else:
+        bits, mul, basis = 128, 0x0000000001000000000000000000013B, 0x6C62272E07BB014262B821756295C58D
+    m = (1 << bits) - 1
+    x = (h * mul + basis) & m
+    x = ((x >> 3) | (x << (bits - 3))) & m
+    return x % (1 << log), fw < rc
+
+
+@pytest.mark.parametrize("k,m", [(31, 15), (45, 21)])
+def test_partial_unitig_routing(gpu_ctx, k, m):
+    # output_sequence (crates/assembler_kmers_merge/src/final_executor.rs:103-246):
+    # lonely / circular / partial, the smaller end bucket, the stored
+    # orientation that makes the hashed end k-mer forward-canonical
+    g = synth.genome(91, 150_000, repeat_frac=0.2, rep_min=300, rep_max=2000)
+    data = synth.reads(g, 92, 20_000, 150, 0.005)
+    _say(f"case partial unitig routing k={k} m={m} bytes={data.size}")
+    gpu_ctx.bucketing(data, _params(k, m, 5, 2, 2))
+    gpu_ctx.kmers_merge()
+    seqs, _, fl = gpu_ctx.unitigs()
+    r = gpu_ctx.partial_unitigs(5, len(seqs))
+    n_partial = 0
+    for i, s in enumerate(seqs):
+        back, fwd, circ = bool(fl[i] & 1), bool(fl[i] & 2), bool(fl[i] & 4)
+        if circ:
+            assert r["kind"][i] == 1
+            continue
+        if not (back or fwd):
+            assert r["kind"][i] == 0 and r["bucket"][i] == 0xFFFF
+            continue
+        n_partial += 1
+        assert r["kind"][i] == 2
+        lb, lfw = _seqhash_bucket(s[:k], k, 5) if back else (0xFFFF, True)
+        rb, rfw = _seqhash_bucket(s[-k:], k, 5) if fwd else (0xFFFF, False)
+        hash_beginning = lb <= rb
+        should_rc = (not lfw) if hash_beginning else (not rfw)
+        assert r["bucket"][i] == min(lb, rb)
+        assert r["rc"][i] == should_rc
+        hash_ending = (not hash_beginning) ^ should_rc
+        assert r["flags"][i] == (2 if hash_ending else 0) | (4 if back and fwd else 0)
+        assert r["last_align"][i] == ((len(s) - k) % 4 if hash_ending else 0)
+        # the stored read (reverse-complemented when rc) has its hashed k-mer,
+        # at the end when HASH_ENDING, in forward-canonical orientation
+        stored = s if not should_rc else s[::-1].translate(bytes.maketrans(b"ACGT", b"TGCA"))
+        end_kmer = stored[-k:] if hash_ending else stored[:k]
+        assert _seqhash_bucket(end_kmer, k, 5)[1]
+    assert n_partial > 100
+
+
+def _canonical_keys(seq: bytes, k: int) -> np.ndarray:
+    code = (np.frombuffer(seq, np.uint8).astype(np.uint64) >> np.uint64(1)) & np.uint64(3)  # A0 C1 T2 G3
+    win = np.lib.stride_tricks.sliding_window_view(code, k)
+    sh = (np.arange(k, dtype=np.uint64) * np.uint64(2))
+    fw = np.bitwise_or.reduce(win << sh, axis=1)
+    rc = np.bitwise_or.reduce((win ^ np.uint64(2)) << sh[::-1], axis=1)
+    return np.minimum(fw, rc)
+
+
+def test_colored_unitigs_fasta_and_colormap(gpu_ctx, tmp_path):
+    # colored FASTA headers " C:{subset hex}:{count}" (crates/colors/src/managers/multiple.rs:707-740)
+    # and the color map file: every unitig k-mer's subset, read back through
+    # the color map, is the oracle's color set of that k-mer
+    from tests.helpers import bitmaps_from_sets
+    wl = synth.config3(0.01, 32)
+    starts, colors = wl.colors
+    _say(f"case colored unitigs + colormap genomes=32 bytes={wl.data.size}")
+    gpu_ctx.bucketing(wl.data, _params(31, 15, wl.buckets_log, wl.second_buckets_log, 1), colors=(starts, colors))
+    gpu_ctx.kmers_merge()
+    seqs, _, _ = gpu_ctx.unitigs(joined=True)
+    fa, cm = str(tmp_path / "g.fa"), str(tmp_path / "g.colors.dat")
+    names = [f"genome_{i}.fa" for i in range(len(colors))]
+    assert gpu_ctx.write_unitigs_fasta(fa) == len(seqs)
+    n_sub = gpu_ctx.colormap_write(cm, names)
+    got_names, subsets = native.colormap_read(cm)
+    assert got_names == names and len(subsets) == n_sub
+    assert len({tuple(s) for s in subsets}) == n_sub  # one id per distinct set
+    o = O.bucketing_colored(wl.data, 31, 15, wl.buckets_log, wl.second_buckets_log, starts, colors, threads=4)
+    o.merge(31, wl.buckets_log, wl.second_buckets_log, HASH_SEQ, 1, 4)
+    osol = o.solid()
+    off, cols = o.solid_colors()
+    bm = bitmaps_from_sets(off, cols, 1)
+    want = dict(zip(osol["key_lo"].tolist(), bm[:, 0].tolist()))
+    lines = open(fa, "rb").read().split(b"\n")
+    heads, body = lines[0::2][:len(seqs)], lines[1::2][:len(seqs)]
+    assert body == seqs
+    seen = 0
+    for i, (h, s) in enumerate(zip(heads, body)):
+        f = h.split(b" ")
+        assert f[0] == b">%d" % i and f[1] == b"LN:i:%d" % len(s)
+        ids = []
+        for tag in f[2:]:
+            c, sid, cnt = tag.split(b":")
+            assert c == b"C" and int(cnt) > 0
+            ids += [int(sid, 16)] * int(cnt)
+        keys = _canonical_keys(s, 31)
+        assert len(ids) == keys.size
+        for key, sid in zip(keys.tolist(), ids):
+            assert sum(1 << c for c in subsets[sid]) == want[key]
+        seen += keys.size
+    assert seen == len(want)  # every solid k-mer is on exactly one unitig
+
+
+def _write_queries_fasta(path, q: np.ndarray) -> None:
+    recs = q.tobytes().split(b"\n")[:-1]
+    open(path, "wb").write(b"".join(b">q%d\n%s\n" % (i, r) for i, r in enumerate(recs)))
+
+
+@pytest.mark.parametrize("colored", [True, False])
+def test_query_graph_from_files(gpu_ctx, tmp_path, colored):
+    # build -> unitig FASTA (+ color map) on disk -> ggcat_gpu_query_graph from
+    # the files: the same counts as querying the build in place, and as the
+    # oracle, written in the reference's CSV / JSON lines
+    from ggcat_amd.query import format_csv, format_jsonl
+    gs = synth.config3_genomes(0.01, 24)
+    data, starts = synth.contigs_joined(gs, 505)
+    cols = np.arange(24, dtype=np.uint32)
+    q = synth.queries(gs, 68, 250, 100, 4000, 0.02)
+    _say(f"case query from graph files colored={colored} bytes={data.size}")
+    qk, mt, cc = _query_parity(gpu_ctx, data, 31, 15, 5, 3, 1, HASH_SEQ, (starts, cols) if colored else None, q)
+    gpu_ctx.unitigs(joined=True)
+    fa, qf = str(tmp_path / "graph.fa"), str(tmp_path / "queries.fa")
+    gpu_ctx.write_unitigs_fasta(fa)
+    names = [f"g{i}.fa" for i in range(24)] + []
+    names[3] = 'odd"name\\x'
+    if colored:
+        gpu_ctx.colormap_write(str(tmp_path / "graph.colors.dat"), names)
+    _write_queries_fasta(qf, q)
+    out = gpu_ctx.query_graph(fa, qf, str(tmp_path / "result"), 31, colors=colored)
+    assert out == str(tmp_path / ("result.jsonl" if colored else "result.csv"))
+    text = open(out).read()
+    if colored:
+        assert text == format_jsonl(qk, cc)
+        out2 = gpu_ctx.query_graph(fa, qf, str(tmp_path / "named.jsonl.gz"), 31, colors=True, names=True)
+        import gzip
+        esc = [n.replace('"', '\\"').replace("\\", "\\\\") for n in names]
+        assert gzip.open(out2, "rt").read() == format_jsonl(qk, cc, esc)
+    else:
+        assert text == format_csv(qk, mt)
 
 
 def test_autotune_and_graphs_keep_results(gpu_ctx):
