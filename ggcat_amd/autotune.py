@@ -21,7 +21,7 @@ import numpy as np
 from .native import Params
 
 KNOBS = ("item_occ", "lds_max_occ", "merge_wgs_per_cu", "part_occ", "lds_table_log", "table_pct", "no_dedup",
-         "unit_bytes", "use_graphs", "stage_tile_cap", "rec_cap",
+         "unit_bytes", "use_graphs", "unit_cap", "rec_cap",
          "merge_threads")
 
 
@@ -35,13 +35,13 @@ class Schedule:
     merge_wgs_per_cu: int = 0
     use_graphs: int = 0
     rec_cap: int = 0
-    stage_tile_cap: int = 0
+    unit_cap: int = 0
     merge_threads: int = 0
 
     def apply(self, ctx) -> None:
         ctx.set_tuning(item_occ=self.item_occ, unit_bytes=self.unit_bytes, table_pct=self.table_pct, lds_table_log=self.lds_table_log,
                        no_dedup=self.no_dedup, merge_wgs_per_cu=self.merge_wgs_per_cu, use_graphs=self.use_graphs,
-                       rec_cap=self.rec_cap, stage_tile_cap=self.stage_tile_cap, merge_threads=self.merge_threads)
+                       rec_cap=self.rec_cap, unit_cap=self.unit_cap, merge_threads=self.merge_threads)
 
 
 def default_grid(k: int) -> list[Schedule]:

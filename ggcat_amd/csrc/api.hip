@@ -45,7 +45,7 @@ struct ggcat_gpu_ctx {
     // bucketing buffers
     DevBuf input, seg_bits, unit_count, unit_cursor, unit_off, unit_kmers, scan_part, slots, pool, pool_cursor,
         slot_color, color_starts, colors, error_flag, stage_slots, stage_unit, stage_rank, stage_color, stage_cursor, defer,
-        defer_count, kmer_total;
+        defer_count, kmer_total, spill_slots, spill_color;  // stage_*: the unit regions, the spill list's units and ranks
     // merge buffers
     DevBuf items, n_items, work, parts, redo, redo_count, out_key, out_val, out_sub, chunk_cursor, chunk_fill_dev,
         fix_key, fix_val, fix_sub, fix_cursor, out_colors, fix_colors;
@@ -54,6 +54,7 @@ struct ggcat_gpu_ctx {
     bool have_records = false, have_solid = false;
     uint32_t n_sub = 0, n_units = 0, xb = 0, slot_words = 0, key_words = 1;
     uint64_t n_records = 0, pool_words = 0, n_solid = 0, n_input = 0, kmer_occ = 0, n_big = 0, n_fix = 0;
+    uint64_t n_spill = 0;  // records of the last bucketing past their unit's region
     uint32_t n_chunks_used = 0;  // output chunks of the last merge
     uint32_t n_passes = 0;  // kernel passes of the last merge
     std::vector<uint32_t> chunk_fill;  // host copy of their fill counts
@@ -430,7 +431,9 @@ void ggcat_gpu_destroy(ggcat_gpu_ctx* ctx) {
                       &ctx->q_seq, &ctx->q_off, &ctx->q_matched, &ctx->q_counts, &ctx->q_work,
                       &ctx->u_nb, &ctx->u_ext, &ctx->u_link, &ctx->u_hbytes, &ctx->u_hcount, &ctx->u_hoff,
                       &ctx->u_hidx, &ctx->u_seq, &ctx->u_sub, &ctx->u_flags, &ctx->u_visited, &ctx->u_rest,
-                      &ctx->u_rest_n, &ctx->u_rep, &ctx->u_tab, &ctx->u_path};
+                      &ctx->u_rest_n, &ctx->u_rep, &ctx->u_tab, &ctx->u_path, &ctx->stage_slots, &ctx->stage_unit,
+                      &ctx->stage_rank, &ctx->stage_color, &ctx->stage_cursor, &ctx->defer, &ctx->defer_count,
+                      &ctx->kmer_total, &ctx->spill_slots, &ctx->spill_color};
     if (ctx->server) (void)ggcat_gpu_serve_close(ctx);
     if (ctx->graph_a) (void)hipGraphExecDestroy(ctx->graph_a);
     for (DevBuf* b : bufs)
@@ -551,25 +554,26 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     ENSURE(ctx->defer_count, 8);
     ENSURE(ctx->kmer_total, 8);
     ENSURE(ctx->defer, std::max<uint64_t>(n_tiles, 1) * 24);
-    A.stage_cursor = P_<unsigned long long>(ctx->stage_cursor);
+    A.spill_cursor = P_<unsigned long long>(ctx->stage_cursor);
     A.defer = P_<uint64_t>(ctx->defer);
     A.defer_count = P_<uint32_t>(ctx->defer_count);
     A.kmer_total = P_<unsigned long long>(ctx->kmer_total);
     ENSURE(ctx->scan_part, (((size_t)n_units + SCAN_CHUNK - 1) / SCAN_CHUNK + 1) * 8);
 
     // Pass 1: clear the unit counts, k-mer counts, group-start bitmap and
-    // cursors; stage every record in tile order and count its unit; resolve
-    // the deferred far-start records.  With graphs enabled this sequence is
-    // captured once per argument set and replayed (fills are kernels, never
-    // memsets).  Staging is sized for the expected record count (about 2
-    // per w+1 windows) and grown to the counted size if that was short.
-    // tile regions hold 2.5 records per w+1 windows (about 2 on random
-    // sequence); the rest of a tile and deferred records go to the overflow
-    const uint32_t tile_cap = ctx->tune.stage_tile_cap ? std::min<uint32_t>(T, ctx->tune.stage_tile_cap)
-                                                       : std::min<uint32_t>(T, (T * 5 + 2 * (w + 1) - 1) / (2 * (w + 1)));
-    A.tile_cap = tile_cap;
-    const uint64_t tile_slots = n_tiles * tile_cap;
-    uint64_t ovf_cap = std::max<uint64_t>(tile_slots / 64 + 65536, ctx->stage_unit.cap / 4 > tile_slots ? ctx->stage_unit.cap / 4 - tile_slots : 0);
+    // cursors; build every record, count it in its unit and write it to its
+    // unit's region; resolve the deferred far-start records.  With graphs
+    // enabled this sequence is captured once per argument set and replayed
+    // (fills are kernels, never memsets).  A unit's region holds 2.5x the
+    // records expected per unit (2 per w+1 windows on random sequence); the
+    // rest spill to a list sized for 1/64 of the input bytes, grown to the
+    // counted size when that was short.
+    const uint64_t est_records = n_bytes * 2 / (w + 1) + 1;
+    const uint32_t unit_cap =
+        ctx->tune.unit_cap ? ctx->tune.unit_cap
+                           : (uint32_t)std::max<uint64_t>(16, std::min<uint64_t>(1u << 16, ((est_records * 5 / 2) / n_units + 15) / 8 * 8));
+    A.unit_cap = unit_cap;
+    uint64_t spill_cap = std::max<uint64_t>(n_bytes / 64 + 65536, ctx->stage_unit.cap / 4);
     uint64_t pool_cap = std::max<uint64_t>(ctx->pool.cap / 4, n_bytes / 64 + 65536);
     const uint32_t small_grid = (uint32_t)std::min<uint64_t>((std::max<uint64_t>(n_tiles, 1) + 255) / 256, 65535);
     auto pass_1 = [&]() {
@@ -590,17 +594,23 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     uint64_t staged = 0, pool_used = 0;
     uint32_t errf = 0;
     for (int attempt = 0;; attempt++) {
-        const uint64_t stage_cap = tile_slots + ovf_cap;
-        ENSURE(ctx->stage_slots, stage_cap * slot_words * 4);
-        ENSURE(ctx->stage_unit, stage_cap * 4);
-        ENSURE(ctx->stage_rank, stage_cap * 4);
-        if (n_colors) ENSURE(ctx->stage_color, stage_cap * 4);
+        const uint64_t region_slots = (uint64_t)n_units * unit_cap;
+        ENSURE(ctx->stage_slots, region_slots * slot_words * 4);
+        ENSURE(ctx->spill_slots, spill_cap * slot_words * 4);
+        ENSURE(ctx->stage_unit, spill_cap * 4);
+        ENSURE(ctx->stage_rank, spill_cap * 4);
+        if (n_colors) {
+            ENSURE(ctx->stage_color, region_slots * 4);
+            ENSURE(ctx->spill_color, spill_cap * 4);
+        }
         ENSURE(ctx->pool, pool_cap * 4);
-        A.stage_slots = P_<uint32_t>(ctx->stage_slots);
-        A.stage_unit = P_<uint32_t>(ctx->stage_unit);
-        A.stage_rank = P_<uint32_t>(ctx->stage_rank);
-        A.stage_color = n_colors ? P_<uint32_t>(ctx->stage_color) : nullptr;
-        A.stage_cap = stage_cap;
+        A.region = P_<uint32_t>(ctx->stage_slots);
+        A.region_color = n_colors ? P_<uint32_t>(ctx->stage_color) : nullptr;
+        A.spill_slots = P_<uint32_t>(ctx->spill_slots);
+        A.spill_unit = P_<uint32_t>(ctx->stage_unit);
+        A.spill_rank = P_<uint32_t>(ctx->stage_rank);
+        A.spill_color = n_colors ? P_<uint32_t>(ctx->spill_color) : nullptr;
+        A.spill_cap = spill_cap;
         A.pool = P_<uint32_t>(ctx->pool);
         A.pool_cap = pool_cap;
         HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
@@ -638,12 +648,13 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
         HIPCHK(hipStreamSynchronize(ctx->stream));
         if (errf & 8) return fail(ctx, GGCAT_GPU_ENOMEM, "a physical unit exceeds 2^32 k-mer occurrences");
         if (errf) return fail(ctx, GGCAT_GPU_EHIP, "bucketing: a group start was not found");
-        if (staged <= ovf_cap && pool_used <= pool_cap) break;
-        if (attempt == 1) return fail(ctx, GGCAT_GPU_ENOMEM, "bucketing staging overflow");
-        ovf_cap = std::max(ovf_cap, staged + 1024);
+        if (staged <= spill_cap && pool_used <= pool_cap) break;
+        if (attempt == 1) return fail(ctx, GGCAT_GPU_ENOMEM, "bucketing spill list overflow");
+        spill_cap = std::max(spill_cap, staged + 1024);
         pool_cap = std::max(pool_cap, pool_used + 1024);
     }
     ctx->pool_words = pool_used;
+    ctx->n_spill = staged;
 
     // Unit offsets, then pass 2: every staged record to its unit's slots.
     const int rs = device_scan(ctx, A.sub_count, P_<uint64_t>(ctx->unit_off), n_units);
@@ -660,10 +671,13 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     A.n_slots = total;
     A.slot_color = n_colors ? P_<uint32_t>(ctx->slot_color) : nullptr;
     HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
-    const uint64_t n_staged = tile_slots + staged;
-    if (n_staged) {
-        const uint32_t grid = (uint32_t)std::min<uint64_t>((n_staged + 255) / 256, (uint64_t)ctx->n_cu * 64);
-        hipLaunchKernelGGL(bucket_scatter_kernel, dim3(grid), dim3(256), 0, ctx->stream, A, n_staged);
+    if (total) {
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(((uint64_t)n_units + 3) / 4, (uint64_t)ctx->n_cu * 32);
+        hipLaunchKernelGGL(unit_compact_kernel, dim3(grid), dim3(256), 0, ctx->stream, A, n_units);
+    }
+    if (staged) {
+        const uint32_t grid = (uint32_t)std::min<uint64_t>((staged + 255) / 256, (uint64_t)ctx->n_cu * 64);
+        hipLaunchKernelGGL(spill_place_kernel, dim3(grid), dim3(256), 0, ctx->stream, A, staged);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
@@ -947,13 +961,14 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     M.lds_try = ctx->tune.lds_max_occ ? ctx->tune.lds_max_occ : M.lds_cap * 16;
     const uint32_t item_occ = ctx->tune.item_occ ? ctx->tune.item_occ : M.lds_cap / 2;
     // identical records counted once (seq-hash keys up to k = 32, inline slots)
-    M.dedup = (p.hash_kind == GGCAT_GPU_HASH_SEQ && p.k <= 32 && ctx->slot_words <= 8 && !ctx->tune.no_dedup) ? 1 : 0;
+    M.dedup = (ctx->slot_words <= 8 && !ctx->tune.no_dedup) ? 1 : 0;
+    M.tag_kmers = (p.hash_kind == GGCAT_GPU_HASH_SEQ && p.k <= 32) ? 1 : 0;
     // workgroup size of the merge (128 or 256 threads); a chunk of records
     // (rec_cap / 2) has at most one record per thread
     const uint32_t merge_threads = ctx->tune.merge_threads == 128 ? 128u : 256u;
     M.rec_cap = std::min<uint32_t>(ctx->tune.rec_cap ? ctx->tune.rec_cap : REC_CAP_DEFAULT, 2 * merge_threads);
     M.rec_lds_off = (uint32_t)(((size_t)M.lds_cap * entry_bytes + 64 + 15) / 16 * 16);
-    const size_t lds_bytes = M.dedup ? M.rec_lds_off + rec_table_bytes(ctx->slot_words, p.k, M.rec_cap) : (size_t)M.lds_cap * entry_bytes + 64;
+    const size_t lds_bytes = M.dedup ? M.rec_lds_off + rec_table_bytes(ctx->slot_words, M.tag_kmers ? p.k : 0xFFFFu, M.rec_cap) : (size_t)M.lds_cap * entry_bytes + 64;
     M.pool_words = ctx->pool_words;
     set_rk_constants(M, p.k);
 

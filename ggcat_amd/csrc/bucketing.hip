@@ -19,11 +19,14 @@
 // minimizer may be longer; their start is found in the global group-start
 // bitmap once pass 1 has written all of it.
 //
-//   pass 1 (bucket_stage_kernel)  : group-start bitmap; every record built and
-//                                   staged in tile order, counted per unit
+//   pass 1 (bucket_stage_kernel)  : group-start bitmap; every record built,
+//                                   counted in its unit (the count is its rank)
+//                                   and written to its unit's region at that
+//                                   rank, or to the spill list past the region
 //          (bucket_defer_kernel)  : records whose group starts before the halo
-//   scan   (scan_*_kernel)        : unit offsets
-//   pass 2 (bucket_scatter_kernel): each staged record to its unit's next slot
+//   scan   (scan_*_kernel)        : unit offsets of the dense arena
+//   pass 2 (unit_compact_kernel)  : each unit's region copied to its offsets
+//          (spill_place_kernel)   : spilled records to unit offset + rank
 //
 // A group = maximal run of consecutive windows sharing the minimizer (value,
 // and position when unique).  Its super-k-mer is the set of k-mers touching
@@ -49,16 +52,18 @@ struct BucketArgs {
     uint32_t* sub_count;  // [n_units] records per unit
     unsigned long long* unit_ck;  // [n_units] records << 32 | k-mer occurrences, counted in pass 1
     uint64_t n_slots;  // slots in the arena
-    // Staging (records in tile order before the scatter by unit): tile i owns positions [i * tile_cap, (i + 1) * tile_cap);
-    // records beyond that and deferred records go to the overflow region
-    // that follows, [n_tiles * tile_cap, stage_cap).
-    uint32_t* stage_slots;  // [stage_cap * slot_words]
-    uint32_t* stage_unit;  // [stage_cap] unit, or NO_UNIT for a hole
-    uint32_t* stage_rank;  // [stage_cap] rank of the record inside its unit
-    uint32_t* stage_color;  // [stage_cap] (colored builds)
-    unsigned long long* stage_cursor;  // positions used in the overflow region
-    uint64_t stage_cap;
-    uint32_t tile_cap;
+    // Pass 1 writes record `rank` of unit u to its region slot
+    // u * unit_cap + rank; records of rank >= unit_cap are appended to the
+    // spill list with their unit and rank.
+    uint32_t unit_cap;
+    uint32_t* region;  // [n_units * unit_cap * slot_words]
+    uint32_t* region_color;  // [n_units * unit_cap] (colored builds)
+    uint32_t* spill_slots;  // [spill_cap * slot_words]
+    uint32_t* spill_unit;  // [spill_cap]
+    uint32_t* spill_rank;  // [spill_cap]
+    uint32_t* spill_color;  // [spill_cap] (colored builds)
+    unsigned long long* spill_cursor;  // spilled records (may exceed spill_cap: the host retries)
+    uint64_t spill_cap;
     uint64_t* defer;  // [n_tiles * 3] records whose group starts before the tile: end window, minimizer, last flag
     unsigned int* defer_count;
     unsigned long long* kmer_total;  // k-mer occurrences of all records
@@ -707,8 +712,60 @@ GG_DEV bool write_record(const Tile& t, const RecInfo& r, uint32_t* out) {
     return true;
 }
 
-// Records staged per lane and pass of the record loop; their unit atomics
-// are all in flight before the records are written.
+// Slot words (up to 8) of an inline record whose bases are in the tile's
+// LDS, built in registers; false when the record takes the generic path
+// (write_record: long records, bases before the tile, slots above 8 words).
+GG_DEV bool record_words(const Tile& t, const RecInfo& r, uint4* w4) {
+    const BucketArgs& A = *t.A;
+    const uint32_t SW = A.slot_words;
+    const uint32_t len = (uint32_t)(r.end - r.start);
+    const int64_t rs = r.start - t.borg;
+    if (SW > 8 || len > (SW - 1) * 16 || rs < 0) return false;
+    const uint32_t nw = (len + 15) / 16;
+#pragma unroll
+    for (uint32_t g = 0; g < 2; g++) {
+        if (4 * g >= SW) break;
+        uint32_t wv[4];
+#pragma unroll
+        for (uint32_t j = 0; j < 4; j++) {
+            const uint32_t q = 4 * g + j;
+            uint32_t wq = 0;
+            if (q == 0) {
+                wq = len | r.meta;
+            } else if (q - 1 < nw) {
+                const uint32_t qi = q - 1;
+                wq = !r.rc ? extract16(t, rs + 16 * qi) : rev_comp16(extract16(t, (r.end - t.borg) - 16 * (int64_t)qi - 16));
+                const uint32_t rem = len - 16 * qi;
+                if (rem < 16) wq &= (1u << (2 * rem)) - 1u;
+            }
+            wv[j] = wq;
+        }
+        w4[g] = make_uint4(wv[0], wv[1], wv[2], wv[3]);
+    }
+    return true;
+}
+
+// Where record `rank` of `unit` goes: its region slot, or the next spill
+// entry (nullptr when the spill list is full: the host retries with the
+// counted size).  *color_at receives the matching color entry.
+GG_DEV uint32_t* record_dest(const BucketArgs& A, uint32_t unit, uint32_t rank, uint32_t** color_at) {
+    const uint32_t SW = A.slot_words;
+    if (rank < A.unit_cap) {
+        const uint64_t slot = (uint64_t)unit * A.unit_cap + rank;
+        *color_at = A.region_color ? A.region_color + slot : nullptr;
+        return A.region + slot * SW;
+    }
+    const unsigned long long pos = atomicAdd(A.spill_cursor, 1ULL);
+    *color_at = nullptr;
+    if (pos >= A.spill_cap) return nullptr;
+    A.spill_unit[pos] = unit;
+    A.spill_rank[pos] = rank;
+    if (A.spill_color) *color_at = A.spill_color + pos;
+    return A.spill_slots + pos * SW;
+}
+
+// Records handled per lane and pass of the record loop; their unit atomics
+// are all in flight while their slot words are built.
 constexpr uint32_t STAGE_R = 2;
 
 // Pass 1 (one wavefront per tile): window minima and group ends; every record
@@ -727,7 +784,6 @@ __global__ __launch_bounds__(64) void bucket_stage_kernel(BucketArgs A) {
     const uint32_t SW = A.slot_words;
     __shared__ uint16_t ends[1024];
     __shared__ uint32_t n_ends;
-    __shared__ unsigned long long s_ovf;
     if (WC == 16) {
         uint32_t x0, segm;
         int64_t j0;
@@ -769,31 +825,15 @@ __global__ __launch_bounds__(64) void bucket_stage_kernel(BucketArgs A) {
         }
     }
     const uint32_t n_end = n_ends;
-    const uint32_t cap = A.tile_cap;
-    const uint64_t tbase = (uint64_t)blockIdx.x * cap;
-    if (lane == 0) {
-        s_ovf = 0;
-        if (n_end > cap) s_ovf = (uint64_t)A.n_tiles * cap + atomicAdd(A.stage_cursor, (unsigned long long)(n_end - cap));
-    }
-    __syncthreads();
-    const uint64_t ovf = s_ovf;
-    for (uint32_t e = n_end + lane; e < cap; e += 64) A.stage_unit[tbase + e] = NO_UNIT;
     for (uint32_t e0 = 0; e0 < n_end; e0 += 64 * STAGE_R) {
         RecInfo ri[STAGE_R];
-        uint64_t pos[STAGE_R];
         unsigned long long old[STAGE_R];
 #pragma unroll
         for (uint32_t r = 0; r < STAGE_R; r++) {
             const uint32_t ei = e0 + 64 * r + lane;
             ri[r].ok = false;
-            pos[r] = ~0ULL;
             old[r] = 0;
             if (ei >= n_end) continue;
-            pos[r] = ei < cap ? tbase + ei : ovf + (ei - cap);
-            if (pos[r] >= A.stage_cap) {  // overflow region full: the host retries with the counted size
-                pos[r] = ~0ULL;
-                continue;
-            }
             const uint32_t x = ends[ei];
             ri[r] = rec_info(t, x);
             if (ri[r].ok) {
@@ -810,20 +850,25 @@ __global__ __launch_bounds__(64) void bucket_stage_kernel(BucketArgs A) {
                 }
             }
         }
+        uint4 wv[STAGE_R][2];
+        bool in_regs[STAGE_R];
+#pragma unroll
+        for (uint32_t r = 0; r < STAGE_R; r++) in_regs[r] = ri[r].ok && record_words(t, ri[r], wv[r]);
 #pragma unroll
         for (uint32_t r = 0; r < STAGE_R; r++) {
-            if (pos[r] == ~0ULL) continue;
-            if (!ri[r].ok) {
-                A.stage_unit[pos[r]] = NO_UNIT;
-                continue;
+            if (!ri[r].ok) continue;
+            uint32_t* color_at;
+            uint32_t* out = record_dest(A, ri[r].unit, rank_of(A, old[r], ri[r].nk), &color_at);
+            if (!out) continue;  // spill list full: retried
+            if (in_regs[r]) {
+                uint4* o4 = reinterpret_cast<uint4*>(out);
+                o4[0] = wv[r][0];
+                if (SW == 8) o4[1] = wv[r][1];
+            } else if (!write_record(t, ri[r], out)) {
+                continue;  // pool full: retried
             }
-            if (!write_record(t, ri[r], A.stage_slots + pos[r] * SW)) continue;  // pool full: retried
-            A.stage_unit[pos[r]] = ri[r].unit;
-            if (A.n_colors) A.stage_color[pos[r]] = color_of(A, ri[r].start);
+            if (color_at) *color_at = color_of(A, ri[r].start);
         }
-#pragma unroll
-        for (uint32_t r = 0; r < STAGE_R; r++)
-            if (pos[r] != ~0ULL && ri[r].ok) A.stage_rank[pos[r]] = rank_of(A, old[r], ri[r].nk);
     }
 }
 
@@ -855,11 +900,11 @@ __global__ void bucket_defer_kernel(BucketArgs A) {
     // a non-unique minimizer: forward orientation, no minimizer position
     const uint32_t flags = (include_first ? 1u : 0u) | ((include_last ? 1u : 0u) << 1);
     const uint32_t unit = unit_of(v, A.b, A.sb, A.xb);
-    const uint64_t pos = (uint64_t)A.n_tiles * A.tile_cap + atomicAdd(A.stage_cursor, 1ULL);
-    if (pos >= A.stage_cap) return;  // the host retries with the counted size
     unsigned long long old;
     count_record(A, unit, len - k + 1, &old);
-    uint32_t* out = A.stage_slots + pos * SW;
+    uint32_t* color_at;
+    uint32_t* out = record_dest(A, unit, rank_of(A, old, len - k + 1), &color_at);
+    if (!out) return;  // spill list full: the host retries with the counted size
     Tile none;  // no LDS codes: every base comes from the input
     none.A = &A;
     none.borg = 0;
@@ -872,9 +917,7 @@ __global__ void bucket_defer_kernel(BucketArgs A) {
         for (uint32_t q = 1; q < SW; q++) out[q] = 0;
         pack_bases(none, start, end, false, out + 1);
     }
-    A.stage_unit[pos] = ok ? unit : NO_UNIT;
-    if (A.n_colors) A.stage_color[pos] = color_of(A, start);
-    A.stage_rank[pos] = rank_of(A, old, len - k + 1);
+    if (ok && color_at) *color_at = color_of(A, start);
 }
 
 // Splits the packed per-unit counts into records and k-mer occurrences.
@@ -916,7 +959,7 @@ __global__ void sub_totals_kernel(const uint32_t* unit_count, const uint32_t* un
 // Clears the cursors of pass 1 (a kernel, so the pass can be captured in a graph).
 __global__ void bucket_reset_kernel(BucketArgs A) {
     if (threadIdx.x == 0) {
-        *A.stage_cursor = 0;
+        *A.spill_cursor = 0;
         *A.pool_cursor = 0;
         *A.kmer_total = 0;
         *A.defer_count = 0;
@@ -924,22 +967,41 @@ __global__ void bucket_reset_kernel(BucketArgs A) {
     }
 }
 
-// Pass 2: every staged record to slot unit offset + rank (no atomics).
-__global__ void bucket_scatter_kernel(BucketArgs A, uint64_t n_staged) {
+// Pass 2a: each unit's region (its first min(count, unit_cap) records) to
+// its place in the dense unit-major arena; one wavefront per unit, 16-byte
+// copies.
+__global__ __launch_bounds__(256) void unit_compact_kernel(BucketArgs A, uint32_t n_units) {
     const uint32_t SW = A.slot_words;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n_staged;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t u = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); u < n_units; u += waves) {
+        const uint64_t o0 = A.sub_off[u], o1 = A.sub_off[u + 1];
+        const uint64_t cnt = o1 - o0;
+        const uint32_t n = cnt < A.unit_cap ? (uint32_t)cnt : A.unit_cap;
+        const uint4* src = reinterpret_cast<const uint4*>(A.region + (uint64_t)u * A.unit_cap * SW);
+        uint4* dst = reinterpret_cast<uint4*>(A.slots + o0 * SW);
+        const uint32_t n4 = n * (SW / 4);
+        for (uint32_t i = lane; i < n4; i += 64) dst[i] = src[i];
+        if (A.region_color)
+            for (uint32_t i = lane; i < n; i += 64) A.slot_color[o0 + i] = A.region_color[(uint64_t)u * A.unit_cap + i];
+    }
+}
+
+// Pass 2b: spilled records (rank >= unit_cap) to unit offset + rank.
+__global__ void spill_place_kernel(BucketArgs A, uint64_t n_spill) {
+    const uint32_t SW = A.slot_words;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n_spill;
          i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t unit = A.stage_unit[i];
-        if (unit == NO_UNIT) continue;
-        const uint64_t slot = A.sub_off[unit] + A.stage_rank[i];
+        const uint32_t unit = A.spill_unit[i];
+        const uint64_t slot = A.sub_off[unit] + A.spill_rank[i];
         if (slot >= A.sub_off[unit + 1]) {
             atomicOr(A.error_flag, 1u);
             continue;
         }
-        const uint4* src = reinterpret_cast<const uint4*>(A.stage_slots + i * SW);
+        const uint4* src = reinterpret_cast<const uint4*>(A.spill_slots + i * SW);
         uint4* dst = reinterpret_cast<uint4*>(A.slots + slot * SW);
         for (uint32_t g = 0; g < SW / 4; g++) dst[g] = src[g];
-        if (A.n_colors) A.slot_color[slot] = A.stage_color[i];
+        if (A.spill_color) A.slot_color[slot] = A.spill_color[i];
     }
 }
 

@@ -58,7 +58,9 @@ struct MergeArgs {
     uint32_t lds_cap;  // table entries in LDS (power of two)
     uint32_t lds_try;  // first pass: items with more occurrences go straight to partitioned passes
     uint32_t table_pct;  // table slots per 100 occurrences of a whole-item pass (power-of-two rounded)
-    uint32_t dedup;  // identical records of an entry are inserted once with their count (seq-hash, k <= 32)
+    uint32_t dedup;  // identical records of an entry are inserted once with their count
+    uint32_t tag_kmers;  // dedup: every k-mer position of the distinct records is a thread's work item
+                         // (seq-hash, k <= 32); otherwise each distinct record's k-mers are rolled by one thread
     uint32_t rec_lds_off;  // byte offset of the record table in dynamic LDS
     uint32_t rec_cap;  // record table slots
     uint32_t claim;  // work entries claimed per atomic on the work cursor
@@ -412,7 +414,8 @@ GG_DEV RecTable rec_table(const MergeArgs& A, uint8_t* lds) {
     return R;
 }
 
-// Bytes of dynamic LDS the record table needs.
+// Bytes of dynamic LDS the record table needs (k = 0xFFFF: no per-k-mer tags,
+// the rolling mode of 128-bit keys).
 __host__ __device__ inline uint32_t rec_table_bytes(uint32_t slot_words, uint32_t k, uint32_t rec_cap) {
     return (rec_cap * slot_words + 3 * rec_cap + 6) * 4 + (rec_cap / 2) * rec_max_kmers(slot_words, k) * 2 + 4;
 }
@@ -449,11 +452,16 @@ GG_DEV void insert_item_dedup(const MergeArgs& A, uint64_t r0, uint64_t r1, cons
                               uint32_t pre_color) {
     const uint32_t W = A.color_words, SW = A.slot_words, k = A.k, RC = A.rec_cap;
     const uint32_t part_mask = (1u << w.log_parts) - 1;
-    auto insert_key = [&](uint64_t key, uint32_t bits, uint32_t color) {
+    // one occurrence (or cnt identical ones) of a k-mer of either key width
+    auto insert_key = [&](uint64_t lo, uint64_t hi, uint32_t bits, uint32_t color) {
         if (*(volatile uint32_t*)overflow) return;
-        if (part_mask && ((tab_hash64(key) >> 16) & part_mask) != w.part) return;
+        if (part_mask) {
+            const uint32_t h = (A.key_words == 1) ? tab_hash64(lo) : hash128(lo, hi);
+            if (((h >> 16) & part_mask) != w.part) return;
+        }
         uint32_t at = 0;
-        const uint32_t res = tab_insert64(klo, vals, mask, key, bits, probe_limit, at);
+        const uint32_t res = (A.key_words == 1) ? tab_insert64(klo, vals, mask, lo, bits, probe_limit, at)
+                                                : tab_insert128(klo, khi, st, vals, mask, lo, hi, bits, probe_limit, at);
         if (res == 2) {
             atomicOr(overflow, 1u);
             return;
@@ -509,13 +517,30 @@ GG_DEV void insert_item_dedup(const MergeArgs& A, uint64_t r0, uint64_t r1, cons
             }
         }
         if (direct) {
-            for_each_kmer(A, slot, [&](uint64_t lo, uint64_t, bool is_fwd, uint32_t idx, uint32_t last) {
-                insert_key(lo, occurrence_bits(meta, is_fwd, idx, last), color);
+            for_each_kmer(A, slot, [&](uint64_t lo, uint64_t hi, bool is_fwd, uint32_t idx, uint32_t last) {
+                insert_key(lo, hi, occurrence_bits(meta, is_fwd, idx, last), color);
             });
         }
     }
 phase_a_done:
     __syncthreads();
+    if (!A.tag_kmers) {
+        // 128-bit keys: each thread rolls the k-mers of the distinct records
+        // in its table slots (rec_cap / blockDim of them) and frees the slots
+        const uint32_t per = RC > blockDim.x ? RC / blockDim.x : 1u;
+        const uint32_t s0 = threadIdx.x * per;
+        for (uint32_t q = 0; q < per; q++) {
+            const uint32_t si = s0 + q;
+            if (si >= RC || R.state[si] != 2) continue;
+            const uint32_t* rw = R.key + si * SW;
+            const uint32_t cnt = R.cnt[si], color = W ? R.col[si] : 0u, meta = rw[0];
+            for_each_kmer(A, rw, [&](uint64_t lo, uint64_t hi, bool is_fwd, uint32_t idx, uint32_t last) {
+                insert_key(lo, hi, occurrence_bits(meta, is_fwd, idx, last) - 1u + cnt, color);
+            });
+            R.state[si] = 0;
+        }
+        return;
+    }
     // phase B: one exclusive scan over the table slots of their k-mer counts
     // places the distinct records' k-mers (each position tagged with its
     // record's slot and k-mer index); each thread owns rec_cap / blockDim

@@ -68,7 +68,7 @@ class Tuning(C.Structure):
         ("no_dedup", C.c_uint32),
         ("unit_bytes", C.c_uint32),
         ("use_graphs", C.c_uint32),
-        ("stage_tile_cap", C.c_uint32),
+        ("unit_cap", C.c_uint32),
         ("rec_cap", C.c_uint32),
         ("merge_threads", C.c_uint32),
     ]
@@ -354,10 +354,10 @@ class GpuContext:
 
     def set_tuning(self, item_occ: int = 0, lds_max_occ: int = 0, merge_wgs_per_cu: int = 0,
                    part_occ: int = 0, lds_table_log: int = 0, table_pct: int = 0, no_dedup: int = 0,
-                   unit_bytes: int = 0, use_graphs: int = 0, stage_tile_cap: int = 0, rec_cap: int = 0,
+                   unit_bytes: int = 0, use_graphs: int = 0, unit_cap: int = 0, rec_cap: int = 0,
                    merge_threads: int = 0) -> None:
         t = Tuning(item_occ, lds_max_occ, merge_wgs_per_cu, part_occ, lds_table_log, table_pct, no_dedup, unit_bytes,
-                   use_graphs, stage_tile_cap, rec_cap, merge_threads)
+                   use_graphs, unit_cap, rec_cap, merge_threads)
         self._check(self.lib.ggcat_gpu_set_tuning(self.h, C.byref(t)), "ggcat_gpu_set_tuning")
 
     def records(self) -> Records:

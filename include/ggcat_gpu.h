@@ -187,9 +187,10 @@ int ggcat_gpu_solid_colors(ggcat_gpu_ctx* ctx, uint32_t* color_words, uint32_t* 
  *   use_graphs        1: capture the bucketing staging pass (clears, stage
  *                     and deferred-record kernels) in a HIP graph per
  *                     argument set and replay it on later calls
- *   stage_tile_cap    bucketing: staging positions per tile before records
- *                     spill to the shared overflow region (default 2.5
- *                     records per w+1 windows); the result never depends on it
+ *   unit_cap          bucketing: slots of each unit's region, where records
+ *                     are written as they are counted (default 2.5x the
+ *                     records expected per unit); records past it go to a
+ *                     spill list; the result never depends on it
  *   rec_cap           merge: identical-record table slots (power of two in
  *                     [16, 512], default 512); records are deduplicated in
  *                     chunks of rec_cap / 2 (at most one record per thread)
@@ -204,7 +205,7 @@ typedef struct {
     uint32_t no_dedup;
     uint32_t unit_bytes;
     uint32_t use_graphs;
-    uint32_t stage_tile_cap;
+    uint32_t unit_cap;
     uint32_t rec_cap;
     uint32_t merge_threads;
 } ggcat_gpu_tuning;

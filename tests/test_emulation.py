@@ -79,12 +79,12 @@ def test_colored_merge_emulated(emu_ctx):
         emu_ctx.set_tuning()
 
 
-@pytest.mark.parametrize("tile_cap", [1, 7])
-def test_staging_overflow_emulated(emu_ctx, tile_cap):
-    # nearly every tile spills records to the shared overflow region
+@pytest.mark.parametrize("unit_cap", [1, 7])
+def test_unit_region_spill_emulated(emu_ctx, unit_cap):
+    # nearly every unit spills records past its region
     rng = np.random.default_rng(57)
     data = np.concatenate([joined(adversarial_records(rng)[:12]), synth.reads(synth.genome(3, 4000), 8, 60, 120, 0.01)])
-    emu_ctx.set_tuning(merge_wgs_per_cu=2, stage_tile_cap=tile_cap)
+    emu_ctx.set_tuning(merge_wgs_per_cu=2, unit_cap=unit_cap)
     try:
         _check(emu_ctx, data, 31, 15, 4, 2, 1)
         seps = np.flatnonzero(data == 10) + 1  # color ranges start at record boundaries

@@ -207,11 +207,13 @@ def test_schedule_knobs_do_not_change_results(gpu_ctx, tune):
         gpu_ctx.set_tuning()
 
 
-@pytest.mark.parametrize("tile_cap", [1, 24])
-def test_staging_overflow_region(gpu_ctx, tile_cap):
+@pytest.mark.parametrize("unit_cap", [1, 24])
+def test_unit_region_spill(gpu_ctx, unit_cap):
+    # tiny unit regions: most records go through the spill list (with
+    # unit_cap=1 the list's first size is too small and the pass is redone)
     wl = synth.config1(0.01)
-    _say(f"case stage_tile_cap={tile_cap} bytes={wl.data.size}")
-    gpu_ctx.set_tuning(stage_tile_cap=tile_cap)
+    _say(f"case unit_cap={unit_cap} bytes={wl.data.size}")
+    gpu_ctx.set_tuning(unit_cap=unit_cap)
     try:
         _run_both(gpu_ctx, wl.data, 31, 15, wl.buckets_log, wl.second_buckets_log, exact=False)
     finally:
