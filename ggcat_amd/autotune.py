@@ -64,6 +64,10 @@ def load_grid(best: Schedule, k: int) -> list[Schedule]:
     the work-item size, which together set how full the LDS tables run."""
     lds = 1 << (best.lds_table_log or (11 if k <= 32 else 10))
     out = []
+    if k <= 32:  # identical records counted once, each distinct record rolled by one thread
+        s = Schedule(**asdict(best))
+        s.no_dedup = 2
+        out.append(s)
     for pct, occ in itertools.product((100, 200, 400), (lds // 2, lds, 2 * lds)):
         if (pct, occ) == (best.table_pct or 200, best.item_occ or lds // 2):
             continue

@@ -961,8 +961,8 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     M.lds_try = ctx->tune.lds_max_occ ? ctx->tune.lds_max_occ : M.lds_cap * 16;
     const uint32_t item_occ = ctx->tune.item_occ ? ctx->tune.item_occ : M.lds_cap / 2;
     // identical records counted once (seq-hash keys up to k = 32, inline slots)
-    M.dedup = (ctx->slot_words <= 8 && !ctx->tune.no_dedup) ? 1 : 0;
-    M.tag_kmers = (p.hash_kind == GGCAT_GPU_HASH_SEQ && p.k <= 32) ? 1 : 0;
+    M.dedup = (ctx->slot_words <= 8 && ctx->tune.no_dedup != 1) ? 1 : 0;
+    M.tag_kmers = (p.hash_kind == GGCAT_GPU_HASH_SEQ && p.k <= 32 && ctx->tune.no_dedup != 2) ? 1 : 0;
     // workgroup size of the merge (128 or 256 threads); a chunk of records
     // (rec_cap / 2) has at most one record per thread
     const uint32_t merge_threads = ctx->tune.merge_threads == 128 ? 128u : 256u;

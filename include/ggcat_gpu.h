@@ -181,7 +181,10 @@ int ggcat_gpu_solid_colors(ggcat_gpu_ctx* ctx, uint32_t* color_words, uint32_t* 
  *                     every k-mer is distinct; lower values trade overflow
  *                     redos for smaller tables)
  *   no_dedup          1: insert every record's k-mers even when identical
- *                     records could be counted once (seq-hash, k <= 32)
+ *                     records could be counted once; 2: count identical
+ *                     records once but let one thread roll each distinct
+ *                     record's k-mers (the mode of 128-bit keys) instead of
+ *                     spreading single k-mer positions over the threads
  *   unit_bytes        bucketing: input bytes per physical unit (sub-bucket
  *                     refinement); default 2000, 1000 for colored builds
  *   use_graphs        1: capture the bucketing staging pass (clears, stage

@@ -220,6 +220,19 @@ def test_unit_region_spill(gpu_ctx, unit_cap):
         gpu_ctx.set_tuning()
 
 
+@pytest.mark.parametrize("mode", [1, 2])
+def test_dedup_modes_keep_results(gpu_ctx, mode):
+    # every record's k-mers inserted (1), or identical records counted once
+    # and each distinct record rolled by one thread (2)
+    wl = synth.config1(0.01)
+    _say(f"case no_dedup={mode} bytes={wl.data.size}")
+    gpu_ctx.set_tuning(no_dedup=mode)
+    try:
+        _run_both(gpu_ctx, wl.data, 31, 15, wl.buckets_log, wl.second_buckets_log, exact=False)
+    finally:
+        gpu_ctx.set_tuning()
+
+
 def test_build_from_files_fastq(gpu_ctx, tmp_path):
     import gzip
 
