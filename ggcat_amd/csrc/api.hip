@@ -5,6 +5,7 @@
 #include "merge.hip"
 #include "query.hip"
 #include "unitigs.hip"
+#include "parse.hip"
 #include "fasta.cpp"
 #include "colormap.cpp"
 
@@ -14,7 +15,12 @@
 #include <deque>
 #include <map>
 #include <memory>
+#include <chrono>
 #include <mutex>
+#include <thread>
+#include <fcntl.h>
+#include <sys/stat.h>
+#include <unistd.h>
 #include <cstring>
 #include <string>
 #include <tuple>
@@ -61,7 +67,8 @@ struct ggcat_gpu_ctx {
     uint64_t n_tail = 0;  // entries appended after the chunks (combined fixups)
     bool colored = false;
     uint32_t color_words = 0;  // bitmap words per solid entry of a colored build
-    double ms[8] = {0};
+    double ms[9] = {0};
+    uint64_t input_path = 0, flush_cuts = 0;  // last build_from_files: 1 parsed in HBM, 2 host reader
     ggcat_gpu_tuning tune{};  // zero fields select the defaults
     hipGraphExec_t graph_a = nullptr;  // captured bucketing staging pass (graphs enabled)
     struct GraphKeyA {
@@ -85,6 +92,10 @@ struct ggcat_gpu_ctx {
     std::vector<uint8_t> cyc_seq, cyc_flags;
     std::vector<uint32_t> cyc_sub;
     DevBuf u_path;  // colored builds: table position of every unitig k-mer, in written order
+    // input files read straight into HBM: raw bytes, parse state per chunk,
+    // and a pinned two-buffer staging ring for the uploads
+    DevBuf raw, p_stats, p_carry, p_keep, p_keep_off, p_flags, p_bits;
+    void* pin_ring = nullptr;
     std::vector<uint32_t> cyc_pos;  // the same for the unitigs of cycles (walked on the host)
     // color subsets of the solid table: ids for the color map and the colored
     // FASTA headers (assigned on first use after a merge)
@@ -433,7 +444,9 @@ void ggcat_gpu_destroy(ggcat_gpu_ctx* ctx) {
                       &ctx->u_hidx, &ctx->u_seq, &ctx->u_sub, &ctx->u_flags, &ctx->u_visited, &ctx->u_rest,
                       &ctx->u_rest_n, &ctx->u_rep, &ctx->u_tab, &ctx->u_path, &ctx->stage_slots, &ctx->stage_unit,
                       &ctx->stage_rank, &ctx->stage_color, &ctx->stage_cursor, &ctx->defer, &ctx->defer_count,
-                      &ctx->kmer_total, &ctx->spill_slots, &ctx->spill_color};
+                      &ctx->kmer_total, &ctx->spill_slots, &ctx->spill_color, &ctx->raw, &ctx->p_stats,
+                      &ctx->p_carry, &ctx->p_keep, &ctx->p_keep_off, &ctx->p_flags, &ctx->p_bits};
+    if (ctx->pin_ring) (void)hipHostFree(ctx->pin_ring);
     if (ctx->server) (void)ggcat_gpu_serve_close(ctx);
     if (ctx->graph_a) (void)hipGraphExecDestroy(ctx->graph_a);
     for (DevBuf* b : bufs)
@@ -1254,21 +1267,22 @@ int ggcat_gpu_set_tuning(ggcat_gpu_ctx* ctx, const ggcat_gpu_tuning* t) {
         return fail(ctx, GGCAT_GPU_EINVAL, "merge_threads must be 0, 128 or 256");
     if (t->rec_cap && (t->rec_cap < 16 || t->rec_cap > 512 || (t->rec_cap & (t->rec_cap - 1))))
         return fail(ctx, GGCAT_GPU_EINVAL, "rec_cap must be 0 or a power of two in [16, 512]");
+    if (t->read_block && t->read_block < 256) return fail(ctx, GGCAT_GPU_EINVAL, "read_block must be 0 or >= 256");
     ctx->tune = *t;
     return GGCAT_GPU_OK;
 }
 
 int ggcat_gpu_timings(ggcat_gpu_ctx* ctx, double* ms, uint32_t n) {
     if (!ctx || !ms) return GGCAT_GPU_EINVAL;
-    for (uint32_t i = 0; i < n && i < 8; i++) ms[i] = ctx->ms[i];
+    for (uint32_t i = 0; i < n && i < 9; i++) ms[i] = ctx->ms[i];
     return GGCAT_GPU_OK;
 }
 
 int ggcat_gpu_counters(ggcat_gpu_ctx* ctx, uint64_t* out, uint32_t n) {
     if (!ctx || !out) return GGCAT_GPU_EINVAL;
-    uint64_t v[8] = {ctx->n_input, ctx->n_records, ctx->kmer_occ + ctx->n_records * (ctx->P.k - 1), ctx->kmer_occ,
-                     ctx->n_big, ctx->n_solid, ctx->n_fix, ctx->n_passes};
-    for (uint32_t i = 0; i < n && i < 8; i++) out[i] = v[i];
+    uint64_t v[10] = {ctx->n_input, ctx->n_records, ctx->kmer_occ + ctx->n_records * (ctx->P.k - 1), ctx->kmer_occ,
+                      ctx->n_big,   ctx->n_solid,   ctx->n_fix,   ctx->n_passes, ctx->input_path, ctx->flush_cuts};
+    for (uint32_t i = 0; i < n && i < 10; i++) out[i] = v[i];
     return GGCAT_GPU_OK;
 }
 
@@ -1345,7 +1359,8 @@ int ggcat_gpu_read_files(ggcat_gpu_ctx* ctx, const char* const* paths, uint32_t 
         ps.emplace_back(paths[i]);
     }
     gg_host::ReadResult R;
-    if (!gg_host::read_files(ps, k, colors != 0, R)) return fail(ctx, GGCAT_GPU_EINVAL, R.error);
+    if (!gg_host::read_files(ps, k, colors != 0, R, ctx ? ctx->tune.read_block : 0))
+        return fail(ctx, GGCAT_GPU_EINVAL, R.error);
     *joined = (uint8_t*)malloc(R.joined.size() + 16);
     if (!*joined) return fail(ctx, GGCAT_GPU_ENOMEM, "host allocation failed");
     memcpy(*joined, R.joined.data(), R.joined.size());
@@ -1381,6 +1396,228 @@ int ggcat_gpu_plan_build(const ggcat_gpu_build_options* o, uint64_t estimated_ba
     return GGCAT_GPU_OK;
 }
 
+namespace {
+
+constexpr uint64_t PIN_CHUNK = 64ull << 20;  // bytes per staging buffer (two of them)
+
+// Plain FASTA / FASTQ files read straight into HBM (parallel reads into a
+// pinned two-buffer ring, each upload overlapping the next read) and parsed
+// there (parse.hip) into the joined layout at ctx->input; every file's parse
+// is queued behind its upload, so it overlaps the reads of the files after
+// it.  Returns GGCAT_GPU_EAGAIN when the files need the host reader:
+// compressed, not a regular file, or a layout parse.hip leaves to it.
+int device_read_files(ggcat_gpu_ctx* ctx, const std::vector<std::string>& paths, uint32_t k, bool colors,
+                      uint64_t* n_joined, std::vector<uint64_t>& color_starts, std::vector<uint32_t>& color_ids,
+                      uint64_t* est_bases) {
+    const size_t nf = paths.size();
+    std::vector<uint64_t> sizes(nf);
+    std::vector<int> types(nf);
+    uint64_t total = 0, max_chunks = 1;
+    *est_bases = 0;
+    for (size_t i = 0; i < nf; i++) {
+        types[i] = gg_host::file_type(paths[i]);
+        if (types[i] < 0) return fail(ctx, GGCAT_GPU_EINVAL, "cannot recognize file type of '" + paths[i] + "'");
+        const std::string ext = gg_host::last_ext(paths[i]);
+        if (ext == "gz" || ext == "lz4" || ext == "bz2" || ext == "xz" || ext == "zst" || ext == "zstd")
+            return GGCAT_GPU_EAGAIN;
+        struct stat st;
+        if (stat(paths[i].c_str(), &st) != 0 || !S_ISREG(st.st_mode)) return GGCAT_GPU_EAGAIN;
+        sizes[i] = (uint64_t)st.st_size;
+        total += sizes[i];
+        *est_bases += sizes[i];  // get_estimated_bases_count of an uncompressed file
+        max_chunks = std::max<uint64_t>(max_chunks, (sizes[i] + PARSE_CHUNK - 1) / PARSE_CHUNK);
+    }
+    if (max_chunks >= 0xFFFFFFFFull) return GGCAT_GPU_EAGAIN;
+    const uint64_t block = ctx->tune.read_block ? ctx->tune.read_block : (uint64_t)gg_host::READ_BLOCK;
+    const uint64_t copyback = k > 1 ? k - 1 : 0;
+    const uint64_t flush = std::max<uint64_t>(block, 2 * copyback);
+    const uint64_t out_cap = total + nf + 16;  // every file's bytes and its closing '\n'
+    const uint64_t bit_words = out_cap / 64 + 2;
+    HIPCHK(hipSetDevice(ctx->device));
+    ENSURE(ctx->raw, total + 16);
+    ENSURE(ctx->input, out_cap);
+    // counters: running output offset, then per file its flags and output start
+    ENSURE(ctx->p_flags, 8 * (1 + 2 * nf));
+    ENSURE(ctx->p_bits, bit_words * 8);
+    ENSURE(ctx->p_stats, max_chunks * sizeof(ChunkStats));
+    ENSURE(ctx->p_carry, max_chunks * sizeof(ChunkCarry));
+    ENSURE(ctx->p_keep, max_chunks * 4);
+    ENSURE(ctx->p_keep_off, (max_chunks + 1) * 8);
+    if (!ctx->pin_ring) {
+        if (hipHostMalloc(&ctx->pin_ring, 2 * PIN_CHUNK, hipHostMallocDefault) != hipSuccess) {
+            ctx->pin_ring = nullptr;
+            return GGCAT_GPU_EAGAIN;
+        }
+    }
+    uint64_t* d_base = P_<uint64_t>(ctx->p_flags);
+    unsigned int* d_flags = reinterpret_cast<unsigned int*>(d_base + 1);  // nf words (2 per u64 slot)
+    uint64_t* d_file_start = d_base + 1 + nf;
+    const uint32_t zgrid = (uint32_t)std::min<uint64_t>((bit_words + 255) / 256, (uint64_t)ctx->n_cu * 16);
+    hipLaunchKernelGGL(parse_zero_kernel, dim3(1), dim3(256), 0, ctx->stream, P_<unsigned long long>(ctx->p_flags),
+                       (uint64_t)(1 + 2 * nf));
+    hipLaunchKernelGGL(parse_zero_kernel, dim3(zgrid), dim3(256), 0, ctx->stream, P_<unsigned long long>(ctx->p_bits),
+                       bit_words);
+    HIPCHK(hipGetLastError());
+    uint8_t* ring = static_cast<uint8_t*>(ctx->pin_ring);
+    uint8_t* d_raw = P_<uint8_t>(ctx->raw);
+    bool pending[2] = {false, false};
+    uint64_t file_off = 0;
+    uint32_t j = 0;
+    const uint32_t n_threads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    for (size_t i = 0; i < nf; i++) {
+        // upload: chunk j of the concatenated files goes through ring buffer j % 2
+        const int fd = open(paths[i].c_str(), O_RDONLY);
+        if (fd < 0) return fail(ctx, GGCAT_GPU_EINVAL, "cannot open '" + paths[i] + "'");
+        bool ok = true;
+        for (uint64_t o = 0; o < sizes[i] && ok; o += PIN_CHUNK, j++) {
+            const uint32_t b = j & 1;
+            if (pending[b]) HIPCHK(hipEventSynchronize(ctx->ev[13 + b]));
+            const uint64_t len = std::min<uint64_t>(PIN_CHUNK, sizes[i] - o);
+            const uint64_t slice = (len + n_threads - 1) / n_threads;
+            std::vector<char> good(n_threads, 1);
+            std::vector<std::thread> th;
+            for (uint32_t q = 0; q < n_threads; q++) {
+                th.emplace_back([&, q] {
+                    const uint64_t hi = std::min(len, (q + 1) * slice);
+                    for (uint64_t a = q * slice; a < hi;) {
+                        const ssize_t r = pread(fd, ring + b * PIN_CHUNK + a, hi - a, (off_t)(o + a));
+                        if (r <= 0) {
+                            good[q] = 0;
+                            return;
+                        }
+                        a += (uint64_t)r;
+                    }
+                });
+            }
+            for (auto& x : th) x.join();
+            for (char g : good) ok = ok && g;
+            if (!ok) break;
+            HIPCHK(hipMemcpyAsync(d_raw + file_off + o, ring + b * PIN_CHUNK, len, hipMemcpyHostToDevice, ctx->stream));
+            HIPCHK(hipEventRecord(ctx->ev[13 + b], ctx->stream));
+            pending[b] = true;
+        }
+        close(fd);
+        if (!ok) return fail(ctx, GGCAT_GPU_EINVAL, "error while reading '" + paths[i] + "'");
+        // parse, queued behind the upload
+        ParseArgs A{};
+        A.raw = d_raw + file_off;
+        A.n = sizes[i];
+        A.fastq = types[i] == 1 ? 1u : 0u;
+        A.n_chunks = (sizes[i] + PARSE_CHUNK - 1) / PARSE_CHUNK;
+        A.block = block;
+        A.flush = flush;
+        A.stats = P_<ChunkStats>(ctx->p_stats);
+        A.carry = P_<ChunkCarry>(ctx->p_carry);
+        A.keep = P_<uint32_t>(ctx->p_keep);
+        A.keep_off = P_<uint64_t>(ctx->p_keep_off);
+        A.out = P_<uint8_t>(ctx->input);
+        A.base = d_base;
+        A.bits = P_<unsigned long long>(ctx->p_bits);
+        A.flags = d_flags + i;
+        if (A.n_chunks) {
+            const uint32_t grid = (uint32_t)std::min<uint64_t>(A.n_chunks, (uint64_t)ctx->n_cu * 8);
+            hipLaunchKernelGGL(parse_stats_kernel, dim3(grid), dim3(256), 0, ctx->stream, A);
+            hipLaunchKernelGGL(parse_carry_kernel, dim3(1), dim3(1024), 0, ctx->stream, A);
+            hipLaunchKernelGGL(parse_emit_kernel<false>, dim3(grid), dim3(256), 0, ctx->stream, A);
+            HIPCHK(hipGetLastError());
+            const int r = device_scan(ctx, A.keep, P_<uint64_t>(ctx->p_keep_off), (uint32_t)A.n_chunks);
+            if (r != GGCAT_GPU_OK) return r;
+            hipLaunchKernelGGL(parse_emit_kernel<true>, dim3(grid), dim3(256), 0, ctx->stream, A);
+        }
+        hipLaunchKernelGGL(parse_finish_kernel, dim3(1), dim3(64), 0, ctx->stream, A, d_file_start + i);
+        HIPCHK(hipGetLastError());
+        file_off += sizes[i];
+    }
+    std::vector<uint64_t> counters(1 + 2 * nf);
+    HIPCHK(hipMemcpyAsync(counters.data(), d_base, counters.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    uint64_t n_out = counters[0];
+    const unsigned int* flags = reinterpret_cast<const unsigned int*>(counters.data() + 1);
+    std::vector<uint64_t> starts(counters.begin() + 1 + nf, counters.end());
+    unsigned int any = 0;
+    for (size_t i = 0; i < nf; i++) any |= flags[i];
+    if (any & 1u) return GGCAT_GPU_EAGAIN;
+    if (any & 2u) {
+        // FASTA records of F bytes or more: find them from per-window newline
+        // summaries, walk their flush points, insert the cuts
+        const uint64_t window = std::min<uint64_t>(1ull << 20, flush);
+        const uint64_t n_win = (n_out + window - 1) / window;
+        ENSURE(ctx->p_stats, n_win * 16);
+        unsigned long long* d_first = P_<unsigned long long>(ctx->p_stats);
+        hipLaunchKernelGGL(newline_summary_kernel, dim3((uint32_t)n_win), dim3(256), 0, ctx->stream,
+                           P_<uint8_t>(ctx->input), n_out, window, d_first, d_first + n_win);
+        HIPCHK(hipGetLastError());
+        std::vector<uint64_t> fl(2 * n_win);
+        HIPCHK(hipMemcpyAsync(fl.data(), d_first, fl.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        std::vector<uint64_t> rs, rl;
+        uint64_t prev_end = 0;  // start of the record after the last '\n' seen
+        for (uint64_t w = 0; w < n_win; w++) {
+            if (!fl[w]) continue;
+            const uint64_t nl = fl[w] - 1;
+            if (nl - prev_end >= flush) {
+                rs.push_back(prev_end);
+                rl.push_back(nl - prev_end);
+            }
+            prev_end = fl[n_win + w];  // last '\n' + 1
+        }
+        if (!rs.empty()) {
+            // each cut advances a record by at least F - (k - 1) >= F / 2
+            const uint64_t cap = 2 * n_out / flush + rs.size() + 16;
+            if (cap >= 0xFFFFFFFFull) return GGCAT_GPU_EAGAIN;
+            ENSURE(ctx->p_carry, (2 * rs.size() + cap) * 8);
+            ENSURE(ctx->p_keep, 8);
+            uint64_t* d_rs = P_<uint64_t>(ctx->p_carry);
+            uint64_t* d_cuts = d_rs + 2 * rs.size();
+            unsigned int* d_n = P_<unsigned int>(ctx->p_keep);
+            HIPCHK(hipMemcpyAsync(d_rs, rs.data(), rs.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+            HIPCHK(hipMemcpyAsync(d_rs + rs.size(), rl.data(), rl.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+            hipLaunchKernelGGL(parse_zero_kernel, dim3(1), dim3(64), 0, ctx->stream,
+                               reinterpret_cast<unsigned long long*>(d_n), (uint64_t)1);
+            hipLaunchKernelGGL(flush_split_kernel, dim3((uint32_t)std::min<size_t>(rs.size(), 65535)), dim3(64), 0,
+                               ctx->stream, d_rs, d_rs + rs.size(), (uint32_t)rs.size(),
+                               P_<const unsigned long long>(ctx->p_bits), flush, copyback, d_cuts, d_n,
+                               (uint32_t)cap);
+            HIPCHK(hipGetLastError());
+            unsigned int n_cuts = 0;
+            HIPCHK(hipMemcpyAsync(&n_cuts, d_n, 4, hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            if (n_cuts > cap) return fail(ctx, GGCAT_GPU_EHIP, "flush cut list overflow");
+            std::vector<uint64_t> cuts(n_cuts);
+            HIPCHK(hipMemcpy(cuts.data(), d_cuts, (size_t)n_cuts * 8, hipMemcpyDeviceToHost));
+            std::sort(cuts.begin(), cuts.end());
+            HIPCHK(hipMemcpyAsync(d_cuts, cuts.data(), (size_t)n_cuts * 8, hipMemcpyHostToDevice, ctx->stream));
+            const uint64_t grow = (uint64_t)n_cuts * (1 + copyback);
+            ENSURE(ctx->raw, n_out + grow + 16);  // the raw files are parsed: reuse their buffer
+            const uint32_t egrid = (uint32_t)std::min<uint64_t>((n_out + 4095) / 4096, (uint64_t)ctx->n_cu * 16);
+            hipLaunchKernelGGL(flush_expand_kernel, dim3(std::max(1u, egrid)), dim3(256), 0, ctx->stream,
+                               P_<uint8_t>(ctx->input), n_out, d_cuts, n_cuts, copyback, P_<uint8_t>(ctx->raw));
+            if (n_cuts)
+                hipLaunchKernelGGL(flush_insert_kernel, dim3(std::min(n_cuts, 65535u)), dim3(256), 0, ctx->stream,
+                                   P_<uint8_t>(ctx->input), d_cuts, n_cuts, copyback, P_<uint8_t>(ctx->raw));
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            std::swap(ctx->raw, ctx->input);
+            for (auto& s : starts) {
+                const uint64_t before = (uint64_t)(std::upper_bound(cuts.begin(), cuts.end(), s) - cuts.begin());
+                s += before * (1 + copyback);
+            }
+            n_out += grow;
+            ctx->flush_cuts = n_cuts;
+        }
+    }
+    color_starts.clear();
+    color_ids.clear();
+    if (colors) {
+        color_starts = starts;
+        for (size_t i = 0; i < nf; i++) color_ids.push_back((uint32_t)i);
+    }
+    *n_joined = n_out;
+    return GGCAT_GPU_OK;
+}
+
+}  // namespace
+
 int ggcat_gpu_build_from_files(ggcat_gpu_ctx* ctx, const char* const* paths, uint32_t n_paths,
                                const ggcat_gpu_build_options* options, ggcat_gpu_params* used) {
     if (!ctx || !options) return GGCAT_GPU_EINVAL;
@@ -1390,8 +1627,38 @@ int ggcat_gpu_build_from_files(ggcat_gpu_ctx* ctx, const char* const* paths, uin
         if (!paths || !paths[i]) return fail(ctx, GGCAT_GPU_EINVAL, "null path");
         ps.emplace_back(paths[i]);
     }
+    if (ctx->server) return fail(ctx, GGCAT_GPU_ESTATE, "close the query server before a new build");
+    const auto t_read = std::chrono::steady_clock::now();
+    auto read_ms = [&] {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_read).count();
+    };
+    ctx->input_path = 0;
+    ctx->flush_cuts = 0;
+    {
+        // plain files: read into HBM and parsed there
+        uint64_t n_joined = 0, est = 0;
+        std::vector<uint64_t> cs;
+        std::vector<uint32_t> ci;
+        int r = ctx->tune.host_reader ? GGCAT_GPU_EAGAIN
+                                      : device_read_files(ctx, ps, o.k, o.colors != 0, &n_joined, cs, ci, &est);
+        if (r == GGCAT_GPU_OK) {
+            ctx->ms[8] = read_ms();
+            ctx->input_path = 1;
+            ggcat_gpu_params p{};
+            r = ggcat_gpu_plan_build(&o, est, &p);
+            if (r != GGCAT_GPU_OK) return fail(ctx, r, "invalid build options");
+            if (used) *used = p;
+            r = ggcat_gpu_bucketing(ctx, P_<uint8_t>(ctx->input), n_joined, 1, o.colors ? cs.data() : nullptr,
+                                    o.colors ? ci.data() : nullptr, o.colors ? (uint32_t)ci.size() : 0, &p);
+            if (r != GGCAT_GPU_OK) return r;
+            return ggcat_gpu_kmers_merge(ctx);
+        }
+        if (r != GGCAT_GPU_EAGAIN) return r;
+    }
     gg_host::ReadResult R;
-    if (!gg_host::read_files(ps, o.k, o.colors != 0, R)) return fail(ctx, GGCAT_GPU_EINVAL, R.error);
+    if (!gg_host::read_files(ps, o.k, o.colors != 0, R, ctx->tune.read_block)) return fail(ctx, GGCAT_GPU_EINVAL, R.error);
+    ctx->ms[8] = read_ms();
+    ctx->input_path = 2;
     ggcat_gpu_params p{};
     int r = ggcat_gpu_plan_build(&o, R.estimated_bases, &p);
     if (r != GGCAT_GPU_OK) return fail(ctx, r, "invalid build options");

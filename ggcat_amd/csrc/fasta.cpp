@@ -61,15 +61,19 @@ inline std::string last_ext(const std::string& path) {
     return path.substr(dot + 1);
 }
 
-// Reads the file in 4 MiB blocks (lines_reader.rs:33-56, buffer size
-// DEFAULT_OUTPUT_BUFFER_SIZE) and calls cb(line, partial, finished) exactly as
+// The reference reads files in blocks of DEFAULT_OUTPUT_BUFFER_SIZE bytes
+// (crates/config/src/lib.rs:53, 4 MiB); block = 0 selects that size, others exist
+// so tests can place block ends in small files.
+constexpr size_t READ_BLOCK = 4u << 20;
+
+// Reads the file in blocks (lines_reader.rs:33-56) and calls cb(line, partial, finished) exactly as
 // LinesReader::process_lines does (lines_reader.rs:147-219): one call per
 // complete line ('\n' and one '\r' before it removed), one call with
 // partial = true for a non-empty line piece cut by the block end (a trailing
 // '\r' removed), and a final call (empty, false, true).
 template <class F>
-bool for_each_line_piece(ByteSource& f, F&& cb) {
-    const size_t BLOCK = 4u << 20;
+bool for_each_line_piece(ByteSource& f, F&& cb, size_t block) {
+    const size_t BLOCK = block ? block : READ_BLOCK;
     std::vector<uint8_t> buf(BLOCK);
     while (true) {
         const long r = f.read(buf.data(), BLOCK);
@@ -97,7 +101,7 @@ bool for_each_line_piece(ByteSource& f, F&& cb) {
     }
 }
 
-inline bool read_file(const std::string& path, uint32_t k, ReadResult& R) {
+inline bool read_file(const std::string& path, uint32_t k, ReadResult& R, size_t block = 0) {
     const int type = file_type(path);
     if (type < 0) {
         R.error = "cannot recognize file type of '" + path + "'";
@@ -129,7 +133,7 @@ inline bool read_file(const std::string& path, uint32_t k, ReadResult& R) {
         // every complete line until a piece cut by a block end arrives; the
         // final call is swallowed too while it is set.
         const size_t copyback = k > 1 ? k - 1 : 0;
-        const size_t flush_size = std::max<size_t>(4u << 20, 2 * copyback);
+        const size_t flush_size = std::max<size_t>(block ? block : READ_BLOCK, 2 * copyback);
         bool on_comment = false, ident = false, new_line = true;
         ok = for_each_line_piece(f, [&](const uint8_t* line, size_t len, bool partial, bool finished) {
             if (on_comment) {
@@ -150,7 +154,7 @@ inline bool read_file(const std::string& path, uint32_t k, ReadResult& R) {
                 seq.erase(seq.begin(), seq.end() - copyback);
             }
             new_line = !partial;
-        });
+        }, block);
     } else {
         // process_fastq (sequences_reader.rs:178-240): identifier, sequence,
         // '+' line, quality; the sequence is emitted when its quality line
@@ -176,7 +180,7 @@ inline bool read_file(const std::string& path, uint32_t k, ReadResult& R) {
                 seq.insert(seq.end(), line, line + len);
             }
             if (!partial) state = (state + 1) % 3;
-        });
+        }, block);
     }
     src.reset();
     if (!ok) {
@@ -190,13 +194,14 @@ inline bool read_file(const std::string& path, uint32_t k, ReadResult& R) {
 // result, and concatenated in the order given (the same bytes, offsets and
 // colors as reading them one after another).  The error reported is the
 // first failing file's in that order.
-inline bool read_files(const std::vector<std::string>& paths, uint32_t k, bool colors, ReadResult& R) {
+inline bool read_files(const std::vector<std::string>& paths, uint32_t k, bool colors, ReadResult& R,
+                       size_t block = 0) {
     const size_t n = paths.size();
     std::vector<ReadResult> part(n);
     std::vector<char> ok(n, 0);
     std::atomic<size_t> next{0};
     auto work = [&] {
-        for (size_t i; (i = next.fetch_add(1)) < n;) ok[i] = read_file(paths[i], k, part[i]) ? 1 : 0;
+        for (size_t i; (i = next.fetch_add(1)) < n;) ok[i] = read_file(paths[i], k, part[i], block) ? 1 : 0;
     };
     const size_t nt = std::min<size_t>(n, 16);
     std::vector<std::thread> th;

@@ -57,7 +57,7 @@ class BuildOptions(C.Structure):
 
 
 class Tuning(C.Structure):
-    """ggcat_gpu_tuning: merge schedule knobs (zero = default); never change results."""
+    """ggcat_gpu_tuning: schedule knobs (zero = default); only read_block changes results."""
     _fields_ = [
         ("item_occ", C.c_uint32),
         ("lds_max_occ", C.c_uint32),
@@ -71,6 +71,8 @@ class Tuning(C.Structure):
         ("unit_cap", C.c_uint32),
         ("rec_cap", C.c_uint32),
         ("merge_threads", C.c_uint32),
+        ("read_block", C.c_uint32),
+        ("host_reader", C.c_uint32),
     ]
 
 
@@ -355,9 +357,9 @@ class GpuContext:
     def set_tuning(self, item_occ: int = 0, lds_max_occ: int = 0, merge_wgs_per_cu: int = 0,
                    part_occ: int = 0, lds_table_log: int = 0, table_pct: int = 0, no_dedup: int = 0,
                    unit_bytes: int = 0, use_graphs: int = 0, unit_cap: int = 0, rec_cap: int = 0,
-                   merge_threads: int = 0) -> None:
+                   merge_threads: int = 0, read_block: int = 0, host_reader: int = 0) -> None:
         t = Tuning(item_occ, lds_max_occ, merge_wgs_per_cu, part_occ, lds_table_log, table_pct, no_dedup, unit_bytes,
-                   use_graphs, unit_cap, rec_cap, merge_threads)
+                   use_graphs, unit_cap, rec_cap, merge_threads, read_block, host_reader)
         self._check(self.lib.ggcat_gpu_set_tuning(self.h, C.byref(t)), "ggcat_gpu_set_tuning")
 
     def records(self) -> Records:
@@ -585,11 +587,11 @@ class GpuContext:
         self._check(self.lib.ggcat_gpu_device_free(self.h, C.c_void_p(ptr)), "device_free")
 
     def timings(self) -> np.ndarray:
-        out = np.zeros(8, np.float64)
-        self._check(self.lib.ggcat_gpu_timings(self.h, _ptr(out, _f64p), 8), "timings")
+        out = np.zeros(9, np.float64)
+        self._check(self.lib.ggcat_gpu_timings(self.h, _ptr(out, _f64p), 9), "timings")
         return out
 
     def counters(self) -> np.ndarray:
-        out = np.zeros(8, np.uint64)
-        self._check(self.lib.ggcat_gpu_counters(self.h, _ptr(out, _u64p), 8), "counters")
+        out = np.zeros(10, np.uint64)
+        self._check(self.lib.ggcat_gpu_counters(self.h, _ptr(out, _u64p), 10), "counters")
         return out

@@ -197,7 +197,14 @@ int ggcat_gpu_solid_colors(ggcat_gpu_ctx* ctx, uint32_t* color_words, uint32_t* 
  *   rec_cap           merge: identical-record table slots (power of two in
  *                     [16, 512], default 512); records are deduplicated in
  *                     chunks of rec_cap / 2 (at most one record per thread)
- *   merge_threads     threads per merge workgroup: 128 or 256 (default) */
+ *   merge_threads     threads per merge workgroup: 128 or 256 (default)
+ *   read_block        input files: the read block of the reference's line
+ *                     reader (0 = its DEFAULT_OUTPUT_BUFFER_SIZE, 4 MiB, else
+ *                     at least 256); results are those of the reference run
+ *                     with that block size (it decides where long FASTA
+ *                     records are cut), so only tests set it
+ *   host_reader       1: read input files on the host even when they could be
+ *                     read into HBM and parsed there (same records) */
 typedef struct {
     uint32_t item_occ;
     uint32_t lds_max_occ;
@@ -211,20 +218,26 @@ typedef struct {
     uint32_t unit_cap;
     uint32_t rec_cap;
     uint32_t merge_threads;
+    uint32_t read_block;
+    uint32_t host_reader;
 } ggcat_gpu_tuning;
 int ggcat_gpu_set_tuning(ggcat_gpu_ctx* ctx, const ggcat_gpu_tuning* tuning);
 
 /* Device-side timings of the last calls, in milliseconds:
  * [0] bucketing staging pass, [1] scan, [2] bucketing scatter pass,
  * [3] merge (whole-item pass), [4] merge (partitioned passes),
- * [5] whole bucketing call, [6] whole merge call, [7] unitig kernels. */
+ * [5] whole bucketing call, [6] whole merge call, [7] unitig kernels,
+ * [8] reading the input of the last build_from_files (wall clock: files
+ * read into HBM and parsed there, or read on the host). */
 int ggcat_gpu_timings(ggcat_gpu_ctx* ctx, double* ms, uint32_t n);
 
 /* Counters of the last calls: [0] input bytes, [1] records,
  * [2] sum of record lengths (bases), [3] k-mer occurrences,
  * [4] work items split into hash-partitioned passes, [5] solid entries,
  * [6] k-mers shared between two units of a sub-bucket (combined on the
- * host after the merge), [7] merge kernel passes (1 + partition rounds). */
+ * host after the merge), [7] merge kernel passes (1 + partition rounds),
+ * [8] how the last build_from_files read its input (1: into HBM, parsed on
+ * the device; 2: host reader), [9] long FASTA records cut on the device. */
 int ggcat_gpu_counters(ggcat_gpu_ctx* ctx, uint64_t* out, uint32_t n);
 
 /* Reads FASTA/FASTQ files (optionally gzip-compressed) into the joined layout.

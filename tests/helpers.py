@@ -168,3 +168,78 @@ def edge_records_inline(rng: np.random.Generator, k: int) -> list[bytes]:
     for _ in range(120):
         recs.append(rnd(int(rng.integers(k - 2, 400))))
     return recs
+
+
+# Input files for the device parse path: the corner cases of the reference's
+# line reader (sequences_reader.rs / lines_reader.rs) at a given read block.
+
+def random_bases(rng, n: int) -> bytes:
+    s = rng.choice(np.frombuffer(b"ACGTacgtN", np.uint8), n, p=[.23, .23, .23, .23, .02, .02, .02, .01, .01])
+    return s.tobytes()
+
+
+def wrap_lines(seq: bytes, width: int, eol: bytes) -> bytes:
+    return b"".join(seq[i:i + width] + eol for i in range(0, len(seq), width))
+
+
+def fasta_text(rng, n_rec: int, eol: bytes, block: int, long_header: bool = True) -> bytes:
+    out = bytearray()
+    out += wrap_lines(random_bases(rng, 150), 60, eol)  # sequence before the first header
+    out += eol  # an empty line
+    for r in range(n_rec):
+        hdr = b">rec%d" % r + (b" x" * (block // 2 + 50) if long_header and r == 2 else b"")
+        out += hdr + eol
+        width = int(rng.choice([50, 60, 61, 80]))
+        out += wrap_lines(random_bases(rng, int(rng.integers(0, 3200))), width, eol)
+        if r % 4 == 3:
+            out += eol
+    return bytes(out)
+
+
+def lone_cr_at_block_ends(data: bytes, block: int) -> bytes:
+    """A '\\r' in the middle of sequence lines at a few block ends."""
+    b = bytearray(data)
+    for e in range(block - 1, len(b) - 2, 3 * block):
+        line0 = b.rfind(b"\n", 0, e) + 1
+        if e > line0 and b[line0] not in b">;" and b[e] not in b"\r\n" and b[e + 1] not in b"\r\n":
+            b[e] = ord("\r")
+    return bytes(b)
+
+
+def fastq_text(rng, n: int, eol: bytes) -> bytes:
+    out = bytearray()
+    for i in range(n):
+        s = random_bases(rng, int(rng.integers(60, 300)))
+        out += b"@r%d" % i + eol + s + eol + b"+" + eol + b"I" * len(s) + (eol if i + 1 < n else b"")
+    return bytes(out)
+
+
+def parse_case_files(tmp_path, block: int) -> list[str]:
+    rng = np.random.default_rng(block)
+    files = {
+        "multi.fa": fasta_text(rng, 12, b"\n", block),
+        "crlf.fasta": lone_cr_at_block_ends(fasta_text(rng, 10, b"\r\n", block), block),
+        "oneline.fa": b">one\n" + random_bases(rng, 5000) + b"\r",  # one line, no '\n' at the end
+        "reads.fq": lone_cr_at_block_ends(fastq_text(rng, 40, b"\r\n"), block),
+        "empty.fa": b"",
+        "tail.fna": fasta_text(rng, 3, b"\n", block, long_header=False)[:-1],
+    }
+    paths = []
+    for name, data in files.items():
+        p = tmp_path / name
+        p.write_bytes(data)
+        paths.append(str(p))
+    return paths
+
+
+def build_both_readers(ctx, paths, opts, block: int, **tuning) -> list:
+    """build_from_files with the device parse path, then with the host
+    reader: (input path, flush cuts, records, solid table) of each."""
+    out = []
+    for host in (0, 1):
+        ctx.set_tuning(read_block=block, host_reader=host, **tuning)
+        ctx.build_from_files(paths, opts)
+        c = ctx.counters()
+        s = (colored_solid_array(ctx.solid(), ctx.solid_colors()) if opts.colors else solid_array(ctx.solid()))
+        out.append((int(c[8]), int(c[9]), record_tuples(ctx.records()), s))
+    return out

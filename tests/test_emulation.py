@@ -13,8 +13,9 @@ from ggcat_amd import synth
 from ggcat_amd.native import HASH_RK128, HASH_SEQ, GpuContext, Params
 from oracle import binding as O
 from tests.emu import build as emu_build
-from tests.helpers import (adversarial_records, bitmaps_from_sets, colored_solid_array, joined, record_tuples,
-                           solid_array)
+from tests.helpers import (adversarial_records, bitmaps_from_sets, colored_solid_array, fasta_text, joined,
+                           build_both_readers, parse_case_files, random_bases, record_tuples, solid_array,
+                           wrap_lines)
 
 
 @pytest.fixture(scope="module")
@@ -95,3 +96,51 @@ def test_unit_region_spill_emulated(emu_ctx, unit_cap):
         assert record_tuples(emu_ctx.records()) == record_tuples(o.records())
     finally:
         emu_ctx.set_tuning()
+
+
+# ---------------------------------------------------------------------------
+# Input files read into device memory and parsed there (parse.hip) against the
+# host reader (fasta.cpp, the restatement of sequences_reader.rs): the same
+# records and solid k-mers for the corner cases of the reference's line
+# reader, at read blocks small enough to put many block ends in small files.
+
+@pytest.mark.parametrize("block,k,m,kind", [(512, 31, 15, HASH_SEQ), (1000, 63, 21, HASH_RK128)])
+@pytest.mark.parametrize("colors", [0, 1])
+def test_device_parse_matches_host_reader_emulated(emu_ctx, tmp_path, block, k, m, kind, colors):
+    from ggcat_amd.native import BuildOptions
+    paths = parse_case_files(tmp_path, block)
+    opts = BuildOptions(k=k, m=m, min_multiplicity=1, hash_kind=kind, colors=colors)
+    try:
+        dev, host = build_both_readers(emu_ctx, paths, opts, block, merge_wgs_per_cu=2)
+        (path_d, cuts, rec_d, sol_d), (path_h, _, rec_h, sol_h) = dev, host
+    finally:
+        emu_ctx.set_tuning()
+    assert (path_d, path_h) == (1, 2)
+    assert cuts > 0  # records longer than the flush size were cut on the device
+    assert rec_d == rec_h
+    assert np.array_equal(sol_d, sol_h)
+
+
+@pytest.mark.parametrize("case", ["comment", "cr_line_at_block_end"])
+def test_device_parse_falls_back_to_host_reader_emulated(emu_ctx, tmp_path, case):
+    from ggcat_amd.native import BuildOptions
+    block = 512
+    rng = np.random.default_rng(9)
+    data = bytearray(fasta_text(rng, 6, b"\n", block, long_header=False))
+    if case == "comment":
+        data[data.index(b">rec3"):data.index(b">rec3") + 1] = b";"
+    else:
+        # a line that is a lone '\r' at a block end, then a header: the
+        # reference sees no piece for it and takes the header as a new line
+        data = bytearray(b">a\n" + random_bases(rng, block - 5) + b"\n\r>late\n" + wrap_lines(random_bases(rng, 500), 60, b"\n"))
+        assert data[block - 1] == ord("\r") and data[block - 2] == ord("\n")
+    p = tmp_path / "x.fa"
+    p.write_bytes(bytes(data))
+    opts = BuildOptions(k=31, m=15, min_multiplicity=1)
+    try:
+        dev, host = build_both_readers(emu_ctx, [str(p)], opts, block, merge_wgs_per_cu=2)
+        (path_d, _, rec_d, sol_d), (_, _, rec_h, sol_h) = dev, host
+    finally:
+        emu_ctx.set_tuning()
+    assert path_d == 2
+    assert rec_d == rec_h and np.array_equal(sol_d, sol_h)

@@ -255,6 +255,51 @@ def test_build_from_files_fastq(gpu_ctx, tmp_path):
     assert np.array_equal(solid_array(gs), solid_array(o.solid()))
 
 
+@pytest.mark.parametrize("block,k,m,kind", [(512, 31, 15, HASH_SEQ), (1000, 63, 21, HASH_RK128)])
+@pytest.mark.parametrize("colors", [0, 1])
+def test_device_parse_matches_host_reader(gpu_ctx, tmp_path, block, k, m, kind, colors):
+    # plain files read into HBM and parsed there against the host reader, at
+    # read blocks that put block ends (lone '\r', cut lines) and flush cuts
+    # in small files
+    from ggcat_amd.native import BuildOptions
+    from tests.helpers import build_both_readers, parse_case_files
+    paths = parse_case_files(tmp_path, block)
+    _say(f"case device parse block={block} k={k} colors={colors}")
+    opts = BuildOptions(k=k, m=m, min_multiplicity=1, hash_kind=kind, colors=colors)
+    try:
+        dev, host = build_both_readers(gpu_ctx, paths, opts, block)
+    finally:
+        gpu_ctx.set_tuning()
+    assert (dev[0], host[0]) == (1, 2) and dev[1] > 0
+    assert dev[2] == host[2]
+    assert np.array_equal(dev[3], host[3])
+
+
+def test_device_parse_long_records_against_oracle(gpu_ctx, tmp_path):
+    # the reference's own flush size: a 9 Mbase record (cut twice at 4 MiB
+    # with k - 1 bases of overlap), shorter records and a FASTQ file, read
+    # into HBM; the oracle runs on the host reader's joined layout
+    from ggcat_amd.native import BuildOptions, read_files
+    from tests.helpers import wrap_lines
+    g = synth.genome(21, 9_000_000).tobytes()
+    fa = b">chr\n" + wrap_lines(g, 80, b"\n") + b">small\n" + wrap_lines(g[:5000], 70, b"\r\n")
+    reads = synth.reads(synth.genome(22, 50_000), 23, 2000, 150, 0.01).tobytes().split(b"\n")[:-1]
+    fq = b"".join(b"@r%d\n%s\n+\n%s\n" % (i, r, b"I" * len(r)) for i, r in enumerate(reads))
+    paths = [tmp_path / "genome.fa", tmp_path / "reads.fastq"]
+    paths[0].write_bytes(fa)
+    paths[1].write_bytes(fq)
+    paths = [str(p) for p in paths]
+    _say("case device parse 9 Mbase record + fastq")
+    used = gpu_ctx.build_from_files(paths, BuildOptions(k=31, m=15, min_multiplicity=1))
+    c = gpu_ctx.counters()
+    assert int(c[8]) == 1 and int(c[9]) == 2
+    inp = read_files(paths, 31)
+    o = O.bucketing(inp.joined, 31, 15, used.buckets_log, used.second_buckets_log, threads=4)
+    assert np.array_equal(record_digests(gpu_ctx.records()), record_digests(o.records()))
+    o.merge(31, used.buckets_log, used.second_buckets_log, HASH_SEQ, 1, 4)
+    assert np.array_equal(solid_array(gpu_ctx.solid()), solid_array(o.solid()))
+
+
 @pytest.mark.parametrize("n_genomes,scale", [(64, 0.02), (256, 0.005)])
 def test_colored_contigs(gpu_ctx, n_genomes, scale):
     # one color per genome file of error-free contigs (the colored build's shape)
