@@ -52,9 +52,13 @@ def default_grid(k: int) -> list[Schedule]:
     out = []
     for ub, rc, lg in itertools.product((1400, 2000, 3000), (256, 512), logs):
         out.append(Schedule(unit_bytes=ub, table_pct=200, lds_table_log=lg, rec_cap=rc, use_graphs=1))
-    # smaller workgroups with smaller tables over smaller units
+    # smaller tables over smaller units, in workgroups of 128 and 256 threads
+    # (more resident workgroups per CU)
     out.append(Schedule(unit_bytes=1000, table_pct=200, lds_table_log=lg - 1, rec_cap=128, use_graphs=1,
                         merge_threads=128))
+    for rc in (128, 256):
+        out.append(Schedule(unit_bytes=1000, table_pct=200, lds_table_log=lg - 1, rec_cap=rc, use_graphs=1,
+                            merge_threads=256))
     return out
 
 

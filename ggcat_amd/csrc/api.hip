@@ -1007,8 +1007,11 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
 
     // resident workgroups per CU: what the 160 KiB LDS holds with the
     // kernel's static LDS, at most 8 (32 waves of 64)
+    // the seq-hash k <= 32 dedup path runs a variant with its switches compiled in
+    const bool mode1 = key_words == 1 && W == 0 && M.dedup && M.tag_kmers && ctx->slot_words == 4;
+    void (*merge_kernel)(MergeArgs) = mode1 ? merge_lds_seq64_kernel : merge_lds_kernel;
     hipFuncAttributes fa{};
-    HIPCHK(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(merge_lds_kernel)));
+    HIPCHK(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(merge_kernel)));
     const size_t wg_lds = (lds_bytes + fa.sharedSizeBytes + 511) / 512 * 512;  // allocation granule
     const uint32_t fit =
         (uint32_t)std::max<size_t>(1, std::min<size_t>(2048 / merge_threads, (160 * 1024) / wg_lds));
@@ -1068,7 +1071,7 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
             HIPCHK(hipMemsetAsync(ctx->work.p, 0, 8, ctx->stream));
             HIPCHK(hipMemsetAsync(ctx->redo_count.p, 0, 8, ctx->stream));
             if (pass) HIPCHK(hipEventRecord(ctx->ev[8], ctx->stream));
-            hipLaunchKernelGGL(merge_lds_kernel, dim3(lds_wgs), dim3(merge_threads), lds_bytes, ctx->stream, M);
+            hipLaunchKernelGGL(merge_kernel, dim3(lds_wgs), dim3(merge_threads), lds_bytes, ctx->stream, M);
             HIPCHK(hipGetLastError());
             n_passes_run++;
             HIPCHK(hipEventRecord(ctx->ev[pass ? 9 : 7], ctx->stream));

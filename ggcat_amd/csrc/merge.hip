@@ -97,6 +97,21 @@ struct MergeArgs {
 
 GG_DEV uint32_t hash128(uint64_t lo, uint64_t hi) { return tab_hash64(lo ^ (hi * 0x9E3779B97F4A7C15ULL)); }
 
+// Kernel variants: MODE 0 reads every switch from the arguments; MODE 1 is
+// the seq-hash k <= 32 path (64-bit keys, 4-word slots, record dedup with one
+// k-mer position per thread, no colors) with those switches compiled in, which keeps the
+// 128-bit and color code out of its register allocation.
+template <int MODE>
+struct Mode {
+    static GG_DEV uint32_t key_words(const MergeArgs& A) { return MODE == 1 ? 1u : A.key_words; }
+    static GG_DEV uint32_t color_words(const MergeArgs& A) { return MODE == 1 ? 0u : A.color_words; }
+    static GG_DEV bool seq64(const MergeArgs& A) { return MODE == 1 || (A.kind == 0 && A.k <= 32); }
+    static GG_DEV bool dedup(const MergeArgs& A) { return MODE == 1 || A.dedup; }
+    static GG_DEV bool tag_kmers(const MergeArgs& A) { return MODE == 1 || A.tag_kmers; }
+    static GG_DEV uint32_t slot_words(const MergeArgs& A) { return MODE == 1 ? 4u : A.slot_words; }
+    static constexpr uint32_t MAX_SW = MODE == 1 ? 4u : 8u;  // register copies of a slot
+};
+
 // Insert one occurrence.  Returns 1 when this call claimed a new slot, 0 when
 // the key was present, 2 when no slot was found within min(mask+1,
 // probe_limit) probes.
@@ -167,7 +182,7 @@ GG_DEV uint32_t tab_insert128(KP klo, KP khi, SP st, VP vals, uint32_t mask, uin
 }
 
 // Iterate the k-mers of one slot; cb(lo, hi, is_forward, idx, last_idx).
-template <class CB>
+template <int MODE, class CB>
 GG_DEV void for_each_kmer(const MergeArgs& A, const uint32_t* slot, CB&& cb) {
     const uint32_t meta = slot[0];
     const uint32_t* bw;
@@ -182,7 +197,7 @@ GG_DEV void for_each_kmer(const MergeArgs& A, const uint32_t* slot, CB&& cb) {
         bw = A.pool + off;
     } else {
         len = meta_len(meta);
-        if (len > (A.slot_words - 1) * 16) {
+        if (len > (Mode<MODE>::slot_words(A) - 1) * 16) {
             atomicOr(A.error_flag, 2u);
             return;
         }
@@ -191,7 +206,7 @@ GG_DEV void for_each_kmer(const MergeArgs& A, const uint32_t* slot, CB&& cb) {
     const uint32_t k = A.k;
     if (len < k) return;
     const uint32_t last = len - k;
-    if (A.kind == 0 && k <= 32) {
+    if (Mode<MODE>::seq64(A)) {
         const uint64_t mask = (k == 32) ? ~0ULL : ((1ULL << (2 * k)) - 1);
         uint64_t fw = 0, rc = 0;
         uint32_t word = 0;
@@ -267,10 +282,10 @@ GG_DEV void close_chunk(const MergeArgs& A, EmitState& es) {
 // Emit solid entries among the table slots claimed by the entry (`claimed`,
 // n of them); cross-marked entries holding only one flag bit go to the
 // fixup list.  Wavefronts take 64 claimed slots at a time.
-template <class KP, class SP, class VP>
+template <int MODE, class KP, class SP, class VP>
 GG_DEV void emit_solid(const MergeArgs& A, uint32_t sub, KP klo, KP khi, SP st, VP vals, const uint32_t* cbits,
                        const uint16_t* claimed, uint32_t n, EmitState& es) {
-    const uint32_t W = A.color_words;
+    const uint32_t W = Mode<MODE>::color_words(A), KW = Mode<MODE>::key_words(A);
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t below = (1ULL << lane) - 1;
     for (uint32_t i0 = wave * 64; i0 < n; i0 += blockDim.x) {
@@ -279,7 +294,7 @@ GG_DEV void emit_solid(const MergeArgs& A, uint32_t sub, KP klo, KP khi, SP st, 
         bool solid = false, fix = false;
         uint32_t outv = 0;
         if (j < n) {
-            bool occupied = (A.key_words == 1) ? (klo[i] != EMPTY64) : (st[i] == 2);
+            bool occupied = (KW == 1) ? (klo[i] != EMPTY64) : (st[i] == 2);
             if (occupied) {
                 uint32_t v = vals[i];
                 uint32_t cnt = v & CNT_MASK, fl = v >> 30;
@@ -301,7 +316,7 @@ GG_DEV void emit_solid(const MergeArgs& A, uint32_t sub, KP klo, KP khi, SP st, 
             }
             if (main && es.chunk < A.n_chunks) {
                 const uint64_t pos = (uint64_t)es.chunk * OUT_CHUNK + es.fill + __popcll(bal & below);
-                if (A.key_words == 1) {
+                if (KW == 1) {
                     A.out_key[pos] = klo[i];
                 } else {
                     A.out_key[2 * pos] = klo[i];
@@ -320,7 +335,7 @@ GG_DEV void emit_solid(const MergeArgs& A, uint32_t sub, KP klo, KP khi, SP st, 
             if (fix) {
                 uint64_t pos = basef + __popcll(balf & below);
                 if (pos < A.fix_cap) {
-                    if (A.key_words == 1) {
+                    if (KW == 1) {
                         A.fix_key[pos] = klo[i];
                     } else {
                         A.fix_key[2 * pos] = klo[i];
@@ -339,25 +354,25 @@ GG_DEV void emit_solid(const MergeArgs& A, uint32_t sub, KP klo, KP khi, SP st, 
 // work entry's partition (bits 16.. of the table hash; the slot index uses
 // the low bits).  An insert that needs more than `probe_limit` probes sets
 // *overflow and insertion stops (the entry is then redone as finer parts).
-template <class KP, class SP, class VP>
+template <int MODE, class KP, class SP, class VP>
 GG_DEV void insert_item(const MergeArgs& A, uint64_t r0, uint64_t r1, const PartWork& w, KP klo, KP khi, SP st,
                         VP vals, uint32_t* cbits, uint32_t mask, uint32_t probe_limit, uint32_t* overflow,
                         uint16_t* claimed, uint32_t* n_claimed) {
-    const uint32_t W = A.color_words;
+    const uint32_t W = Mode<MODE>::color_words(A), KW = Mode<MODE>::key_words(A);
     const uint32_t SW = A.slot_words;
     const uint32_t part_mask = (1u << w.log_parts) - 1;
     for (uint64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
         const uint32_t* slot = A.slots + r * SW;
         const uint32_t meta = slot[0];
         const uint32_t color = W ? A.slot_color[r] : 0u;
-        for_each_kmer(A, slot, [&](uint64_t lo, uint64_t hi, bool is_fwd, uint32_t idx, uint32_t last) {
+        for_each_kmer<MODE>(A, slot, [&](uint64_t lo, uint64_t hi, bool is_fwd, uint32_t idx, uint32_t last) {
             if (*(volatile uint32_t*)overflow) return;
             if (part_mask) {
-                const uint32_t h = (A.key_words == 1) ? tab_hash64(lo) : hash128(lo, hi);
+                const uint32_t h = (KW == 1) ? tab_hash64(lo) : hash128(lo, hi);
                 if (((h >> 16) & part_mask) != w.part) return;
             }
             uint32_t bits = occurrence_bits(meta, is_fwd, idx, last), at = 0;
-            uint32_t res = (A.key_words == 1)
+            uint32_t res = (KW == 1)
                                ? tab_insert64(klo, vals, mask, lo, bits, probe_limit, at)
                                : tab_insert128(klo, khi, st, vals, mask, lo, hi, bits, probe_limit, at);
             if (res == 2) {
@@ -445,22 +460,23 @@ GG_DEV uint64_t kmer_at(const uint32_t* bw, uint32_t i, uint32_t k, bool* fwd) {
     return fw < rc ? fw : rc;
 }
 
-template <class KP, class SP, class VP>
+template <int MODE, class KP, class SP, class VP>
 GG_DEV void insert_item_dedup(const MergeArgs& A, uint64_t r0, uint64_t r1, const PartWork& w, KP klo, KP khi, SP st,
                               VP vals, uint32_t* cbits, uint32_t mask, uint32_t probe_limit, uint32_t* overflow,
                               uint16_t* claimed, uint32_t* n_claimed, const RecTable& R, const uint32_t* pre,
                               uint32_t pre_color) {
-    const uint32_t W = A.color_words, SW = A.slot_words, k = A.k, RC = A.rec_cap;
+    const uint32_t W = Mode<MODE>::color_words(A), KW = Mode<MODE>::key_words(A);
+    const uint32_t SW = Mode<MODE>::slot_words(A), k = A.k, RC = A.rec_cap;
     const uint32_t part_mask = (1u << w.log_parts) - 1;
     // one occurrence (or cnt identical ones) of a k-mer of either key width
     auto insert_key = [&](uint64_t lo, uint64_t hi, uint32_t bits, uint32_t color) {
         if (*(volatile uint32_t*)overflow) return;
         if (part_mask) {
-            const uint32_t h = (A.key_words == 1) ? tab_hash64(lo) : hash128(lo, hi);
+            const uint32_t h = (KW == 1) ? tab_hash64(lo) : hash128(lo, hi);
             if (((h >> 16) & part_mask) != w.part) return;
         }
         uint32_t at = 0;
-        const uint32_t res = (A.key_words == 1) ? tab_insert64(klo, vals, mask, lo, bits, probe_limit, at)
+        const uint32_t res = (KW == 1) ? tab_insert64(klo, vals, mask, lo, bits, probe_limit, at)
                                                 : tab_insert128(klo, khi, st, vals, mask, lo, hi, bits, probe_limit, at);
         if (res == 2) {
             atomicOr(overflow, 1u);
@@ -482,7 +498,7 @@ GG_DEV void insert_item_dedup(const MergeArgs& A, uint64_t r0, uint64_t r1, cons
         const uint32_t color = pre_color;
         bool direct = (meta & META_LONG) != 0;
         if (!direct) {
-            uint32_t wv[8];
+            uint32_t wv[Mode<MODE>::MAX_SW];
             for (uint32_t q = 0; q < SW; q++) wv[q] = pre[q];
             uint32_t i = rec_hash(wv, SW, color) & (RC - 1);
             uint32_t probes = 0, spins = 0;
@@ -517,14 +533,14 @@ GG_DEV void insert_item_dedup(const MergeArgs& A, uint64_t r0, uint64_t r1, cons
             }
         }
         if (direct) {
-            for_each_kmer(A, slot, [&](uint64_t lo, uint64_t hi, bool is_fwd, uint32_t idx, uint32_t last) {
+            for_each_kmer<MODE>(A, slot, [&](uint64_t lo, uint64_t hi, bool is_fwd, uint32_t idx, uint32_t last) {
                 insert_key(lo, hi, occurrence_bits(meta, is_fwd, idx, last), color);
             });
         }
     }
 phase_a_done:
     __syncthreads();
-    if (!A.tag_kmers) {
+    if (!Mode<MODE>::tag_kmers(A)) {
         // 128-bit keys: each thread rolls the k-mers of the distinct records
         // in its table slots (rec_cap / blockDim of them) and frees the slots
         const uint32_t per = RC > blockDim.x ? RC / blockDim.x : 1u;
@@ -534,7 +550,7 @@ phase_a_done:
             if (si >= RC || R.state[si] != 2) continue;
             const uint32_t* rw = R.key + si * SW;
             const uint32_t cnt = R.cnt[si], color = W ? R.col[si] : 0u, meta = rw[0];
-            for_each_kmer(A, rw, [&](uint64_t lo, uint64_t hi, bool is_fwd, uint32_t idx, uint32_t last) {
+            for_each_kmer<MODE>(A, rw, [&](uint64_t lo, uint64_t hi, bool is_fwd, uint32_t idx, uint32_t last) {
                 insert_key(lo, hi, occurrence_bits(meta, is_fwd, idx, last) - 1u + cnt, color);
             });
             R.state[si] = 0;
@@ -612,7 +628,15 @@ phase_a_done:
                 }
             }
         }
-        if (cl) claimed[atomicAdd(n_claimed, 1u)] = (uint16_t)at;
+        // one LDS atomic per wavefront for the slots it claimed
+        const uint64_t bal = __ballot(cl);
+        if (bal) {
+            const uint32_t lane = threadIdx.x & 63, first = (uint32_t)__builtin_ctzll(bal);
+            uint32_t base = 0;
+            if (lane == first) base = atomicAdd(n_claimed, (uint32_t)__popcll(bal));
+            base = __shfl(base, (int)first);
+            if (cl) claimed[base + (uint32_t)__popcll(bal & ((1ULL << lane) - 1))] = (uint16_t)at;
+        }
     }
 }
 
@@ -650,7 +674,8 @@ GG_DEV void pipe_item(const MergeArgs& A, Pipe& p) {
 
 // Persistent work-stealing merge: each workgroup takes work entries from a
 // global cursor, thread 0 keeping the next entry's metadata in flight.
-__global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
+template <int MODE>
+GG_DEV void merge_lds_body(const MergeArgs& A) {
     extern __shared__ __align__(16) uint8_t lds[];
     __shared__ uint32_t s_overflow;
     __shared__ Fetched s_cur;
@@ -658,10 +683,11 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
     const uint32_t cap_max = A.lds_cap;  // the host allocates this many entries plus a margin
     uint64_t* klo = reinterpret_cast<uint64_t*>(lds);
     uint64_t* khi = klo + cap_max;  // only used for 128-bit keys
-    uint32_t* vals = reinterpret_cast<uint32_t*>(lds + (size_t)cap_max * 8 * A.key_words);
+    const uint32_t KW = Mode<MODE>::key_words(A);
+    uint32_t* vals = reinterpret_cast<uint32_t*>(lds + (size_t)cap_max * 8 * KW);
     uint32_t* st = vals + cap_max;  // only used for 128-bit keys
-    uint32_t* cbits = vals + (size_t)cap_max * A.key_words;  // color_words per entry (colored builds)
-    const uint32_t W = A.color_words;
+    uint32_t* cbits = vals + (size_t)cap_max * KW;  // color_words per entry (colored builds)
+    const uint32_t W = Mode<MODE>::color_words(A);
     uint16_t* claimed = reinterpret_cast<uint16_t*>(cbits + (size_t)cap_max * W);  // slots claimed by the entry
     __shared__ uint32_t s_claimed;
     const bool leader = threadIdx.x == 0;
@@ -669,16 +695,17 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
     for (uint32_t i = threadIdx.x; i < cap_max; i += blockDim.x) {
         klo[i] = EMPTY64;
         vals[i] = 0;
-        if (A.key_words == 2) st[i] = 0;
+        if (KW == 2) st[i] = 0;
     }
     for (uint32_t i = threadIdx.x; i < cap_max * W; i += blockDim.x) cbits[i] = 0;
-    if (A.dedup) {
+    if (Mode<MODE>::dedup(A)) {
         // record table slots start free; each chunk's phase B frees them again
         const RecTable R = rec_table(A, lds);
         for (uint32_t i = threadIdx.x; i < A.rec_cap; i += blockDim.x) R.state[i] = 0;
     }
     // each thread's record of the current chunk (dedup path)
-    const uint32_t chunk = A.rec_cap / 2, SW = A.slot_words;
+    const uint32_t chunk = A.rec_cap / 2, SW = Mode<MODE>::slot_words(A);
+    constexpr uint32_t MSW = Mode<MODE>::MAX_SW;
     auto load_rec = [&](uint64_t r, uint64_t r_end, uint32_t* wv, uint32_t& col) {
         if (threadIdx.x < chunk && r < r_end) {
             const uint32_t* sp = A.slots + r * SW;
@@ -694,7 +721,7 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
             col = W ? A.slot_color[r] : 0u;
         }
     };
-    uint32_t cw[8] = {0, 0, 0, 0, 0, 0, 0, 0}, ccol = 0;
+    uint32_t cw[MSW] = {}, ccol = 0;
     Pipe pp{};
     if (leader) {
         s_claimed = 0;
@@ -730,7 +757,7 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
         while (cap < want && cap < cap_max) cap <<= 1;
         if (w.log_parts) cap = cap_max;
         const uint32_t mask = cap - 1;
-        if (A.dedup) {
+        if (Mode<MODE>::dedup(A)) {
             // records in chunks of rec_cap / 2 <= 256 (the record table's load
             // factor stays at most one half), one record per thread; identical
             // records of one chunk are counted once, the k-mer table
@@ -739,17 +766,17 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
             load_rec(it.r0 + threadIdx.x, it.r1, cw, ccol);
             for (uint64_t c0 = it.r0; c0 < it.r1; c0 += chunk) {
                 const uint64_t c1 = c0 + chunk < it.r1 ? c0 + chunk : it.r1;
-                uint32_t nw[8] = {0, 0, 0, 0, 0, 0, 0, 0}, ncol = 0;
+                uint32_t nw[MSW] = {}, ncol = 0;
                 load_rec(c1 + threadIdx.x, it.r1, nw, ncol);
-                insert_item_dedup(A, c0, c1, w, klo, khi, st, vals, cbits, mask, 96u, &s_overflow, claimed,
+                insert_item_dedup<MODE>(A, c0, c1, w, klo, khi, st, vals, cbits, mask, 96u, &s_overflow, claimed,
                                   &s_claimed, rec_table(A, lds), cw, ccol);
 #pragma unroll
-                for (uint32_t q = 0; q < 8; q++) cw[q] = nw[q];
+                for (uint32_t q = 0; q < MSW; q++) cw[q] = nw[q];
                 ccol = ncol;
                 __syncthreads();  // the next chunk reuses the record table
             }
         } else {
-            insert_item(A, it.r0, it.r1, w, klo, khi, st, vals, cbits, mask, 96u, &s_overflow, claimed, &s_claimed);
+            insert_item<MODE>(A, it.r0, it.r1, w, klo, khi, st, vals, cbits, mask, 96u, &s_overflow, claimed, &s_claimed);
         }
         if (leader) pipe_item(A, pp);
         __syncthreads();
@@ -758,14 +785,14 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
             // more distinct k-mers than the table holds: redone as finer parts
             if (leader) A.redo[atomicAdd(A.redo_count, 1u)] = PartWork{w.item, w.part, w.log_parts, it.occ};
         } else {
-            emit_solid(A, it.sub, klo, khi, st, vals, cbits, claimed, n_cl, es);
+            emit_solid<MODE>(A, it.sub, klo, khi, st, vals, cbits, claimed, n_cl, es);
         }
         __syncthreads();
         for (uint32_t j = threadIdx.x; j < n_cl; j += blockDim.x) {
             const uint32_t i = claimed[j];
             klo[i] = EMPTY64;
             vals[i] = 0;
-            if (A.key_words == 2) st[i] = 0;
+            if (KW == 2) st[i] = 0;
             for (uint32_t q = 0; q < W; q++) cbits[i * W + q] = 0;
         }
         if (leader) s_claimed = 0;
@@ -773,6 +800,12 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) {
     }
     close_chunk(A, es);
 }
+
+__global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) { merge_lds_body<0>(A); }
+
+// The seq-hash k <= 32 variant is held to 64 VGPRs: eight waves per SIMD when
+// the LDS tables allow that many workgroups.
+__global__ __launch_bounds__(256, 8) void merge_lds_seq64_kernel(MergeArgs A) { merge_lds_body<1>(A); }
 
 // One thread per sub-bucket: runs of its units become work items of at most
 // `group_occ` occurrences (a single unit above that is an item of its own),

@@ -233,6 +233,20 @@ def test_dedup_modes_keep_results(gpu_ctx, mode):
         gpu_ctx.set_tuning()
 
 
+@pytest.mark.parametrize("threads,rec_cap,log", [(128, 128, 10), (256, 256, 10), (256, 512, 11)])
+def test_merge_workgroup_shapes(gpu_ctx, threads, rec_cap, log):
+    # the seq-hash k <= 32 merge variant under the workgroup sizes, record
+    # table sizes and LDS table sizes the tuner chooses between
+    wl = synth.config2(0.004)
+    _say(f"case merge threads={threads} rec_cap={rec_cap} lds_table_log={log} bytes={wl.data.size}")
+    gpu_ctx.set_tuning(merge_threads=threads, rec_cap=rec_cap, lds_table_log=log, unit_bytes=1000, table_pct=400,
+                       item_occ=512)
+    try:
+        _run_both(gpu_ctx, wl.data, 31, 15, wl.buckets_log, wl.second_buckets_log, exact=False)
+    finally:
+        gpu_ctx.set_tuning()
+
+
 def test_build_from_files_fastq(gpu_ctx, tmp_path):
     import gzip
 
