@@ -43,13 +43,14 @@ def kernel_bytes(cnt: np.ndarray, key_words: int) -> dict:
     """Algorithmic HBM bytes per launch of each kernel.
     cnt: [input bytes, records, sum record bases, k-mer occurrences, big, solid, ...].
     The super-k-mer arena is sum ceil(l/4) + 8 B of metadata per record; the
-    staging pass reads the input and writes the arena once, the scatter moves
-    it (read + write), the merge reads it and writes the solid entries."""
+    staging pass reads the input and writes every record once (into its
+    unit's region), the compaction moves the regions into the dense arena
+    (read + write), the merge reads it and writes the solid entries."""
     in_b, n_rec, sum_len, _, _, n_solid = (float(x) for x in cnt[:6])
     arena = (sum_len + 1.5 * n_rec) / 4.0 + 8.0 * n_rec
     return {
         "bucket_stage": in_b + arena,
-        "bucket_scatter": 2.0 * arena,
+        "unit_compact": 2.0 * arena,
         "merge": arena + n_solid * (8.0 * key_words + 8.0),
     }
 
@@ -58,8 +59,8 @@ def kernel_bytes(cnt: np.ndarray, key_words: int) -> dict:
 # scaled by the counters' reading for a 1 GiB copy (profiles/ holds the
 # commands and the per-kernel table)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_config2.json")
-TRAFFIC_KERNELS = {"bucket_stage": "gg::bucket_stage_kernel<16>", "bucket_scatter": "gg::bucket_scatter_kernel",
-                   "merge": "gg::merge_lds_kernel"}
+TRAFFIC_KERNELS = {"bucket_stage": "gg::bucket_stage_kernel<16>", "unit_compact": "gg::unit_compact_kernel",
+                   "merge": "gg::merge_lds_seq64_kernel"}
 
 
 def measured_traffic(workload: str, scale: float, kernel: str):
@@ -283,7 +284,7 @@ def serve_measurement(ctx, rate_hz: float, n_requests: int) -> dict:
 
 
 def measure(ctx, wl, steps: int, warmup: int, tune: bool, check: bool, threads: int, scale: float,
-            sample_cap: int = 256 << 20) -> dict:
+            sample_cap: int = 256 << 20, schedule: dict | None = None) -> dict:
     """Upload the workload, pick the schedule on a sample, gate it against the
     oracle on that sample, then time `steps` bucketing + merge passes over the
     device-resident input.  Leaves the last step's solid table in ctx."""
@@ -300,7 +301,13 @@ def measure(ctx, wl, steps: int, warmup: int, tune: bool, check: bool, threads: 
     sample = wl.data[:cut]
     tuning = None
     ctx.set_tuning()
-    if tune:
+    if schedule is not None:
+        # a fixed schedule (e.g. the one an earlier run's tuner picked)
+        from ggcat_amd.autotune import Schedule
+
+        Schedule(**schedule).apply(ctx)
+        tuning = {"schedule": schedule, "fixed": True}
+    elif tune:
         # schedule picked on a bounded sample of the same input; every
         # candidate's solid table must have the same digest
         from ggcat_amd.autotune import autotune
@@ -333,7 +340,7 @@ def measure(ctx, wl, steps: int, warmup: int, tune: bool, check: bool, threads: 
     n, kw = int(cnt[5]), (1 if (wl.k <= 32 and wl.hash_kind == 0) else 2)
     ms = np.mean(np.array(per_kernel), axis=0)
     kb = kernel_bytes(cnt, kw)
-    kern_ms = {"bucket_stage": ms[0], "bucket_scatter": ms[2], "merge": ms[3] + ms[4]}
+    kern_ms = {"bucket_stage": ms[0], "unit_compact": ms[2], "merge": ms[3] + ms[4]}
     dom = max(kern_ms, key=lambda x: kern_ms[x])
     achieved = kb[dom] / (kern_ms[dom] * 1e-3) / 1e9
     traffic, traffic_src = measured_traffic(wl.name, scale, dom)
@@ -385,6 +392,8 @@ def main() -> None:
     ap.add_argument("--no-k63", action="store_true", help="skip the k=63 rabin-karp128 measurement (config 4)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson request rate of the serving run (1/s)")
     ap.add_argument("--serve-requests", type=int, default=60)
+    ap.add_argument("--schedule", type=str, default=None,
+                    help="JSON object of schedule fields to use instead of the tuner's choice (config 2 line)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 1 GiB device-to-device copy first (HBM counter calibration under rocprofv3 --pmc)")
     a = ap.parse_args()
@@ -407,23 +416,25 @@ def main() -> None:
         ctx.free(dst)
         log("[bench] calibration copy (copy16_kernel): 1073741824 bytes read, 1073741824 bytes written")
     threads = min(os.cpu_count() or 1, 64)
-    out = measure(ctx, wl, a.steps, a.warmup, not a.no_autotune, not a.no_check, threads, a.scale)
+    out = measure(ctx, wl, a.steps, a.warmup, not a.no_autotune, not a.no_check, threads, a.scale,
+                  schedule=json.loads(a.schedule) if a.schedule else None)
     n = out["counters"]["solid_entries"]
-    # unitig chains inside the sub-buckets of the last step's graph (device
-    # kernels only; the sequences stay in HBM)
-    t = time.perf_counter()
-    n_unitigs, unitig_bytes = ctx.unitig_counts()
-    k_ms = float(ctx.timings()[7])
-    out["unitigs"] = {"unitigs": n_unitigs, "bytes": unitig_bytes, "kernels_ms": round(k_ms, 2),
-                      "call_ms": round((time.perf_counter() - t) * 1e3, 1),
-                      "solid_kmers_per_s": round(n / (k_ms / 1e3), 1) if k_ms > 0 else None}
-    # maximal unitigs of the whole solid set (joined across sub-buckets)
-    t = time.perf_counter()
-    n_max, max_bytes = ctx.unitig_counts(joined=True)
-    k_ms = float(ctx.timings()[7])
-    out["maximal_unitigs"] = {"unitigs": n_max, "bytes": max_bytes, "kernels_ms": round(k_ms, 2),
-                              "call_ms": round((time.perf_counter() - t) * 1e3, 1),
-                              "solid_kmers_per_s": round(n / (k_ms / 1e3), 1) if k_ms > 0 else None}
+    if wl.hash_kind == 0:  # unitigs walk k-mers, which seq-hash keys are
+        # unitig chains inside the sub-buckets of the last step's graph (device
+        # kernels only; the sequences stay in HBM)
+        t = time.perf_counter()
+        n_unitigs, unitig_bytes = ctx.unitig_counts()
+        k_ms = float(ctx.timings()[7])
+        out["unitigs"] = {"unitigs": n_unitigs, "bytes": unitig_bytes, "kernels_ms": round(k_ms, 2),
+                          "call_ms": round((time.perf_counter() - t) * 1e3, 1),
+                          "solid_kmers_per_s": round(n / (k_ms / 1e3), 1) if k_ms > 0 else None}
+        # maximal unitigs of the whole solid set (joined across sub-buckets)
+        t = time.perf_counter()
+        n_max, max_bytes = ctx.unitig_counts(joined=True)
+        k_ms = float(ctx.timings()[7])
+        out["maximal_unitigs"] = {"unitigs": n_max, "bytes": max_bytes, "kernels_ms": round(k_ms, 2),
+                                  "call_ms": round((time.perf_counter() - t) * 1e3, 1),
+                                  "solid_kmers_per_s": round(n / (k_ms / 1e3), 1) if k_ms > 0 else None}
     schedule = (out.get("tuning") or {}).get("schedule")
     if a.config == 2 and not a.no_k63:
         # the 128-bit key path at full size: k = 63, rabin-karp128, 15 Gbases

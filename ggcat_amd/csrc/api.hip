@@ -1007,9 +1007,13 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
 
     // resident workgroups per CU: what the 160 KiB LDS holds with the
     // kernel's static LDS, at most 8 (32 waves of 64)
-    // the seq-hash k <= 32 dedup path runs a variant with its switches compiled in
+    // the seq-hash k <= 32 dedup path runs a variant with its switches compiled in,
     const bool mode1 = key_words == 1 && W == 0 && M.dedup && M.tag_kmers && ctx->slot_words == 4;
-    void (*merge_kernel)(MergeArgs) = mode1 ? merge_lds_seq64_kernel : merge_lds_kernel;
+    // and the uncolored rabin-karp128 dedup path another
+    const bool mode2 = key_words == 2 && W == 0 && M.dedup && !M.tag_kmers && ctx->slot_words == 8 &&
+                       p.hash_kind == GGCAT_GPU_HASH_RK128;
+    void (*merge_kernel)(MergeArgs) =
+        mode1 ? merge_lds_seq64_kernel : mode2 ? merge_lds_rk128_kernel : merge_lds_kernel;
     hipFuncAttributes fa{};
     HIPCHK(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(merge_kernel)));
     const size_t wg_lds = (lds_bytes + fa.sharedSizeBytes + 511) / 512 * 512;  // allocation granule

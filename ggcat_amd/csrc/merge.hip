@@ -99,16 +99,19 @@ GG_DEV uint32_t hash128(uint64_t lo, uint64_t hi) { return tab_hash64(lo ^ (hi *
 
 // Kernel variants: MODE 0 reads every switch from the arguments; MODE 1 is
 // the seq-hash k <= 32 path (64-bit keys, 4-word slots, record dedup with one
-// k-mer position per thread, no colors) with those switches compiled in, which keeps the
-// 128-bit and color code out of its register allocation.
+// k-mer position per thread, no colors) and MODE 2 the uncolored
+// rabin-karp128 path (128-bit keys, 8-word slots, record dedup with each
+// distinct record rolled by one thread), with those switches compiled in,
+// which keeps the other paths' code out of their register allocation.
 template <int MODE>
 struct Mode {
-    static GG_DEV uint32_t key_words(const MergeArgs& A) { return MODE == 1 ? 1u : A.key_words; }
-    static GG_DEV uint32_t color_words(const MergeArgs& A) { return MODE == 1 ? 0u : A.color_words; }
-    static GG_DEV bool seq64(const MergeArgs& A) { return MODE == 1 || (A.kind == 0 && A.k <= 32); }
-    static GG_DEV bool dedup(const MergeArgs& A) { return MODE == 1 || A.dedup; }
-    static GG_DEV bool tag_kmers(const MergeArgs& A) { return MODE == 1 || A.tag_kmers; }
-    static GG_DEV uint32_t slot_words(const MergeArgs& A) { return MODE == 1 ? 4u : A.slot_words; }
+    static GG_DEV uint32_t key_words(const MergeArgs& A) { return MODE == 1 ? 1u : MODE == 2 ? 2u : A.key_words; }
+    static GG_DEV uint32_t color_words(const MergeArgs& A) { return MODE != 0 ? 0u : A.color_words; }
+    static GG_DEV bool seq64(const MergeArgs& A) { return MODE == 1 || (MODE == 0 && A.kind == 0 && A.k <= 32); }
+    static GG_DEV bool seq128(const MergeArgs& A) { return MODE == 0 && A.kind == 0; }
+    static GG_DEV bool dedup(const MergeArgs& A) { return MODE != 0 || A.dedup; }
+    static GG_DEV bool tag_kmers(const MergeArgs& A) { return MODE == 1 || (MODE == 0 && A.tag_kmers); }
+    static GG_DEV uint32_t slot_words(const MergeArgs& A) { return MODE == 1 ? 4u : MODE == 2 ? 8u : A.slot_words; }
     static constexpr uint32_t MAX_SW = MODE == 1 ? 4u : 8u;  // register copies of a slot
 };
 
@@ -217,7 +220,7 @@ GG_DEV void for_each_kmer(const MergeArgs& A, const uint32_t* slot, CB&& cb) {
             rc = ((rc << 2) | (c ^ 2)) & mask;
             if (i + 1 >= k) cb(fw < rc ? fw : rc, 0ULL, fw < rc, i + 1 - k, last);
         }
-    } else if (A.kind == 0) {
+    } else if (Mode<MODE>::seq128(A)) {
         const u128_t mask = (k == 64) ? ~(u128_t)0 : (((u128_t)1 << (2 * k)) - 1);
         u128_t fw = 0, rc = 0;
         uint32_t word = 0;
@@ -806,6 +809,8 @@ __global__ __launch_bounds__(256) void merge_lds_kernel(MergeArgs A) { merge_lds
 // The seq-hash k <= 32 variant is held to 64 VGPRs: eight waves per SIMD when
 // the LDS tables allow that many workgroups.
 __global__ __launch_bounds__(256, 8) void merge_lds_seq64_kernel(MergeArgs A) { merge_lds_body<1>(A); }
+
+__global__ __launch_bounds__(256) void merge_lds_rk128_kernel(MergeArgs A) { merge_lds_body<2>(A); }
 
 // One thread per sub-bucket: runs of its units become work items of at most
 // `group_occ` occurrences (a single unit above that is an item of its own),

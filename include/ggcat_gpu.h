@@ -224,7 +224,7 @@ typedef struct {
 int ggcat_gpu_set_tuning(ggcat_gpu_ctx* ctx, const ggcat_gpu_tuning* tuning);
 
 /* Device-side timings of the last calls, in milliseconds:
- * [0] bucketing staging pass, [1] scan, [2] bucketing scatter pass,
+ * [0] bucketing staging pass, [1] scan, [2] bucketing region compaction,
  * [3] merge (whole-item pass), [4] merge (partitioned passes),
  * [5] whole bucketing call, [6] whole merge call, [7] unitig kernels,
  * [8] reading the input of the last build_from_files (wall clock: files
