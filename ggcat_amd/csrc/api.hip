@@ -518,8 +518,10 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
         d_in = P_<uint8_t>(ctx->input);
     }
     // w == 16 forms window minima in registers; its tile of 960 windows (plus
-    // the w + 1 halo) is exactly 62 blocks of 16, one per lane
-    const bool regs16 = (w == 16 && p.k <= 47);
+    // the w + 1 halo) is exactly 62 blocks of 16, one per lane.  It keeps the
+    // low 32 bits of each window minimum, which must name the unique bit,
+    // bucket, second bucket and unit.
+    const bool regs16 = (w == 16 && p.k <= 47 && 1 + p.buckets_log + p.second_buckets_log + xb <= 32);
     const uint32_t T = regs16 ? 960 : TILE_T;
     const uint64_t n_tiles = (n_bytes + T - 1) / T;
     const uint64_t seg_words = n_tiles * T / 32 + 2;

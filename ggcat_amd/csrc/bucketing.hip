@@ -103,16 +103,31 @@ __host__ __device__ inline TileGeom tile_geom(uint32_t T, uint32_t w, uint32_t k
     uint32_t o = 0;
     g.off_g = o; o += 40 * 8;
     g.off_mm = o; o += regs ? 0 : g.NM * 8;
-    // wv / wp / fwd are padded after every 16 entries (Tile::WV, WP, FWD) so
-    // that lanes writing 16-window blocks hit distinct LDS banks
-    g.off_wv = o; o += (g.NWB * w + g.NWB * w / 16 + 1) * 8;
+    if (regs) {
+        // register path (w == 16): per window the low 32 bits of its minimum
+        // (all the record needs: unique bit, bucket, second bucket and unit
+        // bits) and the minimizer's offset in the window as a byte, per m-mer
+        // its strand as a bit; each lane's 16 windows padded by one word so
+        // that lanes' stores hit distinct LDS banks
+        g.off_wv = o; o += (g.NWB * 17 + 1) * 4;
+        o = (o + 7) / 8 * 8;
+    } else {
+        // wv / wp / fwd are padded after every 16 entries (Tile::WV, WP, FWD)
+        // so that lanes writing 16-window blocks hit distinct LDS banks
+        g.off_wv = o; o += (g.NWB * w + g.NWB * w / 16 + 1) * 8;
+    }
     g.off_pk = o; o += g.NB / 16 * 4;
     g.off_inv = o; o += g.NB / 32 * 4;
     g.off_seg = o; o += (g.NW / 64 + 2) * 8;
     g.off_ok = o; o += (g.NW / 64 + 2) * 8;
     g.off_last = o; o += (g.NW / 64 + 2) * 8;
-    g.off_wp = o; o += (g.NWB * w + 2 * (g.NWB * w / 16) + 2) * 2;
-    g.off_fwd = o; o += g.NM + 4 * (g.NM / 16) + 4;
+    if (regs) {
+        g.off_wp = o; o += g.NWB * 20 + 4;
+        g.off_fwd = o; o += (g.NWB + 2) * 2;
+    } else {
+        g.off_wp = o; o += (g.NWB * w + 2 * (g.NWB * w / 16) + 2) * 2;
+        g.off_fwd = o; o += g.NM + 4 * (g.NM / 16) + 4;
+    }
     g.bytes = (o + 15) / 16 * 16;
     return g;
 }
@@ -138,6 +153,10 @@ struct Tile {
     uint32_t* ok;
     uint32_t* last;
     uint64_t* gt;
+    bool regs;  // the w == 16 layout: wv32 / wp8 / fwd16
+    uint32_t* wv32;
+    uint8_t* wp8;
+    uint16_t* fwd16;
 
     GG_DEV uint32_t code_rel(int64_t r) const { return (pk[r >> 4] >> ((r & 15) * 2)) & 3; }
     GG_DEV bool inv_rel(int64_t r) const { return (inv[r >> 5] >> (r & 31)) & 1; }
@@ -162,6 +181,11 @@ struct Tile {
     GG_DEV uint64_t& WV(uint32_t x) const { return wv[x + (x >> 4)]; }
     GG_DEV uint16_t& WP(uint32_t x) const { return wp[x + 2 * (x >> 4)]; }
     GG_DEV uint8_t& FWD(uint32_t x) const { return fwd[x + 4 * (x >> 4)]; }
+    // either layout: window minimum (the w == 16 path keeps its low 32 bits),
+    // its m-mer position, and the strand flag of m-mer p
+    GG_DEV uint64_t wval(uint32_t x) const { return regs ? (uint64_t)wv32[x + (x >> 4)] : WV(x); }
+    GG_DEV uint32_t wpos(uint32_t x) const { return regs ? x + wp8[x + 4 * (x >> 4)] : WP(x); }
+    GG_DEV bool fwd_at(uint32_t p) const { return regs ? ((fwd16[p >> 4] >> (p & 15)) & 1u) != 0 : FWD(p) != 0; }
 };
 
 // 16 codes of relative positions [r, r+16) packed (position r at bits 0..1);
@@ -332,6 +356,9 @@ GG_DEV void tile_minima_w16(Tile& t) {
     const uint32_t p0 = q * 16;
     uint32_t segm = 0, okm = 0, lastm = 0;
     bool need_prev = false;
+    // the lane's first and last window minima (compared across lanes below)
+    uint64_t first_v = 0, last_v = 0;
+    uint32_t first_p = 0, last_p = 0;
     // every lane q <= NWB hashes m-mer block q once; lane q takes block q+1
     // from lane q+1 by a lane shift instead of hashing it again
     uint64_t sv[16], nv[16];
@@ -357,13 +384,7 @@ GG_DEV void tile_minima_w16(Tile& t) {
             sp[i] = p0 + i;
             fbits |= (fw < rc ? 1u : 0u) << i;
         }
-        // 16 flag bytes as four 32-bit stores (FWD(p0) is 4-byte aligned)
-        uint32_t* f4 = reinterpret_cast<uint32_t*>(&t.FWD(p0));
-#pragma unroll
-        for (int g = 0; g < 4; g++) {
-            const uint32_t b = fbits >> (4 * g);
-            f4[g] = (b & 1u) | ((b >> 1) & 1u) << 8 | ((b >> 2) & 1u) << 16 | ((b >> 3) & 1u) << 24;
-        }
+        t.fwd16[q] = (uint16_t)fbits;  // strands of m-mers p0 .. p0 + 15
     } else {
 #pragma unroll
         for (int i = 0; i < 16; i++) {
@@ -382,17 +403,25 @@ GG_DEV void tile_minima_w16(Tile& t) {
         {
             const uint32_t p1 = p0 + 16;
             uint64_t pv = 0, prv = 0;
-            uint32_t pp = 0, prp = 0, prp_pair = 0;
+            uint32_t pp = 0, prp = 0;
+            uint32_t offs[4] = {0u, 0u, 0u, 0u};  // minimizer offsets in the window, one byte each
+            uint32_t* wv_q = t.wv32 + q * 17;
 #pragma unroll
             for (int i = 0; i < 16; i++) {
                 const uint64_t v = nv[i];  // m-mer p1 + i
                 uint64_t rv = sv[i];
                 uint32_t rp = sp[i];
                 if (i > 0) mm_combine(sv[i], sp[i], pv, pp, rv, rp);
-                t.WV(p0 + i) = rv;
-                // positions in pairs as 32-bit stores (WP(p0) is 4-byte aligned)
-                if (i & 1) reinterpret_cast<uint32_t*>(&t.WP(p0))[i >> 1] = (prp_pair & 0xFFFFu) | (rp << 16);
-                prp_pair = rp;
+                wv_q[i] = (uint32_t)rv;
+                offs[i >> 2] |= (rp - (p0 + i)) << (8 * (i & 3));
+                if (i == 0) {
+                    first_v = rv;
+                    first_p = rp;
+                }
+                if (i == 15) {
+                    last_v = rv;
+                    last_p = rp;
+                }
                 // flags of window x = p0 + i (bases rb + x .. rb + x + k - 2)
                 const uint32_t x = p0 + i;
                 if (x >= 1 && x < G.NW) {
@@ -421,14 +450,16 @@ GG_DEV void tile_minima_w16(Tile& t) {
                     mm_combine(pv, pp, v, p1 + i, pv, pp);
                 }
             }
+            uint32_t* o4 = reinterpret_cast<uint32_t*>(t.wp8 + 20 * q);
+#pragma unroll
+            for (int j = 0; j < 4; j++) o4[j] = offs[j];
         }
     }
-    __syncthreads();
-    if (need_prev) {
-        // window p0 starts a group when its minimum differs from window p0 - 1
-        const uint64_t v1 = t.WV(p0), v0 = t.WV(p0 - 1);
-        if (v0 != v1 || ((v1 & 1) && t.WP(p0) != t.WP(p0 - 1))) segm |= 1u;
-    }
+    // window p0 starts a group when its minimum differs from window p0 - 1,
+    // the previous lane's last window (every lane takes part in the shift)
+    const uint64_t prev_v = lane_prev64(last_v);
+    const uint32_t prev_p = lane_prev32(last_p);
+    if (need_prev && (prev_v != first_v || ((first_v & 1) && first_p != prev_p))) segm |= 1u;
     // bitmap words past the last block stay zero
     const uint32_t npieces = (G.NW / 64 + 2) * 4;
     for (uint32_t pc = q; pc < npieces; pc += 64) {
@@ -511,6 +542,10 @@ GG_DEV void tile_init(Tile& t, const BucketArgs* A, uint64_t tile, uint8_t* lds,
     t.pk = reinterpret_cast<uint32_t*>(lds + t.G.off_pk);
     t.inv = reinterpret_cast<uint32_t*>(lds + t.G.off_inv);
     t.fwd = lds + t.G.off_fwd;
+    t.regs = regs;
+    t.wv32 = reinterpret_cast<uint32_t*>(lds + t.G.off_wv);
+    t.wp8 = lds + t.G.off_wp;
+    t.fwd16 = reinterpret_cast<uint16_t*>(lds + t.G.off_fwd);
     t.seg = reinterpret_cast<uint32_t*>(lds + t.G.off_seg);
     t.ok = reinterpret_cast<uint32_t*>(lds + t.G.off_ok);
     t.last = reinterpret_cast<uint32_t*>(lds + t.G.off_last);
@@ -633,15 +668,15 @@ GG_DEV RecInfo rec_info(const Tile& t, uint32_t x) {
         }
     }
     if (a < 0) return r;  // group starts before the halo: deferred
-    const uint64_t v = t.WV(x);
+    const uint64_t v = t.wval(x);
     const bool unique = v & 1;
     const bool include_first = invalid_abs(t, a - 1);
     const bool include_last = t.bit(t.last, x);
     r.start = include_first ? a : a - 1;
     r.end = include_last ? bwin + k - 1 : bwin + k;
-    const uint32_t prel = t.WP(x);
+    const uint32_t prel = t.wpos(x);
     const int64_t mmabs = t.wlo + prel;
-    r.rc = A.canonical && unique && !t.FWD(prel);
+    r.rc = A.canonical && unique && !t.fwd_at(prel);
     uint32_t minpos = 0;
     if (unique) minpos = r.rc ? (uint32_t)(r.end - mmabs - m) : (uint32_t)(mmabs - r.start);
     const uint32_t flags = ((include_first ? 1u : 0u) << (r.rc ? 1 : 0)) | ((include_last ? 1u : 0u) << (r.rc ? 0 : 1));
@@ -654,11 +689,11 @@ GG_DEV RecInfo rec_info(const Tile& t, uint32_t x) {
     uint32_t cross = 0;
     if (unique && A.xb) {
         if (!include_first) {
-            const uint64_t vl = t.WV(a - t.wlo - 1);
+            const uint64_t vl = t.wval((uint32_t)(a - t.wlo - 1));
             if (sub_of(vl, A.b, A.sb) == sub && unit_of(vl, A.b, A.sb, A.xb) != r.unit) cross |= r.rc ? META_XEND : META_XBEGIN;
         }
         if (!include_last) {
-            const uint64_t vr = t.WV(x + 1);
+            const uint64_t vr = t.wval(x + 1);
             if (sub_of(vr, A.b, A.sb) == sub && unit_of(vr, A.b, A.sb, A.xb) != r.unit) cross |= r.rc ? META_XBEGIN : META_XEND;
         }
     }
@@ -843,7 +878,7 @@ __global__ __launch_bounds__(64) void bucket_stage_kernel(BucketArgs A) {
                 const uint32_t d = atomicAdd(A.defer_count, 1u);
                 if (d < A.n_tiles) {
                     A.defer[3 * (uint64_t)d] = (uint64_t)(t.t0 + (int64_t)x - w);
-                    A.defer[3 * (uint64_t)d + 1] = t.WV(x);
+                    A.defer[3 * (uint64_t)d + 1] = t.wval(x);
                     A.defer[3 * (uint64_t)d + 2] = t.bit(t.last, x) ? 1 : 0;
                 } else {
                     atomicOr(A.error_flag, 4u);

@@ -61,6 +61,13 @@ GG_DEV uint32_t lane_next32(uint32_t x) {
 GG_DEV uint64_t lane_next64(uint64_t x) {
     return (uint64_t)lane_next32((uint32_t)x) | ((uint64_t)lane_next32((uint32_t)(x >> 32)) << 32);
 }
+// Value of the previous lane (lane - 1): DPP wave_shr:1; lane 0 gets 0.
+GG_DEV uint32_t lane_prev32(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x138, 0xF, 0xF, false);
+}
+GG_DEV uint64_t lane_prev64(uint64_t x) {
+    return (uint64_t)lane_prev32((uint32_t)x) | ((uint64_t)lane_prev32((uint32_t)(x >> 32)) << 32);
+}
 
 GG_DEV uint32_t tab_hash64(uint64_t x) {
     x ^= x >> 29;

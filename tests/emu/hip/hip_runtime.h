@@ -111,11 +111,11 @@ template <class T> inline T __shfl(T v, int src) {
 inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int, int, bool) {
     unsigned w = threadIdx.x / 64, l = threadIdx.x % 64;
     static int slots[16][64];
-    if (ctrl != 0x130) std::abort();
+    if (ctrl != 0x130 && ctrl != 0x138) std::abort();  // wave_shl:1 / wave_shr:1
     emu::wbar[w]->arrive_and_wait();
     slots[w][l] = src;
     emu::wbar[w]->arrive_and_wait();
-    const int r = (l + 1 < 64) ? slots[w][l + 1] : old;
+    const int r = ctrl == 0x130 ? ((l + 1 < 64) ? slots[w][l + 1] : old) : (l >= 1 ? slots[w][l - 1] : old);
     emu::wbar[w]->arrive_and_wait();
     return r;
 }
