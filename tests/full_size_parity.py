@@ -2,9 +2,11 @@
 m=15, s=2): the GPU's solid table against the oracle's, compared through
 order-free digests (entry count, xor and sum of a 64-bit mix of sub-bucket,
 key, multiplicity and flags, multiplicity sum).  Not collected by pytest: it
-needs a GPU, ~1.5 minutes and ~90 GB of host memory for the oracle.
+needs a GPU, ~1.5 minutes and ~90 GB of host memory for the oracle.  An
+optional JSON schedule (ggcat_gpu_set_tuning fields, e.g. the one bench.py's
+tuner picks) runs the GPU side under that schedule.
 
-    python tests/full_size_parity.py
+    python tests/full_size_parity.py ['{"unit_bytes": 1000, ...}']
 """
 import json
 import os
@@ -27,9 +29,16 @@ def digest(sub, lo, hi, mult, flags):
 wl = synth.config2(1.0)
 P = Params(31, 15, wl.buckets_log, wl.second_buckets_log, 0, 0, 2)
 ctx = GpuContext(0)
+schedule = json.loads(sys.argv[1]) if len(sys.argv) > 1 else {}
+if schedule:
+    ctx.set_tuning(**schedule)
+print("[parity] schedule", schedule or "default", flush=True)
 p, n = ctx.upload(wl.data)
 ctx.bucketing(None, P, device_ptr=p, n_bytes=n)
 ctx.kmers_merge()
+c = ctx.counters()
+print(f"[parity] gpu records {int(c[1])}, items in partitioned passes {int(c[4])}, merge passes {int(c[7])}",
+      flush=True)
 s = ctx.solid()
 g = digest(s.sub, s.keys[:, 0], np.zeros_like(s.keys[:, 0]), s.mult, s.flags)
 print("[parity] gpu", g, flush=True)
@@ -43,5 +52,6 @@ print(f"[parity] oracle merge done {time.time() - t:.1f} s", flush=True)
 os_ = o.solid()
 e = digest(os_["sub"], os_["key_lo"], os_["key_hi"], os_["mult"], os_["flags"])
 print("[parity] oracle", e, flush=True)
-print(json.dumps({"equal": g == e, "gpu": g, "oracle": e}), flush=True)
+print(json.dumps({"equal": g == e, "gpu": g, "oracle": e, "schedule": schedule, "split_items": int(c[4])}),
+      flush=True)
 sys.exit(0 if g == e else 1)

@@ -243,3 +243,62 @@ def build_both_readers(ctx, paths, opts, block: int, **tuning) -> list:
         s = (colored_solid_array(ctx.solid(), ctx.solid_colors()) if opts.colors else solid_array(ctx.solid()))
         out.append((int(c[8]), int(c[9]), record_tuples(ctx.records()), s))
     return out
+
+
+def query_by_lookup(queries: np.ndarray, keys: np.ndarray, bitmaps: np.ndarray | None, k: int):
+    """Query counts by direct lookup (seq-hash keys, k <= 32): for each query
+    record of the joined layout, query_kmers = len - (k - 1), matched = its
+    all-ACGT k-mers whose canonical 2-bit key is in `keys`, and per color the
+    matched k-mers whose graph k-mer's color bitmap holds that color.  keys
+    must be sorted, bitmaps (n, W) uint32 in the same order (or None)."""
+    q = np.ascontiguousarray(queries, np.uint8)
+    nl = np.flatnonzero(q == 10)
+    nq = nl.size
+    starts = np.concatenate(([0], nl[:-1] + 1))
+    lens = nl - starts
+    qk = np.maximum(lens.astype(np.int64) - (k - 1), 0).astype(np.uint64)
+    up = q & 0xDF
+    valid = (up == ord("A")) | (up == ord("C")) | (up == ord("G")) | (up == ord("T"))
+    code = ((q >> 1) & 3).astype(np.uint64)
+    n = q.size - k + 1
+    if n <= 0:
+        return qk, np.zeros(nq, np.uint64), None
+    fw = np.zeros(n, np.uint64)
+    rc = np.zeros(n, np.uint64)
+    for i in range(k):
+        fw |= code[i:i + n] << np.uint64(2 * i)
+        rc |= (code[k - 1 - i:k - 1 - i + n] ^ np.uint64(2)) << np.uint64(2 * i)
+    key = np.minimum(fw, rc)
+    bad = np.concatenate(([0], np.cumsum(~valid, dtype=np.int64)))
+    ok = (bad[k:k + n] - bad[:n]) == 0
+    pos = np.searchsorted(keys, key)
+    pos_c = np.minimum(pos, max(keys.size - 1, 0))
+    hit = ok & (keys.size > 0) & (keys[pos_c] == key) if keys.size else np.zeros(n, bool)
+    qid = np.concatenate(([0], np.cumsum(q[:n - 1] == 10, dtype=np.int64)))  # record holding position p
+    mt = np.bincount(qid[hit], minlength=nq).astype(np.uint64)
+    cc = None
+    if bitmaps is not None:
+        W = bitmaps.shape[1]
+        rows = bitmaps[pos_c[hit]]
+        hq = qid[hit]
+        cc = np.zeros((nq, 32 * W), np.uint64)
+        for w in range(W):
+            col = rows[:, w]
+            for b in range(32):
+                m = ((col >> np.uint32(b)) & 1).astype(bool)
+                if m.any():
+                    cc[:, 32 * w + b] = np.bincount(hq[m], minlength=nq)
+    return qk, mt, cc
+
+
+def graph_index(keys: np.ndarray, bitmaps: np.ndarray | None):
+    """Solid entries (a boundary k-mer can be held by two sub-buckets) ->
+    sorted unique keys and, per key, the OR of its entries' color bitmaps."""
+    u, first, inv, cnt = np.unique(keys, return_index=True, return_inverse=True, return_counts=True)
+    if bitmaps is None:
+        return u, None
+    bm = bitmaps[first].copy()
+    dup = cnt[inv] > 1
+    if dup.any():
+        np.bitwise_or.at(bm, inv[dup], bitmaps[dup])
+    return u, bm

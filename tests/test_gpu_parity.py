@@ -398,6 +398,29 @@ def test_query_colored_and_serving_mix(gpu_ctx):
     assert rep.max_admitted >= 2  # requests overlapped on the device
 
 
+def test_query_serving_graph_full_scale(gpu_ctx):
+    # the serving workload at its real size: the 256-genome colored graph
+    # (665 Mbases, ~30 M solid k-mers, 256 colors, 8 bitmap words) and a
+    # request of 2000 queries of 100..10000 bp; per-query and per-color counts
+    # against a direct lookup over the same solid k-mers (the graph itself is
+    # checked against the oracle at smaller sizes)
+    from tests.helpers import graph_index, query_by_lookup
+    wl = synth.config3(1.0)
+    p = _params(wl.k, wl.m, wl.buckets_log, wl.second_buckets_log, wl.min_mult)
+    _say(f"case serving graph full scale bytes={wl.data.size}")
+    gpu_ctx.bucketing(wl.data, p, colors=wl.colors)
+    gpu_ctx.kmers_merge()
+    gpu_ctx.index_build()
+    q = synth.queries(synth.config3_genomes(1.0), 77, 2000, 100, 10_000, 0.02)
+    qk, mt, cc = gpu_ctx.query(q)
+    s = gpu_ctx.solid()
+    keys, bm = graph_index(s.keys[:, 0], gpu_ctx.solid_colors())
+    wqk, wmt, wcc = query_by_lookup(q, keys, bm, wl.k)
+    assert np.array_equal(qk, wqk) and np.array_equal(mt, wmt)
+    assert np.array_equal(cc.astype(np.uint64), wcc[:, :cc.shape[1]])
+    assert int(mt.sum()) > 0 and cc.shape[1] == 256
+
+
 @pytest.mark.parametrize("k,m,rep,err", [(31, 15, 0.2, 0.005), (21, 9, 0.0, 0.01), (45, 21, 0.2, 0.005)])
 def test_unitigs_and_extension_flags(gpu_ctx, k, m, rep, err):
     from tests.test_unitigs import canon_unitigs

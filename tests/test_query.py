@@ -67,6 +67,29 @@ def test_oracle_query_matches_bruteforce(s):
     assert int(got[1].sum()) > 0
 
 
+def test_lookup_query_matches_oracle():
+    # the vectorized lookup counter the GPU tests use at full scale, pinned
+    # against the oracle's query on a colored graph
+    from tests.helpers import bitmaps_from_sets, graph_index, query_by_lookup
+    k = 31
+    anc = synth.genome(32, 8000)
+    gs = [synth.descendant(anc, 50 + i, 0.02, 0.002) for i in range(6)]
+    data, starts = synth.contigs_joined(gs, 3, 1500, 2500)
+    colors = np.array([5, 0, 3, 1, 2, 4], np.uint32)
+    r = O.bucketing_colored(data, k, 15, 4, 1, starts, colors, threads=2)
+    r.merge(k, 4, 1, 0, 1, 2)
+    q = synth.queries(gs, 9, 40, 10, 600, 0.03)
+    q = np.concatenate([q, np.frombuffer(b"acgtacgtacgtacgtacgtacgtacgtacgNNacgt\n\nA\n", np.uint8)])
+    s = r.solid()
+    off, cols = r.solid_colors()
+    keys, bm = graph_index(s["key_lo"], bitmaps_from_sets(off, cols, 1))
+    got = query_by_lookup(q, keys, bm, k)
+    want = r.query(q, k, 0, 6)
+    assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+    assert np.array_equal(got[2][:, :6], want[2])
+    assert int(got[1].sum()) > 0
+
+
 def test_csv_format_known_answer():
     qk = np.array([120, 0, 3], np.uint64)
     mt = np.array([64, 0, 1], np.uint64)
