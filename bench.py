@@ -41,17 +41,21 @@ def make_workload(config: int, scale: float):
 
 def kernel_bytes(cnt: np.ndarray, key_words: int) -> dict:
     """Algorithmic HBM bytes per launch of each kernel.
-    cnt: [input bytes, records, sum record bases, k-mer occurrences, big, solid, ...].
+    cnt: [input bytes, records, sum record bases, k-mer occurrences, big, solid,
+    ..., stored records (counter 10)].
     The super-k-mer arena is sum ceil(l/4) + 8 B of metadata per record; the
     staging pass reads the input and writes every record once (into its
-    unit's region), the compaction moves the regions into the dense arena
-    (read + write), the merge reads it and writes the solid entries."""
+    unit's region); the compaction reads the regions and writes the stored
+    records (identical records of a unit once) into the dense arena; the
+    merge reads the stored arena and writes the solid entries."""
     in_b, n_rec, sum_len, _, _, n_solid = (float(x) for x in cnt[:6])
+    n_stored = float(cnt[10]) if len(cnt) > 10 and cnt[10] else n_rec
     arena = (sum_len + 1.5 * n_rec) / 4.0 + 8.0 * n_rec
+    stored = arena * (n_stored / n_rec) if n_rec else 0.0
     return {
         "bucket_stage": in_b + arena,
-        "unit_compact": 2.0 * arena,
-        "merge": arena + n_solid * (8.0 * key_words + 8.0),
+        "unit_compact": arena + stored,
+        "merge": stored + n_solid * (8.0 * key_words + 8.0),
     }
 
 
@@ -59,8 +63,9 @@ def kernel_bytes(cnt: np.ndarray, key_words: int) -> dict:
 # scaled by the counters' reading for a 1 GiB copy (profiles/ holds the
 # commands and the per-kernel table)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_config2.json")
-TRAFFIC_KERNELS = {"bucket_stage": "gg::bucket_stage_kernel<16>", "unit_compact": "gg::unit_compact_kernel",
-                   "merge": "gg::merge_lds_seq64_kernel"}
+TRAFFIC_KERNELS = {"bucket_stage": ("gg::bucket_stage_kernel<16>",),
+                   "unit_compact": ("gg::unit_dedup_kernel", "gg::unit_compact_kernel"),
+                   "merge": ("gg::merge_lds_seq64_kernel",)}
 
 
 def measured_traffic(workload: str, scale: float, kernel: str):
@@ -68,10 +73,12 @@ def measured_traffic(workload: str, scale: float, kernel: str):
     the committed counter runs of the same workload, else (None, None)."""
     if not workload.startswith("config2") or scale != 1.0 or not os.path.exists(TRAFFIC_FILE):
         return None, None
-    t = json.load(open(TRAFFIC_FILE)).get(TRAFFIC_KERNELS.get(kernel, ""))
-    if not t:
+    table = json.load(open(TRAFFIC_FILE))
+    parts = [table.get(name) for name in TRAFFIC_KERNELS.get(kernel, ())]
+    if not parts or not all(parts):
         return None, None
-    return round(t["read_gb"] + t["write_gb"], 3), "profiles/hbm_traffic_config2.md (rocprofv3 --pmc, GB per launch)"
+    gb = sum(t["read_gb"] + t["write_gb"] for t in parts)
+    return round(gb, 3), "profiles/hbm_traffic_config2.md (rocprofv3 --pmc, GB per launch)"
 
 
 def kernel_rooflines(kb: dict, kern_ms: dict, workload: str, scale: float) -> dict:
@@ -283,11 +290,23 @@ def serve_measurement(ctx, rate_hz: float, n_requests: int) -> dict:
             "graph_build_s": round(t_build, 3), "index_build_ms": round(t_index * 1e3, 1)}
 
 
+def shape_sample(config: int, scale: float, full_bytes: int, cap: int) -> tuple[np.ndarray, str]:
+    """A read set of the same configuration with its genome scaled so that it
+    holds about `cap` bytes: coverage, read length, error rate and repeat
+    content as at full size, so work items (and the items that need
+    partitioned passes) look like the full-size ones."""
+    f = min(1.0, cap / max(full_bytes, 1))
+    s = make_workload(config, scale * f)
+    return s.data, s.name
+
+
 def measure(ctx, wl, steps: int, warmup: int, tune: bool, check: bool, threads: int, scale: float,
-            sample_cap: int = 256 << 20, schedule: dict | None = None) -> dict:
-    """Upload the workload, pick the schedule on a sample, gate it against the
-    oracle on that sample, then time `steps` bucketing + merge passes over the
-    device-resident input.  Leaves the last step's solid table in ctx."""
+            sample: np.ndarray | None = None, sample_name: str | None = None,
+            schedule: dict | None = None) -> dict:
+    """Upload the workload, pick the schedule on a sample (default: the first
+    256 MiB of the reads), gate it against the oracle on that sample, then
+    time `steps` bucketing + merge passes over the device-resident input.
+    Leaves the last step's solid table in ctx."""
     from ggcat_amd.native import Params
 
     params = Params(wl.k, wl.m, wl.buckets_log, wl.second_buckets_log, 0, wl.hash_kind, wl.min_mult)
@@ -295,10 +314,12 @@ def measure(ctx, wl, steps: int, warmup: int, tune: bool, check: bool, threads: 
     dptr, nbytes = ctx.upload(wl.data)
     h2d_s = time.perf_counter() - t
     log(f"[bench] {wl.name}: input uploaded to HBM in {h2d_s * 1e3:.1f} ms")
-    cut = min(wl.data.size, sample_cap)
-    while cut < wl.data.size and wl.data[cut - 1] != ord("\n"):
-        cut += 1
-    sample = wl.data[:cut]
+    if sample is None:
+        cut = min(wl.data.size, 256 << 20)
+        while cut < wl.data.size and wl.data[cut - 1] != ord("\n"):
+            cut += 1
+        sample, sample_name = wl.data[:cut], f"first {cut} bytes of {wl.name}"
+    cut = int(sample.size)
     tuning = None
     ctx.set_tuning()
     if schedule is not None:
@@ -314,11 +335,13 @@ def measure(ctx, wl, steps: int, warmup: int, tune: bool, check: bool, threads: 
 
         t = time.perf_counter()
         tuning = autotune(ctx, sample, params, colored=False)
+        tuning["sample"] = sample_name
         log(f"[bench] autotune: {tuning['tried']} schedules on {cut} bytes in {time.perf_counter() - t:.1f} s -> "
             f"{tuning['schedule']}")
     gate = None
     if check:
         gate = parity_gate(ctx, wl, params, sample, threads)
+        gate["sample"] = sample_name
         log(f"[bench] parity gate ok (schedule that is timed): {gate}")
 
     def step():
@@ -368,8 +391,9 @@ def measure(ctx, wl, steps: int, warmup: int, tune: bool, check: bool, threads: 
                      "algorithmic_gb": round(kb[dom] / 1e9, 3), "traffic_source": traffic_src},
         "kernels_ms": {k: round(float(v), 3) for k, v in kern_ms.items()},
         "kernels_roofline": kernel_rooflines(kb, kern_ms, wl.name, scale),
-        "counters": {"records": int(cnt[1]), "record_bases": int(cnt[2]), "kmer_occurrences": int(cnt[3]),
-                     "split_items": int(cnt[4]), "solid_entries": int(n), "merge_passes": int(cnt[7])},
+        "counters": {"records": int(cnt[1]), "stored_records": int(cnt[10]), "record_bases": int(cnt[2]),
+                     "kmer_occurrences": int(cnt[3]), "split_items": int(cnt[4]), "solid_entries": int(n),
+                     "merge_passes": int(cnt[7])},
         "h2d_ms": round(h2d_s * 1e3, 1),
         "tuning": tuning,
         "parity_gate": gate,
@@ -416,8 +440,12 @@ def main() -> None:
         ctx.free(dst)
         log("[bench] calibration copy (copy16_kernel): 1073741824 bytes read, 1073741824 bytes written")
     threads = min(os.cpu_count() or 1, 64)
+    # the schedule is tuned and gated on the same configuration with its genome
+    # scaled to a 256 MiB read set (same coverage)
+    sample, sample_name = shape_sample(a.config, a.scale, wl.data.size, 256 << 20)
     out = measure(ctx, wl, a.steps, a.warmup, not a.no_autotune, not a.no_check, threads, a.scale,
-                  schedule=json.loads(a.schedule) if a.schedule else None)
+                  sample=sample, sample_name=sample_name, schedule=json.loads(a.schedule) if a.schedule else None)
+    del sample
     n = out["counters"]["solid_entries"]
     if wl.hash_kind == 0:  # unitigs walk k-mers, which seq-hash keys are
         # unitig chains inside the sub-buckets of the last step's graph (device
@@ -442,8 +470,10 @@ def main() -> None:
         wl4 = make_workload(4, a.scale)
         log(f"[bench] workload {wl4.name}: {wl4.data.size} bytes, {wl4.bases} bases "
             f"({time.perf_counter() - t:.1f} s to generate)")
+        sample, sample_name = shape_sample(4, a.scale, wl4.data.size, 64 << 20)
         out["config4"] = measure(ctx, wl4, max(1, min(a.steps, 2)), 1, not a.no_autotune, not a.no_check, threads,
-                                 a.scale, sample_cap=64 << 20)
+                                 a.scale, sample=sample, sample_name=sample_name)
+        del sample
         del wl4
     # the file-to-table run uses the same read set, so the same schedule
     ctx.set_tuning(**schedule) if schedule else ctx.set_tuning()

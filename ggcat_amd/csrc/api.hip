@@ -51,7 +51,8 @@ struct ggcat_gpu_ctx {
     // bucketing buffers
     DevBuf input, seg_bits, unit_count, unit_cursor, unit_off, unit_kmers, scan_part, slots, pool, pool_cursor,
         slot_color, color_starts, colors, error_flag, stage_slots, stage_unit, stage_rank, stage_color, stage_cursor, defer,
-        defer_count, kmer_total, spill_slots, spill_color;  // stage_*: the unit regions, the spill list's units and ranks
+        defer_count, kmer_total, spill_slots, spill_color,  // stage_*: the unit regions, the spill list's units and ranks
+        unit_dcount, unit_dregion;  // per unit after deduplication
     // merge buffers
     DevBuf items, n_items, work, parts, redo, redo_count, out_key, out_val, out_sub, chunk_cursor, chunk_fill_dev,
         fix_key, fix_val, fix_sub, fix_cursor, out_colors, fix_colors;
@@ -60,6 +61,7 @@ struct ggcat_gpu_ctx {
     bool have_records = false, have_solid = false;
     uint32_t n_sub = 0, n_units = 0, xb = 0, slot_words = 0, key_words = 1;
     uint64_t n_records = 0, pool_words = 0, n_solid = 0, n_input = 0, kmer_occ = 0, n_big = 0, n_fix = 0;
+    uint64_t n_stored = 0;  // arena slots: records with identical copies stored once
     uint64_t n_spill = 0;  // records of the last bucketing past their unit's region
     uint32_t n_chunks_used = 0;  // output chunks of the last merge
     uint32_t n_passes = 0;  // kernel passes of the last merge
@@ -434,6 +436,7 @@ void ggcat_gpu_destroy(ggcat_gpu_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevBuf* bufs[] = {&ctx->input, &ctx->seg_bits, &ctx->unit_count, &ctx->unit_cursor, &ctx->unit_off,
                       &ctx->unit_kmers, &ctx->scan_part, &ctx->slots, &ctx->pool, &ctx->pool_cursor,
+                      &ctx->unit_dcount, &ctx->unit_dregion,
                       &ctx->slot_color, &ctx->color_starts, &ctx->colors, &ctx->error_flag, &ctx->items,
                       &ctx->n_items, &ctx->work, &ctx->parts, &ctx->redo, &ctx->redo_count, &ctx->out_key,
                       &ctx->out_val, &ctx->out_sub, &ctx->chunk_cursor, &ctx->chunk_fill_dev, &ctx->fix_key,
@@ -530,6 +533,8 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     ENSURE(ctx->unit_cursor, (size_t)n_units * 8);
     ENSURE(ctx->unit_kmers, (size_t)n_units * 4);
     ENSURE(ctx->unit_off, ((size_t)n_units + 1) * 8);
+    ENSURE(ctx->unit_dcount, (size_t)n_units * 4);
+    ENSURE(ctx->unit_dregion, (size_t)n_units * 4);
     ENSURE(ctx->pool_cursor, 8);
     ENSURE(ctx->error_flag, 8);
     if (n_colors) {
@@ -557,6 +562,8 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     A.unit_ck = P_<unsigned long long>(ctx->unit_cursor);
     A.sub_off = P_<uint64_t>(ctx->unit_off);
     A.sub_kmers = P_<uint32_t>(ctx->unit_kmers);
+    A.unit_dcount = P_<uint32_t>(ctx->unit_dcount);
+    A.unit_dregion = P_<uint32_t>(ctx->unit_dregion);
     A.slot_words = slot_words;
     A.pool_cursor = P_<unsigned long long>(ctx->pool_cursor);
     A.color_starts = n_colors ? P_<uint64_t>(ctx->color_starts) : nullptr;
@@ -567,7 +574,7 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     const TileGeom G = tile_geom(T, w, p.k, p.m, regs16);
     ENSURE(ctx->stage_cursor, 8);
     ENSURE(ctx->defer_count, 8);
-    ENSURE(ctx->kmer_total, 8);
+    ENSURE(ctx->kmer_total, 16);  // k-mer occurrences, records
     ENSURE(ctx->defer, std::max<uint64_t>(n_tiles, 1) * 24);
     A.spill_cursor = P_<unsigned long long>(ctx->stage_cursor);
     A.defer = P_<uint64_t>(ctx->defer);
@@ -671,15 +678,24 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     ctx->pool_words = pool_used;
     ctx->n_spill = staged;
 
-    // Unit offsets, then pass 2: every staged record to its unit's slots.
-    const int rs = device_scan(ctx, A.sub_count, P_<uint64_t>(ctx->unit_off), n_units);
+    // Pass 2: identical records of a unit counted once, unit offsets of the
+    // stored records, then every stored record to its unit's slots.
+    if (n_units) {
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(n_units, (uint64_t)ctx->n_cu * 64);
+        hipLaunchKernelGGL(unit_dedup_kernel, dim3(grid), dim3(64), dedup_lds_bytes(slot_words, n_colors != 0),
+                           ctx->stream, A, n_units);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipEventRecord(ctx->ev[15], ctx->stream));
+    const int rs = device_scan(ctx, A.unit_dcount, P_<uint64_t>(ctx->unit_off), n_units);
     if (rs != GGCAT_GPU_OK) return rs;
     HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
-    uint64_t total = 0, kmer_total = 0;
+    uint64_t total = 0, kmer_total[2] = {0, 0};
     HIPCHK(hipMemcpyAsync(&total, P_<uint64_t>(ctx->unit_off) + n_units, 8, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(&kmer_total, A.kmer_total, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(kmer_total, A.kmer_total, 16, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    ctx->n_records = total;
+    ctx->n_records = kmer_total[1];
+    ctx->n_stored = total;
     ENSURE(ctx->slots, std::max<uint64_t>(total, 1) * slot_words * 4);
     if (n_colors) ENSURE(ctx->slot_color, std::max<uint64_t>(total, 1) * 4);
     A.slots = P_<uint32_t>(ctx->slots);
@@ -702,12 +718,14 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     float t;
     HIPCHK(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[1]));
     ctx->ms[0] = t;
-    HIPCHK(hipEventElapsedTime(&t, ctx->ev[1], ctx->ev[2]));
+    HIPCHK(hipEventElapsedTime(&t, ctx->ev[15], ctx->ev[2]));
     ctx->ms[1] = t;
+    float t_dedup;
+    HIPCHK(hipEventElapsedTime(&t_dedup, ctx->ev[1], ctx->ev[15]));
     HIPCHK(hipEventElapsedTime(&t, ctx->ev[3], ctx->ev[4]));
-    ctx->ms[2] = t;
+    ctx->ms[2] = t + t_dedup;
     ctx->ms[5] = ctx->ms[0] + ctx->ms[1] + ctx->ms[2];
-    ctx->kmer_occ = kmer_total;  // sizes the merge output
+    ctx->kmer_occ = kmer_total[0];  // sizes the merge output
     ctx->have_records = true;
     return GGCAT_GPU_OK;
 }
@@ -815,19 +833,21 @@ int ggcat_gpu_records_copy(ggcat_gpu_ctx* ctx, uint32_t* sub, uint32_t* len, uin
     if (!ctx->have_records) return fail(ctx, GGCAT_GPU_ESTATE, "no bucketing result");
     HIPCHK(hipSetDevice(ctx->device));
     const uint32_t SW = ctx->slot_words;
-    std::vector<uint32_t> slots(ctx->n_records * SW), pool(ctx->pool_words), cols;
+    std::vector<uint32_t> slots(ctx->n_stored * SW), pool(ctx->pool_words), cols;
     std::vector<uint64_t> uoff(ctx->n_units + 1);
     if (!slots.empty()) HIPCHK(hipMemcpy(slots.data(), ctx->slots.p, slots.size() * 4, hipMemcpyDeviceToHost));
     if (!pool.empty()) HIPCHK(hipMemcpy(pool.data(), ctx->pool.p, pool.size() * 4, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(uoff.data(), ctx->unit_off.p, uoff.size() * 8, hipMemcpyDeviceToHost));
-    if (ctx->colored && ctx->n_records) {
-        cols.resize(ctx->n_records);
+    if (ctx->colored && ctx->n_stored) {
+        cols.resize(ctx->n_stored);
         HIPCHK(hipMemcpy(cols.data(), ctx->slot_color.p, cols.size() * 4, hipMemcpyDeviceToHost));
     }
     static const char LET[4] = {'A', 'C', 'T', 'G'};
-    uint64_t bpos = 0;
+    // every stored record as many times as its multiplicity
+    uint64_t bpos = 0, out_r = 0;
     uint32_t u = 0;
-    for (uint64_t r = 0; r < ctx->n_records; r++) {
+    for (uint64_t s = 0; s < ctx->n_stored; s++) {
+        const uint64_t r = s;
         while (u + 1 < uoff.size() && uoff[u + 1] <= r) u++;
         const uint32_t* sl = slots.data() + r * SW;
         uint32_t meta = sl[0];
@@ -840,16 +860,20 @@ int ggcat_gpu_records_copy(ggcat_gpu_ctx* ctx, uint32_t* sub, uint32_t* len, uin
             L = meta & 0xFF;
             bw = sl + 1;
         }
-        if (sub) sub[r] = u >> ctx->xb;
-        if (len) len[r] = L;
-        if (flags) flags[r] = (meta >> 8) & 3;
-        if (minpos) minpos[r] = (meta >> 10) & 0xFF;
-        if (color) color[r] = ctx->colored ? cols[r] : 0;
-        if (off) off[r] = bpos;
-        if (bases)
-            for (uint32_t i = 0; i < L; i++) bases[bpos + i] = LET[(bw[i >> 4] >> (2 * (i & 15))) & 3];
-        bpos += L;
+        for (uint32_t c = meta_mult(meta); c--; out_r++) {
+            if (out_r >= ctx->n_records) return fail(ctx, GGCAT_GPU_EHIP, "stored records exceed the record count");
+            if (sub) sub[out_r] = u >> ctx->xb;
+            if (len) len[out_r] = L;
+            if (flags) flags[out_r] = (meta >> 8) & 3;
+            if (minpos) minpos[out_r] = (meta >> 10) & 0xFF;
+            if (color) color[out_r] = ctx->colored ? cols[r] : 0;
+            if (off) off[out_r] = bpos;
+            if (bases)
+                for (uint32_t i = 0; i < L; i++) bases[bpos + i] = LET[(bw[i >> 4] >> (2 * (i & 15))) & 3];
+            bpos += L;
+        }
     }
+    if (out_r != ctx->n_records) return fail(ctx, GGCAT_GPU_EHIP, "stored records do not add up to the record count");
     return GGCAT_GPU_OK;
 }
 
@@ -887,13 +911,13 @@ int ggcat_gpu_records_serialize(ggcat_gpu_ctx* ctx, uint8_t* out, uint64_t cap, 
     const uint32_t n_buckets = ctx->n_sub >> sb;
     uint32_t min_size_log = 0;
     while ((1u << min_size_log) < k) min_size_log++;  // k.next_power_of_two().ilog2()
-    std::vector<uint32_t> slots(ctx->n_records * SW), pool(ctx->pool_words), cols;
+    std::vector<uint32_t> slots(ctx->n_stored * SW), pool(ctx->pool_words), cols;
     std::vector<uint64_t> uoff(ctx->n_units + 1);
     if (!slots.empty()) HIPCHK(hipMemcpy(slots.data(), ctx->slots.p, slots.size() * 4, hipMemcpyDeviceToHost));
     if (!pool.empty()) HIPCHK(hipMemcpy(pool.data(), ctx->pool.p, pool.size() * 4, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(uoff.data(), ctx->unit_off.p, uoff.size() * 8, hipMemcpyDeviceToHost));
-    if (ctx->colored && ctx->n_records) {
-        cols.resize(ctx->n_records);
+    if (ctx->colored && ctx->n_stored) {
+        cols.resize(ctx->n_stored);
         HIPCHK(hipMemcpy(cols.data(), ctx->slot_color.p, cols.size() * 4, hipMemcpyDeviceToHost));
     }
     // Records are stored by unit, so the records of a bucket (its 2^sb
@@ -902,7 +926,7 @@ int ggcat_gpu_records_serialize(ggcat_gpu_ctx* ctx, uint8_t* out, uint64_t cap, 
     std::vector<uint64_t> boff(n_buckets + 1, 0);
     uint32_t u = 0, cur_bucket = 0;
     uint32_t last_color = 0xFFFFFFFFu;  // MinBkSingleColor::default()
-    for (uint64_t r = 0; r < ctx->n_records; r++) {
+    for (uint64_t r = 0; r < ctx->n_stored; r++) {
         while (u + 1 < uoff.size() && uoff[u + 1] <= r) u++;
         const uint32_t sub = u >> ctx->xb, bucket = sub >> sb;
         if (bucket >= n_buckets) return fail(ctx, GGCAT_GPU_EHIP, "record outside the bucket range");
@@ -922,6 +946,7 @@ int ggcat_gpu_records_serialize(ggcat_gpu_ctx* ctx, uint8_t* out, uint64_t cap, 
             bw = sl + 1;
         }
         if (L < k) return fail(ctx, GGCAT_GPU_EHIP, "record shorter than k");
+        for (uint32_t c = meta_mult(meta); c--;) {  // the record once per copy (no multiplicity field)
         // creads_utils.rs:374-432 write_to: second bucket byte, no
         // multiplicity, length with minimizer position and flags
         // (compressed_read.rs:435-467 encode_length), packed bases, color.
@@ -936,6 +961,7 @@ int ggcat_gpu_records_serialize(ggcat_gpu_ctx* ctx, uint8_t* out, uint64_t cap, 
             // previous record's color, otherwise color + 1
             put_varint(o, cols[r] == last_color ? 0 : (uint64_t)cols[r] + 1);
             last_color = cols[r];
+        }
         }
     }
     while (cur_bucket < n_buckets) boff[++cur_bucket] = o.size();
@@ -1011,8 +1037,8 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     // kernel's static LDS, at most 8 (32 waves of 64)
     // the seq-hash k <= 32 dedup path runs a variant with its switches compiled in,
     const bool mode1 = key_words == 1 && W == 0 && M.dedup && M.tag_kmers && ctx->slot_words == 4;
-    // and the uncolored rabin-karp128 dedup path another
-    const bool mode2 = key_words == 2 && W == 0 && M.dedup && !M.tag_kmers && ctx->slot_words == 8 &&
+    // and the uncolored rabin-karp128 path another
+    const bool mode2 = key_words == 2 && W == 0 && !M.tag_kmers && ctx->slot_words == 8 &&
                        p.hash_kind == GGCAT_GPU_HASH_RK128;
     void (*merge_kernel)(MergeArgs) =
         mode1 ? merge_lds_seq64_kernel : mode2 ? merge_lds_rk128_kernel : merge_lds_kernel;
@@ -1289,9 +1315,10 @@ int ggcat_gpu_timings(ggcat_gpu_ctx* ctx, double* ms, uint32_t n) {
 
 int ggcat_gpu_counters(ggcat_gpu_ctx* ctx, uint64_t* out, uint32_t n) {
     if (!ctx || !out) return GGCAT_GPU_EINVAL;
-    uint64_t v[10] = {ctx->n_input, ctx->n_records, ctx->kmer_occ + ctx->n_records * (ctx->P.k - 1), ctx->kmer_occ,
-                      ctx->n_big,   ctx->n_solid,   ctx->n_fix,   ctx->n_passes, ctx->input_path, ctx->flush_cuts};
-    for (uint32_t i = 0; i < n && i < 10; i++) out[i] = v[i];
+    uint64_t v[11] = {ctx->n_input, ctx->n_records, ctx->kmer_occ + ctx->n_records * (ctx->P.k - 1), ctx->kmer_occ,
+                      ctx->n_big,   ctx->n_solid,   ctx->n_fix,   ctx->n_passes, ctx->input_path, ctx->flush_cuts,
+                      ctx->n_stored};
+    for (uint32_t i = 0; i < n && i < 11; i++) out[i] = v[i];
     return GGCAT_GPU_OK;
 }
 

@@ -74,6 +74,10 @@ struct BucketArgs {
     unsigned long long* pool_cursor;
     uint64_t pool_cap;  // words
     uint32_t* sub_kmers;  // [n_units] k-mer occurrences
+    // after deduplication: records stored per unit (distinct region records
+    // plus spilled ones), of which distinct region records
+    uint32_t* unit_dcount;
+    uint32_t* unit_dregion;
     const uint64_t* color_starts;
     const uint32_t* colors;
     uint32_t n_colors;
@@ -956,24 +960,33 @@ __global__ void bucket_defer_kernel(BucketArgs A) {
 }
 
 // Splits the packed per-unit counts into records and k-mer occurrences.
-// Also sums the k-mer occurrences into *total (cleared by the reset kernel;
-// blocks of 256 threads).
+// Also sums the k-mer occurrences into total[0] and the records into
+// total[1] (cleared by the reset kernel; blocks of 256 threads).
 __global__ __launch_bounds__(256) void unit_split_kernel(const unsigned long long* ck, uint32_t* count, uint32_t* kmers,
                                                         uint32_t n, unsigned long long* total) {
-    __shared__ unsigned long long part[4];
-    unsigned long long sum = 0;
+    __shared__ unsigned long long part[4], rpart[4];
+    unsigned long long sum = 0, rsum = 0;  // k-mer occurrences, records
     for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
         const unsigned long long c = ck[u];
         count[u] = (uint32_t)(c >> 32);
         kmers[u] = (uint32_t)c;
         sum += (uint32_t)c;
+        rsum += c >> 32;
     }
-    for (uint32_t d = 32; d; d >>= 1) sum += __shfl_down(sum, d);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = sum;
+    for (uint32_t d = 32; d; d >>= 1) {
+        sum += __shfl_down(sum, d);
+        rsum += __shfl_down(rsum, d);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        part[threadIdx.x >> 6] = sum;
+        rpart[threadIdx.x >> 6] = rsum;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         const unsigned long long t = part[0] + part[1] + part[2] + part[3];
+        const unsigned long long r = rpart[0] + rpart[1] + rpart[2] + rpart[3];
         if (t) atomicAdd(total, t);
+        if (r) atomicAdd(total + 1, r);
     }
 }
 
@@ -996,23 +1009,174 @@ __global__ void bucket_reset_kernel(BucketArgs A) {
     if (threadIdx.x == 0) {
         *A.spill_cursor = 0;
         *A.pool_cursor = 0;
-        *A.kmer_total = 0;
+        A.kmer_total[0] = 0;
+        A.kmer_total[1] = 0;
         *A.defer_count = 0;
         *A.error_flag = 0;
     }
 }
 
-// Pass 2a: each unit's region (its first min(count, unit_cap) records) to
-// its place in the dense unit-major arena; one wavefront per unit, 16-byte
-// copies.
+// Pass 2a: identical records of a unit counted once, as the reference's
+// compactor merges identical super-k-mers of a bucket
+// (crates/minimizer_bucketing/src/compactor.rs:128-196: same bases, length
+// and flags; multiplicities summed).  Identical super-k-mers share their
+// minimizer, so all copies of a record live in one unit.  One 64-thread
+// workgroup per unit of 2..DEDUP_MAX region records (others are kept as they
+// are): the records are loaded into LDS with 16-byte loads, each is inserted
+// into an LDS table keyed by its slot words and color, a record equal to an
+// earlier inserted one adds to that one's multiplicity (up to MAX_MULT per
+// stored record), and the stored records are written back, in region order,
+// to the front of the region with multiplicity - 1 in their meta word.
+// Spilled records (rank >= unit_cap) are stored as they are.
+constexpr uint32_t DEDUP_MAX = 256;
+constexpr uint32_t DEDUP_TABLE = 512;
+
+GG_DEV uint32_t slot_hash(const uint32_t* w, uint32_t sw, uint32_t color) {
+    uint32_t h = 0x9E3779B9u ^ color;
+    for (uint32_t i = 0; i < sw; i++) {
+        h ^= w[i];
+        h *= 0x85EBCA6Bu;
+        h ^= h >> 13;
+    }
+    h *= 0xC2B2AE35u;
+    return h ^ (h >> 16);
+}
+
+// Dynamic LDS of unit_dedup_kernel: the records' slot words, then (colored
+// builds) their colors.
+__host__ __device__ inline uint32_t dedup_lds_bytes(uint32_t slot_words, bool colored) {
+    return DEDUP_MAX * slot_words * 4 + (colored ? DEDUP_MAX * 4 : 0);
+}
+
+__global__ __launch_bounds__(64) void unit_dedup_kernel(BucketArgs A, uint32_t n_units) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    __shared__ uint32_t mult[DEDUP_MAX];  // 0: a copy of another record
+    __shared__ uint32_t table[DEDUP_TABLE];  // record index + 1, 0 = free
+    const uint32_t SW = A.slot_words, lane = threadIdx.x;
+    uint32_t* rec = reinterpret_cast<uint32_t*>(lds);  // slot words
+    uint32_t* col = rec + DEDUP_MAX * SW;  // colored builds only
+    // a unit's region is loaded into registers (at most 8 16-byte pieces per
+    // lane) while the previous unit is processed
+    uint4 p0 = {}, p1 = {}, p2 = {}, p3 = {}, p4 = {}, p5 = {}, p6 = {}, p7 = {};
+#define GG_DEDUP_PREFETCH(UU, CNT)                                                                 \
+    do {                                                                                           \
+        const uint32_t n_ = (CNT) < A.unit_cap ? (CNT) : A.unit_cap;                               \
+        if (n_ <= DEDUP_MAX && n_ >= 2) {                                                          \
+            const uint4* s_ = reinterpret_cast<const uint4*>(A.region + (uint64_t)(UU) * A.unit_cap * SW); \
+            const uint32_t m_ = n_ * (SW / 4);                                                     \
+            if (lane < m_) p0 = s_[lane];                                                          \
+            if (lane + 64 < m_) p1 = s_[lane + 64];                                                \
+            if (lane + 128 < m_) p2 = s_[lane + 128];                                              \
+            if (lane + 192 < m_) p3 = s_[lane + 192];                                              \
+            if (lane + 256 < m_) p4 = s_[lane + 256];                                              \
+            if (lane + 320 < m_) p5 = s_[lane + 320];                                              \
+            if (lane + 384 < m_) p6 = s_[lane + 384];                                              \
+            if (lane + 448 < m_) p7 = s_[lane + 448];                                              \
+        }                                                                                          \
+    } while (0)
+    uint32_t u = blockIdx.x;
+    uint32_t cnt = u < n_units ? A.sub_count[u] : 0u;
+    if (u < n_units) GG_DEDUP_PREFETCH(u, cnt);
+    for (; u < n_units; u += gridDim.x) {
+        const uint32_t n = cnt < A.unit_cap ? cnt : A.unit_cap;
+        const uint32_t spilled = cnt - n;
+        uint32_t* reg = A.region + (uint64_t)u * A.unit_cap * SW;
+        uint32_t* regc = A.region_color ? A.region_color + (uint64_t)u * A.unit_cap : nullptr;
+        const uint32_t un = u + gridDim.x;
+        const uint32_t cnt_next = un < n_units ? A.sub_count[un] : 0u;
+        if (n > DEDUP_MAX || n < 2) {
+            if (lane == 0) {
+                A.unit_dregion[u] = n;
+                A.unit_dcount[u] = cnt;
+            }
+            cnt = cnt_next;
+            if (un < n_units) GG_DEDUP_PREFETCH(un, cnt);
+            continue;
+        }
+        const uint32_t n4 = n * (SW / 4);
+        uint4* r4 = reinterpret_cast<uint4*>(rec);
+        if (lane < n4) r4[lane] = p0;
+        if (lane + 64 < n4) r4[lane + 64] = p1;
+        if (lane + 128 < n4) r4[lane + 128] = p2;
+        if (lane + 192 < n4) r4[lane + 192] = p3;
+        if (lane + 256 < n4) r4[lane + 256] = p4;
+        if (lane + 320 < n4) r4[lane + 320] = p5;
+        if (lane + 384 < n4) r4[lane + 384] = p6;
+        if (lane + 448 < n4) r4[lane + 448] = p7;
+        for (uint32_t i = lane; i < n; i += 64) {
+            if (regc) col[i] = regc[i];
+            mult[i] = 1;
+        }
+        for (uint32_t i = lane; i < DEDUP_TABLE; i += 64) table[i] = 0;
+        __syncthreads();
+        // the next unit's records are in flight meanwhile
+        cnt = cnt_next;
+        if (un < n_units) GG_DEDUP_PREFETCH(un, cnt);
+        for (uint32_t i = lane; i < n; i += 64) {
+            const uint32_t* w = rec + i * SW;
+            if (w[0] & META_LONG) continue;  // pool records: bases elsewhere, kept as they are
+            const uint32_t ci = regc ? col[i] : 0u;
+            uint32_t slot = slot_hash(w, SW, ci) & (DEDUP_TABLE - 1);
+            for (uint32_t probe = 0; probe < DEDUP_TABLE; probe++) {
+                uint32_t cur = table[slot];
+                if (cur == 0) {
+                    cur = atomicCAS(&table[slot], 0u, i + 1);
+                    if (cur == 0) break;  // first of its kind: stored
+                }
+                const uint32_t j = cur - 1;
+                const uint32_t* o = rec + j * SW;
+                bool eq = !regc || col[j] == ci;
+                for (uint32_t q = 0; q < SW && eq; q++) eq = o[q] == w[q];
+                if (eq) {
+                    // a copy of record j: counted there while j has room
+                    uint32_t m = mult[j];
+                    while (m < MAX_MULT) {
+                        const uint32_t seen = atomicCAS(&mult[j], m, m + 1);
+                        if (seen == m) break;
+                        m = seen;
+                    }
+                    if (m < MAX_MULT) mult[i] = 0;
+                    break;
+                }
+                slot = (slot + 1) & (DEDUP_TABLE - 1);
+            }
+        }
+        __syncthreads();
+        // stored records to the front of the region, in region order
+        uint32_t placed = 0;
+        for (uint32_t i0 = 0; i0 < n; i0 += 64) {
+            const uint32_t i = i0 + lane;
+            const uint32_t m = i < n ? mult[i] : 0u;
+            const uint64_t bal = __ballot(m != 0);
+            if (m) {
+                const uint32_t dst = placed + (uint32_t)__popcll(bal & ((1ULL << lane) - 1));
+                const uint32_t* w = rec + i * SW;
+                uint32_t* d = reg + (uint64_t)dst * SW;
+                d[0] = w[0] | ((m - 1) << META_MULT_SHIFT);
+                for (uint32_t q = 1; q < SW; q++) d[q] = w[q];
+                if (regc) regc[dst] = col[i];
+            }
+            placed += (uint32_t)__popcll(bal);
+        }
+        if (lane == 0) {
+            A.unit_dregion[u] = placed;
+            A.unit_dcount[u] = placed + spilled;
+        }
+        __syncthreads();  // LDS reused by the next unit
+    }
+#undef GG_DEDUP_PREFETCH
+}
+
+// Pass 2b: each unit's stored region records (the region's first
+// unit_dregion[u] slots) to their place in the dense unit-major arena; one
+// wavefront per unit, 16-byte copies.
 __global__ __launch_bounds__(256) void unit_compact_kernel(BucketArgs A, uint32_t n_units) {
     const uint32_t SW = A.slot_words;
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t waves = gridDim.x * (blockDim.x >> 6);
     for (uint32_t u = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); u < n_units; u += waves) {
-        const uint64_t o0 = A.sub_off[u], o1 = A.sub_off[u + 1];
-        const uint64_t cnt = o1 - o0;
-        const uint32_t n = cnt < A.unit_cap ? (uint32_t)cnt : A.unit_cap;
+        const uint32_t n = A.unit_dregion[u];
+        const uint64_t o0 = A.sub_off[u];
         const uint4* src = reinterpret_cast<const uint4*>(A.region + (uint64_t)u * A.unit_cap * SW);
         uint4* dst = reinterpret_cast<uint4*>(A.slots + o0 * SW);
         const uint32_t n4 = n * (SW / 4);
@@ -1022,13 +1186,14 @@ __global__ __launch_bounds__(256) void unit_compact_kernel(BucketArgs A, uint32_
     }
 }
 
-// Pass 2b: spilled records (rank >= unit_cap) to unit offset + rank.
+// Pass 2c: spilled records (rank >= unit_cap) after their unit's stored
+// region records.
 __global__ void spill_place_kernel(BucketArgs A, uint64_t n_spill) {
     const uint32_t SW = A.slot_words;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n_spill;
          i += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t unit = A.spill_unit[i];
-        const uint64_t slot = A.sub_off[unit] + A.spill_rank[i];
+        const uint64_t slot = A.sub_off[unit] + A.unit_dregion[unit] + (A.spill_rank[i] - A.unit_cap);
         if (slot >= A.sub_off[unit + 1]) {
             atomicOr(A.error_flag, 1u);
             continue;

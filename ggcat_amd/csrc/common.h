@@ -18,6 +18,8 @@ typedef unsigned __int128 u128_t;
 //   bit  20      first k-mer (stored orientation) is shared with a group of
 //                the same sub-bucket that lives in another physical unit
 //   bit  21      same for the last k-mer
+//   bits 22..31  multiplicity - 1: identical records of a unit are stored
+//                once (crates/minimizer_bucketing/src/compactor.rs:128-196)
 constexpr uint32_t META_LONG = 1u << 18;
 constexpr uint32_t META_RC = 1u << 19;
 constexpr uint32_t META_XBEGIN = 1u << 20;
@@ -26,6 +28,9 @@ constexpr uint32_t META_XEND = 1u << 21;
 GG_DEV uint32_t meta_len(uint32_t m) { return m & 0xFF; }
 GG_DEV uint32_t meta_flags(uint32_t m) { return (m >> 8) & 3; }
 GG_DEV uint32_t meta_minpos(uint32_t m) { return (m >> 10) & 0xFF; }
+constexpr uint32_t META_MULT_SHIFT = 22;
+constexpr uint32_t MAX_MULT = 1024;  // per stored record
+__host__ __device__ inline uint32_t meta_mult(uint32_t m) { return ((m >> META_MULT_SHIFT) & 0x3FF) + 1; }
 
 // ntHash constants (crates/hashes/src/nthash_base.rs:49-61)
 constexpr uint64_t NT_MULT = 0x397f178c6ae330f9ULL;

@@ -105,14 +105,15 @@ GG_DEV uint32_t hash128(uint64_t lo, uint64_t hi) { return tab_hash64(lo ^ (hi *
 // which keeps the other paths' code out of their register allocation.
 template <int MODE>
 struct Mode {
-    static GG_DEV uint32_t key_words(const MergeArgs& A) { return MODE == 1 ? 1u : MODE == 2 ? 2u : A.key_words; }
+    static constexpr bool S64 = MODE == 1;  // seq-hash k <= 32, 4-word slots, uncolored
+    static GG_DEV uint32_t key_words(const MergeArgs& A) { return S64 ? 1u : MODE == 2 ? 2u : A.key_words; }
     static GG_DEV uint32_t color_words(const MergeArgs& A) { return MODE != 0 ? 0u : A.color_words; }
-    static GG_DEV bool seq64(const MergeArgs& A) { return MODE == 1 || (MODE == 0 && A.kind == 0 && A.k <= 32); }
+    static GG_DEV bool seq64(const MergeArgs& A) { return S64 || (MODE == 0 && A.kind == 0 && A.k <= 32); }
     static GG_DEV bool seq128(const MergeArgs& A) { return MODE == 0 && A.kind == 0; }
-    static GG_DEV bool dedup(const MergeArgs& A) { return MODE != 0 || A.dedup; }
-    static GG_DEV bool tag_kmers(const MergeArgs& A) { return MODE == 1 || (MODE == 0 && A.tag_kmers); }
-    static GG_DEV uint32_t slot_words(const MergeArgs& A) { return MODE == 1 ? 4u : MODE == 2 ? 8u : A.slot_words; }
-    static constexpr uint32_t MAX_SW = MODE == 1 ? 4u : 8u;  // register copies of a slot
+    static GG_DEV bool dedup(const MergeArgs& A) { return S64 || A.dedup; }
+    static GG_DEV bool tag_kmers(const MergeArgs& A) { return S64 || (MODE == 0 && A.tag_kmers); }
+    static GG_DEV uint32_t slot_words(const MergeArgs& A) { return S64 ? 4u : MODE == 2 ? 8u : A.slot_words; }
+    static constexpr uint32_t MAX_SW = S64 ? 4u : 8u;  // register copies of a slot
 };
 
 // Insert one occurrence.  Returns 1 when this call claimed a new slot, 0 when
@@ -368,13 +369,14 @@ GG_DEV void insert_item(const MergeArgs& A, uint64_t r0, uint64_t r1, const Part
         const uint32_t* slot = A.slots + r * SW;
         const uint32_t meta = slot[0];
         const uint32_t color = W ? A.slot_color[r] : 0u;
+        const uint32_t extra = meta_mult(meta) - 1;  // identical copies stored once
         for_each_kmer<MODE>(A, slot, [&](uint64_t lo, uint64_t hi, bool is_fwd, uint32_t idx, uint32_t last) {
             if (*(volatile uint32_t*)overflow) return;
             if (part_mask) {
                 const uint32_t h = (KW == 1) ? tab_hash64(lo) : hash128(lo, hi);
                 if (((h >> 16) & part_mask) != w.part) return;
             }
-            uint32_t bits = occurrence_bits(meta, is_fwd, idx, last), at = 0;
+            uint32_t bits = occurrence_bits(meta, is_fwd, idx, last) + extra, at = 0;
             uint32_t res = (KW == 1)
                                ? tab_insert64(klo, vals, mask, lo, bits, probe_limit, at)
                                : tab_insert128(klo, khi, st, vals, mask, lo, hi, bits, probe_limit, at);
@@ -512,7 +514,7 @@ GG_DEV void insert_item_dedup(const MergeArgs& A, uint64_t r0, uint64_t r1, cons
                     if (atomicCAS(&R.state[i], 0u, 1u) == 0u) {
                         for (uint32_t q = 0; q < SW; q++) R.key[i * SW + q] = wv[q];
                         R.col[i] = color;
-                        R.cnt[i] = 1;
+                        R.cnt[i] = meta_mult(meta);
                         __threadfence_block();
                         atomicExch(&R.state[i], 2u);
                         done = true;
@@ -521,7 +523,7 @@ GG_DEV void insert_item_dedup(const MergeArgs& A, uint64_t r0, uint64_t r1, cons
                     bool eq = R.col[i] == color;
                     for (uint32_t q = 0; q < SW && eq; q++) eq = R.key[i * SW + q] == wv[q];
                     if (eq) {
-                        atomicAdd(&R.cnt[i], 1u);
+                        atomicAdd(&R.cnt[i], meta_mult(meta));
                         done = true;
                     } else if (++probes >= 32) {
                         direct = true;
@@ -536,8 +538,9 @@ GG_DEV void insert_item_dedup(const MergeArgs& A, uint64_t r0, uint64_t r1, cons
             }
         }
         if (direct) {
+            const uint32_t extra = meta_mult(meta) - 1;
             for_each_kmer<MODE>(A, slot, [&](uint64_t lo, uint64_t hi, bool is_fwd, uint32_t idx, uint32_t last) {
-                insert_key(lo, hi, occurrence_bits(meta, is_fwd, idx, last), color);
+                insert_key(lo, hi, occurrence_bits(meta, is_fwd, idx, last) + extra, color);
             });
         }
     }

@@ -592,6 +592,6 @@ class GpuContext:
         return out
 
     def counters(self) -> np.ndarray:
-        out = np.zeros(10, np.uint64)
-        self._check(self.lib.ggcat_gpu_counters(self.h, _ptr(out, _u64p), 10), "counters")
+        out = np.zeros(11, np.uint64)
+        self._check(self.lib.ggcat_gpu_counters(self.h, _ptr(out, _u64p), 11), "counters")
         return out

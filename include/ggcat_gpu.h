@@ -224,7 +224,7 @@ typedef struct {
 int ggcat_gpu_set_tuning(ggcat_gpu_ctx* ctx, const ggcat_gpu_tuning* tuning);
 
 /* Device-side timings of the last calls, in milliseconds:
- * [0] bucketing staging pass, [1] scan, [2] bucketing region compaction,
+ * [0] bucketing staging pass, [1] scan, [2] bucketing deduplication + region compaction,
  * [3] merge (whole-item pass), [4] merge (partitioned passes),
  * [5] whole bucketing call, [6] whole merge call, [7] unitig kernels,
  * [8] reading the input of the last build_from_files (wall clock: files
@@ -237,7 +237,9 @@ int ggcat_gpu_timings(ggcat_gpu_ctx* ctx, double* ms, uint32_t n);
  * [6] k-mers shared between two units of a sub-bucket (combined on the
  * host after the merge), [7] merge kernel passes (1 + partition rounds),
  * [8] how the last build_from_files read its input (1: into HBM, parsed on
- * the device; 2: host reader), [9] long FASTA records cut on the device. */
+ * the device; 2: host reader), [9] long FASTA records cut on the device,
+ * [10] records stored after identical copies in a unit were counted once
+ * (the arena's slots; [1] counts every copy). */
 int ggcat_gpu_counters(ggcat_gpu_ctx* ctx, uint64_t* out, uint32_t n);
 
 /* Reads FASTA/FASTQ files (optionally gzip-compressed) into the joined layout.

@@ -144,3 +144,33 @@ def test_device_parse_falls_back_to_host_reader_emulated(emu_ctx, tmp_path, case
         emu_ctx.set_tuning()
     assert path_d == 2
     assert rec_d == rec_h and np.array_equal(sol_d, sol_h)
+
+
+def test_compaction_counts_identical_records_emulated(emu_ctx):
+    # identical records of a unit are stored once with their multiplicity
+    # (the reference's compactor); the decoded records, the solid table and
+    # the colored solid table are unchanged, and copies of different colors
+    # stay apart
+    g = synth.genome(3, 8000)
+    data = synth.reads(g, 4, 800, 120, 0.005)
+    emu_ctx.set_tuning(merge_wgs_per_cu=2)
+    try:
+        _check(emu_ctx, data, 31, 15, 3, 2, 2)
+        c = emu_ctx.counters()
+        assert 0 < int(c[10]) < int(c[1]) // 1.5  # copies were folded
+        # the same reads under two colors: nothing folds across them
+        both = np.concatenate([data, data])
+        starts = np.array([0, data.size], np.uint64)
+        colors = np.array([1, 0], np.uint32)
+        emu_ctx.bucketing(both, Params(31, 15, 3, 2, 0, HASH_SEQ, 2), colors=(starts, colors))
+        c2 = emu_ctx.counters()
+        assert int(c2[10]) == 2 * int(c[10])
+        o = O.bucketing_colored(both, 31, 15, 3, 2, starts, colors, threads=2)
+        assert record_tuples(emu_ctx.records()) == record_tuples(o.records())
+        emu_ctx.kmers_merge()
+        o.merge(31, 3, 2, HASH_SEQ, 2, 2)
+        got = colored_solid_array(emu_ctx.solid(), emu_ctx.solid_colors())
+        off, cols = o.solid_colors()
+        assert np.array_equal(got, colored_solid_array(o.solid(), bitmaps_from_sets(off, cols, 1)))
+    finally:
+        emu_ctx.set_tuning()
