@@ -6,6 +6,7 @@
 #include "query.hip"
 #include "unitigs.hip"
 #include "parse.hip"
+#include "resplit.hip"
 #include "fasta.cpp"
 #include "colormap.cpp"
 
@@ -40,6 +41,8 @@ struct DevBuf {
     size_t cap = 0;
 };
 
+constexpr uint64_t PIN_CHUNK = 64ull << 20;  // bytes per pinned staging buffer (two of them)
+
 }  // namespace
 
 struct ggcat_gpu_ctx {
@@ -62,6 +65,8 @@ struct ggcat_gpu_ctx {
     uint32_t n_sub = 0, n_units = 0, xb = 0, slot_words = 0, key_words = 1;
     uint64_t n_records = 0, pool_words = 0, n_solid = 0, n_input = 0, kmer_occ = 0, n_big = 0, n_fix = 0;
     uint64_t n_stored = 0;  // arena slots: records with identical copies stored once
+    uint32_t n_sub_all = 0;  // sub-buckets including the resplit buckets of outliers
+    bool resplit_done = false;  // the arena holds resplit buckets (ggcat_gpu_resplit_outliers)
     uint64_t n_spill = 0;  // records of the last bucketing past their unit's region
     uint32_t n_chunks_used = 0;  // output chunks of the last merge
     uint32_t n_passes = 0;  // kernel passes of the last merge
@@ -98,6 +103,8 @@ struct ggcat_gpu_ctx {
     // and a pinned two-buffer staging ring for the uploads
     DevBuf raw, p_stats, p_carry, p_keep, p_keep_off, p_flags, p_bits;
     void* pin_ring = nullptr;
+    hipStream_t copy_stream = nullptr;  // host-to-device uploads of host input, beside the compute stream
+    hipEvent_t up_ev[2] = {nullptr, nullptr};  // last upload out of each staging buffer
     std::vector<uint32_t> cyc_pos;  // the same for the unitigs of cycles (walked on the host)
     // color subsets of the solid table: ids for the color map and the colored
     // FASTA headers (assigned on first use after a merge)
@@ -417,7 +424,9 @@ int ggcat_gpu_create(int device, ggcat_gpu_ctx** out) {
     ctx->device = device;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking);
     for (int i = 0; i < 16 && e == hipSuccess; i++) e = hipEventCreate(&ctx->ev[i]);
+    for (int i = 0; i < 2 && e == hipSuccess; i++) e = hipEventCreateWithFlags(&ctx->up_ev[i], hipEventDisableTiming);
     hipDeviceProp_t prop;
     if (e == hipSuccess && hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->n_cu = prop.multiProcessorCount;
     // the no-allocation entry point's scratch array: its address on this device
@@ -455,6 +464,9 @@ void ggcat_gpu_destroy(ggcat_gpu_ctx* ctx) {
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (int i = 0; i < 16; i++) (void)hipEventDestroy(ctx->ev[i]);
+    for (int i = 0; i < 2; i++)
+        if (ctx->up_ev[i]) (void)hipEventDestroy(ctx->up_ev[i]);
+    if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -498,6 +510,8 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     uint32_t slot_words = 4;
     while (slot_words < sw) slot_words <<= 1;
     ctx->n_sub = n_sub;
+    ctx->n_sub_all = n_sub;
+    ctx->resplit_done = false;
     ctx->xb = xb;
     ctx->n_units = n_units;
     ctx->slot_words = slot_words;
@@ -515,10 +529,23 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     }
 
     const uint8_t* d_in = data;
+    // Host input of at least two upload chunks goes through the pinned
+    // staging ring on the copy stream, chunk by chunk, while the staging pass
+    // runs on the compute stream over the tiles already resident (first
+    // attempt of pass 1); smaller input is copied whole first.
+    const uint64_t up_chunk = ctx->tune.upload_chunk ? std::min<uint64_t>(ctx->tune.upload_chunk, PIN_CHUNK) : PIN_CHUNK;
+    bool chunked = false;
     if (!input_on_device) {
         ENSURE(ctx->input, n_bytes + 16);
-        HIPCHK(hipMemcpy(ctx->input.p, data, n_bytes, hipMemcpyHostToDevice));
         d_in = P_<uint8_t>(ctx->input);
+        chunked = n_bytes >= 2 * up_chunk;
+        if (chunked && !ctx->pin_ring &&
+            hipHostMalloc(&ctx->pin_ring, 2 * PIN_CHUNK, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->pin_ring = nullptr;
+            chunked = false;
+        }
+        if (!chunked) HIPCHK(hipMemcpy(ctx->input.p, data, n_bytes, hipMemcpyHostToDevice));
     }
     // w == 16 forms window minima in registers; its tile of 960 windows (plus
     // the w + 1 halo) is exactly 62 blocks of 16, one per lane.  It keeps the
@@ -613,6 +640,65 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
         hipLaunchKernelGGL(unit_split_kernel, dim3(std::min<uint32_t>((n_units + 255) / 256, 4096)), dim3(256), 0,
                            ctx->stream, A.unit_ck, A.sub_count, A.sub_kmers, n_units, A.kmer_total);
     };
+    // The same pass over host input uploaded chunk by chunk: chunk j is copied
+    // into staging buffer j % 2 by host threads (once the upload that last
+    // read that buffer is done) and uploaded on the copy stream; the compute
+    // stream waits for it and stages every tile whose bytes, with the margin
+    // a tile reads past its windows, are now resident.  Tiles are launched
+    // with their absolute index (A.tile0), so the records are the ones of a
+    // single launch.
+    auto pass_1_chunked = [&]() -> int {
+        hipLaunchKernelGGL(fill32_kernel, dim3(256), dim3(256), 0, ctx->stream, reinterpret_cast<uint32_t*>(A.unit_ck),
+                           0u, 2 * (uint64_t)n_units);
+        hipLaunchKernelGGL(fill32_kernel, dim3(256), dim3(256), 0, ctx->stream, A.seg_bits, 0u, (uint64_t)seg_words);
+        hipLaunchKernelGGL(bucket_reset_kernel, dim3(1), dim3(64), 0, ctx->stream, A);
+        HIPCHK(hipGetLastError());
+        uint8_t* ring = static_cast<uint8_t*>(ctx->pin_ring);
+        uint8_t* d_dst = P_<uint8_t>(ctx->input);
+        const uint64_t margin = 2 * (uint64_t)T + 4 * (uint64_t)p.k + 4096;
+        const uint32_t n_threads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+        bool pending[2] = {false, false};
+        uint64_t tiles_done = 0;
+        uint32_t j = 0;
+        for (uint64_t o = 0; o < n_bytes; o += up_chunk, j++) {
+            const uint32_t b = j & 1;
+            if (pending[b]) HIPCHK(hipEventSynchronize(ctx->up_ev[b]));
+            const uint64_t len = std::min<uint64_t>(up_chunk, n_bytes - o);
+            uint8_t* stage_buf = ring + b * PIN_CHUNK;
+            const uint64_t slice = (len + n_threads - 1) / n_threads;
+            std::vector<std::thread> th;
+            for (uint32_t q = 1; q < n_threads; q++)
+                th.emplace_back([&, q] {
+                    const uint64_t lo = std::min(len, q * slice), hi = std::min(len, (q + 1) * slice);
+                    if (hi > lo) memcpy(stage_buf + lo, data + o + lo, hi - lo);
+                });
+            memcpy(stage_buf, data + o, std::min(len, slice));
+            for (auto& x : th) x.join();
+            HIPCHK(hipMemcpyAsync(d_dst + o, stage_buf, len, hipMemcpyHostToDevice, ctx->copy_stream));
+            HIPCHK(hipEventRecord(ctx->up_ev[b], ctx->copy_stream));
+            pending[b] = true;
+            HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->up_ev[b], 0));
+            const uint64_t resident = o + len;
+            uint64_t safe = resident == n_bytes ? n_tiles : (resident > margin ? (resident - margin) / T : 0);
+            safe = std::min<uint64_t>(safe, n_tiles);
+            if (safe > tiles_done) {
+                A.tile0 = tiles_done;
+                const dim3 grid((uint32_t)(safe - tiles_done));
+                if (regs16)
+                    hipLaunchKernelGGL(bucket_stage_kernel<16>, grid, dim3(64), G.bytes, ctx->stream, A);
+                else
+                    hipLaunchKernelGGL(bucket_stage_kernel<0>, grid, dim3(64), G.bytes, ctx->stream, A);
+                HIPCHK(hipGetLastError());
+                tiles_done = safe;
+            }
+        }
+        A.tile0 = 0;
+        if (n_tiles) hipLaunchKernelGGL(bucket_defer_kernel, dim3(small_grid), dim3(256), 0, ctx->stream, A);
+        hipLaunchKernelGGL(unit_split_kernel, dim3(std::min<uint32_t>((n_units + 255) / 256, 4096)), dim3(256), 0,
+                           ctx->stream, A.unit_ck, A.sub_count, A.sub_kmers, n_units, A.kmer_total);
+        HIPCHK(hipGetLastError());
+        return GGCAT_GPU_OK;
+    };
     uint64_t staged = 0, pool_used = 0;
     uint32_t errf = 0;
     for (int attempt = 0;; attempt++) {
@@ -636,7 +722,10 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
         A.pool = P_<uint32_t>(ctx->pool);
         A.pool_cap = pool_cap;
         HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
-        if (ctx->tune.use_graphs) {
+        if (attempt == 0 && chunked) {
+            const int r = pass_1_chunked();
+            if (r != GGCAT_GPU_OK) return r;
+        } else if (ctx->tune.use_graphs) {
             ggcat_gpu_ctx::GraphKeyA key;
             memset(&key, 0, sizeof key);
             key.args = A;
@@ -740,11 +829,14 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
 // resplit into next_pow2(records / average) clamped to [4, 1024] buckets.
 // The reference orders by compressed bucket-file size and counts compacted
 // records; this uses the serialized record bytes and the records as staged.
-int ggcat_gpu_outliers(ggcat_gpu_ctx* ctx, uint32_t threads, uint32_t* n_outliers, uint32_t* sub, uint32_t* resplit,
-                       uint64_t* records, uint32_t cap) {
-    if (!ctx || !n_outliers) return GGCAT_GPU_EINVAL;
-    if (!ctx->have_records) return fail(ctx, GGCAT_GPU_ESTATE, "no bucketing result");
-    HIPCHK(hipSetDevice(ctx->device));
+namespace {
+struct OutlierInfo {
+    uint32_t sub, resplit;
+    uint64_t records;
+};
+
+int find_outliers(ggcat_gpu_ctx* ctx, uint32_t threads, std::vector<OutlierInfo>& found) {
+    found.clear();
     const uint32_t n_sub = ctx->n_sub, sb = ctx->P.second_buckets_log;
     std::vector<uint64_t> rec(n_sub), bas(n_sub);
     DevBuf tmp;
@@ -793,7 +885,6 @@ int ggcat_gpu_outliers(ggcat_gpu_ctx* ctx, uint32_t threads, uint32_t* n_outlier
         }
     }
     uint64_t tot_seq = 0, tot_cnt = 0;
-    uint32_t n = 0;
     for (uint32_t b : order) {
         for (uint32_t s = b << sb; s < (b + 1) << sb; s++)
             if (rec[s]) {
@@ -806,23 +897,289 @@ int ggcat_gpu_outliers(ggcat_gpu_ctx* ctx, uint32_t threads, uint32_t* n_outlier
             uint64_t c = 1;
             while (c < rec[s] / avg) c <<= 1;
             c = std::min<uint64_t>(std::max<uint64_t>(c, 4), 1024);
-            if (n < cap) {
-                if (sub) sub[n] = s;
-                if (resplit) resplit[n] = (uint32_t)c;
-                if (records) records[n] = rec[s];
-            }
-            n++;
+            found.push_back(OutlierInfo{s, (uint32_t)c, rec[s]});
         }
     }
-    *n_outliers = n;
     return GGCAT_GPU_OK;
+}
+
+void free_buf(DevBuf& b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+}  // namespace
+
+int ggcat_gpu_outliers(ggcat_gpu_ctx* ctx, uint32_t threads, uint32_t* n_outliers, uint32_t* sub, uint32_t* resplit,
+                       uint64_t* records, uint32_t cap) {
+    if (!ctx || !n_outliers) return GGCAT_GPU_EINVAL;
+    if (!ctx->have_records) return fail(ctx, GGCAT_GPU_ESTATE, "no bucketing result");
+    if (ctx->resplit_done) return fail(ctx, GGCAT_GPU_ESTATE, "the bucketing's outliers were resplit already");
+    HIPCHK(hipSetDevice(ctx->device));
+    std::vector<OutlierInfo> found;
+    const int rc = find_outliers(ctx, threads, found);
+    if (rc != GGCAT_GPU_OK) return rc;
+    for (uint32_t i = 0; i < found.size() && i < cap; i++) {
+        if (sub) sub[i] = found[i].sub;
+        if (resplit) resplit[i] = found[i].resplit;
+        if (records) records[i] = found[i].records;
+    }
+    *n_outliers = (uint32_t)found.size();
+    return GGCAT_GPU_OK;
+}
+
+namespace {
+// The listed sub-buckets resplit as crates/kmers_transform/src/resplitter.rs:47-183
+// does (resplit.hip): the arena is rebuilt with their units emptied and one
+// unit per resplit bucket appended after the bucketing's units, as extra
+// sub-buckets n_sub + base_i .. n_sub + base_i + count_i (duplicates bucket
+// last).  Counts are powers of two.
+int resplit_subs(ggcat_gpu_ctx* ctx, const std::vector<OutlierInfo>& found) {
+    int rc = GGCAT_GPU_OK;
+    const uint32_t n_out = (uint32_t)found.size();
+    const uint32_t n_sub = ctx->n_sub, xb = ctx->xb, n_units = ctx->n_units, SW = ctx->slot_words, k = ctx->P.k;
+    std::vector<uint32_t> logs(n_out), base(n_out);
+    uint64_t n_extra = 0;
+    for (uint32_t i = 0; i < n_out; i++) {
+        uint32_t lg = 0;
+        while ((1u << lg) < found[i].resplit) lg++;
+        logs[i] = lg;
+        base[i] = (uint32_t)n_extra;
+        n_extra += (1u << lg) + 1;
+    }
+    if (n_out == 0) return GGCAT_GPU_OK;
+    const uint64_t n_units_all = (uint64_t)(n_sub + n_extra) << xb;
+    if (n_units_all > (1ull << 31)) return fail(ctx, GGCAT_GPU_ENOMEM, "too many resplit units");
+    // each outlier's stored records: one contiguous range of the unit-major arena
+    std::vector<uint64_t> prefix(n_out + 1, 0), first(n_out);
+    std::vector<uint8_t> is_out(n_sub, 0);
+    for (uint32_t i = 0; i < n_out; i++) {
+        uint64_t ab[2];
+        HIPCHK(hipMemcpy(&ab[0], P_<uint64_t>(ctx->unit_off) + ((uint64_t)found[i].sub << xb), 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&ab[1], P_<uint64_t>(ctx->unit_off) + ((uint64_t)(found[i].sub + 1) << xb), 8,
+                         hipMemcpyDeviceToHost));
+        first[i] = ab[0];
+        prefix[i + 1] = prefix[i] + (ab[1] - ab[0]);
+        is_out[found[i].sub] = 1;
+    }
+    const uint32_t m2 = k > 11 ? k - 10 : std::min<uint32_t>(ctx->P.m, 2);
+    if (k - m2 > RS_RING) return fail(ctx, GGCAT_GPU_EINVAL, "resplit window wider than supported");
+    DevBuf meta_b, isout_b, ext_b, need_b, cur_b;
+    auto cleanup = [&]() {
+        free_buf(meta_b);
+        free_buf(isout_b);
+        free_buf(ext_b);
+        free_buf(need_b);
+        free_buf(cur_b);
+    };
+    // host tables: prefix [n_out + 1], first [n_out], logs [n_out], base [n_out]
+    const size_t meta_bytes = (size_t)(2 * n_out + 1) * 8 + (size_t)2 * n_out * 4;
+    std::vector<uint8_t> mt(meta_bytes);
+    memcpy(mt.data(), prefix.data(), (n_out + 1) * 8);
+    memcpy(mt.data() + (n_out + 1) * 8, first.data(), n_out * 8);
+    memcpy(mt.data() + (2 * n_out + 1) * 8, logs.data(), n_out * 4);
+    memcpy(mt.data() + (2 * n_out + 1) * 8 + n_out * 4, base.data(), n_out * 4);
+    if ((rc = ensure(ctx, meta_b, meta_bytes)) != GGCAT_GPU_OK || (rc = ensure(ctx, isout_b, n_sub)) != GGCAT_GPU_OK ||
+        (rc = ensure(ctx, ext_b, (size_t)n_extra * 12)) != GGCAT_GPU_OK ||
+        (rc = ensure(ctx, need_b, 48)) != GGCAT_GPU_OK) {
+        cleanup();
+        return rc;
+    }
+#define RS_CHK(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) {                                                                        \
+            cleanup();                                                                                 \
+            return fail(ctx, GGCAT_GPU_EHIP, std::string("resplit: ") + #expr + ": " + hipGetErrorString(e_)); \
+        }                                                                                              \
+    } while (0)
+    RS_CHK(hipMemcpyAsync(meta_b.p, mt.data(), meta_bytes, hipMemcpyHostToDevice, ctx->stream));
+    RS_CHK(hipMemcpyAsync(isout_b.p, is_out.data(), n_sub, hipMemcpyHostToDevice, ctx->stream));
+    RS_CHK(hipMemsetAsync(ext_b.p, 0, (size_t)n_extra * 12, ctx->stream));
+    RS_CHK(hipMemsetAsync(need_b.p, 0, 48, ctx->stream));
+    RS_CHK(hipMemsetAsync(ctx->error_flag.p, 0, 4, ctx->stream));
+    ResplitArgs R{};
+    R.slots = P_<uint32_t>(ctx->slots);
+    R.slot_color = ctx->colored ? P_<uint32_t>(ctx->slot_color) : nullptr;
+    R.pool = P_<uint32_t>(ctx->pool);
+    R.pool_words = ctx->pool_words;
+    R.SW = SW;
+    R.k = k;
+    R.m2 = m2;
+    R.w2 = k - m2;
+    R.n_out = n_out;
+    R.rec_prefix = reinterpret_cast<const uint64_t*>(meta_b.p);
+    R.rec_first = R.rec_prefix + n_out + 1;
+    R.log_buckets = reinterpret_cast<const uint32_t*>(R.rec_first + n_out);
+    R.ext_base = R.log_buckets + n_out;
+    R.ext_records = P_<uint32_t>(ext_b);
+    R.ext_kmers = R.ext_records + n_extra;
+    R.ext_cursor = R.ext_kmers + n_extra;
+    R.pool_need = P_<unsigned long long>(need_b);
+    R.sums = R.pool_need + 2;
+    R.error_flag = P_<uint32_t>(ctx->error_flag);
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((prefix[n_out] + 255) / 256, 4096));
+    hipLaunchKernelGGL(resplit_count_kernel, dim3(grid), dim3(256), 0, ctx->stream, R);
+    RS_CHK(hipGetLastError());
+    uint64_t pool_need = 0, sums[4] = {0, 0, 0, 0};
+    uint32_t errf = 0;
+    RS_CHK(hipMemcpyAsync(&pool_need, need_b.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+    RS_CHK(hipMemcpyAsync(sums, R.sums, 32, hipMemcpyDeviceToHost, ctx->stream));
+    RS_CHK(hipMemcpyAsync(&errf, ctx->error_flag.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    RS_CHK(hipStreamSynchronize(ctx->stream));
+    if (errf) {
+        cleanup();
+        return fail(ctx, GGCAT_GPU_EHIP, "resplit: a record does not fit its slot or the pool");
+    }
+    // the rebuilt unit arrays and arena
+    DevBuf dcount2, count2, kmers2, off2, slots2, color2, pool2;
+    auto cleanup2 = [&]() {
+        free_buf(dcount2);
+        free_buf(count2);
+        free_buf(kmers2);
+        free_buf(off2);
+        free_buf(slots2);
+        free_buf(color2);
+        free_buf(pool2);
+    };
+    if ((rc = ensure(ctx, dcount2, n_units_all * 4)) != GGCAT_GPU_OK ||
+        (rc = ensure(ctx, count2, n_units_all * 4)) != GGCAT_GPU_OK ||
+        (rc = ensure(ctx, kmers2, n_units_all * 4)) != GGCAT_GPU_OK ||
+        (rc = ensure(ctx, off2, (n_units_all + 1) * 8)) != GGCAT_GPU_OK) {
+        cleanup();
+        cleanup2();
+        return rc;
+    }
+    hipLaunchKernelGGL(resplit_units_kernel, dim3((uint32_t)std::min<uint64_t>((n_units_all + 255) / 256, 65535)),
+                       dim3(256), 0, ctx->stream, P_<uint32_t>(ctx->unit_dcount), P_<uint32_t>(ctx->unit_count),
+                       P_<uint32_t>(ctx->unit_kmers), P_<uint8_t>(isout_b), n_units, (uint32_t)n_units_all, xb,
+                       n_sub, R.ext_records, R.ext_kmers, P_<uint32_t>(dcount2), P_<uint32_t>(count2),
+                       P_<uint32_t>(kmers2));
+    RS_CHK(hipGetLastError());
+    if ((rc = device_scan(ctx, P_<uint32_t>(dcount2), P_<uint64_t>(off2), (uint32_t)n_units_all)) != GGCAT_GPU_OK) {
+        cleanup();
+        cleanup2();
+        return rc;
+    }
+    uint64_t total2 = 0;
+    RS_CHK(hipMemcpyAsync(&total2, P_<uint64_t>(off2) + n_units_all, 8, hipMemcpyDeviceToHost, ctx->stream));
+    RS_CHK(hipStreamSynchronize(ctx->stream));
+    const uint64_t pool_cap2 = ctx->pool_words + pool_need;
+    if ((rc = ensure(ctx, slots2, std::max<uint64_t>(total2, 1) * SW * 4)) != GGCAT_GPU_OK ||
+        (ctx->colored && (rc = ensure(ctx, color2, std::max<uint64_t>(total2, 1) * 4)) != GGCAT_GPU_OK) ||
+        (pool_need && pool_cap2 * 4 > ctx->pool.cap && (rc = ensure(ctx, pool2, pool_cap2 * 4)) != GGCAT_GPU_OK)) {
+        cleanup();
+        cleanup2();
+        return rc;
+    }
+    if (pool2.p && ctx->pool_words)
+        RS_CHK(hipMemcpyAsync(pool2.p, ctx->pool.p, ctx->pool_words * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    {
+        const uint32_t g = (uint32_t)std::min<uint64_t>(((uint64_t)n_units + 3) / 4, (uint64_t)ctx->n_cu * 32);
+        hipLaunchKernelGGL(resplit_copy_kernel, dim3(std::max<uint32_t>(g, 1)), dim3(256), 0, ctx->stream,
+                           P_<uint32_t>(ctx->slots), R.slot_color, P_<uint64_t>(ctx->unit_off), P_<uint64_t>(off2),
+                           P_<uint8_t>(isout_b), n_units, xb, SW, P_<uint32_t>(slots2),
+                           ctx->colored ? P_<uint32_t>(color2) : nullptr);
+        RS_CHK(hipGetLastError());
+    }
+    const uint64_t pool_used = ctx->pool_words;
+    RS_CHK(hipMemcpyAsync(R.pool_need + 1, &pool_used, 8, hipMemcpyHostToDevice, ctx->stream));
+    R.unit_off = P_<uint64_t>(off2);
+    R.first_extra_unit = n_sub;
+    R.xb = xb;
+    R.out_slots = P_<uint32_t>(slots2);
+    R.out_color = ctx->colored ? P_<uint32_t>(color2) : nullptr;
+    R.out_pool = pool2.p ? P_<uint32_t>(pool2) : P_<uint32_t>(ctx->pool);
+    R.pool_cursor = R.pool_need + 1;
+    R.pool_cap = pool2.p ? pool2.cap / 4 : ctx->pool.cap / 4;
+    hipLaunchKernelGGL(resplit_write_kernel, dim3(grid), dim3(256), 0, ctx->stream, R);
+    RS_CHK(hipGetLastError());
+    RS_CHK(hipMemcpyAsync(&errf, ctx->error_flag.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    RS_CHK(hipStreamSynchronize(ctx->stream));
+#undef RS_CHK
+    if (errf) {
+        cleanup();
+        cleanup2();
+        return fail(ctx, GGCAT_GPU_EHIP, "resplit: a piece outside its unit or the pool");
+    }
+    cleanup();
+    std::swap(ctx->unit_dcount, dcount2);
+    std::swap(ctx->unit_count, count2);
+    std::swap(ctx->unit_kmers, kmers2);
+    std::swap(ctx->unit_off, off2);
+    std::swap(ctx->slots, slots2);
+    if (ctx->colored) std::swap(ctx->slot_color, color2);
+    if (pool2.p) std::swap(ctx->pool, pool2);
+    cleanup2();  // the replaced buffers
+    // unit_dregion / unit_cursor keep the bucketing's size; the next
+    // bucketing sizes every unit array again
+    ctx->pool_words = pool_used + pool_need;
+    ctx->n_units = (uint32_t)n_units_all;
+    ctx->n_sub_all = (uint32_t)(n_sub + n_extra);
+    ctx->n_stored = total2;
+    // record copies and k-mer occurrences: the outliers' records out, their pieces in
+    ctx->n_records = ctx->n_records - sums[0] + sums[1];
+    ctx->kmer_occ = ctx->kmer_occ - sums[2] + sums[3];
+    ctx->resplit_done = true;
+    ctx->have_solid = ctx->have_index = ctx->have_unitigs = false;
+    return GGCAT_GPU_OK;
+}
+
+// First resplit sub-bucket of each listed sub-bucket (resplit_subs numbering).
+void resplit_firsts(const ggcat_gpu_ctx* ctx, const std::vector<OutlierInfo>& found, uint32_t* first_sub,
+                    uint32_t cap) {
+    uint32_t next = ctx->n_sub;
+    for (uint32_t i = 0; i < found.size(); i++) {
+        uint32_t c = 1;
+        while (c < found[i].resplit) c <<= 1;
+        if (first_sub && i < cap) first_sub[i] = next;
+        next += c + 1;
+    }
+}
+}  // namespace
+
+int ggcat_gpu_resplit_outliers(ggcat_gpu_ctx* ctx, uint32_t threads, uint32_t* n_outliers, uint32_t* sub,
+                               uint32_t* resplit, uint32_t* first_sub, uint32_t cap) {
+    if (!ctx || !n_outliers) return GGCAT_GPU_EINVAL;
+    if (!ctx->have_records) return fail(ctx, GGCAT_GPU_ESTATE, "no bucketing result");
+    if (ctx->resplit_done) return fail(ctx, GGCAT_GPU_ESTATE, "the bucketing's outliers were resplit already");
+    HIPCHK(hipSetDevice(ctx->device));
+    std::vector<OutlierInfo> found;
+    int rc = find_outliers(ctx, threads, found);
+    if (rc != GGCAT_GPU_OK) return rc;
+    *n_outliers = (uint32_t)found.size();
+    for (uint32_t i = 0; i < found.size() && i < cap; i++) {
+        if (sub) sub[i] = found[i].sub;
+        if (resplit) resplit[i] = found[i].resplit;
+    }
+    resplit_firsts(ctx, found, first_sub, cap);
+    return resplit_subs(ctx, found);
+}
+
+int ggcat_gpu_resplit(ggcat_gpu_ctx* ctx, uint32_t n, const uint32_t* sub, const uint32_t* count,
+                      uint32_t* first_sub) {
+    if (!ctx || (n && (!sub || !count))) return GGCAT_GPU_EINVAL;
+    if (!ctx->have_records) return fail(ctx, GGCAT_GPU_ESTATE, "no bucketing result");
+    if (ctx->resplit_done) return fail(ctx, GGCAT_GPU_ESTATE, "the bucketing's outliers were resplit already");
+    HIPCHK(hipSetDevice(ctx->device));
+    std::vector<OutlierInfo> list;
+    std::vector<uint8_t> seen(ctx->n_sub, 0);
+    for (uint32_t i = 0; i < n; i++) {
+        if (sub[i] >= ctx->n_sub || seen[sub[i]]) return fail(ctx, GGCAT_GPU_EINVAL, "resplit: sub-bucket out of range or listed twice");
+        if (count[i] < 1 || count[i] > 1024 || (count[i] & (count[i] - 1)))
+            return fail(ctx, GGCAT_GPU_EINVAL, "resplit: bucket count must be a power of two in [1, 1024]");
+        seen[sub[i]] = 1;
+        list.push_back(OutlierInfo{sub[i], count[i], 0});
+    }
+    resplit_firsts(ctx, list, first_sub, n);
+    return resplit_subs(ctx, list);
 }
 
 int ggcat_gpu_records_info(ggcat_gpu_ctx* ctx, uint64_t* n_records, uint32_t* n_sub, uint64_t* n_bases) {
     if (!ctx) return GGCAT_GPU_EINVAL;
     if (!ctx->have_records) return fail(ctx, GGCAT_GPU_ESTATE, "no bucketing result");
     if (n_records) *n_records = ctx->n_records;
-    if (n_sub) *n_sub = ctx->n_sub;
+    if (n_sub) *n_sub = ctx->n_sub_all;
     if (n_bases) *n_bases = ctx->kmer_occ + ctx->n_records * (ctx->P.k - 1);
     return GGCAT_GPU_OK;
 }
@@ -906,6 +1263,8 @@ int ggcat_gpu_records_serialize(ggcat_gpu_ctx* ctx, uint8_t* out, uint64_t cap, 
                                 uint64_t* n_bytes) {
     if (!ctx) return GGCAT_GPU_EINVAL;
     if (!ctx->have_records) return fail(ctx, GGCAT_GPU_ESTATE, "no bucketing result");
+    if (ctx->resplit_done)
+        return fail(ctx, GGCAT_GPU_ESTATE, "resplit buckets have no bucket file of the first bucketing");
     HIPCHK(hipSetDevice(ctx->device));
     const uint32_t SW = ctx->slot_words, k = ctx->P.k, sb = ctx->P.second_buckets_log;
     const uint32_t n_buckets = ctx->n_sub >> sb;
@@ -1024,8 +1383,8 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     M.error_flag = P_<uint32_t>(ctx->error_flag);
     HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->n_items.p, 0, 8, ctx->stream));
-    hipLaunchKernelGGL(make_items_kernel, dim3((ctx->n_sub + 255) / 256), dim3(256), 0, ctx->stream,
-                       P_<uint32_t>(ctx->unit_kmers), P_<uint64_t>(ctx->unit_off), ctx->n_sub, ctx->xb, item_occ,
+    hipLaunchKernelGGL(make_items_kernel, dim3((ctx->n_sub_all + 255) / 256), dim3(256), 0, ctx->stream,
+                       P_<uint32_t>(ctx->unit_kmers), P_<uint64_t>(ctx->unit_off), ctx->n_sub_all, ctx->xb, item_occ,
                        P_<WorkItem>(ctx->items), P_<unsigned int>(ctx->n_items));
     HIPCHK(hipGetLastError());
     uint32_t n_items = 0;
@@ -1303,6 +1662,7 @@ int ggcat_gpu_set_tuning(ggcat_gpu_ctx* ctx, const ggcat_gpu_tuning* t) {
     if (t->rec_cap && (t->rec_cap < 16 || t->rec_cap > 512 || (t->rec_cap & (t->rec_cap - 1))))
         return fail(ctx, GGCAT_GPU_EINVAL, "rec_cap must be 0 or a power of two in [16, 512]");
     if (t->read_block && t->read_block < 256) return fail(ctx, GGCAT_GPU_EINVAL, "read_block must be 0 or >= 256");
+    if (t->upload_chunk && t->upload_chunk < 4096) return fail(ctx, GGCAT_GPU_EINVAL, "upload_chunk must be 0 or >= 4096");
     ctx->tune = *t;
     return GGCAT_GPU_OK;
 }
@@ -1434,7 +1794,6 @@ int ggcat_gpu_plan_build(const ggcat_gpu_build_options* o, uint64_t estimated_ba
 
 namespace {
 
-constexpr uint64_t PIN_CHUNK = 64ull << 20;  // bytes per staging buffer (two of them)
 
 // Plain FASTA / FASTQ files read straight into HBM (parallel reads into a
 // pinned two-buffer ring, each upload overlapping the next read) and parsed

@@ -48,6 +48,7 @@ struct BucketArgs {
     uint32_t canonical;
     uint32_t T;  // windows per tile (multiple of 64)
     uint64_t n_tiles;
+    uint64_t tile0;  // index of the launch's first tile (launches over part of the tiles)
     uint32_t* seg_bits;  // [ceil(n/32)+2] group-start bitmap (written by pass 1)
     uint32_t* sub_count;  // [n_units] records per unit
     unsigned long long* unit_ck;  // [n_units] records << 32 | k-mer occurrences, counted in pass 1
@@ -816,7 +817,7 @@ template <int WC>
 __global__ __launch_bounds__(64) void bucket_stage_kernel(BucketArgs A) {
     extern __shared__ __align__(16) uint8_t lds[];
     Tile t;
-    tile_init(t, &A, blockIdx.x, lds, WC != 0);
+    tile_init(t, &A, A.tile0 + blockIdx.x, lds, WC != 0);
     tile_compute<WC>(t);
     const uint32_t lane = threadIdx.x;
     const uint32_t w = A.w;

@@ -73,6 +73,7 @@ class Tuning(C.Structure):
         ("merge_threads", C.c_uint32),
         ("read_block", C.c_uint32),
         ("host_reader", C.c_uint32),
+        ("upload_chunk", C.c_uint32),
     ]
 
 
@@ -125,6 +126,8 @@ def gpu_lib(path: str = GPU_LIB) -> C.CDLL:
     lib.ggcat_gpu_colormap_close.argtypes = [V]
     lib.ggcat_gpu_partial_unitigs.argtypes = [V, C.c_uint32, _u8p, C.POINTER(C.c_uint16), _u8p, _u8p, _u8p]
     lib.ggcat_gpu_outliers.argtypes = [V, C.c_uint32, _u32p, _u32p, _u32p, _u64p, C.c_uint32]
+    lib.ggcat_gpu_resplit_outliers.argtypes = [V, C.c_uint32, _u32p, _u32p, _u32p, _u32p, C.c_uint32]
+    lib.ggcat_gpu_resplit.argtypes = [V, C.c_uint32, _u32p, _u32p, _u32p]
     lib.ggcat_gpu_query_graph.argtypes = [V, C.c_char_p, C.c_char_p, C.c_char_p, C.c_uint32, C.c_uint32, C.c_int,
                                           C.c_uint32, C.c_int, C.c_uint32, C.c_uint32, C.c_char_p, C.c_uint64]
     lib.ggcat_gpu_colormap_close.restype = None
@@ -164,7 +167,8 @@ def gpu_lib(path: str = GPU_LIB) -> C.CDLL:
                  "ggcat_gpu_extension_copy", "ggcat_gpu_copy_to_host", "ggcat_gpu_query_device_noalloc",
                  "ggcat_gpu_color_subsets", "ggcat_gpu_colormap_write", "ggcat_gpu_colormap_write_sets",
                  "ggcat_gpu_colormap_open", "ggcat_gpu_colormap_info", "ggcat_gpu_colormap_subset",
-                 "ggcat_gpu_query_graph", "ggcat_gpu_partial_unitigs", "ggcat_gpu_outliers"):
+                 "ggcat_gpu_query_graph", "ggcat_gpu_partial_unitigs", "ggcat_gpu_outliers",
+                 "ggcat_gpu_resplit_outliers", "ggcat_gpu_resplit"):
         getattr(lib, name).restype = C.c_int
     _gpu[path] = lib
     return lib
@@ -357,9 +361,10 @@ class GpuContext:
     def set_tuning(self, item_occ: int = 0, lds_max_occ: int = 0, merge_wgs_per_cu: int = 0,
                    part_occ: int = 0, lds_table_log: int = 0, table_pct: int = 0, no_dedup: int = 0,
                    unit_bytes: int = 0, use_graphs: int = 0, unit_cap: int = 0, rec_cap: int = 0,
-                   merge_threads: int = 0, read_block: int = 0, host_reader: int = 0) -> None:
+                   merge_threads: int = 0, read_block: int = 0, host_reader: int = 0,
+                   upload_chunk: int = 0) -> None:
         t = Tuning(item_occ, lds_max_occ, merge_wgs_per_cu, part_occ, lds_table_log, table_pct, no_dedup, unit_bytes,
-                   use_graphs, unit_cap, rec_cap, merge_threads, read_block, host_reader)
+                   use_graphs, unit_cap, rec_cap, merge_threads, read_block, host_reader, upload_chunk)
         self._check(self.lib.ggcat_gpu_set_tuning(self.h, C.byref(t)), "ggcat_gpu_set_tuning")
 
     def records(self) -> Records:
@@ -451,6 +456,30 @@ class GpuContext:
         self._check(self.lib.ggcat_gpu_outliers(self.h, threads, C.byref(n), _ptr(s, _u32p), _ptr(r, _u32p),
                                                 _ptr(c, _u64p), n.value), "outliers")
         return [(int(s[i]), int(r[i]), int(c[i])) for i in range(n.value)]
+
+    def resplit_outliers(self, threads: int) -> list[tuple[int, int, int]]:
+        """Resplits the outlier sub-buckets of the last bucketing (found as
+        outliers(threads) finds them); returns (sub-bucket, resplit bucket
+        count, first resplit sub-bucket id) of each.  The next merge counts
+        every resplit bucket as a sub-bucket of its own."""
+        nr, ns, nb = C.c_uint64(), C.c_uint32(), C.c_uint64()
+        self._check(self.lib.ggcat_gpu_records_info(self.h, C.byref(nr), C.byref(ns), C.byref(nb)), "records_info")
+        cap = int(ns.value) + 1
+        n = C.c_uint32()
+        s, r, f = np.zeros(cap, np.uint32), np.zeros(cap, np.uint32), np.zeros(cap, np.uint32)
+        self._check(self.lib.ggcat_gpu_resplit_outliers(self.h, threads, C.byref(n), _ptr(s, _u32p), _ptr(r, _u32p),
+                                                        _ptr(f, _u32p), cap), "resplit_outliers")
+        return [(int(s[i]), int(r[i]), int(f[i])) for i in range(n.value)]
+
+    def resplit(self, subs: list[tuple[int, int]]) -> list[int]:
+        """Resplits the listed (sub-bucket, bucket count) pairs of the last
+        bucketing; returns the first resplit sub-bucket id of each."""
+        s = np.array([x[0] for x in subs] + [0], np.uint32)
+        c = np.array([x[1] for x in subs] + [0], np.uint32)
+        f = np.zeros(len(subs) + 1, np.uint32)
+        self._check(self.lib.ggcat_gpu_resplit(self.h, len(subs), _ptr(s, _u32p), _ptr(c, _u32p), _ptr(f, _u32p)),
+                    "resplit")
+        return [int(x) for x in f[:len(subs)]]
 
     def partial_unitigs(self, buckets_log: int, n: int) -> dict:
         """Routing of the last per-sub-bucket unitigs (n of them) as the

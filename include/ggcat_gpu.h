@@ -94,13 +94,41 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
  * The order uses the serialized record bytes of each bucket and the records
  * as bucketed (the reference uses compressed file sizes and compacted
  * counts).  Writes up to cap entries (sub-bucket, resplit bucket count,
- * records); *n_outliers = all of them.  Detection only: the merge treats
- * outliers like every other sub-bucket. */
+ * records); *n_outliers = all of them.  Detection only: without
+ * ggcat_gpu_resplit_outliers the merge treats outliers like every other
+ * sub-bucket. */
 int ggcat_gpu_outliers(ggcat_gpu_ctx* ctx, uint32_t threads, uint32_t* n_outliers, uint32_t* sub, uint32_t* resplit,
                        uint64_t* records, uint32_t cap);
 
-/* Sizes of the last bucketing result: records, sub-buckets, and the bytes
- * needed by ggcat_gpu_records_copy for the bases buffer. */
+/* Resplits the outliers of the last bucketing (those ggcat_gpu_outliers
+ * finds with `threads`) as crates/kmers_transform/src/resplitter.rs:47-183
+ * does before their merge: every stored record of an outlier is split again
+ * by process_sequence (crates/assembler_minimizer_bucketing/src/lib.rs:
+ * 152-270) with m' = k - 10 (min(m, 2) for k <= 11,
+ * crates/assembler_kmers_merge/src/lib.rs:236-246), into the outlier's
+ * resplit bucket count of normal buckets plus a duplicates bucket, keeping
+ * its multiplicity and color, with its include flags as the new ends'.
+ * Outlier i's buckets become sub-buckets first_sub[i] .. first_sub[i] +
+ * resplit[i] (the duplicates bucket last), numbered from the bucketing's
+ * sub-bucket count in outlier order; the next ggcat_gpu_kmers_merge counts
+ * each as a sub-bucket of its own (the reference runs the same MapEntry
+ * rules on them).  Writes up to cap entries; *n_outliers = all of them.
+ * Once per bucketing; ggcat_gpu_records_serialize and ggcat_gpu_outliers
+ * then return GGCAT_GPU_ESTATE. */
+int ggcat_gpu_resplit_outliers(ggcat_gpu_ctx* ctx, uint32_t threads, uint32_t* n_outliers, uint32_t* sub,
+                               uint32_t* resplit, uint32_t* first_sub, uint32_t cap);
+
+/* The resplit of ggcat_gpu_resplit_outliers applied to the n listed
+ * sub-buckets of the last bucketing (distinct, below the bucketing's
+ * sub-bucket count) with count[i] normal buckets each (a power of two in
+ * [1, 1024]) instead of the detected outliers: sub-bucket i's buckets become
+ * first_sub[i] .. first_sub[i] + count[i].  Once per bucketing. */
+int ggcat_gpu_resplit(ggcat_gpu_ctx* ctx, uint32_t n, const uint32_t* sub, const uint32_t* count,
+                      uint32_t* first_sub);
+
+/* Sizes of the last bucketing result: records, sub-buckets (resplit
+ * buckets included), and the bytes needed by ggcat_gpu_records_copy for the
+ * bases buffer. */
 int ggcat_gpu_records_info(ggcat_gpu_ctx* ctx, uint64_t* n_records, uint32_t* n_sub, uint64_t* n_bases);
 
 /* Decoded copy of the last bucketing result (test/inspection path):
@@ -204,7 +232,12 @@ int ggcat_gpu_solid_colors(ggcat_gpu_ctx* ctx, uint32_t* color_words, uint32_t* 
  *                     with that block size (it decides where long FASTA
  *                     records are cut), so only tests set it
  *   host_reader       1: read input files on the host even when they could be
- *                     read into HBM and parsed there (same records) */
+ *                     read into HBM and parsed there (same records)
+ *   upload_chunk      host input of ggcat_gpu_bucketing: bytes per upload
+ *                     chunk of the pipeline that stages resident tiles while
+ *                     later chunks upload (0 = 64 MiB, the pinned staging
+ *                     buffer; at most that); input shorter than two chunks
+ *                     is copied whole before the staging pass */
 typedef struct {
     uint32_t item_occ;
     uint32_t lds_max_occ;
@@ -220,11 +253,12 @@ typedef struct {
     uint32_t merge_threads;
     uint32_t read_block;
     uint32_t host_reader;
+    uint32_t upload_chunk;
 } ggcat_gpu_tuning;
 int ggcat_gpu_set_tuning(ggcat_gpu_ctx* ctx, const ggcat_gpu_tuning* tuning);
 
 /* Device-side timings of the last calls, in milliseconds:
- * [0] bucketing staging pass, [1] scan, [2] bucketing deduplication + region compaction,
+ * [0] bucketing staging pass (with the uploads of chunked host input), [1] scan, [2] bucketing deduplication + region compaction,
  * [3] merge (whole-item pass), [4] merge (partitioned passes),
  * [5] whole bucketing call, [6] whole merge call, [7] unitig kernels,
  * [8] reading the input of the last build_from_files (wall clock: files
@@ -239,7 +273,9 @@ int ggcat_gpu_timings(ggcat_gpu_ctx* ctx, double* ms, uint32_t n);
  * [8] how the last build_from_files read its input (1: into HBM, parsed on
  * the device; 2: host reader), [9] long FASTA records cut on the device,
  * [10] records stored after identical copies in a unit were counted once
- * (the arena's slots; [1] counts every copy). */
+ * (the arena's slots; [1] counts every copy; copies past their unit's
+ * region are stored apart, so this count can vary by a few records between
+ * runs while every result stays the same). */
 int ggcat_gpu_counters(ggcat_gpu_ctx* ctx, uint64_t* out, uint32_t n);
 
 /* Reads FASTA/FASTQ files (optionally gzip-compressed) into the joined layout.

@@ -51,6 +51,8 @@ def lib() -> C.CDLL:
     L.orc_rec_get.restype = None
     L.orc_merge.argtypes = [V, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_uint32, C.c_int]
     L.orc_merge.restype = None
+    L.orc_resplit.argtypes = [V, C.c_uint32, C.c_uint32, _u32p, _u32p, C.c_size_t, C.c_uint32]
+    L.orc_resplit.restype = None
     L.orc_solid_count.argtypes = [V]
     L.orc_solid_count.restype = C.c_size_t
     L.orc_solid_get.argtypes = [V, _u32p, _u64p, _u64p, _u32p, _u8p]
@@ -142,6 +144,14 @@ class Result:
 
     def merge(self, k: int, b: int, sb: int, kind: int, min_mult: int, threads: int = 1) -> None:
         lib().orc_merge(self.h, k, b, sb, kind, min_mult, threads)
+
+    def resplit(self, k: int, m: int, outliers: list[tuple[int, int]], first_extra: int) -> None:
+        """Replaces the records of each (sub-bucket, resplit bucket count)
+        outlier by its resplit records, numbered from first_extra (each
+        outlier's count normal buckets, then its duplicates bucket)."""
+        subs = np.array([o[0] for o in outliers], np.uint32)
+        cnts = np.array([o[1] for o in outliers], np.uint32)
+        lib().orc_resplit(self.h, k, m, _p(subs, _u32p), _p(cnts, _u32p), len(outliers), first_extra)
 
     def merge_sorting(self, k: int, m: int, b: int, sb: int, kind: int, min_mult: int, threads: int = 1) -> dict:
         """The sorting extender's rule on the normal sub-buckets: entries

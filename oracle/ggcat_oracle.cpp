@@ -10,6 +10,10 @@
 //   ntHash m-mer hash (nthash_m)          pinned: closed form + RC symmetry + roll==direct
 //   BatchMinQueue restatement (minqueue)  pinned: brute-force closed-form window minimum
 //   process_sequence / splits (bucket_run) pinned: super-k-mer cover property + brute force
+//   outlier resplit (resplit_records)     pinned: independent pure-Python restatement (brute-force
+//                                         windows with the record's include flags) + the solid
+//                                         k-mers and multiplicities of an outlier equal before and
+//                                         after its resplit
 //   seq-hash u64/u128, rabin-karp128      pinned: invertibility / roll==direct / RC symmetry
 //   MapEntry merge (merge_subbucket)      pinned: union of solid sets == exact k-mer counts
 //   compaction (compact)                  pinned: multiset expansion equality
@@ -231,13 +235,18 @@ inline uint8_t normalize(uint8_t c) {
     return 'N';
 }
 
-// crates/assembler_minimizer_bucketing/src/lib.rs:171-270 process_sequence,
-// applied to one N-free run (include_first = include_last = true, as set by
-// preprocess_dna_sequence :149-150).
-void bucket_run(const uint8_t* seq, size_t L, const Params& P, uint32_t color, RecordSet& out,
-                std::vector<uint64_t>& fw, std::vector<uint64_t>& rcv, std::vector<Item>& items,
-                BatchMinQueue& q) {
+// crates/assembler_minimizer_bucketing/src/lib.rs:171-270 process_sequence
+// with SEPARATE_DUPLICATES over one N-free sequence.  include_first /
+// include_last come from preprocess_dna_sequence (:149-150, both true) for
+// an input run, or from reprocess_sequence (:152-168: the record's
+// READ_FLAG_INCL_BEGIN / READ_FLAG_INCL_END) when a record is split again;
+// an end that is not included is skipped by the minimizer queue (:187-188).
+// Every record emitted carries `mult`.
+void split_sequence(const uint8_t* seq, size_t L, const Params& P, uint32_t color, uint32_t mult,
+                    bool include_first0, bool include_last0, RecordSet& out, std::vector<uint64_t>& fw,
+                    std::vector<uint64_t>& rcv, std::vector<Item>& items, BatchMinQueue& q) {
     const uint32_t k = P.k, m = P.m;
+    if (L < m) return;
     size_t n = L - m + 1;
     fw.resize(n);
     rcv.resize(n);
@@ -251,10 +260,11 @@ void bucket_run(const uint8_t* seq, size_t L, const Params& P, uint32_t color, R
         items[i].x.is_forward = fw[i] < rcv[i];
     }
     size_t last_index = 1;
-    bool include_first = true;
-    const bool include_last_pre = true;
+    bool include_first = include_first0;
+    const bool include_last_pre = include_last0;
     const uint32_t dup_bucket = 1u << P.b;
-    q.get_minimizer_splits(items.data(), n, 0, 0, [&](size_t index, const Item& mh, bool is_last) {
+    const size_t skip_before = include_first0 ? 0 : 1, skip_after = include_last0 ? 0 : 1;
+    q.get_minimizer_splits(items.data(), n, skip_before, skip_after, [&](size_t index, const Item& mh, bool is_last) {
         bool include_last = include_last_pre && is_last;
         uint32_t bucket;
         bool rc;
@@ -278,7 +288,7 @@ void bucket_run(const uint8_t* seq, size_t L, const Params& P, uint32_t color, R
         out.len.push_back((uint32_t)(s1 - s0));
         out.flags.push_back(flags);
         out.minpos.push_back(minimizer_pos);
-        out.mult.push_back(1);
+        out.mult.push_back(mult);
         out.color.push_back(color);
         out.off.push_back(out.bases.size());
         if (rc) {
@@ -289,6 +299,41 @@ void bucket_run(const uint8_t* seq, size_t L, const Params& P, uint32_t color, R
         last_index = index;
         include_first = false;
     });
+}
+
+// One N-free run of the input (include_first = include_last = true).
+void bucket_run(const uint8_t* seq, size_t L, const Params& P, uint32_t color, RecordSet& out,
+                std::vector<uint64_t>& fw, std::vector<uint64_t>& rcv, std::vector<Item>& items,
+                BatchMinQueue& q) {
+    split_sequence(seq, L, P, color, 1, true, true, out, fw, rcv, items, q);
+}
+
+// Resplit of an outlier sub-bucket (crates/kmers_transform/src/resplitter.rs:47-183):
+// every record, in its stored orientation and with its multiplicity and
+// color, goes through process_sequence again with the resplit minimizer
+// length (crates/assembler_kmers_merge/src/lib.rs:236-246: m' = k - 10 when
+// k > 11, else min(m, 2)), 2^log_buckets normal buckets and one duplicates
+// bucket (index 2^log_buckets, new_with_duplicates at
+// crates/assembler_kmers_merge/src/lib.rs:120-128), no second bucket, and
+// include_first / include_last from the record's flags (reprocess_sequence).
+// Output records carry sub = base + resplit bucket.
+inline uint32_t resplit_m(uint32_t k, uint32_t m) { return k > 11 ? k - 10 : std::min<uint32_t>(m, 2); }
+
+void resplit_records(const RecordSet& R, const std::vector<size_t>& idx, uint32_t k, uint32_t m,
+                     uint32_t log_buckets, uint32_t base, RecordSet& out) {
+    Params P{k, resplit_m(k, m), log_buckets, 0, 1};
+    std::vector<uint64_t> fw, rcv;
+    std::vector<Item> items;
+    BatchMinQueue q(P.k - P.m);
+    RecordSet part;
+    for (size_t r : idx) {
+        part.clear();
+        const uint8_t fl = R.flags[r];
+        split_sequence(R.bases.data() + R.off[r], R.len[r], P, R.color[r], R.mult[r], (fl & 1) != 0, (fl & 2) != 0,
+                       part, fw, rcv, items, q);
+        for (auto& s : part.sub) s += base;
+        out.append(part);
+    }
 }
 
 // crates/minimizer_bucketing/src/sequences_splitter.rs:15-40 (runs >= k of
@@ -907,11 +952,51 @@ void orc_rec_colors(void* hv, uint32_t* color) {
     memcpy(color, R.color.data(), R.color.size() * 4);
 }
 
-// Per-sub-bucket MapEntry merge over the handle's records, `threads` workers.
+// Replaces the records of each listed outlier sub-bucket by its resplit
+// records (resplit_records).  Outlier i is split into counts[i] (a power of
+// two) normal buckets plus its duplicates bucket, which get the sub-bucket
+// ids base_i .. base_i + counts[i] (the duplicates bucket last), with
+// base_0 = first_extra and base_{i+1} = base_i + counts[i] + 1.
+void orc_resplit(void* hv, uint32_t k, uint32_t m, const uint32_t* subs, const uint32_t* counts, size_t n,
+                 uint32_t first_extra) {
+    Handle* h = (Handle*)hv;
+    RecordSet& R = h->rec;
+    std::unordered_map<uint32_t, size_t> which;
+    for (size_t i = 0; i < n; i++) which[subs[i]] = i;
+    std::vector<std::vector<size_t>> idx(n);
+    RecordSet kept;
+    for (size_t r = 0; r < R.sub.size(); r++) {
+        auto it = which.find(R.sub[r]);
+        if (it != which.end()) {
+            idx[it->second].push_back(r);
+            continue;
+        }
+        kept.sub.push_back(R.sub[r]);
+        kept.len.push_back(R.len[r]);
+        kept.flags.push_back(R.flags[r]);
+        kept.minpos.push_back(R.minpos[r]);
+        kept.mult.push_back(R.mult[r]);
+        kept.color.push_back(R.color[r]);
+        kept.off.push_back(kept.bases.size());
+        kept.bases.insert(kept.bases.end(), R.bases.begin() + R.off[r], R.bases.begin() + R.off[r] + R.len[r]);
+    }
+    uint32_t base = first_extra;
+    for (size_t i = 0; i < n; i++) {
+        uint32_t lg = 0;
+        while ((1u << lg) < counts[i]) lg++;
+        resplit_records(R, idx[i], k, m, lg, base, kept);
+        base += (1u << lg) + 1;
+    }
+    R = std::move(kept);
+}
+
+// Per-sub-bucket MapEntry merge over the handle's records, `threads` workers
+// (sub-bucket ids past the bucketing's, from a resplit, included).
 void orc_merge(void* hv, uint32_t k, uint32_t b, uint32_t sb, int kind, uint32_t min_mult, int threads) {
     Handle* h = (Handle*)hv;
     auto t0 = std::chrono::steady_clock::now();
     uint32_t n_sub = ((1u << b) + 1) << sb;
+    for (uint32_t s : h->rec.sub) n_sub = std::max(n_sub, s + 1);
     auto groups = group_by_sub(h->rec, n_sub);
     if (threads < 1) threads = 1;
     std::vector<SolidSet> parts(threads);

@@ -174,3 +174,79 @@ def test_compaction_counts_identical_records_emulated(emu_ctx):
         assert np.array_equal(got, colored_solid_array(o.solid(), bitmaps_from_sets(off, cols, 1)))
     finally:
         emu_ctx.set_tuning()
+
+
+@pytest.mark.parametrize("k,m,kind", [(31, 15, HASH_SEQ), (63, 21, HASH_RK128)])
+def test_resplit_emulated(emu_ctx, k, m, kind):
+    # the three largest sub-buckets and the largest duplicates-bucket
+    # sub-bucket of a repeat-rich input (with tandem repeats: long pool
+    # records) resplit on the device as the oracle resplits them; then the
+    # records and the solid table, every resplit bucket a sub-bucket of its own
+    rng = np.random.default_rng(83)
+    g = synth.genome(81, 30_000, repeat_frac=0.3, rep_min=500, rep_max=2000)
+    data = np.concatenate([synth.reads(g, 82, 500, 150, 0.005), joined(adversarial_records(rng)[:30])])
+    b, sb = 3, 1
+    n_sub = ((1 << b) + 1) << sb
+    emu_ctx.set_tuning(merge_wgs_per_cu=2)
+    try:
+        emu_ctx.bucketing(data, Params(k, m, b, sb, 0, kind, 2))
+        o = O.bucketing(data, k, m, b, sb, threads=2)
+        r = o.records()
+        cnt = np.bincount(r["sub"].astype(np.int64), minlength=n_sub)
+        normal = [int(s) for s in np.argsort(-cnt[:(1 << b) << sb], kind="stable")[:3]]
+        dup = int(((1 << b) << sb) + np.argmax(cnt[(1 << b) << sb:]))
+        assert cnt[dup] > 0
+        lst = [(normal[0], 4), (dup, 8), (normal[1], 1), (normal[2], 16)]
+        first = emu_ctx.resplit(lst)
+        assert first == [n_sub, n_sub + 5, n_sub + 14, n_sub + 16]
+        o.resplit(k, m, lst, n_sub)
+        got_r, want_r = record_tuples(emu_ctx.records()), record_tuples(o.records())
+        assert len(got_r) == len(want_r) and got_r == want_r
+        assert any(t[0] >= n_sub for t in got_r)
+        emu_ctx.kmers_merge()
+        o.merge(k, b, sb, kind, 2, 2)
+        assert np.array_equal(solid_array(emu_ctx.solid()), solid_array(o.solid()))
+        with pytest.raises(Exception):
+            emu_ctx.resplit(lst)  # once per bucketing
+    finally:
+        emu_ctx.set_tuning()
+
+
+def test_resplit_colored_emulated(emu_ctx):
+    # pieces keep their record's color: colored solid table after a resplit
+    gs = synth.config3_genomes(0.002, 12)
+    data, starts = synth.contigs_joined(gs, 5, 3000, 9000)
+    colors = (np.arange(12, dtype=np.uint32) * 29) % 60
+    b, sb = 3, 2
+    n_sub = ((1 << b) + 1) << sb
+    emu_ctx.set_tuning(merge_wgs_per_cu=2)
+    try:
+        emu_ctx.bucketing(data, Params(31, 15, b, sb, 0, HASH_SEQ, 1), colors=(starts, colors))
+        o = O.bucketing_colored(data, 31, 15, b, sb, starts, colors, threads=2)
+        cnt = np.bincount(o.records()["sub"].astype(np.int64), minlength=n_sub)
+        lst = [(int(s), 8) for s in np.argsort(-cnt, kind="stable")[:2]]
+        assert emu_ctx.resplit(lst) == [n_sub, n_sub + 9]
+        o.resplit(31, 15, lst, n_sub)
+        emu_ctx.kmers_merge()
+        o.merge(31, b, sb, HASH_SEQ, 1, 2)
+        got = colored_solid_array(emu_ctx.solid(), emu_ctx.solid_colors())
+        off, cols = o.solid_colors()
+        want = colored_solid_array(o.solid(), bitmaps_from_sets(off, cols, 2))
+        assert np.array_equal(got, want)
+        assert (emu_ctx.solid().sub >= n_sub).sum() > 100
+    finally:
+        emu_ctx.set_tuning()
+
+
+@pytest.mark.parametrize("k,m,kind,chunk", [(31, 15, HASH_SEQ, 16384), (63, 21, HASH_RK128, 20000)])
+def test_chunked_host_upload_emulated(emu_ctx, k, m, kind, chunk):
+    # host input uploaded in chunks while the staging pass runs over the
+    # tiles already resident: the same records and solid table as the oracle
+    g = synth.genome(87, 20_000, repeat_frac=0.2, rep_min=300, rep_max=1000)
+    data = synth.reads(g, 88, 1_200, 150, 0.005)
+    assert data.size > 8 * chunk
+    emu_ctx.set_tuning(merge_wgs_per_cu=2, upload_chunk=chunk)
+    try:
+        _check(emu_ctx, data, k, m, 3, 1, 2, kind)
+    finally:
+        emu_ctx.set_tuning()

@@ -536,6 +536,81 @@ def test_outlier_detection(gpu_ctx):
                                                       ((1 << 4) + 1) << 2, 2, 64)
 
 
+@pytest.mark.parametrize("k,m,kind,chunk", [(31, 15, HASH_SEQ, 1 << 20), (63, 21, HASH_RK128, 1 << 18)])
+def test_chunked_host_upload(gpu_ctx, k, m, kind, chunk):
+    # host input uploaded chunk by chunk on the copy stream while the staging
+    # pass runs over the tiles already resident: same records and solid table
+    g = synth.genome(89, 1_000_000, repeat_frac=0.2)
+    data = synth.reads(g, 90, 40_000, 150, 0.005)
+    assert data.size > 4 * chunk
+    _say(f"case chunked upload k={k} chunk={chunk} bytes={data.size}")
+    gpu_ctx.set_tuning(upload_chunk=chunk)
+    try:
+        _run_both(gpu_ctx, data, k, m, 6, 2, 2, kind, exact=False)
+    finally:
+        gpu_ctx.set_tuning()
+
+
+@pytest.mark.parametrize("k,m,kind", [(31, 15, HASH_SEQ), (63, 21, HASH_RK128)])
+def test_resplit_listed_subbuckets(gpu_ctx, k, m, kind):
+    # the largest sub-buckets of a repeat-rich read set resplit on the device
+    # as the oracle resplits them (crates/kmers_transform/src/resplitter.rs);
+    # every resplit bucket is then counted as a sub-bucket of its own
+    g = synth.genome(85, 300_000, repeat_frac=0.3, rep_min=500, rep_max=3000)
+    data = synth.reads(g, 86, 30_000, 150, 0.005)
+    b, sb = 4, 2
+    n_sub = ((1 << b) + 1) << sb
+    _say(f"case resplit listed k={k} m={m} bytes={data.size}")
+    gpu_ctx.bucketing(data, _params(k, m, b, sb, 2, kind))
+    o = O.bucketing(data, k, m, b, sb, threads=4)
+    orec = o.records()
+    assert int(orec["length"].max()) <= inline_capacity(k, m)  # inline slots only
+    cnt = np.bincount(orec["sub"].astype(np.int64), minlength=n_sub)
+    top = [int(s) for s in np.argsort(-cnt, kind="stable")[:4]]
+    lst = [(top[0], 4), (top[1], 16), (top[2], 1), (top[3], 1024)]
+    assert gpu_ctx.resplit(lst) == [n_sub, n_sub + 5, n_sub + 22, n_sub + 24]
+    o.resplit(k, m, lst, n_sub)
+    assert np.array_equal(record_digests(gpu_ctx.records()), record_digests(o.records()))
+    gpu_ctx.kmers_merge()
+    o.merge(k, b, sb, kind, 2, 4)
+    ga, oa = solid_array(gpu_ctx.solid()), solid_array(o.solid())
+    assert ga.shape == oa.shape and np.array_equal(ga, oa)
+    assert int((gpu_ctx.solid().sub >= n_sub).sum()) > 1000
+
+
+def test_resplit_detected_outliers(gpu_ctx):
+    # outliers found and resplit on the device: tandem-repeat reads in one
+    # duplicates-bucket sub-bucket and 210 000 copies of one 40-base read in
+    # two normal sub-buckets (copies past 1024 per stored record); the oracle
+    # resplits the same list; solid table and per-sub-bucket unitigs
+    from tests.test_unitigs import canon_unitigs
+    rng = np.random.default_rng(3)
+    g = synth.genome(93, 200_000)
+    ac = [(b"AC" * 30)[p:p + 40] for p in rng.integers(0, 2, 220_000)]
+    hot = joined([bytes(g[50_000:50_040])] * 210_000)
+    data = np.concatenate([synth.reads(g, 94, 13_000, 150, 0.005), joined(ac), hot])
+    b, sb = 4, 2
+    n_sub = ((1 << b) + 1) << sb
+    _say(f"case resplit detected outliers bytes={data.size}")
+    gpu_ctx.bucketing(data, _params(31, 15, b, sb, 2))
+    o = O.bucketing(data, 31, 15, b, sb, threads=4)
+    orec = o.records()
+    want = _outliers_restated(orec["sub"].astype(np.int64), orec["length"].astype(np.int64), n_sub, sb, 4)
+    assert [w[0] for w in want] == [64, 4, 9]
+    got = gpu_ctx.resplit_outliers(4)
+    assert [(x[0], x[1]) for x in got] == [(w[0], w[1]) for w in want]
+    o.resplit(31, 15, [(x[0], x[1]) for x in got], n_sub)
+    assert [x[2] for x in got] == [n_sub, n_sub + 9, n_sub + 42]
+    assert np.array_equal(record_digests(gpu_ctx.records()), record_digests(o.records()))
+    gpu_ctx.kmers_merge()
+    o.merge(31, b, sb, HASH_SEQ, 2, 4)
+    ga, oa = solid_array(gpu_ctx.solid()), solid_array(o.solid())
+    assert ga.shape == oa.shape and np.array_equal(ga, oa)
+    assert canon_unitigs(*gpu_ctx.unitigs()) == canon_unitigs(*o.unitigs(31))
+    with pytest.raises(native.GGCATGpuError):
+        gpu_ctx.outliers(4)  # the bucketing's outliers are resplit
+
+
 def _seqhash_bucket(kmer: bytes, k: int, log: int) -> tuple[int, bool]:
     # CanonicalSeqHashFactory::get_bucket(0, log, min(fw, rc)) with wrapping
     # arithmetic (crates/hashes/src/base/cn_seqhash_base.rs:140-150,
