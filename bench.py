@@ -503,7 +503,7 @@ def main() -> None:
         wl4 = make_workload(4, a.scale)
         log(f"[bench] workload {wl4.name}: {wl4.data.size} bytes, {wl4.bases} bases "
             f"({time.perf_counter() - t:.1f} s to generate)")
-        sample, sample_name = shape_sample(4, a.scale, wl4.data.size, 64 << 20)
+        sample, sample_name = shape_sample(4, a.scale, wl4.data.size, 256 << 20)
         out["config4"] = measure(ctx, wl4, max(1, min(a.steps, 2)), 1, not a.no_autotune, not a.no_check, threads,
                                  a.scale, sample=sample, sample_name=sample_name, host_steps=min(a.host_steps, 1))
         del sample

@@ -771,8 +771,17 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     // stored records, then every stored record to its unit's slots.
     if (n_units) {
         const uint32_t grid = (uint32_t)std::min<uint64_t>(n_units, (uint64_t)ctx->n_cu * 64);
-        hipLaunchKernelGGL(unit_dedup_kernel, dim3(grid), dim3(64), dedup_lds_bytes(slot_words, n_colors != 0),
-                           ctx->stream, A, n_units);
+        // units of more than DEDUP_SMALL region records: 256-thread workgroups
+        // (their region must fit the registers that stage it: 8 pieces per thread)
+        const bool wide = unit_cap > DEDUP_SMALL && DEDUP_WIDE * slot_words / 4 <= 8 * 256;
+        hipLaunchKernelGGL(unit_dedup_kernel, dim3(grid), dim3(64),
+                           dedup_lds_bytes(DEDUP_SMALL, slot_words, n_colors != 0), ctx->stream, A, n_units,
+                           wide ? 1u : 0u);
+        if (wide) {
+            const uint32_t wgrid = (uint32_t)std::min<uint64_t>(n_units, (uint64_t)ctx->n_cu * 8);
+            hipLaunchKernelGGL(unit_dedup_wide_kernel, dim3(wgrid), dim3(256),
+                               dedup_lds_bytes(DEDUP_WIDE, slot_words, n_colors != 0), ctx->stream, A, n_units);
+        }
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipEventRecord(ctx->ev[15], ctx->stream));

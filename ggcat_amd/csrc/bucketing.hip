@@ -1021,16 +1021,17 @@ __global__ void bucket_reset_kernel(BucketArgs A) {
 // compactor merges identical super-k-mers of a bucket
 // (crates/minimizer_bucketing/src/compactor.rs:128-196: same bases, length
 // and flags; multiplicities summed).  Identical super-k-mers share their
-// minimizer, so all copies of a record live in one unit.  One 64-thread
-// workgroup per unit of 2..DEDUP_MAX region records (others are kept as they
-// are): the records are loaded into LDS with 16-byte loads, each is inserted
-// into an LDS table keyed by its slot words and color, a record equal to an
-// earlier inserted one adds to that one's multiplicity (up to MAX_MULT per
-// stored record), and the stored records are written back, in region order,
-// to the front of the region with multiplicity - 1 in their meta word.
-// Spilled records (rank >= unit_cap) are stored as they are.
-constexpr uint32_t DEDUP_MAX = 256;
-constexpr uint32_t DEDUP_TABLE = 512;
+// minimizer, so all copies of a record live in one unit.  One workgroup per
+// unit whose region holds 2..DEDUP_WIDE records: 64 threads for units of up to
+// DEDUP_SMALL records, 256 above (two launches; the small one also passes the
+// other units through).  The records are loaded into LDS with 16-byte loads,
+// each is inserted into an LDS table keyed by its slot words and color, a
+// record equal to an earlier inserted one adds to that one's multiplicity (up
+// to MAX_MULT per stored record), and the stored records are written back, in
+// region order, to the front of the region with multiplicity - 1 in their
+// meta word.  Spilled records (rank >= unit_cap) are stored as they are.
+constexpr uint32_t DEDUP_SMALL = 256;
+constexpr uint32_t DEDUP_WIDE = 1024;
 
 GG_DEV uint32_t slot_hash(const uint32_t* w, uint32_t sw, uint32_t color) {
     uint32_t h = 0x9E3779B9u ^ color;
@@ -1043,36 +1044,44 @@ GG_DEV uint32_t slot_hash(const uint32_t* w, uint32_t sw, uint32_t color) {
     return h ^ (h >> 16);
 }
 
-// Dynamic LDS of unit_dedup_kernel: the records' slot words, then (colored
-// builds) their colors.
-__host__ __device__ inline uint32_t dedup_lds_bytes(uint32_t slot_words, bool colored) {
-    return DEDUP_MAX * slot_words * 4 + (colored ? DEDUP_MAX * 4 : 0);
+// Dynamic LDS of a dedup workgroup taking up to max_rec records: their slot
+// words, then (colored builds) their colors.
+__host__ __device__ inline uint32_t dedup_lds_bytes(uint32_t max_rec, uint32_t slot_words, bool colored) {
+    return max_rec * slot_words * 4 + (colored ? max_rec * 4 : 0);
 }
 
-__global__ __launch_bounds__(64) void unit_dedup_kernel(BucketArgs A, uint32_t n_units) {
+// NT threads per workgroup, units of LO < records <= HI (and, with PASS, the
+// other units kept as they are, except those of (DEDUP_SMALL, DEDUP_WIDE]
+// when the wide launch takes them).  At most 8 16-byte pieces per thread:
+// HI * slot_words / 4 <= 8 * NT.
+template <uint32_t NT, uint32_t LO, uint32_t HI, bool PASS>
+GG_DEV void unit_dedup_body(const BucketArgs& A, uint32_t n_units, bool wide) {
     extern __shared__ __align__(16) uint8_t lds[];
-    __shared__ uint32_t mult[DEDUP_MAX];  // 0: a copy of another record
-    __shared__ uint32_t table[DEDUP_TABLE];  // record index + 1, 0 = free
-    const uint32_t SW = A.slot_words, lane = threadIdx.x;
+    __shared__ uint32_t mult[HI];  // 0: a copy of another record
+    __shared__ uint32_t table[2 * HI];  // record index + 1, 0 = free
+    __shared__ uint32_t wave_n[NT / 64];
+    constexpr uint32_t TABLE = 2 * HI;
+    const uint32_t SW = A.slot_words, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     uint32_t* rec = reinterpret_cast<uint32_t*>(lds);  // slot words
-    uint32_t* col = rec + DEDUP_MAX * SW;  // colored builds only
-    // a unit's region is loaded into registers (at most 8 16-byte pieces per
-    // lane) while the previous unit is processed
+    uint32_t* col = rec + HI * SW;  // colored builds only
+    auto mine = [&](uint32_t n) { return n > LO && n <= HI; };
+    // a unit's region is loaded into registers while the previous unit is
+    // processed
     uint4 p0 = {}, p1 = {}, p2 = {}, p3 = {}, p4 = {}, p5 = {}, p6 = {}, p7 = {};
 #define GG_DEDUP_PREFETCH(UU, CNT)                                                                 \
     do {                                                                                           \
         const uint32_t n_ = (CNT) < A.unit_cap ? (CNT) : A.unit_cap;                               \
-        if (n_ <= DEDUP_MAX && n_ >= 2) {                                                          \
+        if (mine(n_)) {                                                                            \
             const uint4* s_ = reinterpret_cast<const uint4*>(A.region + (uint64_t)(UU) * A.unit_cap * SW); \
             const uint32_t m_ = n_ * (SW / 4);                                                     \
-            if (lane < m_) p0 = s_[lane];                                                          \
-            if (lane + 64 < m_) p1 = s_[lane + 64];                                                \
-            if (lane + 128 < m_) p2 = s_[lane + 128];                                              \
-            if (lane + 192 < m_) p3 = s_[lane + 192];                                              \
-            if (lane + 256 < m_) p4 = s_[lane + 256];                                              \
-            if (lane + 320 < m_) p5 = s_[lane + 320];                                              \
-            if (lane + 384 < m_) p6 = s_[lane + 384];                                              \
-            if (lane + 448 < m_) p7 = s_[lane + 448];                                              \
+            if (tid < m_) p0 = s_[tid];                                                            \
+            if (tid + NT < m_) p1 = s_[tid + NT];                                                  \
+            if (tid + 2 * NT < m_) p2 = s_[tid + 2 * NT];                                          \
+            if (tid + 3 * NT < m_) p3 = s_[tid + 3 * NT];                                          \
+            if (tid + 4 * NT < m_) p4 = s_[tid + 4 * NT];                                          \
+            if (tid + 5 * NT < m_) p5 = s_[tid + 5 * NT];                                          \
+            if (tid + 6 * NT < m_) p6 = s_[tid + 6 * NT];                                          \
+            if (tid + 7 * NT < m_) p7 = s_[tid + 7 * NT];                                          \
         }                                                                                          \
     } while (0)
     uint32_t u = blockIdx.x;
@@ -1085,8 +1094,10 @@ __global__ __launch_bounds__(64) void unit_dedup_kernel(BucketArgs A, uint32_t n
         uint32_t* regc = A.region_color ? A.region_color + (uint64_t)u * A.unit_cap : nullptr;
         const uint32_t un = u + gridDim.x;
         const uint32_t cnt_next = un < n_units ? A.sub_count[un] : 0u;
-        if (n > DEDUP_MAX || n < 2) {
-            if (lane == 0) {
+        if (!mine(n)) {
+            // kept as it is (by the small launch for every unit the wide
+            // launch does not take)
+            if (PASS && tid == 0 && !(wide && n > DEDUP_SMALL && n <= DEDUP_WIDE)) {
                 A.unit_dregion[u] = n;
                 A.unit_dcount[u] = cnt;
             }
@@ -1096,29 +1107,29 @@ __global__ __launch_bounds__(64) void unit_dedup_kernel(BucketArgs A, uint32_t n
         }
         const uint32_t n4 = n * (SW / 4);
         uint4* r4 = reinterpret_cast<uint4*>(rec);
-        if (lane < n4) r4[lane] = p0;
-        if (lane + 64 < n4) r4[lane + 64] = p1;
-        if (lane + 128 < n4) r4[lane + 128] = p2;
-        if (lane + 192 < n4) r4[lane + 192] = p3;
-        if (lane + 256 < n4) r4[lane + 256] = p4;
-        if (lane + 320 < n4) r4[lane + 320] = p5;
-        if (lane + 384 < n4) r4[lane + 384] = p6;
-        if (lane + 448 < n4) r4[lane + 448] = p7;
-        for (uint32_t i = lane; i < n; i += 64) {
+        if (tid < n4) r4[tid] = p0;
+        if (tid + NT < n4) r4[tid + NT] = p1;
+        if (tid + 2 * NT < n4) r4[tid + 2 * NT] = p2;
+        if (tid + 3 * NT < n4) r4[tid + 3 * NT] = p3;
+        if (tid + 4 * NT < n4) r4[tid + 4 * NT] = p4;
+        if (tid + 5 * NT < n4) r4[tid + 5 * NT] = p5;
+        if (tid + 6 * NT < n4) r4[tid + 6 * NT] = p6;
+        if (tid + 7 * NT < n4) r4[tid + 7 * NT] = p7;
+        for (uint32_t i = tid; i < n; i += NT) {
             if (regc) col[i] = regc[i];
             mult[i] = 1;
         }
-        for (uint32_t i = lane; i < DEDUP_TABLE; i += 64) table[i] = 0;
+        for (uint32_t i = tid; i < TABLE; i += NT) table[i] = 0;
         __syncthreads();
         // the next unit's records are in flight meanwhile
         cnt = cnt_next;
         if (un < n_units) GG_DEDUP_PREFETCH(un, cnt);
-        for (uint32_t i = lane; i < n; i += 64) {
+        for (uint32_t i = tid; i < n; i += NT) {
             const uint32_t* w = rec + i * SW;
             if (w[0] & META_LONG) continue;  // pool records: bases elsewhere, kept as they are
             const uint32_t ci = regc ? col[i] : 0u;
-            uint32_t slot = slot_hash(w, SW, ci) & (DEDUP_TABLE - 1);
-            for (uint32_t probe = 0; probe < DEDUP_TABLE; probe++) {
+            uint32_t slot = slot_hash(w, SW, ci) & (TABLE - 1);
+            for (uint32_t probe = 0; probe < TABLE; probe++) {
                 uint32_t cur = table[slot];
                 if (cur == 0) {
                     cur = atomicCAS(&table[slot], 0u, i + 1);
@@ -1139,33 +1150,54 @@ __global__ __launch_bounds__(64) void unit_dedup_kernel(BucketArgs A, uint32_t n
                     if (m < MAX_MULT) mult[i] = 0;
                     break;
                 }
-                slot = (slot + 1) & (DEDUP_TABLE - 1);
+                slot = (slot + 1) & (TABLE - 1);
             }
         }
         __syncthreads();
         // stored records to the front of the region, in region order
         uint32_t placed = 0;
-        for (uint32_t i0 = 0; i0 < n; i0 += 64) {
-            const uint32_t i = i0 + lane;
+        for (uint32_t i0 = 0; i0 < n; i0 += NT) {
+            const uint32_t i = i0 + tid;
             const uint32_t m = i < n ? mult[i] : 0u;
             const uint64_t bal = __ballot(m != 0);
+            uint32_t before = 0, total = (uint32_t)__popcll(bal);
+            if (NT > 64) {
+                if (lane == 0) wave_n[wave] = total;
+                __syncthreads();
+                total = 0;
+                for (uint32_t q = 0; q < NT / 64; q++) {
+                    const uint32_t c = wave_n[q];
+                    if (q < wave) before += c;
+                    total += c;
+                }
+            }
             if (m) {
-                const uint32_t dst = placed + (uint32_t)__popcll(bal & ((1ULL << lane) - 1));
+                const uint32_t dst = placed + before + (uint32_t)__popcll(bal & ((1ULL << lane) - 1));
                 const uint32_t* w = rec + i * SW;
                 uint32_t* d = reg + (uint64_t)dst * SW;
                 d[0] = w[0] | ((m - 1) << META_MULT_SHIFT);
                 for (uint32_t q = 1; q < SW; q++) d[q] = w[q];
                 if (regc) regc[dst] = col[i];
             }
-            placed += (uint32_t)__popcll(bal);
+            placed += total;
+            if (NT > 64) __syncthreads();  // wave_n is reused by the next step
         }
-        if (lane == 0) {
+        if (tid == 0) {
             A.unit_dregion[u] = placed;
             A.unit_dcount[u] = placed + spilled;
         }
         __syncthreads();  // LDS reused by the next unit
     }
 #undef GG_DEDUP_PREFETCH
+}
+
+// wide: unit_dedup_wide_kernel runs too and takes the larger units
+__global__ __launch_bounds__(64) void unit_dedup_kernel(BucketArgs A, uint32_t n_units, uint32_t wide) {
+    unit_dedup_body<64, 1, DEDUP_SMALL, true>(A, n_units, wide != 0);
+}
+
+__global__ __launch_bounds__(256) void unit_dedup_wide_kernel(BucketArgs A, uint32_t n_units) {
+    unit_dedup_body<256, DEDUP_SMALL, DEDUP_WIDE, false>(A, n_units, true);
 }
 
 // Pass 2b: each unit's stored region records (the region's first
