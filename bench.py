@@ -64,7 +64,7 @@ def kernel_bytes(cnt: np.ndarray, key_words: int) -> dict:
 # commands and the per-kernel table)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_config2.json")
 TRAFFIC_KERNELS = {"bucket_stage": ("gg::bucket_stage_kernel<16>",),
-                   "unit_compact": ("gg::unit_dedup_kernel", "gg::unit_compact_kernel"),
+                   "unit_compact": ("gg::unit_dedup_kernel", "gg::unit_dedup_wide_kernel", "gg::unit_compact_kernel"),
                    "merge": ("gg::merge_lds_seq64_kernel",)}
 
 
