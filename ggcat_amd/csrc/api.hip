@@ -105,6 +105,7 @@ struct ggcat_gpu_ctx {
     void* pin_ring = nullptr;
     hipStream_t copy_stream = nullptr;  // host-to-device uploads of host input, beside the compute stream
     hipEvent_t up_ev[2] = {nullptr, nullptr};  // last upload out of each staging buffer
+    hipEvent_t dd_ev[2] = {nullptr, nullptr};  // the wide deduplication's start and end on the copy stream
     std::vector<uint32_t> cyc_pos;  // the same for the unitigs of cycles (walked on the host)
     // color subsets of the solid table: ids for the color map and the colored
     // FASTA headers (assigned on first use after a merge)
@@ -427,6 +428,7 @@ int ggcat_gpu_create(int device, ggcat_gpu_ctx** out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking);
     for (int i = 0; i < 16 && e == hipSuccess; i++) e = hipEventCreate(&ctx->ev[i]);
     for (int i = 0; i < 2 && e == hipSuccess; i++) e = hipEventCreateWithFlags(&ctx->up_ev[i], hipEventDisableTiming);
+    for (int i = 0; i < 2 && e == hipSuccess; i++) e = hipEventCreateWithFlags(&ctx->dd_ev[i], hipEventDisableTiming);
     hipDeviceProp_t prop;
     if (e == hipSuccess && hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->n_cu = prop.multiProcessorCount;
     // the no-allocation entry point's scratch array: its address on this device
@@ -464,8 +466,10 @@ void ggcat_gpu_destroy(ggcat_gpu_ctx* ctx) {
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (int i = 0; i < 16; i++) (void)hipEventDestroy(ctx->ev[i]);
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < 2; i++) {
         if (ctx->up_ev[i]) (void)hipEventDestroy(ctx->up_ev[i]);
+        if (ctx->dd_ev[i]) (void)hipEventDestroy(ctx->dd_ev[i]);
+    }
     if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -770,19 +774,31 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     // Pass 2: identical records of a unit counted once, unit offsets of the
     // stored records, then every stored record to its unit's slots.
     if (n_units) {
-        const uint32_t grid = (uint32_t)std::min<uint64_t>(n_units, (uint64_t)ctx->n_cu * 64);
         // units of more than DEDUP_SMALL region records: 256-thread workgroups
-        // (their region must fit the registers that stage it: 8 pieces per thread)
+        // (their region must fit the registers that stage it: 8 pieces per
+        // thread) on the copy stream, beside the wavefront-per-unit launch
+        // (the two take disjoint units); both persistent grids are as large
+        // as the CUs hold at once
         const bool wide = unit_cap > DEDUP_SMALL && DEDUP_WIDE * slot_words / 4 <= 8 * 256;
-        hipLaunchKernelGGL(unit_dedup_kernel, dim3(grid), dim3(64),
-                           dedup_lds_bytes(DEDUP_SMALL, slot_words, n_colors != 0), ctx->stream, A, n_units,
-                           wide ? 1u : 0u);
+        const size_t lds_small = dedup_lds_bytes(DEDUP_SMALL, slot_words, n_colors != 0);
+        const size_t lds_wide = dedup_lds_bytes(DEDUP_WIDE, slot_words, n_colors != 0);
+        int occ = 0;
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(unit_dedup_kernel), 64,
+                                                            lds_small));
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(n_units, (uint64_t)ctx->n_cu * std::max(occ, 1));
         if (wide) {
-            const uint32_t wgrid = (uint32_t)std::min<uint64_t>(n_units, (uint64_t)ctx->n_cu * 8);
-            hipLaunchKernelGGL(unit_dedup_wide_kernel, dim3(wgrid), dim3(256),
-                               dedup_lds_bytes(DEDUP_WIDE, slot_words, n_colors != 0), ctx->stream, A, n_units);
+            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &occ, reinterpret_cast<const void*>(unit_dedup_wide_kernel), 256, lds_wide));
+            const uint32_t wgrid = (uint32_t)std::min<uint64_t>(n_units, (uint64_t)ctx->n_cu * std::max(occ, 1));
+            HIPCHK(hipEventRecord(ctx->dd_ev[0], ctx->stream));
+            HIPCHK(hipStreamWaitEvent(ctx->copy_stream, ctx->dd_ev[0], 0));
+            hipLaunchKernelGGL(unit_dedup_wide_kernel, dim3(wgrid), dim3(256), lds_wide, ctx->copy_stream, A, n_units);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipEventRecord(ctx->dd_ev[1], ctx->copy_stream));
         }
+        hipLaunchKernelGGL(unit_dedup_kernel, dim3(grid), dim3(64), lds_small, ctx->stream, A, n_units, wide ? 1u : 0u);
         HIPCHK(hipGetLastError());
+        if (wide) HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->dd_ev[1], 0));
     }
     HIPCHK(hipEventRecord(ctx->ev[15], ctx->stream));
     const int rs = device_scan(ctx, A.unit_dcount, P_<uint64_t>(ctx->unit_off), n_units);

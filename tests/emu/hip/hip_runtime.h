@@ -165,6 +165,11 @@ inline hipError_t hipEventRecord(hipEvent_t e, hipStream_t) { e->t = std::chrono
 inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
 // every call runs to completion in order, so a stream never has to wait
 inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
+// one resident workgroup per CU (workgroups run one at a time here)
+inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, const void*, int, size_t) {
+    *n = 1;
+    return hipSuccess;
+}
 constexpr hipError_t hipErrorNotReady = 600;
 constexpr hipError_t hipErrorInvalidSymbol = 13;
 // device symbols are host variables here: their address is the symbol's own
