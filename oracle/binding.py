@@ -68,7 +68,7 @@ def lib() -> C.CDLL:
     L.orc_query.restype = C.c_uint64
     L.orc_extension.argtypes = [V, C.c_uint32, _u8p]
     L.orc_extension.restype = None
-    L.orc_unitigs.argtypes = [V, C.c_uint32, C.c_int]
+    L.orc_unitigs.argtypes = [V, C.c_uint32, C.c_int, C.c_int]
     L.orc_unitigs.restype = C.c_size_t
     L.orc_unitigs_bytes.argtypes = [V]
     L.orc_unitigs_bytes.restype = C.c_size_t
@@ -205,12 +205,13 @@ class Result:
         L.orc_extension(self.h, k, _p(out, _u8p))
         return out
 
-    def unitigs(self, k: int, sequential: bool = False, joined: bool = False):
+    def unitigs(self, k: int, sequential: bool = False, joined: bool = False, threads: int = 1):
         """Intra-sub-bucket unitigs of the merged table: (list of ASCII sequences,
         sub-bucket per unitig, flags: bit0 open start, bit1 open end, bit2 circular).
-        joined=True: the maximal unitigs of the whole solid k-mer set."""
+        joined=True: the maximal unitigs of the whole solid k-mer set (one
+        graph: single-threaded); otherwise sub-buckets go to `threads` workers."""
         L = lib()
-        n = L.orc_unitigs(self.h, k, 2 if joined else (1 if sequential else 0))
+        n = L.orc_unitigs(self.h, k, 2 if joined else (1 if sequential else 0), threads)
         seq = np.zeros(L.orc_unitigs_bytes(self.h), np.uint8)
         sub, fl = np.zeros(n, np.uint32), np.zeros(n, np.uint8)
         L.orc_unitigs_get(self.h, _p(seq, _u8p), _p(sub, _u32p), _p(fl, _u8p))

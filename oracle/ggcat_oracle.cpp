@@ -1081,7 +1081,7 @@ void orc_sorting_get(void* hv, uint32_t* sub, uint64_t* key_lo, uint64_t* key_hi
 // node per distinct k-mer over all sub-buckets, no side flagged, sub-bucket
 // 0.  Returns the number of unitigs; the data stays in the handle
 // (orc_unitigs_get).
-size_t orc_unitigs(void* hv, uint32_t k, int mode) {
+size_t orc_unitigs(void* hv, uint32_t k, int mode, int threads) {
     Handle* h = (Handle*)hv;
     const SolidSet& S = h->solid;
     h->unitigs = UnitigOut();
@@ -1099,23 +1099,42 @@ size_t orc_unitigs(void* hv, uint32_t k, int mode) {
     }
     std::map<uint32_t, std::vector<size_t>> by_sub;
     for (size_t i = 0; i < S.sub.size(); i++) by_sub[S.sub[i]].push_back(i);
-    for (auto& kv : by_sub) {
-        SubGraph G;
-        G.k = k;
-        for (size_t i : kv.second) {
-            const u128 key = ((u128)S.key_hi[i] << 64) | S.key_lo[i];
-            G.at.emplace(key, (uint32_t)G.key.size());
-            G.key.push_back(key);
-            G.flags.push_back(S.flags[i]);
-        }
-        if (mode == 0) {
-            unitig_chains(G, kv.first, h->unitigs);
-        } else {
-            std::vector<uint32_t> order(G.key.size());
-            for (uint32_t i = 0; i < order.size(); i++) order[i] = i;
-            std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return G.key[a] < G.key[b]; });
-            unitig_sequential(G, kv.first, order, h->unitigs);
-        }
+    std::vector<const std::pair<const uint32_t, std::vector<size_t>>*> subs;
+    for (auto& kv : by_sub) subs.push_back(&kv);
+    // sub-buckets are independent: `threads` workers take contiguous runs of
+    // them and their outputs are concatenated in sub-bucket order
+    if (threads < 1) threads = 1;
+    std::vector<UnitigOut> parts(threads);
+    std::vector<std::thread> th;
+    for (int t = 0; t < threads; t++) {
+        th.emplace_back([&, t] {
+            const size_t a = subs.size() * t / threads, b = subs.size() * (t + 1) / threads;
+            for (size_t s = a; s < b; s++) {
+                const auto& kv = *subs[s];
+                SubGraph G;
+                G.k = k;
+                for (size_t i : kv.second) {
+                    const u128 key = ((u128)S.key_hi[i] << 64) | S.key_lo[i];
+                    G.at.emplace(key, (uint32_t)G.key.size());
+                    G.key.push_back(key);
+                    G.flags.push_back(S.flags[i]);
+                }
+                if (mode == 0) {
+                    unitig_chains(G, kv.first, parts[t]);
+                } else {
+                    std::vector<uint32_t> order(G.key.size());
+                    for (uint32_t i = 0; i < order.size(); i++) order[i] = i;
+                    std::sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return G.key[x] < G.key[y]; });
+                    unitig_sequential(G, kv.first, order, parts[t]);
+                }
+            }
+        });
+    }
+    for (auto& x : th) x.join();
+    for (auto& p : parts) {
+        h->unitigs.seq.insert(h->unitigs.seq.end(), p.seq.begin(), p.seq.end());
+        h->unitigs.sub.insert(h->unitigs.sub.end(), p.sub.begin(), p.sub.end());
+        h->unitigs.flags.insert(h->unitigs.flags.end(), p.flags.begin(), p.flags.end());
     }
     return h->unitigs.sub.size();
 }

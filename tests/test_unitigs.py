@@ -163,3 +163,17 @@ def test_unitigs_and_extension_emulated():
             sorted(zip(osol["sub"].tolist(), osol["key_lo"].tolist(), o.extension(31).tolist()))
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("sequential", [False, True])
+def test_unitigs_threads_same_output(sequential):
+    # sub-buckets are independent: worker threads give the same unitigs, in
+    # the same order, as one thread
+    g = synth.genome(44, 60_000, repeat_frac=0.2, rep_min=300, rep_max=1500)
+    data = synth.reads(g, 45, 6_000, 150, 0.005)
+    r = O.bucketing(data, 31, 15, 3, 1, threads=2)
+    r.merge(31, 3, 1, 0, 2, 2)
+    one = r.unitigs(31, sequential=sequential)
+    many = r.unitigs(31, sequential=sequential, threads=5)
+    assert one[0] == many[0] and np.array_equal(one[1], many[1]) and np.array_equal(one[2], many[2])
+    assert len(one[0]) > 100
