@@ -13,7 +13,6 @@ from __future__ import annotations
 import itertools
 import json
 import os
-import time
 from dataclasses import asdict, dataclass
 
 import numpy as np
@@ -95,11 +94,14 @@ def _digest(ctx) -> tuple:
         return int(s.mult.size), int(np.bitwise_xor.reduce(h)) if h.size else 0, int(h.sum(dtype=np.uint64))
 
 
-def autotune(ctx, sample: np.ndarray, params: Params, grid: list[Schedule] | None = None, repeats: int = 2,
+def autotune(ctx, sample: np.ndarray, params: Params, grid: list[Schedule] | None = None, repeats: int = 3,
              cache_path: str | None = None, colored: bool = False, bucket_logs: tuple[int, ...] = ()) -> dict:
     """Returns {"schedule": Schedule fields, "ms": best bucketing+merge ms on
     the sample, "tried": n, "buckets_log": recommended b}; applies the best
-    schedule to ctx."""
+    schedule to ctx.  A schedule is timed by the device (the bucketing's
+    kernels and the merge call between its stream events, the lowest of
+    `repeats` runs), so host-side call overheads, which do not grow with the
+    input, do not decide between schedules on a small sample."""
     key = f"k{params.k}-m{params.m}-h{params.hash_kind}-c{int(colored)}"
     if cache_path and os.path.exists(cache_path):
         cached = json.load(open(cache_path)).get(key)
@@ -119,10 +121,10 @@ def autotune(ctx, sample: np.ndarray, params: Params, grid: list[Schedule] | Non
         sch.apply(ctx)
         ts = []
         for _ in range(repeats):
-            t0 = time.perf_counter()
             ctx.bucketing(None, params, device_ptr=ptr, n_bytes=n)
             ctx.kmers_merge()
-            ts.append((time.perf_counter() - t0) * 1e3)
+            tm = ctx.timings()
+            ts.append(float(tm[5] + tm[6]))
         d = _digest(ctx)
         if ref is None:
             ref = d
@@ -144,10 +146,10 @@ def autotune(ctx, sample: np.ndarray, params: Params, grid: list[Schedule] | Non
             p2 = Params(params.k, params.m, b, params.second_buckets_log, params.forward_only, params.hash_kind,
                         params.min_multiplicity)
             best[1].apply(ctx)
-            t0 = time.perf_counter()
             ctx.bucketing(None, p2, device_ptr=ptr, n_bytes=n)
             ctx.kmers_merge()
-            ms = (time.perf_counter() - t0) * 1e3
+            tm = ctx.timings()
+            ms = float(tm[5] + tm[6])
             if ms < best[0]:
                 rec_b = b
     finally:
