@@ -445,6 +445,7 @@ void ggcat_gpu_destroy(ggcat_gpu_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
     DevBuf* bufs[] = {&ctx->input, &ctx->seg_bits, &ctx->unit_count, &ctx->unit_cursor, &ctx->unit_off,
                       &ctx->unit_kmers, &ctx->scan_part, &ctx->slots, &ctx->pool, &ctx->pool_cursor,
                       &ctx->unit_dcount, &ctx->unit_dregion,
@@ -657,6 +658,8 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
         hipLaunchKernelGGL(fill32_kernel, dim3(256), dim3(256), 0, ctx->stream, A.seg_bits, 0u, (uint64_t)seg_words);
         hipLaunchKernelGGL(bucket_reset_kernel, dim3(1), dim3(64), 0, ctx->stream, A);
         HIPCHK(hipGetLastError());
+        // no upload of an earlier call may still read the staging buffers
+        HIPCHK(hipStreamSynchronize(ctx->copy_stream));
         uint8_t* ring = static_cast<uint8_t*>(ctx->pin_ring);
         uint8_t* d_dst = P_<uint8_t>(ctx->input);
         const uint64_t margin = 2 * (uint64_t)T + 4 * (uint64_t)p.k + 4096;
