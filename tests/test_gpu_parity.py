@@ -536,6 +536,24 @@ def test_outlier_detection(gpu_ctx):
                                                       ((1 << 4) + 1) << 2, 2, 64)
 
 
+@pytest.mark.parametrize("k,m,kind,unit_bytes", [(31, 15, HASH_SEQ, 6000), (31, 15, HASH_SEQ, 30000),
+                                                   (63, 21, HASH_RK128, 12000)])
+def test_large_unit_deduplication(gpu_ctx, k, m, kind, unit_bytes):
+    # units of several hundred to a few thousand region records: the
+    # 256-thread deduplication takes those of 257..1024 records, larger ones
+    # are kept as they are; records and solid table equal the oracle's
+    g = synth.genome(97, 300_000, repeat_frac=0.2)
+    data = synth.reads(g, 98, 60_000, 150, 0.005)
+    _say(f"case large units k={k} unit_bytes={unit_bytes} bytes={data.size}")
+    gpu_ctx.set_tuning(unit_bytes=unit_bytes)
+    try:
+        _run_both(gpu_ctx, data, k, m, 4, 1, 2, kind, exact=False)
+        c = gpu_ctx.counters()
+        assert int(c[10]) < int(c[1])  # identical records were folded
+    finally:
+        gpu_ctx.set_tuning()
+
+
 @pytest.mark.parametrize("k,m,kind,chunk", [(31, 15, HASH_SEQ, 1 << 20), (63, 21, HASH_RK128, 1 << 18)])
 def test_chunked_host_upload(gpu_ctx, k, m, kind, chunk):
     # host input uploaded chunk by chunk on the copy stream while the staging
