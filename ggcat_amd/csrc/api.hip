@@ -42,6 +42,7 @@ struct DevBuf {
 };
 
 constexpr uint64_t PIN_CHUNK = 64ull << 20;  // bytes per pinned staging buffer (two of them)
+constexpr uint32_t HOST_COPY_THREADS = 16;  // host threads reading or copying input into a staging buffer
 
 }  // namespace
 
@@ -663,7 +664,7 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
         uint8_t* ring = static_cast<uint8_t*>(ctx->pin_ring);
         uint8_t* d_dst = P_<uint8_t>(ctx->input);
         const uint64_t margin = 2 * (uint64_t)T + 4 * (uint64_t)p.k + 4096;
-        const uint32_t n_threads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+        const uint32_t n_threads = std::max(1u, std::min(HOST_COPY_THREADS, std::thread::hardware_concurrency()));
         bool pending[2] = {false, false};
         uint64_t tiles_done = 0;
         uint32_t j = 0;
@@ -1886,7 +1887,7 @@ int device_read_files(ggcat_gpu_ctx* ctx, const std::vector<std::string>& paths,
     bool pending[2] = {false, false};
     uint64_t file_off = 0;
     uint32_t j = 0;
-    const uint32_t n_threads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    const uint32_t n_threads = std::max(1u, std::min(HOST_COPY_THREADS, std::thread::hardware_concurrency()));
     for (size_t i = 0; i < nf; i++) {
         // upload: chunk j of the concatenated files goes through ring buffer j % 2
         const int fd = open(paths[i].c_str(), O_RDONLY);
