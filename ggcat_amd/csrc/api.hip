@@ -416,7 +416,7 @@ int serve_admit(ggcat_gpu_ctx* ctx) {
 extern "C" {
 
 namespace {
-int noalloc_scratch_setup(int device);
+static int noalloc_scratch_setup(int device);
 }  // namespace
 
 int ggcat_gpu_create(int device, ggcat_gpu_ctx** out) {
@@ -864,7 +864,7 @@ struct OutlierInfo {
     uint64_t records;
 };
 
-int find_outliers(ggcat_gpu_ctx* ctx, uint32_t threads, std::vector<OutlierInfo>& found) {
+static int find_outliers(ggcat_gpu_ctx* ctx, uint32_t threads, std::vector<OutlierInfo>& found) {
     found.clear();
     const uint32_t n_sub = ctx->n_sub, sb = ctx->P.second_buckets_log;
     std::vector<uint64_t> rec(n_sub), bas(n_sub);
@@ -932,7 +932,7 @@ int find_outliers(ggcat_gpu_ctx* ctx, uint32_t threads, std::vector<OutlierInfo>
     return GGCAT_GPU_OK;
 }
 
-void free_buf(DevBuf& b) {
+static void free_buf(DevBuf& b) {
     if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.cap = 0;
@@ -963,7 +963,7 @@ namespace {
 // unit per resplit bucket appended after the bucketing's units, as extra
 // sub-buckets n_sub + base_i .. n_sub + base_i + count_i (duplicates bucket
 // last).  Counts are powers of two.
-int resplit_subs(ggcat_gpu_ctx* ctx, const std::vector<OutlierInfo>& found) {
+static int resplit_subs(ggcat_gpu_ctx* ctx, const std::vector<OutlierInfo>& found) {
     int rc = GGCAT_GPU_OK;
     const uint32_t n_out = (uint32_t)found.size();
     const uint32_t n_sub = ctx->n_sub, xb = ctx->xb, n_units = ctx->n_units, SW = ctx->slot_words, k = ctx->P.k;
@@ -1155,7 +1155,7 @@ int resplit_subs(ggcat_gpu_ctx* ctx, const std::vector<OutlierInfo>& found) {
 }
 
 // First resplit sub-bucket of each listed sub-bucket (resplit_subs numbering).
-void resplit_firsts(const ggcat_gpu_ctx* ctx, const std::vector<OutlierInfo>& found, uint32_t* first_sub,
+static void resplit_firsts(const ggcat_gpu_ctx* ctx, const std::vector<OutlierInfo>& found, uint32_t* first_sub,
                     uint32_t cap) {
     uint32_t next = ctx->n_sub;
     for (uint32_t i = 0; i < found.size(); i++) {
@@ -1267,7 +1267,7 @@ namespace {
 
 // crates/io/src/varint.rs:8-22 (encode_varint): 7 bits per byte, low first,
 // bit 7 set while more bytes follow.
-void put_varint(std::vector<uint8_t>& o, uint64_t v) {
+static void put_varint(std::vector<uint8_t>& o, uint64_t v) {
     do {
         o.push_back((uint8_t)((v & 0x7F) | (v > 0x7F ? 0x80 : 0)));
         v >>= 7;
@@ -1277,7 +1277,7 @@ void put_varint(std::vector<uint8_t>& o, uint64_t v) {
 // crates/io/src/varint.rs:26-57 (encode_varint_flags) with FlagsCount = 2:
 // the flags in bits 6-7 of the first byte, 5 value bits and a continuation
 // bit below them, then plain varint bytes.
-void put_varint_flags2(std::vector<uint8_t>& o, uint64_t v, uint8_t flags) {
+static void put_varint_flags2(std::vector<uint8_t>& o, uint64_t v, uint8_t flags) {
     o.push_back((uint8_t)((flags << 6) | (v & 0x1F) | (v > 0x1F ? 0x20 : 0)));
     v >>= 5;
     while (v) {
@@ -1830,7 +1830,7 @@ namespace {
 // is queued behind its upload, so it overlaps the reads of the files after
 // it.  Returns GGCAT_GPU_EAGAIN when the files need the host reader:
 // compressed, not a regular file, or a layout parse.hip leaves to it.
-int device_read_files(ggcat_gpu_ctx* ctx, const std::vector<std::string>& paths, uint32_t k, bool colors,
+static int device_read_files(ggcat_gpu_ctx* ctx, const std::vector<std::string>& paths, uint32_t k, bool colors,
                       uint64_t* n_joined, std::vector<uint64_t>& color_starts, std::vector<uint32_t>& color_ids,
                       uint64_t* est_bases) {
     const size_t nf = paths.size();
@@ -2340,7 +2340,7 @@ namespace {
 
 // Unitigs of the last merge's solid table: per sub-bucket (joined == false)
 // or over the whole table, joined across sub-buckets (joined == true).
-int build_unitigs(ggcat_gpu_ctx* ctx, bool joined, uint64_t* n_unitigs, uint64_t* n_bytes) {
+static int build_unitigs(ggcat_gpu_ctx* ctx, bool joined, uint64_t* n_unitigs, uint64_t* n_bytes) {
     if (!ctx->have_solid) return fail(ctx, GGCAT_GPU_ESTATE, "no merge result");
     if (ctx->P.hash_kind != GGCAT_GPU_HASH_SEQ)
         return fail(ctx, GGCAT_GPU_EINVAL, "unitig extension needs seq-hash keys (the k-mer is the key)");
@@ -2531,7 +2531,7 @@ namespace {
 // on the table's layout (the reference assigns them first-come,
 // crates/colors/src/colors_memmap_writer.rs:50-59, so its ids vary between
 // runs; the sets do not).
-int ensure_subsets(ggcat_gpu_ctx* ctx) {
+static int ensure_subsets(ggcat_gpu_ctx* ctx) {
     if (ctx->have_subsets) return GGCAT_GPU_OK;
     if (!ctx->have_solid) return fail(ctx, GGCAT_GPU_ESTATE, "no merge result");
     if (!ctx->colored) return fail(ctx, GGCAT_GPU_ESTATE, "the last build was not colored");
@@ -2749,7 +2749,7 @@ namespace {
 
 // CanonicalSeqHash of a k-mer of ASCII bases in sequence orientation:
 // (fw, rc) with base i at bits 2i of fw (crates/hashes/src/base/cn_seqhash_base.rs:22-72).
-void seqhash_pair(const uint8_t* s, uint32_t k, u128_t* fw, u128_t* rc) {
+static void seqhash_pair(const uint8_t* s, uint32_t k, u128_t* fw, u128_t* rc) {
     u128_t f = 0, r = 0;
     for (uint32_t i = 0; i < k; i++) {
         const u128_t c = (s[i] >> 1) & 3;
@@ -2764,7 +2764,7 @@ void seqhash_pair(const uint8_t* s, uint32_t k, u128_t* fw, u128_t* rc) {
 // (crates/hashes/src/base/cn_seqhash_base.rs:140-150, constants
 // crates/hashes/src/cn_seqhash.rs:15-27): hash * MULTIPLIER + BASIS, rotated
 // right by 3, low `log` bits; 64-bit arithmetic for k <= 32, 128-bit above.
-uint32_t seqhash_bucket(u128_t h, uint32_t k, uint32_t log) {
+static uint32_t seqhash_bucket(u128_t h, uint32_t k, uint32_t log) {
     if (k <= 32) {
         uint64_t x = (uint64_t)h * 0x00000100000001b3ULL + 0xcbf29ce484222325ULL;
         x = (x >> 3) | (x << 61);

@@ -40,7 +40,7 @@ void* g_scratch_ptr[NA_MAX_DEVICES] = {};
 
 // Looks the array's device address up on `device` (which must be current).
 // Called by ggcat_gpu_create; repeated calls on a device are no-ops.
-int noalloc_scratch_setup(int device) {
+static int noalloc_scratch_setup(int device) {
     if (device < 0 || device >= NA_MAX_DEVICES) return GGCAT_GPU_EINVAL;
     std::lock_guard<std::mutex> lock(g_scratch_mu);
     if (g_scratch_ptr[device]) return GGCAT_GPU_OK;
