@@ -596,6 +596,34 @@ def test_resplit_listed_subbuckets(gpu_ctx, k, m, kind):
     assert int((gpu_ctx.solid().sub >= n_sub).sum()) > 1000
 
 
+def test_resplit_colored(gpu_ctx):
+    # pieces keep their record's color: the colored solid table after the
+    # largest sub-buckets of a 64-genome build are resplit
+    from tests.helpers import bitmaps_from_sets, colored_solid_array
+    wl = synth.config3(0.02, 64)
+    starts, colors = wl.colors
+    b, sb = wl.buckets_log, wl.second_buckets_log
+    n_sub = ((1 << b) + 1) << sb
+    _say(f"case resplit colored bytes={wl.data.size}")
+    gpu_ctx.bucketing(wl.data, _params(31, 15, b, sb, wl.min_mult), colors=(starts, colors))
+    o = O.bucketing_colored(wl.data, 31, 15, b, sb, starts, colors, threads=4)
+    orec = o.records()
+    assert int(orec["length"].max()) <= inline_capacity(31, 15)  # inline slots only
+    cnt = np.bincount(orec["sub"].astype(np.int64), minlength=n_sub)
+    lst = [(int(s), 16) for s in np.argsort(-cnt, kind="stable")[:3]]
+    assert gpu_ctx.resplit(lst) == [n_sub, n_sub + 17, n_sub + 34]
+    o.resplit(31, 15, lst, n_sub)
+    assert np.array_equal(record_digests(gpu_ctx.records(), True), record_digests(o.records(), True))
+    gpu_ctx.kmers_merge()
+    o.merge(31, b, sb, HASH_SEQ, wl.min_mult, 4)
+    words = gpu_ctx.solid_colors().shape[1]
+    got = colored_solid_array(gpu_ctx.solid(), gpu_ctx.solid_colors())
+    off, cols = o.solid_colors()
+    want = colored_solid_array(o.solid(), bitmaps_from_sets(off, cols, words))
+    assert got.shape == want.shape and np.array_equal(got, want)
+    assert int((gpu_ctx.solid().sub >= n_sub).sum()) > 1000
+
+
 def test_resplit_detected_outliers(gpu_ctx):
     # outliers found and resplit on the device: tandem-repeat reads in one
     # duplicates-bucket sub-bucket and 210 000 copies of one 40-base read in
