@@ -19,9 +19,21 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT, "libggcat_gpu_emu.so")
+LIB_ASAN = os.path.join(OUT, "libggcat_gpu_emu_asan.so")  # built with host sanitizers
+
+
+def _newest_input() -> float:
+    """Latest modification time of everything the library is built from."""
+    paths = [os.path.abspath(__file__), os.path.join(HERE, "hip", "hip_runtime.h")]
+    for d in (os.path.join(ROOT, "ggcat_amd", "csrc"), os.path.join(ROOT, "include")):
+        paths += [os.path.join(d, n) for n in os.listdir(d)]
+    return max(os.path.getmtime(p) for p in paths)
 
 
 def build(sanitize: bool = False) -> str:
+    # a library built from the current sources (without sanitizers) is reused
+    if not sanitize and os.path.exists(LIB) and os.path.getmtime(LIB) > _newest_input():
+        return LIB
     src = os.path.join(OUT, "src")
     if os.path.isdir(src):
         shutil.rmtree(src)
@@ -38,14 +50,15 @@ def build(sanitize: bool = False) -> str:
     flags = ["-std=c++20", "-O1", "-g", "-fPIC", "-shared", "-I", HERE, "-Wno-unused-result"]
     if sanitize:
         flags += ["-fsanitize=address,undefined", "-fno-omit-frame-pointer"]
+    out = LIB_ASAN if sanitize else LIB
     cmd = ["g++", *flags, "-x", "c++", os.path.join(src, "ggcat_amd", "csrc", "api.hip"), "-x", "none", glob,
-           "-o", LIB + ".tmp", "-lpthread", "-lz"]
+           "-o", out + ".tmp", "-lpthread", "-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("emulation build failed")
-    os.replace(LIB + ".tmp", LIB)
-    return LIB
+    os.replace(out + ".tmp", out)
+    return out
 
 
 if __name__ == "__main__":

@@ -152,7 +152,7 @@ def test_compaction_counts_identical_records_emulated(emu_ctx):
     # the colored solid table are unchanged, and copies of different colors
     # stay apart
     g = synth.genome(3, 8000)
-    data = synth.reads(g, 4, 800, 120, 0.005)
+    data = synth.reads(g, 4, 400, 120, 0.005)
     emu_ctx.set_tuning(merge_wgs_per_cu=2)
     try:
         _check(emu_ctx, data, 31, 15, 3, 2, 2)
@@ -184,7 +184,7 @@ def test_resplit_emulated(emu_ctx, k, m, kind):
     # records and the solid table, every resplit bucket a sub-bucket of its own
     rng = np.random.default_rng(83)
     g = synth.genome(81, 30_000, repeat_frac=0.3, rep_min=500, rep_max=2000)
-    data = np.concatenate([synth.reads(g, 82, 500, 150, 0.005), joined(adversarial_records(rng)[:30])])
+    data = np.concatenate([synth.reads(g, 82, 250, 150, 0.005), joined(adversarial_records(rng)[:30])])
     b, sb = 3, 1
     n_sub = ((1 << b) + 1) << sb
     emu_ctx.set_tuning(merge_wgs_per_cu=2)
@@ -238,12 +238,12 @@ def test_resplit_colored_emulated(emu_ctx):
         emu_ctx.set_tuning()
 
 
-@pytest.mark.parametrize("k,m,kind,chunk", [(31, 15, HASH_SEQ, 16384), (63, 21, HASH_RK128, 20000)])
+@pytest.mark.parametrize("k,m,kind,chunk", [(31, 15, HASH_SEQ, 8192), (63, 21, HASH_RK128, 10000)])
 def test_chunked_host_upload_emulated(emu_ctx, k, m, kind, chunk):
     # host input uploaded in chunks while the staging pass runs over the
     # tiles already resident: the same records and solid table as the oracle
     g = synth.genome(87, 20_000, repeat_frac=0.2, rep_min=300, rep_max=1000)
-    data = synth.reads(g, 88, 1_200, 150, 0.005)
+    data = synth.reads(g, 88, 600, 150, 0.005)
     assert data.size > 8 * chunk
     emu_ctx.set_tuning(merge_wgs_per_cu=2, upload_chunk=chunk)
     try:

@@ -147,7 +147,7 @@ def test_joined_unitigs_emulated(tmp_path):
         ctx.set_tuning(merge_wgs_per_cu=2)
         g = synth.genome(83, 12_000, repeat_frac=0.2, rep_min=300, rep_max=1000)
         circ = _random_genome(84, 1500)
-        data = np.concatenate([synth.reads(g, 85, 1500, 150, 0.005), _tiled([circ + circ[:200]])])
+        data = np.concatenate([synth.reads(g, 85, 800, 150, 0.005), _tiled([circ + circ[:200]])])
         ctx.bucketing(data, Params(K, 15, 3, 1, 0, HASH_SEQ, 2))
         ctx.kmers_merge()
         gu = ctx.unitigs(joined=True)

@@ -180,7 +180,7 @@ def emu_ctx():
 def test_serialized_streams_decode_to_oracle_records(emu_ctx, k, m):
     from ggcat_amd.native import Params
     g = synth.genome(91, 20_000, repeat_frac=0.2, rep_min=300, rep_max=1000)
-    data = synth.reads(g, 92, 2000, 150, 0.01)
+    data = synth.reads(g, 92, 800, 150, 0.01)
     emu_ctx.bucketing(data, Params(k, m, 3, 2, 0, HASH_SEQ, 2))
     o = O.bucketing(data, k, m, 3, 2, threads=2)
     assert _device_streams(emu_ctx, k, False, 3) == _oracle_streams(o, k, 2, False)
