@@ -56,7 +56,7 @@ struct ggcat_gpu_ctx {
     DevBuf input, seg_bits, unit_count, unit_cursor, unit_off, unit_kmers, scan_part, slots, pool, pool_cursor,
         slot_color, color_starts, colors, error_flag, stage_slots, stage_unit, stage_rank, stage_color, stage_cursor, defer,
         defer_count, kmer_total, spill_slots, spill_color,  // stage_*: the unit regions, the spill list's units and ranks
-        unit_dcount, unit_dregion;  // per unit after deduplication
+        unit_dcount, unit_dregion, unit_skmers;  // per unit after deduplication
     // merge buffers
     DevBuf items, n_items, work, parts, redo, redo_count, out_key, out_val, out_sub, chunk_cursor, chunk_fill_dev,
         fix_key, fix_val, fix_sub, fix_cursor, out_colors, fix_colors;
@@ -449,7 +449,7 @@ void ggcat_gpu_destroy(ggcat_gpu_ctx* ctx) {
     if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
     DevBuf* bufs[] = {&ctx->input, &ctx->seg_bits, &ctx->unit_count, &ctx->unit_cursor, &ctx->unit_off,
                       &ctx->unit_kmers, &ctx->scan_part, &ctx->slots, &ctx->pool, &ctx->pool_cursor,
-                      &ctx->unit_dcount, &ctx->unit_dregion,
+                      &ctx->unit_dcount, &ctx->unit_dregion, &ctx->unit_skmers,
                       &ctx->slot_color, &ctx->color_starts, &ctx->colors, &ctx->error_flag, &ctx->items,
                       &ctx->n_items, &ctx->work, &ctx->parts, &ctx->redo, &ctx->redo_count, &ctx->out_key,
                       &ctx->out_val, &ctx->out_sub, &ctx->chunk_cursor, &ctx->chunk_fill_dev, &ctx->fix_key,
@@ -568,6 +568,7 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     ENSURE(ctx->unit_off, ((size_t)n_units + 1) * 8);
     ENSURE(ctx->unit_dcount, (size_t)n_units * 4);
     ENSURE(ctx->unit_dregion, (size_t)n_units * 4);
+    ENSURE(ctx->unit_skmers, (size_t)n_units * 4);
     ENSURE(ctx->pool_cursor, 8);
     ENSURE(ctx->error_flag, 8);
     if (n_colors) {
@@ -597,6 +598,7 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
     A.sub_kmers = P_<uint32_t>(ctx->unit_kmers);
     A.unit_dcount = P_<uint32_t>(ctx->unit_dcount);
     A.unit_dregion = P_<uint32_t>(ctx->unit_dregion);
+    A.unit_skmers = P_<uint32_t>(ctx->unit_skmers);
     A.slot_words = slot_words;
     A.pool_cursor = P_<unsigned long long>(ctx->pool_cursor);
     A.color_starts = n_colors ? P_<uint64_t>(ctx->color_starts) : nullptr;
@@ -1061,11 +1063,12 @@ static int resplit_subs(ggcat_gpu_ctx* ctx, const std::vector<OutlierInfo>& foun
         return fail(ctx, GGCAT_GPU_EHIP, "resplit: a record does not fit its slot or the pool");
     }
     // the rebuilt unit arrays and arena
-    DevBuf dcount2, count2, kmers2, off2, slots2, color2, pool2;
+    DevBuf dcount2, count2, kmers2, skmers2, off2, slots2, color2, pool2;
     auto cleanup2 = [&]() {
         free_buf(dcount2);
         free_buf(count2);
         free_buf(kmers2);
+        free_buf(skmers2);
         free_buf(off2);
         free_buf(slots2);
         free_buf(color2);
@@ -1074,6 +1077,7 @@ static int resplit_subs(ggcat_gpu_ctx* ctx, const std::vector<OutlierInfo>& foun
     if ((rc = ensure(ctx, dcount2, n_units_all * 4)) != GGCAT_GPU_OK ||
         (rc = ensure(ctx, count2, n_units_all * 4)) != GGCAT_GPU_OK ||
         (rc = ensure(ctx, kmers2, n_units_all * 4)) != GGCAT_GPU_OK ||
+        (rc = ensure(ctx, skmers2, n_units_all * 4)) != GGCAT_GPU_OK ||
         (rc = ensure(ctx, off2, (n_units_all + 1) * 8)) != GGCAT_GPU_OK) {
         cleanup();
         cleanup2();
@@ -1081,9 +1085,9 @@ static int resplit_subs(ggcat_gpu_ctx* ctx, const std::vector<OutlierInfo>& foun
     }
     hipLaunchKernelGGL(resplit_units_kernel, dim3((uint32_t)std::min<uint64_t>((n_units_all + 255) / 256, 65535)),
                        dim3(256), 0, ctx->stream, P_<uint32_t>(ctx->unit_dcount), P_<uint32_t>(ctx->unit_count),
-                       P_<uint32_t>(ctx->unit_kmers), P_<uint8_t>(isout_b), n_units, (uint32_t)n_units_all, xb,
-                       n_sub, R.ext_records, R.ext_kmers, P_<uint32_t>(dcount2), P_<uint32_t>(count2),
-                       P_<uint32_t>(kmers2));
+                       P_<uint32_t>(ctx->unit_kmers), P_<uint32_t>(ctx->unit_skmers), P_<uint8_t>(isout_b), n_units,
+                       (uint32_t)n_units_all, xb, n_sub, R.ext_records, R.ext_kmers, P_<uint32_t>(dcount2),
+                       P_<uint32_t>(count2), P_<uint32_t>(kmers2), P_<uint32_t>(skmers2));
     RS_CHK(hipGetLastError());
     if ((rc = device_scan(ctx, P_<uint32_t>(dcount2), P_<uint64_t>(off2), (uint32_t)n_units_all)) != GGCAT_GPU_OK) {
         cleanup();
@@ -1135,6 +1139,7 @@ static int resplit_subs(ggcat_gpu_ctx* ctx, const std::vector<OutlierInfo>& foun
     std::swap(ctx->unit_dcount, dcount2);
     std::swap(ctx->unit_count, count2);
     std::swap(ctx->unit_kmers, kmers2);
+    std::swap(ctx->unit_skmers, skmers2);
     std::swap(ctx->unit_off, off2);
     std::swap(ctx->slots, slots2);
     if (ctx->colored) std::swap(ctx->slot_color, color2);
@@ -1413,7 +1418,7 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->n_items.p, 0, 8, ctx->stream));
     hipLaunchKernelGGL(make_items_kernel, dim3((ctx->n_sub_all + 255) / 256), dim3(256), 0, ctx->stream,
-                       P_<uint32_t>(ctx->unit_kmers), P_<uint64_t>(ctx->unit_off), ctx->n_sub_all, ctx->xb, item_occ,
+                       P_<uint32_t>(ctx->unit_skmers), P_<uint64_t>(ctx->unit_off), ctx->n_sub_all, ctx->xb, item_occ,
                        P_<WorkItem>(ctx->items), P_<unsigned int>(ctx->n_items));
     HIPCHK(hipGetLastError());
     uint32_t n_items = 0;

@@ -79,6 +79,9 @@ struct BucketArgs {
     // plus spilled ones), of which distinct region records
     uint32_t* unit_dcount;
     uint32_t* unit_dregion;
+    // k-mer positions per unit once identical records are stored once (the
+    // inserts the merge makes; sizes its work items and tables)
+    uint32_t* unit_skmers;
     const uint64_t* color_starts;
     const uint32_t* colors;
     uint32_t n_colors;
@@ -1060,6 +1063,7 @@ GG_DEV void unit_dedup_body(const BucketArgs& A, uint32_t n_units, bool wide) {
     __shared__ uint32_t mult[HI];  // 0: a copy of another record
     __shared__ uint32_t table[2 * HI];  // record index + 1, 0 = free
     __shared__ uint32_t wave_n[NT / 64];
+    __shared__ uint32_t folded_k;  // k-mer positions of the copies folded into earlier records
     constexpr uint32_t TABLE = 2 * HI;
     const uint32_t SW = A.slot_words, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     uint32_t* rec = reinterpret_cast<uint32_t*>(lds);  // slot words
@@ -1100,6 +1104,7 @@ GG_DEV void unit_dedup_body(const BucketArgs& A, uint32_t n_units, bool wide) {
             if (PASS && tid == 0 && !(wide && n > DEDUP_SMALL && n <= DEDUP_WIDE)) {
                 A.unit_dregion[u] = n;
                 A.unit_dcount[u] = cnt;
+                A.unit_skmers[u] = A.sub_kmers[u];
             }
             cnt = cnt_next;
             if (un < n_units) GG_DEDUP_PREFETCH(un, cnt);
@@ -1120,6 +1125,7 @@ GG_DEV void unit_dedup_body(const BucketArgs& A, uint32_t n_units, bool wide) {
             mult[i] = 1;
         }
         for (uint32_t i = tid; i < TABLE; i += NT) table[i] = 0;
+        if (tid == 0) folded_k = 0;
         __syncthreads();
         // the next unit's records are in flight meanwhile
         cnt = cnt_next;
@@ -1147,7 +1153,10 @@ GG_DEV void unit_dedup_body(const BucketArgs& A, uint32_t n_units, bool wide) {
                         if (seen == m) break;
                         m = seen;
                     }
-                    if (m < MAX_MULT) mult[i] = 0;
+                    if (m < MAX_MULT) {
+                        mult[i] = 0;
+                        atomicAdd(&folded_k, meta_len(w[0]) - A.k + 1);  // inserts this copy no longer makes
+                    }
                     break;
                 }
                 slot = (slot + 1) & (TABLE - 1);
@@ -1185,6 +1194,7 @@ GG_DEV void unit_dedup_body(const BucketArgs& A, uint32_t n_units, bool wide) {
         if (tid == 0) {
             A.unit_dregion[u] = placed;
             A.unit_dcount[u] = placed + spilled;
+            A.unit_skmers[u] = A.sub_kmers[u] - folded_k;
         }
         __syncthreads();  // LDS reused by the next unit
     }

@@ -266,28 +266,31 @@ __global__ void resplit_write_kernel(ResplitArgs A) {
 }
 
 // Per-unit arrays of the rebuilt arena: the outliers' units emptied, each
-// extra sub-bucket's pieces in its first unit.
+// extra sub-bucket's pieces in its first unit (k-mer occurrences and
+// stored k-mer positions alike: pieces are not deduplicated).
 __global__ void resplit_units_kernel(const uint32_t* dcount, const uint32_t* count, const uint32_t* kmers,
-                                     const uint8_t* is_outlier, uint32_t n_units, uint32_t n_units_all,
-                                     uint32_t xb, uint32_t first_extra, const uint32_t* ext_records,
-                                     const uint32_t* ext_kmers, uint32_t* dcount2, uint32_t* count2,
-                                     uint32_t* kmers2) {
+                                     const uint32_t* skmers, const uint8_t* is_outlier, uint32_t n_units,
+                                     uint32_t n_units_all, uint32_t xb, uint32_t first_extra,
+                                     const uint32_t* ext_records, const uint32_t* ext_kmers, uint32_t* dcount2,
+                                     uint32_t* count2, uint32_t* kmers2, uint32_t* skmers2) {
     for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n_units_all; u += gridDim.x * blockDim.x) {
-        uint32_t d = 0, c = 0, km = 0;
+        uint32_t d = 0, c = 0, km = 0, skm = 0;
         if (u < n_units) {
             if (!is_outlier[u >> xb]) {
                 d = dcount[u];
                 c = count[u];
                 km = kmers[u];
+                skm = skmers[u];
             }
         } else if ((u & ((1u << xb) - 1)) == 0) {
             const uint32_t e = (u >> xb) - first_extra;
             d = c = ext_records[e];
-            km = ext_kmers[e];
+            km = skm = ext_kmers[e];
         }
         dcount2[u] = d;
         count2[u] = c;
         kmers2[u] = km;
+        skmers2[u] = skm;
     }
 }
 
