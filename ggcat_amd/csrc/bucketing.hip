@@ -1130,6 +1130,7 @@ GG_DEV void unit_dedup_body(const BucketArgs& A, uint32_t n_units, bool wide) {
         // the next unit's records are in flight meanwhile
         cnt = cnt_next;
         if (un < n_units) GG_DEDUP_PREFETCH(un, cnt);
+        uint32_t my_folded = 0;
         for (uint32_t i = tid; i < n; i += NT) {
             const uint32_t* w = rec + i * SW;
             if (w[0] & META_LONG) continue;  // pool records: bases elsewhere, kept as they are
@@ -1155,13 +1156,16 @@ GG_DEV void unit_dedup_body(const BucketArgs& A, uint32_t n_units, bool wide) {
                     }
                     if (m < MAX_MULT) {
                         mult[i] = 0;
-                        atomicAdd(&folded_k, meta_len(w[0]) - A.k + 1);  // inserts this copy no longer makes
+                        my_folded += meta_len(w[0]) - A.k + 1;  // inserts this copy no longer makes
                     }
                     break;
                 }
                 slot = (slot + 1) & (TABLE - 1);
             }
         }
+        // one LDS atomic per wavefront for the folded copies' k-mer positions
+        for (uint32_t d = 32; d >= 1; d >>= 1) my_folded += __shfl_down(my_folded, d);
+        if (lane == 0 && my_folded) atomicAdd(&folded_k, my_folded);
         __syncthreads();
         // stored records to the front of the region, in region order
         uint32_t placed = 0;
