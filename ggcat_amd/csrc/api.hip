@@ -67,6 +67,7 @@ struct ggcat_gpu_ctx {
     uint64_t n_records = 0, pool_words = 0, n_solid = 0, n_input = 0, kmer_occ = 0, n_big = 0, n_fix = 0;
     uint64_t n_stored = 0;  // arena slots: records with identical copies stored once
     uint32_t n_sub_all = 0;  // sub-buckets including the resplit buckets of outliers
+    uint32_t dd_max = 0;  // the compaction deduplicated the units of up to this many records
     bool resplit_done = false;  // the arena holds resplit buckets (ggcat_gpu_resplit_outliers)
     uint64_t n_spill = 0;  // records of the last bucketing past their unit's region
     uint32_t n_chunks_used = 0;  // output chunks of the last merge
@@ -786,6 +787,7 @@ int ggcat_gpu_bucketing(ggcat_gpu_ctx* ctx, const uint8_t* data, uint64_t n_byte
         // (the two take disjoint units); both persistent grids are as large
         // as the CUs hold at once
         const bool wide = unit_cap > DEDUP_SMALL && DEDUP_WIDE * slot_words / 4 <= 8 * 256;
+        ctx->dd_max = wide ? DEDUP_WIDE : DEDUP_SMALL;
         const size_t lds_small = dedup_lds_bytes(DEDUP_SMALL, slot_words, n_colors != 0);
         const size_t lds_wide = dedup_lds_bytes(DEDUP_WIDE, slot_words, n_colors != 0);
         int occ = 0;
@@ -1397,6 +1399,8 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     // identical records counted once (seq-hash keys up to k = 32, inline slots)
     M.dedup = (ctx->slot_words <= 8 && ctx->tune.no_dedup != 1) ? 1 : 0;
     M.tag_kmers = (p.hash_kind == GGCAT_GPU_HASH_SEQ && p.k <= 32 && ctx->tune.no_dedup != 2) ? 1 : 0;
+    // work items whose units the compaction deduplicated skip the record hashing
+    M.rec_direct = ctx->tune.no_dedup != 3 ? 1 : 0;
     // workgroup size of the merge (128 or 256 threads); a chunk of records
     // (rec_cap / 2) has at most one record per thread
     const uint32_t merge_threads = ctx->tune.merge_threads == 128 ? 128u : 256u;
@@ -1419,7 +1423,7 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     HIPCHK(hipMemsetAsync(ctx->n_items.p, 0, 8, ctx->stream));
     hipLaunchKernelGGL(make_items_kernel, dim3((ctx->n_sub_all + 255) / 256), dim3(256), 0, ctx->stream,
                        P_<uint32_t>(ctx->unit_skmers), P_<uint64_t>(ctx->unit_off), ctx->n_sub_all, ctx->xb, item_occ,
-                       P_<WorkItem>(ctx->items), P_<unsigned int>(ctx->n_items));
+                       ctx->dd_max, P_<WorkItem>(ctx->items), P_<unsigned int>(ctx->n_items));
     HIPCHK(hipGetLastError());
     uint32_t n_items = 0;
     HIPCHK(hipMemcpyAsync(&n_items, ctx->n_items.p, 4, hipMemcpyDeviceToHost, ctx->stream));

@@ -30,7 +30,8 @@ constexpr uint32_t CNT_MASK = 0x1FFFFFFFu;
 constexpr uint32_t VAL_CROSS = 1u << 29;
 
 struct WorkItem {
-    uint32_t u0, u1;  // physical units [u0, u1)
+    uint32_t u0;  // first physical unit
+    uint32_t dup;  // 1: a unit of the item may still hold identical records (not deduplicated by the compaction)
     uint32_t occ;  // k-mer occurrences
     uint32_t sub;  // sub-bucket
     uint64_t r0, r1;  // its slots [r0, r1)
@@ -63,6 +64,7 @@ struct MergeArgs {
                          // (seq-hash, k <= 32); otherwise each distinct record's k-mers are rolled by one thread
     uint32_t rec_lds_off;  // byte offset of the record table in dynamic LDS
     uint32_t rec_cap;  // record table slots
+    uint32_t rec_direct;  // items without dup: records placed in the record table by position, not hashed
     uint32_t claim;  // work entries claimed per atomic on the work cursor
     const WorkItem* items;
     const PartWork* parts;  // null: first pass, one work entry per item
@@ -469,7 +471,7 @@ template <int MODE, class KP, class SP, class VP>
 GG_DEV void insert_item_dedup(const MergeArgs& A, uint64_t r0, uint64_t r1, const PartWork& w, KP klo, KP khi, SP st,
                               VP vals, uint32_t* cbits, uint32_t mask, uint32_t probe_limit, uint32_t* overflow,
                               uint16_t* claimed, uint32_t* n_claimed, const RecTable& R, const uint32_t* pre,
-                              uint32_t pre_color) {
+                              uint32_t pre_color, bool hashed) {
     const uint32_t W = Mode<MODE>::color_words(A), KW = Mode<MODE>::key_words(A);
     const uint32_t SW = Mode<MODE>::slot_words(A), k = A.k, RC = A.rec_cap;
     const uint32_t part_mask = (1u << w.log_parts) - 1;
@@ -502,7 +504,15 @@ GG_DEV void insert_item_dedup(const MergeArgs& A, uint64_t r0, uint64_t r1, cons
         const uint32_t meta = pre[0];
         const uint32_t color = pre_color;
         bool direct = (meta & META_LONG) != 0;
-        if (!direct) {
+        if (!direct && !hashed) {
+            // the item's records are distinct: thread t's record takes table
+            // slot t (t < rec_cap / 2)
+            const uint32_t i = threadIdx.x;
+            for (uint32_t q = 0; q < SW; q++) R.key[i * SW + q] = pre[q];
+            R.col[i] = color;
+            R.cnt[i] = meta_mult(meta);
+            R.state[i] = 2;
+        } else if (!direct) {
             uint32_t wv[Mode<MODE>::MAX_SW];
             for (uint32_t q = 0; q < SW; q++) wv[q] = pre[q];
             uint32_t i = rec_hash(wv, SW, color) & (RC - 1);
@@ -764,6 +774,7 @@ GG_DEV void merge_lds_body(const MergeArgs& A) {
         if (w.log_parts) cap = cap_max;
         const uint32_t mask = cap - 1;
         if (Mode<MODE>::dedup(A)) {
+            const bool hashed = !A.rec_direct || it.dup;
             // records in chunks of rec_cap / 2 <= 256 (the record table's load
             // factor stays at most one half), one record per thread; identical
             // records of one chunk are counted once, the k-mer table
@@ -775,7 +786,7 @@ GG_DEV void merge_lds_body(const MergeArgs& A) {
                 uint32_t nw[MSW] = {}, ncol = 0;
                 load_rec(c1 + threadIdx.x, it.r1, nw, ncol);
                 insert_item_dedup<MODE>(A, c0, c1, w, klo, khi, st, vals, cbits, mask, 96u, &s_overflow, claimed,
-                                  &s_claimed, rec_table(A, lds), cw, ccol);
+                                  &s_claimed, rec_table(A, lds), cw, ccol, hashed);
 #pragma unroll
                 for (uint32_t q = 0; q < MSW; q++) cw[q] = nw[q];
                 ccol = ncol;
@@ -817,13 +828,15 @@ __global__ __launch_bounds__(256) void merge_lds_rk128_kernel(MergeArgs A) { mer
 
 // One thread per sub-bucket: runs of its units become work items of at most
 // `group_occ` occurrences (a single unit above that is an item of its own),
-// each with its slot range.
+// each with its slot range.  An item is marked dup when one of its units
+// stores more than `dd_max` records: the compaction deduplicates units of up
+// to dd_max records, so only such units can still hold identical records.
 __global__ void make_items_kernel(const uint32_t* unit_kmers, const uint64_t* unit_off, uint32_t n_sub, uint32_t xb,
-                                  uint32_t group_occ, WorkItem* items, unsigned int* n_items) {
+                                  uint32_t group_occ, uint32_t dd_max, WorkItem* items, unsigned int* n_items) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n_sub) return;
     const uint32_t u_begin = s << xb, u_end = (s + 1) << xb;
-    uint32_t start = u_begin, acc = 0;
+    uint32_t start = u_begin, acc = 0, dup = 0;
     for (uint32_t u = u_begin; u < u_end; u++) {
         const uint32_t o = unit_kmers[u];
         if (acc == 0 && o == 0) {
@@ -831,13 +844,15 @@ __global__ void make_items_kernel(const uint32_t* unit_kmers, const uint64_t* un
             continue;
         }
         if (acc > 0 && acc + o > group_occ) {
-            items[atomicAdd(n_items, 1u)] = WorkItem{start, u, acc, s, unit_off[start], unit_off[u]};
+            items[atomicAdd(n_items, 1u)] = WorkItem{start, dup, acc, s, unit_off[start], unit_off[u]};
             start = u;
             acc = 0;
+            dup = 0;
         }
         acc += o;
+        if (unit_off[u + 1] - unit_off[u] > dd_max) dup = 1;
     }
-    if (acc > 0) items[atomicAdd(n_items, 1u)] = WorkItem{start, u_end, acc, s, unit_off[start], unit_off[u_end]};
+    if (acc > 0) items[atomicAdd(n_items, 1u)] = WorkItem{start, dup, acc, s, unit_off[start], unit_off[u_end]};
 }
 
 }  // namespace gg

@@ -212,7 +212,12 @@ int ggcat_gpu_solid_colors(ggcat_gpu_ctx* ctx, uint32_t* color_words, uint32_t* 
  *                     records could be counted once; 2: count identical
  *                     records once but let one thread roll each distinct
  *                     record's k-mers (the mode of 128-bit keys) instead of
- *                     spreading single k-mer positions over the threads
+ *                     spreading single k-mer positions over the threads;
+ *                     3: hash the records of every work item, also of those
+ *                     whose units the bucketing already deduplicated (no
+ *                     unit above the deduplication's 1024-record limit),
+ *                     which otherwise take their records by position
+ *                     (no mode changes a result)
  *   unit_bytes        bucketing: input bytes per physical unit (sub-bucket
  *                     refinement); default 2000, 1000 for colored builds
  *   use_graphs        1: capture the bucketing staging pass (clears, stage

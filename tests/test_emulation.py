@@ -176,6 +176,26 @@ def test_compaction_counts_identical_records_emulated(emu_ctx):
         emu_ctx.set_tuning()
 
 
+@pytest.mark.parametrize("k,m,kind,big", [(31, 15, HASH_SEQ, False), (31, 15, HASH_SEQ, True),
+                                          (63, 21, HASH_RK128, False)])
+def test_direct_record_placement_emulated(emu_ctx, k, m, kind, big):
+    # work items whose units the compaction deduplicated place their records
+    # in the record table by position; items with a unit above the
+    # deduplication's limit (big: two sub-buckets of one unit each, over a
+    # thousand records) still hash them; the solid table is the oracle's
+    # either way
+    g = synth.genome(3, 8000)
+    data = synth.reads(g, 4, 400, 120, 0.005)
+    emu_ctx.set_tuning(merge_wgs_per_cu=2, unit_bytes=(1 << 20) if big else 0)
+    try:
+        b, sb = (1, 0) if big else (3, 2)
+        _check(emu_ctx, data, k, m, b, sb, 2, kind)
+        if big:
+            assert int(emu_ctx.counters()[10]) > 2 * 1024  # a unit stores more than the limit
+    finally:
+        emu_ctx.set_tuning()
+
+
 @pytest.mark.parametrize("k,m,kind", [(31, 15, HASH_SEQ), (63, 21, HASH_RK128)])
 def test_resplit_emulated(emu_ctx, k, m, kind):
     # the three largest sub-buckets and the largest duplicates-bucket

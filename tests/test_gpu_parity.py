@@ -220,15 +220,22 @@ def test_unit_region_spill(gpu_ctx, unit_cap):
         gpu_ctx.set_tuning()
 
 
-@pytest.mark.parametrize("mode", [1, 2])
-def test_dedup_modes_keep_results(gpu_ctx, mode):
-    # every record's k-mers inserted (1), or identical records counted once
-    # and each distinct record rolled by one thread (2)
+@pytest.mark.parametrize("mode,unit_bytes", [(1, 0), (2, 0), (3, 0), (0, 1 << 22), (3, 1 << 22)])
+def test_dedup_modes_keep_results(gpu_ctx, mode, unit_bytes):
+    # every record's k-mers inserted (1), identical records counted once
+    # and each distinct record rolled by one thread (2), or every work item's
+    # records hashed, also where the compaction deduplicated its units (3);
+    # with 128 sub-buckets of one unit each (unit_bytes 4 MiB), units hold
+    # over a thousand records, above the deduplication's limit, and the
+    # default mode hashes those items' records too
     wl = synth.config1(0.01)
-    _say(f"case no_dedup={mode} bytes={wl.data.size}")
-    gpu_ctx.set_tuning(no_dedup=mode)
+    b, sb = (5, 2) if unit_bytes else (wl.buckets_log, wl.second_buckets_log)
+    _say(f"case no_dedup={mode} unit_bytes={unit_bytes} b={b} sb={sb} bytes={wl.data.size}")
+    gpu_ctx.set_tuning(no_dedup=mode, unit_bytes=unit_bytes)
     try:
-        _run_both(gpu_ctx, wl.data, 31, 15, wl.buckets_log, wl.second_buckets_log, exact=False)
+        _run_both(gpu_ctx, wl.data, 31, 15, b, sb, exact=False)
+        if unit_bytes:
+            assert int(gpu_ctx.counters()[10]) > 1024 * (1 << (b + sb)) // 2
     finally:
         gpu_ctx.set_tuning()
 
