@@ -505,9 +505,10 @@ GG_DEV void insert_item_dedup(const MergeArgs& A, uint64_t r0, uint64_t r1, cons
         const uint32_t color = pre_color;
         bool direct = (meta & META_LONG) != 0;
         if (!direct && !hashed) {
-            // the item's records are distinct: thread t's record takes table
-            // slot t (t < rec_cap / 2)
-            const uint32_t i = threadIdx.x;
+            // the item's records are distinct: thread t's record takes the
+            // first of the table slots thread t reads in phase B (t < chunk
+            // <= min(rec_cap, blockDim); both powers of two)
+            const uint32_t i = RC > blockDim.x ? threadIdx.x * (RC / blockDim.x) : threadIdx.x;
             for (uint32_t q = 0; q < SW; q++) R.key[i * SW + q] = pre[q];
             R.col[i] = color;
             R.cnt[i] = meta_mult(meta);
@@ -720,7 +721,8 @@ GG_DEV void merge_lds_body(const MergeArgs& A) {
         for (uint32_t i = threadIdx.x; i < A.rec_cap; i += blockDim.x) R.state[i] = 0;
     }
     // each thread's record of the current chunk (dedup path)
-    const uint32_t chunk = A.rec_cap / 2, SW = Mode<MODE>::slot_words(A);
+    uint32_t chunk = A.rec_cap / 2;
+    const uint32_t SW = Mode<MODE>::slot_words(A);
     constexpr uint32_t MSW = Mode<MODE>::MAX_SW;
     auto load_rec = [&](uint64_t r, uint64_t r_end, uint32_t* wv, uint32_t& col) {
         if (threadIdx.x < chunk && r < r_end) {
@@ -779,7 +781,12 @@ GG_DEV void merge_lds_body(const MergeArgs& A) {
             // factor stays at most one half), one record per thread; identical
             // records of one chunk are counted once, the k-mer table
             // accumulates over the chunks.  Each thread's record of the next
-            // chunk is loaded while the current chunk is processed.
+            // chunk is loaded while the current chunk is processed.  Records
+            // placed by position without per-k-mer tags (each distinct record
+            // rolled by one thread) fill the whole record table: one record
+            // per thread of the workgroup.
+            chunk = (hashed || Mode<MODE>::tag_kmers(A)) ? A.rec_cap / 2
+                                                         : (A.rec_cap < blockDim.x ? A.rec_cap : blockDim.x);
             load_rec(it.r0 + threadIdx.x, it.r1, cw, ccol);
             for (uint64_t c0 = it.r0; c0 < it.r1; c0 += chunk) {
                 const uint64_t c1 = c0 + chunk < it.r1 ? c0 + chunk : it.r1;
