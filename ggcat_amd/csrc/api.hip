@@ -101,6 +101,8 @@ struct ggcat_gpu_ctx {
     std::vector<uint8_t> cyc_seq, cyc_flags;
     std::vector<uint32_t> cyc_sub;
     DevBuf u_path;  // colored builds: table position of every unitig k-mer, in written order
+    DevBuf rk_pos;  // rabin-karp128 per-position start terms of the merge's rolling hash
+    uint32_t rk_pos_k = 0;  // k they were computed for (0: none)
     // input files read straight into HBM: raw bytes, parse state per chunk,
     // and a pinned two-buffer staging ring for the uploads
     DevBuf raw, p_stats, p_carry, p_keep, p_keep_off, p_flags, p_bits;
@@ -462,7 +464,7 @@ void ggcat_gpu_destroy(ggcat_gpu_ctx* ctx) {
                       &ctx->u_rest_n, &ctx->u_rep, &ctx->u_tab, &ctx->u_path, &ctx->stage_slots, &ctx->stage_unit,
                       &ctx->stage_rank, &ctx->stage_color, &ctx->stage_cursor, &ctx->defer, &ctx->defer_count,
                       &ctx->kmer_total, &ctx->spill_slots, &ctx->spill_color, &ctx->raw, &ctx->p_stats,
-                      &ctx->p_carry, &ctx->p_keep, &ctx->p_keep_off, &ctx->p_flags, &ctx->p_bits};
+                      &ctx->p_carry, &ctx->p_keep, &ctx->p_keep_off, &ctx->p_flags, &ctx->p_bits, &ctx->rk_pos};
     if (ctx->pin_ring) (void)hipHostFree(ctx->pin_ring);
     if (ctx->server) (void)ggcat_gpu_serve_close(ctx);
     if (ctx->graph_a) (void)hipGraphExecDestroy(ctx->graph_a);
@@ -1409,6 +1411,26 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     const size_t lds_bytes = M.dedup ? M.rec_lds_off + rec_table_bytes(ctx->slot_words, M.tag_kmers ? p.k : 0xFFFFu, M.rec_cap) : (size_t)M.lds_cap * entry_bytes + 64;
     M.pool_words = ctx->pool_words;
     set_rk_constants(M, p.k);
+    if (p.hash_kind == GGCAT_GPU_HASH_RK128) {
+        // start terms of the rolling hash per base position and base
+        if (ctx->rk_pos_k != p.k) {
+            std::vector<u128_t> tab((size_t)8 * (p.k - 1));
+            for (uint32_t i = 0; i + 1 < p.k; i++) {
+                u128_t pf = 1, pb = M.rk_mult;  // M^(k-2-i), M^(i+1)
+                for (uint32_t e = 0; e + 2 + i < p.k; e++) pf *= M.rk_mult;
+                for (uint32_t e = 0; e < i; e++) pb *= M.rk_mult;
+                for (uint32_t c = 0; c < 4; c++) {
+                    tab[2 * (4 * i + c)] = M.rk_fwd[c] * pf;
+                    tab[2 * (4 * i + c) + 1] = M.rk_bkw[c] * pb;
+                }
+            }
+            ctx->rk_pos_k = 0;
+            ENSURE(ctx->rk_pos, tab.size() * sizeof(u128_t));
+            HIPCHK(hipMemcpy(ctx->rk_pos.p, tab.data(), tab.size() * sizeof(u128_t), hipMemcpyHostToDevice));
+            ctx->rk_pos_k = p.k;
+        }
+        M.rk_pos = P_<u128_t>(ctx->rk_pos);
+    }
 
     // work items: runs of units of one sub-bucket
     ENSURE(ctx->items, (size_t)ctx->n_units * sizeof(WorkItem));

@@ -95,6 +95,9 @@ struct MergeArgs {
     u128_t rk_mult, rk_mult_inv, rk_rmmult;
     u128_t rk_fwd[4], rk_bkw[4];
     u128_t rk_fwd_rm[4], rk_bkw_rm[4];  // rk_fwd[c] * rk_rmmult, rk_bkw[c] * rk_rmmult
+    // per base position i < k - 1 of a k-mer and base c: entry 2(4i + c) is
+    // rk_fwd[c] * rk_mult^(k-2-i), entry 2(4i + c) + 1 is rk_bkw[c] * rk_mult^(i+1)
+    const u128_t* rk_pos;
 };
 
 GG_DEV uint32_t hash128(uint64_t lo, uint64_t hi) { return tab_hash64(lo ^ (hi * 0x9E3779B97F4A7C15ULL)); }
@@ -238,12 +241,19 @@ GG_DEV void for_each_kmer(const MergeArgs& A, const uint32_t* slot, CB&& cb) {
             }
         }
     } else {
-        // rabin-karp128 rolling (cn_rkhash_base.rs:65-110)
+        // rabin-karp128 rolling (cn_rkhash_base.rs:65-110).  The state over
+        // the first k - 1 bases, fh = sum fwd[c_i] M^(k-2-i) and bk = sum
+        // bkw[c_i] M^(i+1) (what the Horner loops of the reference's
+        // init compute), is summed from the per-position table: independent
+        // loads and 128-bit adds instead of two chains of k - 1 dependent
+        // 128-bit multiplies
         u128_t fh = 0, bk = 0;
         auto code = [&](uint32_t i) -> uint32_t { return (bw[i >> 4] >> (2 * (i & 15))) & 3; };
-        for (uint32_t i = 0; i + 1 < k; i++) fh = fh * A.rk_mult + A.rk_fwd[code(i)];
-        for (int i = (int)k - 2; i >= 0; i--) bk = bk * A.rk_mult + A.rk_bkw[code(i)];
-        bk = bk * A.rk_mult;
+        for (uint32_t i = 0; i + 1 < k; i++) {
+            const u128_t* e = A.rk_pos + 2 * (4 * i + code(i));
+            fh += e[0];
+            bk += e[1];
+        }
         for (uint32_t idx = k - 1; idx < len; idx++) {
             uint32_t in = code(idx), out = code(idx - (k - 1));
             u128_t cur_f = fh * A.rk_mult + A.rk_fwd[in];
