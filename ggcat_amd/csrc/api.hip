@@ -1408,7 +1408,13 @@ int ggcat_gpu_kmers_merge(ggcat_gpu_ctx* ctx) {
     const uint32_t merge_threads = ctx->tune.merge_threads == 128 ? 128u : 256u;
     M.rec_cap = std::min<uint32_t>(ctx->tune.rec_cap ? ctx->tune.rec_cap : REC_CAP_DEFAULT, 2 * merge_threads);
     M.rec_lds_off = (uint32_t)(((size_t)M.lds_cap * entry_bytes + 64 + 15) / 16 * 16);
-    const size_t lds_bytes = M.dedup ? M.rec_lds_off + rec_table_bytes(ctx->slot_words, M.tag_kmers ? p.k : 0xFFFFu, M.rec_cap) : (size_t)M.lds_cap * entry_bytes + 64;
+    // the uncolored rabin-karp128 variant spreads the distinct records' k-mers
+    // over the workgroup in segments (tags in the record table's LDS)
+    const bool rk_seg = M.dedup && key_words == 2 && W == 0 && !M.tag_kmers && ctx->slot_words == 8 &&
+                        p.hash_kind == GGCAT_GPU_HASH_RK128;
+    const size_t lds_bytes =
+        M.dedup ? M.rec_lds_off + rec_table_bytes(ctx->slot_words, (M.tag_kmers || rk_seg) ? p.k : 0xFFFFu, M.rec_cap, rk_seg)
+                : (size_t)M.lds_cap * entry_bytes + 64;
     M.pool_words = ctx->pool_words;
     set_rk_constants(M, p.k);
     if (p.hash_kind == GGCAT_GPU_HASH_RK128) {

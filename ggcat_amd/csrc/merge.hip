@@ -190,6 +190,38 @@ GG_DEV uint32_t tab_insert128(KP klo, KP khi, SP st, VP vals, uint32_t mask, uin
     return claimed;
 }
 
+// Rabin-karp128 hashes (cn_rkhash_base.rs:65-110) of the k-mers with index
+// [i0, i1) of packed bases bw; cb(lo, hi, is_forward, idx).  The state over
+// the first k - 1 bases of k-mer i0, fh = sum fwd[c] M^(k-2-i) and
+// bk = sum bkw[c] M^(i+1) over its base offsets i (what the Horner loops of
+// the reference's init compute), is summed from the per-position table:
+// independent loads and 128-bit adds instead of two chains of k - 1
+// dependent 128-bit multiplies; then each k-mer is one rolling step.
+template <class CB>
+GG_DEV void rk_segment(const MergeArgs& A, const uint32_t* bw, uint32_t i0, uint32_t i1, CB&& cb) {
+    const uint32_t k = A.k;
+    auto code = [&](uint32_t i) -> uint32_t { return (bw[i >> 4] >> (2 * (i & 15))) & 3; };
+    u128_t fh = 0, bk = 0;
+    for (uint32_t i = 0; i + 1 < k; i++) {
+        const u128_t* e = A.rk_pos + 2 * (4 * i + code(i0 + i));
+        fh += e[0];
+        bk += e[1];
+    }
+    for (uint32_t x = i0; x < i1; x++) {
+        const uint32_t in = code(x + k - 1), out = code(x);
+        const u128_t cur_f = fh * A.rk_mult + A.rk_fwd[in];
+        fh = cur_f - A.rk_fwd_rm[out];
+        const u128_t cur_b = bk * A.rk_mult_inv + A.rk_bkw_rm[in];
+        bk = cur_b - A.rk_bkw[out];
+        const u128_t key = cur_f < cur_b ? cur_f : cur_b;
+        cb((uint64_t)key, (uint64_t)(key >> 64), cur_f < cur_b, x);
+    }
+}
+
+// k-mers per segment when the k-mers of rabin-karp128 records are spread
+// over the threads of a workgroup (each segment starts from the table)
+constexpr uint32_t RK_SEG = 8;
+
 // Iterate the k-mers of one slot; cb(lo, hi, is_forward, idx, last_idx).
 template <int MODE, class CB>
 GG_DEV void for_each_kmer(const MergeArgs& A, const uint32_t* slot, CB&& cb) {
@@ -241,28 +273,8 @@ GG_DEV void for_each_kmer(const MergeArgs& A, const uint32_t* slot, CB&& cb) {
             }
         }
     } else {
-        // rabin-karp128 rolling (cn_rkhash_base.rs:65-110).  The state over
-        // the first k - 1 bases, fh = sum fwd[c_i] M^(k-2-i) and bk = sum
-        // bkw[c_i] M^(i+1) (what the Horner loops of the reference's
-        // init compute), is summed from the per-position table: independent
-        // loads and 128-bit adds instead of two chains of k - 1 dependent
-        // 128-bit multiplies
-        u128_t fh = 0, bk = 0;
-        auto code = [&](uint32_t i) -> uint32_t { return (bw[i >> 4] >> (2 * (i & 15))) & 3; };
-        for (uint32_t i = 0; i + 1 < k; i++) {
-            const u128_t* e = A.rk_pos + 2 * (4 * i + code(i));
-            fh += e[0];
-            bk += e[1];
-        }
-        for (uint32_t idx = k - 1; idx < len; idx++) {
-            uint32_t in = code(idx), out = code(idx - (k - 1));
-            u128_t cur_f = fh * A.rk_mult + A.rk_fwd[in];
-            fh = cur_f - A.rk_fwd_rm[out];
-            u128_t cur_b = bk * A.rk_mult_inv + A.rk_bkw_rm[in];
-            bk = cur_b - A.rk_bkw[out];
-            u128_t key = cur_f < cur_b ? cur_f : cur_b;
-            cb((uint64_t)key, (uint64_t)(key >> 64), cur_f < cur_b, idx + 1 - k, last);
-        }
+        rk_segment(A, bw, 0, last + 1,
+                   [&](uint64_t lo, uint64_t hi, bool fwd, uint32_t x) { cb(lo, hi, fwd, x, last); });
     }
 }
 
@@ -448,8 +460,13 @@ GG_DEV RecTable rec_table(const MergeArgs& A, uint8_t* lds) {
 
 // Bytes of dynamic LDS the record table needs (k = 0xFFFF: no per-k-mer tags,
 // the rolling mode of 128-bit keys).
-__host__ __device__ inline uint32_t rec_table_bytes(uint32_t slot_words, uint32_t k, uint32_t rec_cap) {
-    return (rec_cap * slot_words + 3 * rec_cap + 6) * 4 + (rec_cap / 2) * rec_max_kmers(slot_words, k) * 2 + 4;
+// With rk_seg the tags are per segment of RK_SEG k-mers, for up to rec_cap
+// records (the uncolored rabin-karp128 variant).
+__host__ __device__ inline uint32_t rec_table_bytes(uint32_t slot_words, uint32_t k, uint32_t rec_cap,
+                                                    bool rk_seg = false) {
+    const uint32_t tags = rk_seg ? rec_cap * ((rec_max_kmers(slot_words, k) + RK_SEG - 1) / RK_SEG)
+                                 : (rec_cap / 2) * rec_max_kmers(slot_words, k);
+    return (rec_cap * slot_words + 3 * rec_cap + 6) * 4 + tags * 2 + 4;
 }
 
 GG_DEV uint32_t rec_hash(const uint32_t* w, uint32_t sw, uint32_t color) {
@@ -567,7 +584,7 @@ GG_DEV void insert_item_dedup(const MergeArgs& A, uint64_t r0, uint64_t r1, cons
     }
 phase_a_done:
     __syncthreads();
-    if (!Mode<MODE>::tag_kmers(A)) {
+    if (!Mode<MODE>::tag_kmers(A) && MODE != 2) {
         // 128-bit keys: each thread rolls the k-mers of the distinct records
         // in its table slots (rec_cap / blockDim of them) and frees the slots
         const uint32_t per = RC > blockDim.x ? RC / blockDim.x : 1u;
@@ -586,9 +603,10 @@ phase_a_done:
     }
     // phase B: one exclusive scan over the table slots of their k-mer counts
     // places the distinct records' k-mers (each position tagged with its
-    // record's slot and k-mer index); each thread owns rec_cap / blockDim
-    // consecutive slots (at least one) and frees them for the next chunk
-    // once read.
+    // record's slot and k-mer index; the rabin-karp128 variant tags segments
+    // of RK_SEG k-mers with their index instead); each thread owns
+    // rec_cap / blockDim consecutive slots (at least one) and frees them for
+    // the next chunk once read.
     {
         const uint32_t per = RC > blockDim.x ? RC / blockDim.x : 1u;
         const uint32_t s0 = threadIdx.x * per;
@@ -599,7 +617,8 @@ phase_a_done:
             const uint32_t si = s0 + q;
             if (si < RC && R.state[si] == 2) {
                 const uint32_t len = meta_len(R.key[si * SW]);
-                pk[q] = len >= k ? len - k + 1 : 0u;
+                const uint32_t nk = len >= k ? len - k + 1 : 0u;
+                pk[q] = MODE == 2 ? (nk + RK_SEG - 1) / RK_SEG : nk;
                 R.state[si] = 0;
             }
             mine += pk[q];
@@ -629,10 +648,24 @@ phase_a_done:
         if (threadIdx.x == 0) R.n[1] = all;
         __syncthreads();
     }
-    // phase B2: every k-mer of the distinct records, one per thread
+    // phase B2: every k-mer of the distinct records, one per thread (the
+    // rabin-karp128 variant: one segment of RK_SEG k-mers per thread)
     const uint32_t total = R.n[1];
     for (uint32_t j0 = 0; j0 < total; j0 += blockDim.x) {
         const uint32_t j = j0 + threadIdx.x;
+        if (MODE == 2) {
+            if (j < total) {
+                const uint32_t v = R.rec_of[j];
+                const uint32_t si = v >> 7, i0 = (v & 127u) * RK_SEG;
+                const uint32_t* rw = R.key + si * SW;
+                const uint32_t cnt = R.cnt[si], meta = rw[0], last = meta_len(meta) - k;
+                const uint32_t i1 = i0 + RK_SEG < last + 1 ? i0 + RK_SEG : last + 1;
+                rk_segment(A, rw + 1, i0, i1, [&](uint64_t lo, uint64_t hi, bool fwd, uint32_t x) {
+                    insert_key(lo, hi, occurrence_bits(meta, fwd, x, last) - 1u + cnt, 0u);
+                });
+            }
+            continue;
+        }
         bool cl = false;
         uint32_t at = 0;
         if (j < total) {
