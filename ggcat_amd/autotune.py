@@ -95,13 +95,15 @@ def _digest(ctx) -> tuple:
 
 
 def autotune(ctx, sample: np.ndarray, params: Params, grid: list[Schedule] | None = None, repeats: int = 3,
-             cache_path: str | None = None, colored: bool = False, bucket_logs: tuple[int, ...] = ()) -> dict:
+             cache_path: str | None = None, colored: bool = False, bucket_logs: tuple[int, ...] = (),
+             min_gain: float = 0.01) -> dict:
     """Returns {"schedule": Schedule fields, "ms": best bucketing+merge ms on
     the sample, "tried": n, "buckets_log": recommended b}; applies the best
     schedule to ctx.  A schedule is timed by the device (the bucketing's
     kernels and the merge call between its stream events, the lowest of
     `repeats` runs), so host-side call overheads, which do not grow with the
-    input, do not decide between schedules on a small sample."""
+    input, do not decide between schedules on a small sample; a schedule
+    must be at least `min_gain` faster than the best so far to replace it."""
     key = f"k{params.k}-m{params.m}-h{params.hash_kind}-c{int(colored)}"
     if cache_path and os.path.exists(cache_path):
         cached = json.load(open(cache_path)).get(key)
@@ -132,7 +134,11 @@ def autotune(ctx, sample: np.ndarray, params: Params, grid: list[Schedule] | Non
             raise RuntimeError(f"schedule {sch} changed the solid table")
         tried += 1
         ms = min(ts)
-        if best is None or ms < best[0]:
+        # a later schedule replaces the best only when it is at least
+        # min_gain faster: of schedules within the sample's timing noise of
+        # each other the one timed first is kept (the second stage times the
+        # first stage's best before its variations)
+        if best is None or ms < best[0] * (1.0 - min_gain):
             best = (ms, sch)
 
     try:

@@ -220,7 +220,7 @@ GG_DEV void rk_segment(const MergeArgs& A, const uint32_t* bw, uint32_t i0, uint
 
 // k-mers per segment when the k-mers of rabin-karp128 records are spread
 // over the threads of a workgroup (each segment starts from the table)
-constexpr uint32_t RK_SEG = 8;
+constexpr uint32_t RK_SEG = 16;
 
 // Iterate the k-mers of one slot; cb(lo, hi, is_forward, idx, last_idx).
 template <int MODE, class CB>
