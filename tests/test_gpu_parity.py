@@ -118,6 +118,25 @@ def test_rk128_k63(gpu_ctx):
     _run_both(gpu_ctx, data, 63, 21, 7, 3, s=2, kind=HASH_RK128)
 
 
+@pytest.mark.parametrize("no_dedup,unit_bytes", [(3, 0), (0, 1 << 22)])
+def test_rk128_k63_hashed_records(gpu_ctx, no_dedup, unit_bytes):
+    # the rabin-karp128 merge's k-mer segments over records that went through
+    # the record table's hashing: every work item (no_dedup=3), or the items
+    # with units above the deduplication's limit (32 sub-buckets of one unit
+    # each, units of 4 MiB of input)
+    g = synth.genome(44, 200_000, repeat_frac=0.3, rep_min=500, rep_max=2000)
+    data = synth.reads(g, 45, 40_000, 150, 0.005)
+    b, sb = (7, 3) if not unit_bytes else (4, 1)
+    _say(f"case rk128 k=63 m=21 no_dedup={no_dedup} unit_bytes={unit_bytes} b={b} sb={sb} bytes={data.size}")
+    gpu_ctx.set_tuning(no_dedup=no_dedup, unit_bytes=unit_bytes)
+    try:
+        _run_both(gpu_ctx, data, 63, 21, b, sb, s=2, kind=HASH_RK128)
+        if unit_bytes:
+            assert int(gpu_ctx.counters()[10]) > 1024 * (1 << (b + sb)) // 2
+    finally:
+        gpu_ctx.set_tuning()
+
+
 def test_seqhash_k63(gpu_ctx):
     g = synth.genome(46, 100_000)
     data = synth.reads(g, 47, 20_000, 150, 0.005)
